@@ -1,0 +1,68 @@
+"""ctypes binding of libadock_hip.so (C ABI in include/adock_hip.h).
+
+The library is built in-tree by adock_amd.build. There is no fallback: if the library is missing or the GPU is not
+visible, every entry point raises.
+"""
+import ctypes
+from pathlib import Path
+
+import torch
+
+LIB_PATH = Path(__file__).resolve().parent / "libadock_hip.so"
+
+P, I, F, U64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_ulonglong
+LIG = [I, I] + [P] * 11 + [I, P]            # AM, KM, ref..rg, P_MAX, gslot
+GRID = [P] * 7 + [I, F, F]                  # grid..gdims, T, spacing, pen
+SF = [P] * 5
+REC = [P, P, P, I]
+
+
+class AdockError(RuntimeError):
+    pass
+
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            raise AdockError(f"{LIB_PATH} is not built: run `python -m adock_amd.build` (hipcc, gfx950)")
+        L = ctypes.CDLL(str(LIB_PATH))
+        L.adock_version.restype = ctypes.c_char_p
+        L.adock_last_error.restype = ctypes.c_char_p
+        L.adock_build_grid.argtypes = [P, P, I, P, P, P, P, P, I, F, F, F, F, I, I, I, P, I, P, P]
+        L.adock_eg.argtypes = LIG + GRID + SF + REC + [I, P, I, P, P, P, P]
+        L.adock_mc.argtypes = LIG + GRID + SF + REC + [I, P, P, I, P, P, I, P, P, P, P, I, I, I, F, F, F, U64, I, P]
+        L.adock_mc_counted.argtypes = (LIG + GRID + SF + REC +
+                                       [I, P, P, I, P, P, I, P, P, P, P, I, I, I, F, F, F, U64, I, P, P])
+        L.adock_set_chains_per_block.argtypes = [I]
+        L.adock_get_chains_per_block.restype = I
+        for name in ("adock_build_grid", "adock_eg", "adock_mc", "adock_mc_counted", "adock_set_chains_per_block"):
+            getattr(L, name).restype = I
+        _LIB = L
+    return _LIB
+
+
+def check(status):
+    if status != 0:
+        raise AdockError(f"libadock_hip status {status}: {lib().adock_last_error().decode()}")
+
+
+def ptr(t):
+    """Device pointer of a CUDA (ROCm) tensor; None -> NULL."""
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise AdockError("adock_amd kernels take device tensors (torch cuda/ROCm)")
+    return t.data_ptr()
+
+
+def stream_of(device):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def require_gpu():
+    if not torch.cuda.is_available():
+        raise AdockError("adock_amd needs a visible GPU (MI355X, gfx950)")

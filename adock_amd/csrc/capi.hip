@@ -1,0 +1,116 @@
+// extern "C" entry points of libadock_hip.so (declared in include/adock_hip.h): argument validation, bucket
+// dispatch, error text. No allocation, no copies, no synchronisation.
+#include <cstdio>
+#include <string>
+
+#include "../../include/adock_hip.h"
+#include "common.hpp"
+
+#include "kernels.hpp"
+
+using namespace adock;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char* msg) {
+    g_err = msg;
+    return code;
+}
+static int hip_status(int st) {
+    if (st != 0) g_err = std::string("HIP launch error: ") + hipGetErrorString((hipError_t)st);
+    return st;
+}
+
+static int check_buckets(int AM, int KM, int P_MAX) {
+    if (AM != 64) return fail(ADOCK_E_BUCKET, "AM must be 64 (at most 64 heavy atoms per ligand)");
+    if (KM != 8 && KM != 16 && KM != 24 && KM != 32) return fail(ADOCK_E_BUCKET, "KM must be one of 8, 16, 24, 32");
+    if (P_MAX < 1 || !lds_budget_ok(P_MAX)) return fail(ADOCK_E_LDS, "P_MAX out of range for the per-wave LDS budget");
+    return 0;
+}
+
+extern "C" {
+
+const char* adock_version(void) { return "adock_hip 0.1 gfx950"; }
+const char* adock_last_error(void) { return g_err.c_str(); }
+
+int adock_build_grid(const float* rxyz, const int* rtype, int n_rec, const float* radius, const int* hyd,
+                     const int* don, const int* acc, const int* lig_types, int n_lt, float ox, float oy, float oz,
+                     float spacing, int nx, int ny, int nz, const float* prm, int per_term, float* out,
+                     adock_stream_t stream) {
+    if (n_rec < 0 || n_lt < 0 || nx < 0 || ny < 0 || nz < 0) return fail(ADOCK_E_ARG, "negative size");
+    if ((long long)nx * ny * nz == 0 || n_lt == 0) return 0;
+    if (!radius || !hyd || !don || !acc || !lig_types || !prm || !out || (n_rec > 0 && (!rxyz || !rtype)))
+        return fail(ADOCK_E_ARG, "null pointer");
+    if (n_lt > 24) return fail(ADOCK_E_TYPES, "at most 24 ligand types per call");
+    if ((long long)nx * ny * nz > (1ll << 31) - 1) return fail(ADOCK_E_ARG, "grid too large");
+    return hip_status(launch_build_grid(rxyz, rtype, n_rec, radius, hyd, don, acc, lig_types, n_lt, ox, oy, oz,
+                                        spacing, nx, ny, nz, prm, per_term, out, (hipStream_t)stream));
+}
+
+#define LIG_DECL                                                                                                     \
+    const float *ref, const int *typ, const unsigned char *tm, const int *ax, const float *axp, const float *axu,    \
+        const int *pr, const int *na, const int *nt, const int *np, const float *rg, int P_MAX, const int *gslot
+#define GRID_DECL                                                                                                    \
+    const float *grid, const float *origin, const float *lo, const float *hi, const int *rec_idx,                    \
+        const long long *goffs, const long long *gdims, int T, float spacing, float pen
+#define SF_DECL const float *prm, const float *rad, const int *hyd, const int *don, const int *acc
+#define REC_DECL const float *rxyz, const int *rtyp, const int *rn, int RN_MAX
+
+static int common_checks(int AM, int KM, int P_MAX, const void* ref, const void* grid, const void* prm, int exact,
+                         const void* rxyz, const void* rn, int RN_MAX, const void* chain_lig, int n_chains) {
+    if (int st = check_buckets(AM, KM, P_MAX)) return st;
+    if (n_chains < 0) return fail(ADOCK_E_ARG, "negative n_chains");
+    if (!ref || !grid || !prm || !chain_lig) return fail(ADOCK_E_ARG, "null pointer");
+    if (exact && (!rxyz || !rn || RN_MAX <= 0)) return fail(ADOCK_E_ARG, "exact mode needs receptor atoms");
+    return 0;
+}
+
+int adock_eg(int AM, int KM, LIG_DECL, GRID_DECL, SF_DECL, REC_DECL, int exact, const int* chain_lig, int n_chains,
+             const float* state, float* e_out, float* g_out, adock_stream_t stream) {
+    if (int st = common_checks(AM, KM, P_MAX, ref, grid, prm, exact, rxyz, rn, RN_MAX, chain_lig, n_chains)) return st;
+    if (n_chains == 0) return 0;
+    if (!state || !e_out || !g_out) return fail(ADOCK_E_ARG, "null pointer");
+    LigArgs L{ref, typ, tm, ax, axp, axu, pr, na, nt, np, rg, P_MAX, gslot};
+    GridArgs G{grid, origin, lo, hi, rec_idx, goffs, gdims, T, spacing, pen};
+    SFArgs S{prm, rad, hyd, don, acc};
+    RecArgs R{rxyz, rtyp, rn, RN_MAX};
+    return hip_status(launch_eg(KM, L, G, S, R, exact, chain_lig, n_chains, state, e_out, g_out, (hipStream_t)stream));
+}
+
+int adock_mc_counted(int AM, int KM, LIG_DECL, GRID_DECL, SF_DECL, REC_DECL, int exact, const int* chain_lig,
+                     const int* chain_id, int n_chains, const int* nbr, const int* nbr_n, int NB_MAX, float* cur,
+                     float* cur_e, float* best, float* best_e, int init_opt, int mc_steps, int bfgs_iters, float temp,
+                     float amp_pos, float amp_rot, unsigned long long seed, int step_offset,
+                     unsigned long long* eval_count, adock_stream_t stream) {
+    if (int st = common_checks(AM, KM, P_MAX, ref, grid, prm, exact, rxyz, rn, RN_MAX, chain_lig, n_chains)) return st;
+    if (n_chains == 0) return 0;
+    if (!chain_id || !cur || !cur_e || !best || !best_e) return fail(ADOCK_E_ARG, "null pointer");
+    if (mc_steps < 0) return fail(ADOCK_E_ARG, "negative mc_steps");
+    if (nbr && (!nbr_n || NB_MAX <= 0)) return fail(ADOCK_E_ARG, "nbr needs nbr_n and NB_MAX > 0");
+    LigArgs L{ref, typ, tm, ax, axp, axu, pr, na, nt, np, rg, P_MAX, gslot};
+    GridArgs G{grid, origin, lo, hi, rec_idx, goffs, gdims, T, spacing, pen};
+    SFArgs S{prm, rad, hyd, don, acc};
+    RecArgs R{rxyz, rtyp, rn, RN_MAX};
+    McArgs M{chain_id, nbr, nbr_n, NB_MAX, cur, cur_e, best, best_e, init_opt, mc_steps, bfgs_iters, temp, amp_pos,
+             amp_rot, seed, step_offset, eval_count};
+    return hip_status(launch_mc(KM, L, G, S, R, exact, chain_lig, n_chains, M, (hipStream_t)stream));
+}
+
+int adock_mc(int AM, int KM, LIG_DECL, GRID_DECL, SF_DECL, REC_DECL, int exact, const int* chain_lig,
+             const int* chain_id, int n_chains, const int* nbr, const int* nbr_n, int NB_MAX, float* cur, float* cur_e,
+             float* best, float* best_e, int init_opt, int mc_steps, int bfgs_iters, float temp, float amp_pos,
+             float amp_rot, unsigned long long seed, int step_offset, adock_stream_t stream) {
+    return adock_mc_counted(AM, KM, ref, typ, tm, ax, axp, axu, pr, na, nt, np, rg, P_MAX, gslot, grid, origin, lo, hi,
+                            rec_idx, goffs, gdims, T, spacing, pen, prm, rad, hyd, don, acc, rxyz, rtyp, rn, RN_MAX,
+                            exact, chain_lig, chain_id, n_chains, nbr, nbr_n, NB_MAX, cur, cur_e, best, best_e,
+                            init_opt, mc_steps, bfgs_iters, temp, amp_pos, amp_rot, seed, step_offset, nullptr,
+                            stream);
+}
+
+int adock_set_chains_per_block(int w) {
+    if (set_waves_per_block(w)) return fail(ADOCK_E_ARG, "chains_per_block must be 1..4");
+    return 0;
+}
+int adock_get_chains_per_block(void) { return get_waves_per_block(); }
+
+}  // extern "C"

@@ -1,0 +1,471 @@
+// Fused docking search kernels for gfx950: pose energy + analytic gradient, Vina-style BFGS, Monte Carlo chains.
+//
+// Same algorithm as the reference's fused search kernel (src/adock/mckernel.py:21-447): torsion-tree forward
+// kinematics, trilinear grid lookup with the quadratic out-of-box penalty, intra-ligand pair terms, gradient in pose
+// space (translation, world-frame torque, torsions), BFGS with Armijo backtracking, MC mutation + Metropolis.
+//
+// MI355X mapping (one wave64 = one chain; W chains per workgroup, independent):
+//   * lane a holds heavy atom a (<= 64 atoms): its reference coordinates, radius, type flags, grid-slot pointer and
+//     the bit mask of torsions that move it live in registers for the whole chain;
+//   * lane 7+k computes sin/cos of torsion k once per evaluation; the forward-kinematics loop broadcasts them with
+//     v_readlane (scalar registers), and the reference axes come from scalar loads;
+//   * per-wave LDS holds world coordinates + radius (float4 per atom), type flags, the pair list (u16 i | j << 8) and
+//     the pair-force accumulators (ds_add_f32); pairs are strided over lanes;
+//   * every reduction (energy, 3 forces, 3 torques, one per torsion, BFGS dot products) is a DPP row scan + row
+//     broadcast, no LDS round trip;
+//   * BFGS vectors are lane-distributed (lane i = component i) and lane i keeps row i of the inverse Hessian in
+//     registers, so H.g and the rank-2 update are register FMAs against readlane-broadcast operands.
+//   Gradient lane layout: 0-2 translation, 3-5 torque, 6 always zero, 7+k torsion k (state layout t(3) q(4) theta:
+//   torsion k sits on the same lane in the state and in the gradient; the zero lane is inert in BFGS).
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace adock {
+
+namespace {
+
+constexpr int AMK = 64;   // heavy-atom bucket handled by these kernels (one atom per lane)
+
+DEVI int wave_lds_bytes(int P_MAX) { return 16 * AMK + 12 * AMK + 4 * AMK + 2 * ((P_MAX + 7) & ~7); }
+
+struct WaveLDS {
+    float4* X4;            // world xyz + radius
+    float* Fx; float* Fy; float* Fz;
+    int* flg;
+    unsigned short* pl;
+};
+
+struct ChainCtx {
+    // lane = atom
+    float rx, ry, rz, rad;
+    int flg;
+    unsigned tmask;
+    bool atom;
+    const float* gp;
+    // lane = torsion
+    int axa, axb;
+    // uniform
+    const float* axp; const float* axu;
+    int na, nt, np;
+    float rg;
+    int ny, nz;
+    float o0, o1, o2, lo0, lo1, lo2, hi0, hi1, hi2, inv_s, pen, fmx0, fmx1, fmx2;
+    int exact;
+    const float* rrx; const int* rrt; int rn; const int* nb; int nbn;
+    const float* sfrad; const int* sfhyd; const int* sfdon; const int* sfacc;
+};
+
+DEVI WaveLDS carve_lds(char* base, int P_MAX) {
+    WaveLDS w;
+    w.X4 = reinterpret_cast<float4*>(base);
+    w.Fx = reinterpret_cast<float*>(base + 16 * AMK);
+    w.Fy = w.Fx + AMK;
+    w.Fz = w.Fy + AMK;
+    w.flg = reinterpret_cast<int*>(base + 28 * AMK);
+    w.pl = reinterpret_cast<unsigned short*>(base + 32 * AMK);
+    (void)P_MAX;
+    return w;
+}
+
+template <int KM>
+DEVI ChainCtx setup_chain(int l, int lane, const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecArgs& R,
+                          int exact, const WaveLDS& w) {
+    ChainCtx c;
+    c.na = __builtin_amdgcn_readfirstlane(L.na[l]);
+    c.nt = __builtin_amdgcn_readfirstlane(L.nt[l]);
+    c.np = __builtin_amdgcn_readfirstlane(L.np[l]);
+    c.rg = L.rg[l];
+    const int ri = __builtin_amdgcn_readfirstlane(G.rec_idx[l]);
+    const int nx = (int)G.gdims[3 * ri];
+    c.ny = (int)G.gdims[3 * ri + 1]; c.nz = (int)G.gdims[3 * ri + 2];
+    c.o0 = G.origin[3 * ri]; c.o1 = G.origin[3 * ri + 1]; c.o2 = G.origin[3 * ri + 2];
+    c.lo0 = G.lo[3 * ri]; c.lo1 = G.lo[3 * ri + 1]; c.lo2 = G.lo[3 * ri + 2];
+    c.hi0 = G.hi[3 * ri]; c.hi1 = G.hi[3 * ri + 1]; c.hi2 = G.hi[3 * ri + 2];
+    c.inv_s = 1.f / G.spacing; c.pen = G.pen;
+    c.fmx0 = (float)(nx - 1) - 1e-4f; c.fmx1 = (float)(c.ny - 1) - 1e-4f; c.fmx2 = (float)(c.nz - 1) - 1e-4f;
+    const long long n3 = (long long)nx * c.ny * c.nz;
+    c.axp = L.axp + (long long)l * KM * 3;
+    c.axu = L.axu + (long long)l * KM * 3;
+    c.atom = lane < c.na;
+    c.rx = c.ry = c.rz = 0.f; c.rad = 0.f; c.flg = 0; c.tmask = 0u;
+    c.gp = G.grid + G.goffs[ri];
+    if (c.atom) {
+        const long long ia = (long long)l * AMK + lane;
+        c.rx = L.ref[3 * ia]; c.ry = L.ref[3 * ia + 1]; c.rz = L.ref[3 * ia + 2];
+        const int t = L.typ[ia];
+        c.rad = S.rad[t];
+        c.flg = S.hyd[t] | (S.don[t] << 1) | (S.acc[t] << 2);
+        c.gp += (long long)L.gslot[ia] * n3;
+        for (int k = 0; k < c.nt; ++k)
+            c.tmask |= (L.tm[((long long)l * KM + k) * AMK + lane] ? 1u : 0u) << k;
+    }
+    c.axa = c.axb = 0;
+    if (lane < c.nt) {
+        c.axa = L.ax[((long long)l * KM + lane) * 2];
+        c.axb = L.ax[((long long)l * KM + lane) * 2 + 1];
+    }
+    w.flg[lane] = c.flg;
+    const int* pr = L.pr + (long long)l * L.P_MAX * 2;
+    for (int q = lane; q < c.np; q += WAVE) w.pl[q] = (unsigned short)(pr[2 * q] | (pr[2 * q + 1] << 8));
+    c.exact = exact;
+    c.rrx = R.rxyz + (long long)ri * R.RN_MAX * 3;
+    c.rrt = R.rtyp + (long long)ri * R.RN_MAX;
+    c.rn = exact ? R.rn[ri] : 0;
+    c.nb = nullptr; c.nbn = 0;
+    c.sfrad = S.rad; c.sfhyd = S.hyd; c.sfdon = S.don; c.sfacc = S.acc;
+    wave_lds_sync();
+    return c;
+}
+
+// Energy and lane-distributed gradient of the lane-distributed state x. Returns E (uniform).
+DEVI float eval_eg(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, int lane, float x, float& gout) {
+    const float t0 = bcast(x, 0), t1 = bcast(x, 1), t2 = bcast(x, 2);
+    float qw = bcast(x, 3), qx = bcast(x, 4), qy = bcast(x, 5), qz = bcast(x, 6);
+    const float nq = fast_rsq(qw * qw + qx * qx + qy * qy + qz * qz);
+    qw *= nq; qx *= nq; qy *= nq; qz *= nq;
+    const float R00 = 1.f - 2.f * (qy * qy + qz * qz), R01 = 2.f * (qx * qy - qz * qw), R02 = 2.f * (qx * qz + qy * qw);
+    const float R10 = 2.f * (qx * qy + qz * qw), R11 = 1.f - 2.f * (qx * qx + qz * qz), R12 = 2.f * (qy * qz - qx * qw);
+    const float R20 = 2.f * (qx * qz - qy * qw), R21 = 2.f * (qy * qz + qx * qw), R22 = 1.f - 2.f * (qx * qx + qy * qy);
+    float sn, cs;
+    sincosf(x, &sn, &cs);   // meaningful on lanes 7 .. 6+nt
+    // forward kinematics, leaf-to-root about the reference axes
+    float px = c.rx, py = c.ry, pz = c.rz;
+    for (int k = 0; k < c.nt; ++k) {
+        const float sk = bcast(sn, 7 + k), ck = bcast(cs, 7 + k);
+        const float ax0 = c.axp[3 * k], ax1 = c.axp[3 * k + 1], ax2 = c.axp[3 * k + 2];
+        const float ux = c.axu[3 * k], uy = c.axu[3 * k + 1], uz = c.axu[3 * k + 2];
+        if ((c.tmask >> k) & 1u) {
+            const float vx = px - ax0, vy = py - ax1, vz = pz - ax2;
+            const float d = (ux * vx + uy * vy + uz * vz) * (1.f - ck);
+            const float cx = uy * vz - uz * vy, cy = uz * vx - ux * vz, cz = ux * vy - uy * vx;
+            px = ax0 + vx * ck + cx * sk + ux * d;
+            py = ax1 + vy * ck + cy * sk + uy * d;
+            pz = ax2 + vz * ck + cz * sk + uz * d;
+        }
+    }
+    const float X0 = R00 * px + R01 * py + R02 * pz + t0;
+    const float X1 = R10 * px + R11 * py + R12 * pz + t1;
+    const float X2 = R20 * px + R21 * py + R22 * pz + t2;
+    float el = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
+    if (c.atom) {
+        if (!c.exact) {
+            const float f0 = (X0 - c.o0) * c.inv_s, f1 = (X1 - c.o1) * c.inv_s, f2 = (X2 - c.o2) * c.inv_s;
+            const float fc0 = fminf(fmaxf(f0, 0.f), c.fmx0), fc1 = fminf(fmaxf(f1, 0.f), c.fmx1),
+                        fc2 = fminf(fmaxf(f2, 0.f), c.fmx2);
+            const float in0 = (f0 > 0.f && f0 < c.fmx0) ? c.inv_s : 0.f;
+            const float in1 = (f1 > 0.f && f1 < c.fmx1) ? c.inv_s : 0.f;
+            const float in2 = (f2 > 0.f && f2 < c.fmx2) ? c.inv_s : 0.f;
+            const int i0 = (int)floorf(fc0), i1 = (int)floorf(fc1), i2 = (int)floorf(fc2);
+            const float wx = fc0 - (float)i0, wy = fc1 - (float)i1, wz = fc2 - (float)i2;
+            const int b000 = (i0 * c.ny + i1) * c.nz + i2;
+            const int bx = b000 + c.ny * c.nz;
+            const float* G = c.gp;
+            const float v000 = G[b000], v001 = G[b000 + 1], v010 = G[b000 + c.nz], v011 = G[b000 + c.nz + 1];
+            const float v100 = G[bx], v101 = G[bx + 1], v110 = G[bx + c.nz], v111 = G[bx + c.nz + 1];
+            const float c00 = v000 + wz * (v001 - v000), c01 = v010 + wz * (v011 - v010);
+            const float c10 = v100 + wz * (v101 - v100), c11 = v110 + wz * (v111 - v110);
+            const float c0 = c00 + wy * (c01 - c00), c1 = c10 + wy * (c11 - c10);
+            el = c0 + wx * (c1 - c0);
+            gx = (c1 - c0) * in0;
+            gy = ((c01 - c00) + wx * ((c11 - c10) - (c01 - c00))) * in1;
+            const float e0 = (v001 - v000) + wy * ((v011 - v010) - (v001 - v000));
+            const float e1 = (v101 - v100) + wy * ((v111 - v110) - (v101 - v100));
+            gz = (e0 + wx * (e1 - e0)) * in2;
+        } else {
+            const float cut2 = sf.cut * sf.cut;
+            const int nloop = c.nb ? c.nbn : c.rn;
+            for (int bb = 0; bb < nloop; ++bb) {
+                const int b = c.nb ? c.nb[bb] : bb;
+                const float rx = X0 - c.rrx[3 * b], ry = X1 - c.rrx[3 * b + 1], rz = X2 - c.rrx[3 * b + 2];
+                const float r2 = rx * rx + ry * ry + rz * rz;
+                if (r2 >= cut2) continue;
+                const int tb = c.rrt[b];
+                const float r = fast_sqrt(r2);
+                const int fb = c.sfhyd[tb] | (c.sfdon[tb] << 1) | (c.sfacc[tb] << 2);
+                float de;
+                el += pair_e(r - c.rad - c.sfrad[tb], pair_flags(c.flg, fb), sf, de);
+                const float s = de * fast_rcp(fmaxf(r, 1e-6f));
+                gx += s * rx; gy += s * ry; gz += s * rz;
+            }
+        }
+        const float l0 = fmaxf(c.lo0 - X0, 0.f), h0 = fmaxf(X0 - c.hi0, 0.f);
+        const float l1 = fmaxf(c.lo1 - X1, 0.f), h1 = fmaxf(X1 - c.hi1, 0.f);
+        const float l2 = fmaxf(c.lo2 - X2, 0.f), h2 = fmaxf(X2 - c.hi2, 0.f);
+        el += c.pen * (l0 * l0 + h0 * h0 + l1 * l1 + h1 * h1 + l2 * l2 + h2 * h2);
+        gx += 2.f * c.pen * (h0 - l0); gy += 2.f * c.pen * (h1 - l1); gz += 2.f * c.pen * (h2 - l2);
+    }
+    w.X4[lane] = make_float4(X0, X1, X2, c.rad);
+    w.Fx[lane] = 0.f; w.Fy[lane] = 0.f; w.Fz[lane] = 0.f;
+    wave_lds_sync();
+    // intra-ligand pairs, strided over lanes; pair forces into LDS
+    for (int q = lane; q < c.np; q += WAVE) {
+        const unsigned pw = w.pl[q];
+        const int i = pw & 0xff, j = pw >> 8;
+        const float4 xi = w.X4[i], xj = w.X4[j];
+        const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+        const float r = fast_sqrt(rx * rx + ry * ry + rz * rz);
+        if (r < sf.cut) {
+            float de;
+            el += pair_e(r - xi.w - xj.w, pair_flags(w.flg[i], w.flg[j]), sf, de);
+            const float s = de * fast_rcp(fmaxf(r, 1e-6f));
+            atomicAdd(&w.Fx[i], s * rx); atomicAdd(&w.Fy[i], s * ry); atomicAdd(&w.Fz[i], s * rz);
+            atomicAdd(&w.Fx[j], -s * rx); atomicAdd(&w.Fy[j], -s * ry); atomicAdd(&w.Fz[j], -s * rz);
+        }
+    }
+    wave_lds_sync();
+    const float fx = gx + w.Fx[lane], fy = gy + w.Fy[lane], fz = gz + w.Fz[lane];
+    // current torsion axes (lane k < nt) from the world coordinates of the axis atoms
+    float pwx = 0.f, pwy = 0.f, pwz = 0.f, uwx = 0.f, uwy = 0.f, uwz = 0.f;
+    if (lane < c.nt) {
+        const float4 A = w.X4[c.axa], B = w.X4[c.axb];
+        const float ux = B.x - A.x, uy = B.y - A.y, uz = B.z - A.z;
+        const float nn = fast_rsq(fmaxf(ux * ux + uy * uy + uz * uz, 1e-12f));
+        uwx = ux * nn; uwy = uy * nn; uwz = uz * nn;
+        pwx = A.x; pwy = A.y; pwz = A.z;
+    }
+    float g = 0.f;
+    const float dX = X0 - t0, dY = X1 - t1, dZ = X2 - t2;
+    const float s0 = wave_sum(fx), s1 = wave_sum(fy), s2 = wave_sum(fz);
+    const float s3 = wave_sum(dY * fz - dZ * fy), s4 = wave_sum(dZ * fx - dX * fz), s5 = wave_sum(dX * fy - dY * fx);
+    g = lane == 0 ? s0 : lane == 1 ? s1 : lane == 2 ? s2 : lane == 3 ? s3 : lane == 4 ? s4 : lane == 5 ? s5 : 0.f;
+    for (int k = 0; k < c.nt; ++k) {
+        const float pxk = bcast(pwx, k), pyk = bcast(pwy, k), pzk = bcast(pwz, k);
+        const float uxk = bcast(uwx, k), uyk = bcast(uwy, k), uzk = bcast(uwz, k);
+        float tv = 0.f;
+        if ((c.tmask >> k) & 1u) {
+            const float vx = X0 - pxk, vy = X1 - pyk, vz = X2 - pzk;
+            tv = (vy * fz - vz * fy) * uxk + (vz * fx - vx * fz) * uyk + (vx * fy - vy * fx) * uzk;
+        }
+        const float gk = wave_sum(tv);
+        if (lane == 7 + k) g = gk;
+    }
+    gout = g;
+    return wave_sum(el);
+}
+
+// x (+) alpha p on the lane-distributed state
+DEVI float conf_inc(float x, float p, float alpha, int lane) {
+    float qw = bcast(x, 3), qx = bcast(x, 4), qy = bcast(x, 5), qz = bcast(x, 6);
+    quat_inc(qw, qx, qy, qz, alpha * bcast(p, 3), alpha * bcast(p, 4), alpha * bcast(p, 5));
+    const float lin = x + alpha * p;
+    return lane == 3 ? qw : lane == 4 ? qx : lane == 5 ? qy : lane == 6 ? qz : lin;
+}
+
+// Vina-style BFGS on the lane-distributed state x (in/out); returns the final energy. One evaluation site: the first
+// pass evaluates x itself, every later pass evaluates one line-search trial.
+template <int KM>
+DEVI float bfgs(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, int lane, float& x, int iters, int& nev) {
+    constexpr int DV = 7 + KM;
+    float h[DV];
+#pragma unroll
+    for (int j = 0; j < DV; ++j) h[j] = (j == lane) ? 1.f : 0.f;
+    const float xo = x;
+    float g = 0.f, p = 0.f, f0 = 0.f, f_orig = 0.f, pg = 0.f, alpha = 1.f;
+    int step = 0, trial = -1;
+    while (true) {
+        const float xe = trial < 0 ? x : conf_inc(x, p, alpha, lane);
+        float ge;
+        const float fe = eval_eg(c, sf, w, lane, xe, ge);
+        ++nev;
+        if (trial < 0) {
+            f0 = fe; f_orig = fe; g = ge;
+            if (iters <= 0) break;
+        } else {
+            const bool ok = fe - f0 < 1e-4f * alpha * pg;
+            if (!ok) alpha *= 0.5f;
+            if (!ok && trial < 9) { ++trial; continue; }
+            const float y = ge - g;
+            f0 = fe; x = xe; g = ge;
+            const float gg = wave_sum(ge * ge);
+            if (!(sqrtf(gg) >= 1e-5f)) break;
+            const float yp = wave_sum(y * p);
+            if (step == 0) {
+                const float yy = wave_sum(y * y);
+                if (fabsf(yy) > EPSF) {
+                    const float dg = alpha * yp / yy;
+#pragma unroll
+                    for (int j = 0; j < DV; ++j) h[j] = (j == lane) ? dg : 0.f;
+                }
+            }
+            if (!(alpha * yp < EPSF)) {
+                float mhy = 0.f;
+#pragma unroll
+                for (int j = 0; j < DV; ++j) mhy += h[j] * bcast(y, j);
+                mhy = -mhy;
+                const float yhy = -wave_sum(y * mhy);
+                const float r = 1.f / (alpha * yp);
+                const float c2 = alpha * alpha * (r * r * yhy + r);
+                const float ar = alpha * r;
+#pragma unroll
+                for (int j = 0; j < DV; ++j) {
+                    const float pj = bcast(p, j), mj = bcast(mhy, j);
+                    h[j] += ar * (mhy * pj + mj * p) + c2 * p * pj;
+                }
+            }
+            if (++step >= iters) break;
+        }
+        float s = 0.f;
+#pragma unroll
+        for (int j = 0; j < DV; ++j) s += h[j] * bcast(g, j);
+        p = -s;
+        pg = wave_sum(p * g);
+        alpha = 1.f;
+        trial = 0;
+    }
+    if (!(f0 <= f_orig)) { f0 = f_orig; x = xo; }
+    return f0;
+}
+
+template <int KM>
+__global__ __launch_bounds__(256) void mc_kernel(LigArgs L, GridArgs G, SFArgs S, RecArgs R, int exact,
+                                                  const int* __restrict__ chain_lig, int n_chains, McArgs M) {
+    extern __shared__ __attribute__((aligned(16))) char lds[];
+    constexpr int SD = 7 + KM;
+    const int lane = lane_id();
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+    const int ch = blockIdx.x * (blockDim.x / WAVE) + wid;
+    if (ch >= n_chains) return;
+    const WaveLDS w = carve_lds(lds + wid * wave_lds_bytes(L.P_MAX), L.P_MAX);
+    const SFConst sf = make_sf(S.prm);
+    const int l = __builtin_amdgcn_readfirstlane(chain_lig[ch]);
+    ChainCtx c = setup_chain<KM>(l, lane, L, G, S, R, exact, w);
+    if (M.nbr) { c.nb = M.nbr + (long long)ch * M.NB_MAX; c.nbn = M.nbr_n[ch]; }
+    const int cid = __builtin_amdgcn_readfirstlane(M.chain_id[ch]);
+    const int S_ = 7 + c.nt;
+    const int iters = M.bfgs_iters > 0 ? M.bfgs_iters : (25 + c.na) / 3;
+    float* gcur = M.cur + (long long)ch * SD;
+    float* gbest = M.best + (long long)ch * SD;
+    float cur = lane < S_ ? gcur[lane] : 0.f;
+    float best = lane < S_ ? gbest[lane] : 0.f;
+    float eb = M.best_e[ch];
+    float ec = M.cur_e[ch];
+    int nev = 0;
+    const unsigned long long seed = M.seed;
+    if (!M.init_opt && ec < eb) { eb = ec; best = cur; }
+    // st = -1 is the optional initial optimisation of cur; st >= 0 are MC steps
+    for (int st = M.init_opt ? -1 : 0; st < M.mc_steps; ++st) {
+        float x = cur;
+        int sg = 0;
+        if (st >= 0) {
+            sg = M.step_offset + st;
+            const int nm = 2 + c.nt;
+            const int which = min((int)(urand(seed, cid, sg, 0) * (float)nm), nm - 1);
+            if (which == 0) {
+                float v0, v1, v2;
+                rand_ball(seed, cid, sg, 10, v0, v1, v2);
+                const float v = lane == 0 ? v0 : lane == 1 ? v1 : v2;
+                if (lane < 3) x += M.amp_pos * v;
+            } else if (which == 1) {
+                float v0, v1, v2;
+                rand_ball(seed, cid, sg, 10, v0, v1, v2);
+                const float sc = M.amp_rot / fmaxf(c.rg, 1.0f);
+                float qw = bcast(x, 3), qx = bcast(x, 4), qy = bcast(x, 5), qz = bcast(x, 6);
+                quat_inc(qw, qx, qy, qz, sc * v0, sc * v1, sc * v2);
+                x = lane == 3 ? qw : lane == 4 ? qx : lane == 5 ? qy : lane == 6 ? qz : x;
+            } else {
+                const float th = (2.f * urand(seed, cid, sg, 1) - 1.f) * 3.14159265f;
+                if (lane == 7 + which - 2) x = th;
+            }
+        }
+        const float en = bfgs<KM>(c, sf, w, lane, x, iters, nev);
+        if (st < 0) {
+            ec = en; cur = x;
+            if (ec < eb) { eb = ec; best = cur; }
+        } else {
+            const bool accept = (en < ec) || (urand(seed, cid, sg, 2) < expf((ec - en) / M.temp));
+            if (accept) {
+                ec = en; cur = x;
+                if (en < eb) { eb = en; best = x; }
+            }
+        }
+    }
+    if (lane < S_) { gcur[lane] = cur; gbest[lane] = best; }
+    if (lane == 0) {
+        M.cur_e[ch] = ec; M.best_e[ch] = eb;
+        if (M.eval_count) atomicAdd(M.eval_count, (unsigned long long)nev);
+    }
+}
+
+template <int KM>
+__global__ __launch_bounds__(256) void eg_kernel(LigArgs L, GridArgs G, SFArgs S, RecArgs R, int exact,
+                                                  const int* __restrict__ chain_lig, int n_chains,
+                                                  const float* __restrict__ state, float* __restrict__ e_out,
+                                                  float* __restrict__ g_out) {
+    extern __shared__ __attribute__((aligned(16))) char lds[];
+    constexpr int SD = 7 + KM, DM = 6 + KM;
+    const int lane = lane_id();
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+    const int ch = blockIdx.x * (blockDim.x / WAVE) + wid;
+    if (ch >= n_chains) return;
+    const WaveLDS w = carve_lds(lds + wid * wave_lds_bytes(L.P_MAX), L.P_MAX);
+    const SFConst sf = make_sf(S.prm);
+    const ChainCtx c = setup_chain<KM>(__builtin_amdgcn_readfirstlane(chain_lig[ch]), lane, L, G, S, R, exact, w);
+    const float x = lane < SD ? state[(long long)ch * SD + lane] : 0.f;
+    float g;
+    const float e = eval_eg(c, sf, w, lane, x, g);
+    if (lane < 6) g_out[(long long)ch * DM + lane] = g;
+    else if (lane >= 7 && lane < 7 + c.nt) g_out[(long long)ch * DM + lane - 1] = g;
+    if (lane == 0) e_out[ch] = e;
+}
+
+int g_waves_per_block = 4;
+
+inline int lds_bytes_per_wave(int P_MAX) { return 16 * AMK + 12 * AMK + 4 * AMK + 2 * ((P_MAX + 7) & ~7); }
+
+template <int KM>
+int launch_mc_t(const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecArgs& R, int exact,
+                const int* chain_lig, int n_chains, const McArgs& M, hipStream_t stream) {
+    const int W = g_waves_per_block;
+    const size_t shm = (size_t)W * lds_bytes_per_wave(L.P_MAX);
+    const unsigned nblk = (unsigned)((n_chains + W - 1) / W);
+    hipLaunchKernelGGL(mc_kernel<KM>, dim3(nblk), dim3(W * WAVE), shm, stream, L, G, S, R, exact, chain_lig,
+                       n_chains, M);
+    return (int)hipGetLastError();
+}
+
+template <int KM>
+int launch_eg_t(const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecArgs& R, int exact,
+                const int* chain_lig, int n_chains, const float* state, float* e_out, float* g_out,
+                hipStream_t stream) {
+    const int W = g_waves_per_block;
+    const size_t shm = (size_t)W * lds_bytes_per_wave(L.P_MAX);
+    const unsigned nblk = (unsigned)((n_chains + W - 1) / W);
+    hipLaunchKernelGGL(eg_kernel<KM>, dim3(nblk), dim3(W * WAVE), shm, stream, L, G, S, R, exact, chain_lig,
+                       n_chains, state, e_out, g_out);
+    return (int)hipGetLastError();
+}
+
+}  // namespace
+
+int set_waves_per_block(int w) {
+    if (w < 1 || w > 4) return -1;
+    g_waves_per_block = w;
+    return 0;
+}
+int get_waves_per_block() { return g_waves_per_block; }
+int lds_budget_ok(int P_MAX) { return (size_t)4 * lds_bytes_per_wave(P_MAX) <= 64 * 1024; }
+
+int launch_mc(int KM, const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecArgs& R, int exact,
+              const int* chain_lig, int n_chains, const McArgs& M, hipStream_t stream) {
+    switch (KM) {
+        case 8: return launch_mc_t<8>(L, G, S, R, exact, chain_lig, n_chains, M, stream);
+        case 16: return launch_mc_t<16>(L, G, S, R, exact, chain_lig, n_chains, M, stream);
+        case 24: return launch_mc_t<24>(L, G, S, R, exact, chain_lig, n_chains, M, stream);
+        case 32: return launch_mc_t<32>(L, G, S, R, exact, chain_lig, n_chains, M, stream);
+        default: return -1;
+    }
+}
+
+int launch_eg(int KM, const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecArgs& R, int exact,
+              const int* chain_lig, int n_chains, const float* state, float* e_out, float* g_out,
+              hipStream_t stream) {
+    switch (KM) {
+        case 8: return launch_eg_t<8>(L, G, S, R, exact, chain_lig, n_chains, state, e_out, g_out, stream);
+        case 16: return launch_eg_t<16>(L, G, S, R, exact, chain_lig, n_chains, state, e_out, g_out, stream);
+        case 24: return launch_eg_t<24>(L, G, S, R, exact, chain_lig, n_chains, state, e_out, g_out, stream);
+        case 32: return launch_eg_t<32>(L, G, S, R, exact, chain_lig, n_chains, state, e_out, g_out, stream);
+        default: return -1;
+    }
+}
+
+}  // namespace adock

@@ -1,0 +1,133 @@
+"""KernelProblem: the drop-in facade over libadock_hip.so for the docking search kernels.
+
+Same constructor input, attributes (AM, KM, L, SD, DM) and methods (pack, unpack, energy_grad, run_mc) with the same
+argument meaning and in-place conventions as the reference's KernelProblem (src/adock/mckernel.py:602-726), so its
+caller (src/adock/search.py:544-671) runs unchanged. Tensors are torch ROCm device tensors; launches go to the current
+stream and are asynchronous, like the reference's.
+"""
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+from .config import search_prm
+from .ligand import pack_ligands
+
+
+def ligand_arrays(prob):
+    """Packed ligand arrays (numpy) of a Problem: (AM, KM, dict)."""
+    slot = prob.gs.slot.cpu().numpy()
+    return pack_ligands(prob.lb.preps, slot, prob.rec_idx.cpu().numpy())
+
+
+class KernelProblem:
+    """State layout per chain: [t(3), q(4) (w,x,y,z), theta(KM)]; gradient [d/dt(3), torque(3), d/dtheta(KM)]."""
+
+    def __init__(self, prob):
+        _lib.require_gpu()
+        self.AM, self.KM, lig = ligand_arrays(prob)
+        if self.AM != 64:
+            raise _lib.AdockError("ligands above 64 heavy atoms are not supported by the HIP kernels")
+        gs = prob.gs
+        dev = gs.flat.device
+        if dev.type != "cuda":
+            raise _lib.AdockError("the grid set must live on the GPU")
+        self.device = dev
+        if int(gs.dims.min()) < 2:
+            raise _lib.AdockError("every grid axis needs at least 2 points")
+        T = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=dev)
+        self.lig = {k: (T(v) if isinstance(v, np.ndarray) else v) for k, v in lig.items()}
+        self.P_MAX = int(lig["P"])
+        self.grid = dict(flat=gs.flat.float().contiguous(), origin=gs.origin.float().contiguous(),
+                         lo=gs.lo.float().contiguous(), hi=gs.hi.float().contiguous(),
+                         rec_idx=prob.rec_idx.to(device=dev, dtype=torch.int32).contiguous(),
+                         offs=gs.offs.to(device=dev, dtype=torch.int64).contiguous(),
+                         dims=gs.dims.to(device=dev, dtype=torch.int64).contiguous(),
+                         T=int(gs.T), spacing=float(gs.spacing), penalty=float(gs.penalty))
+        sp = prob.pot.p
+        self.sf = dict(prm=T(search_prm(sp)), rad=T(np.asarray(sp["radius"], np.float32)),
+                       hyd=T(np.asarray(sp["hydrophobic"], np.int32)), don=T(np.asarray(sp["donor"], np.int32)),
+                       acc=T(np.asarray(sp["acceptor"], np.int32)))
+        if prob.rec is not None:
+            rx, rt, rm = prob.rec
+            self.rec = dict(rxyz=rx.to(dev).float().contiguous(), rtyp=rt.to(device=dev, dtype=torch.int32).contiguous(),
+                            rn=rm.to(dev).sum(1).to(torch.int32).contiguous())
+            self.RN_MAX = int(rx.shape[1])
+        else:
+            self.rec = dict(rxyz=torch.zeros(3, device=dev), rtyp=torch.zeros(1, dtype=torch.int32, device=dev),
+                            rn=torch.zeros(1, dtype=torch.int32, device=dev))
+            self.RN_MAX = 0
+        self.L, self.SD, self.DM = prob.lb.L, 7 + self.KM, 6 + self.KM
+        self.eval_count = torch.zeros(1, dtype=torch.int64, device=dev)
+
+    # ---- state packing ----
+    def pack(self, t, q, theta):
+        """t [N,3], q [N,4], theta [N,K<=KM] -> state [N, SD] float32 contiguous."""
+        N, K = theta.shape
+        s = torch.zeros(N, self.SD, device=t.device, dtype=torch.float32)
+        s[:, :3], s[:, 3:7] = t, q
+        s[:, 7:7 + min(K, self.KM)] = theta[:, :self.KM]
+        return s.contiguous()
+
+    def unpack(self, s, K):
+        th = torch.zeros(s.shape[0], K, device=s.device, dtype=s.dtype)
+        th[:, :min(K, self.KM)] = s[:, 7:7 + min(K, self.KM)]
+        th = torch.remainder(th + math.pi, 2 * math.pi) - math.pi
+        return s[:, :3].clone(), s[:, 3:7] / s[:, 3:7].norm(dim=-1, keepdim=True), th
+
+    # ---- argument groups ----
+    def _args(self, exact):
+        L, G, S, R = self.lig, self.grid, self.sf, self.rec
+        p = _lib.ptr
+        return ([self.AM, self.KM] + [p(L[k]) for k in ("ref", "typ", "tm", "ax", "axp", "axu", "pr", "na", "nt",
+                                                         "np", "rg")]
+                + [self.P_MAX, p(L["gslot"])]
+                + [p(G[k]) for k in ("flat", "origin", "lo", "hi", "rec_idx", "offs", "dims")]
+                + [G["T"], G["spacing"], G["penalty"]]
+                + [p(S[k]) for k in ("prm", "rad", "hyd", "don", "acc")]
+                + [p(R["rxyz"]), p(R["rtyp"]), p(R["rn"]), self.RN_MAX, int(exact)])
+
+    def energy_grad(self, chain_lig, state, exact=False):
+        """chain_lig int32 [N], state [N, SD] (device) -> (e [N], g [N, DM])."""
+        if exact and self.RN_MAX <= 0:
+            raise _lib.AdockError("exact mode needs receptor atoms (Problem.rec)")
+        N = state.shape[0]
+        e = torch.zeros(N, device=self.device)
+        g = torch.zeros(N, self.DM, device=self.device)
+        cl = chain_lig.to(device=self.device, dtype=torch.int32).contiguous()
+        st = state.to(self.device).float().contiguous()
+        self._keep = (cl, st)
+        _lib.check(_lib.lib().adock_eg(*self._args(exact), _lib.ptr(cl), N, _lib.ptr(st), _lib.ptr(e), _lib.ptr(g),
+                                       _lib.stream_of(self.device)))
+        return e, g
+
+    def run_mc(self, chain_lig, chain_id, cur, cur_e, best, best_e, init_opt, mc_steps, bfgs_iters, temp,
+               amp_pos, amp_rot, seed, step_offset, exact=False, nbr=None, nbr_n=None, count_evals=False):
+        """In place on cur/cur_e/best/best_e (float32 contiguous device tensors). exact=True with nbr [N, NB] /
+        nbr_n [N]: each chain's exact inter energy loops only over its receptor neighbour list."""
+        N = cur.shape[0]
+        if N == 0:
+            return
+        if exact and self.RN_MAX <= 0:
+            raise _lib.AdockError("exact mode needs receptor atoms (Problem.rec)")
+        for x in (cur, cur_e, best, best_e):
+            if not (x.is_contiguous() and x.dtype == torch.float32 and x.is_cuda):
+                raise _lib.AdockError("cur/cur_e/best/best_e must be contiguous float32 device tensors")
+        cl = chain_lig.to(device=self.device, dtype=torch.int32).contiguous()
+        ci = chain_id.to(device=self.device, dtype=torch.int32).contiguous()
+        if nbr is not None:
+            nbr = nbr.to(device=self.device, dtype=torch.int32).contiguous()
+            nbr_n = nbr_n.to(device=self.device, dtype=torch.int32).contiguous()
+            nb_args = [_lib.ptr(nbr), _lib.ptr(nbr_n), int(nbr.shape[1])]
+        else:
+            nb_args = [None, None, 0]
+        self._keep = (cl, ci, nbr, nbr_n)
+        args = (self._args(exact) + [_lib.ptr(cl), _lib.ptr(ci), N] + nb_args +
+                [_lib.ptr(cur), _lib.ptr(cur_e), _lib.ptr(best), _lib.ptr(best_e), int(bool(init_opt)), int(mc_steps),
+                 int(bfgs_iters), float(temp), float(amp_pos), float(amp_rot), int(seed) & (2 ** 64 - 1),
+                 int(step_offset)])
+        if count_evals:
+            _lib.check(_lib.lib().adock_mc_counted(*args, _lib.ptr(self.eval_count), _lib.stream_of(self.device)))
+        else:
+            _lib.check(_lib.lib().adock_mc(*args, _lib.stream_of(self.device)))

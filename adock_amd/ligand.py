@@ -1,0 +1,198 @@
+"""Ligand preparation from a heavy-atom graph: rigid-fragment torsion tree, intra-ligand pair list, radius of gyration,
+and the packed per-ligand layout the search kernels read.
+
+Conventions follow the reference so that packed arrays are interchangeable with its `pack_host` output:
+- fragments = connected components after removing rotatable bonds; root = fragment with the smallest maximum BFS
+  depth, ties to the larger heavy-atom count, then the lowest fragment index (src/adock/ligand.py:40-97);
+- torsion k: axis (a, b) from the parent-side atom to the child-side atom, mask = atoms of the child subtree;
+  torsions ordered leaf-to-root by a stable sort on depth (src/adock/ligand.py:98-109);
+- intra pairs: heavy atoms in different fragments and more than 3 bonds apart (src/adock/ligand.py:111-119);
+- ref = coordinates minus the heavy-atom centroid; rg over heavy atoms (src/adock/ligand.py:27-38).
+The reference builds the same from an RDKit molecule; here the graph comes straight from the synthetic generator.
+"""
+import math
+from collections import deque
+
+import numpy as np
+
+
+class LigandPrep:
+    """Heavy-atom ligand with its torsion tree.
+
+    xyz [n,3] float, types [n] int (XS index, -1 = hydrogen), bonds [(i, j)], rot_bonds [(i, j)] (i < j)."""
+
+    def __init__(self, xyz, types, bonds, rot_bonds, n_tors_vina=None):
+        xyz = np.asarray(xyz, np.float64)
+        self.n = len(xyz)
+        self.types = np.asarray(types, np.int64)
+        self.heavy = self.types >= 0
+        self.bonds = [(int(min(i, j)), int(max(i, j))) for i, j in bonds]
+        self.rot_bonds = sorted((int(min(i, j)), int(max(i, j))) for i, j in rot_bonds)
+        self.center = xyz[self.heavy].mean(0)
+        self.ref = (xyz - self.center).astype(np.float32)
+        self._build_tree()
+        self._build_pairs()
+        h = self.ref[self.heavy]
+        self.rg = float(np.sqrt(((h - h.mean(0)) ** 2).sum(1).mean())) if len(h) > 1 else 1.0
+        self.n_tors_vina = self.n_tors if n_tors_vina is None else n_tors_vina
+
+    def _adjacency(self, skip=()):
+        adj = [[] for _ in range(self.n)]
+        skip = set(skip)
+        for i, j in self.bonds:
+            if (i, j) in skip:
+                continue
+            adj[i].append(j)
+            adj[j].append(i)
+        return adj
+
+    def _build_tree(self):
+        n = self.n
+        adj = self._adjacency(self.rot_bonds)
+        frag = -np.ones(n, dtype=np.int64)
+        nf = 0
+        for s in range(n):
+            if frag[s] >= 0:
+                continue
+            frag[s] = nf
+            dq = deque([s])
+            while dq:
+                u = dq.popleft()
+                for v in adj[u]:
+                    if frag[v] < 0:
+                        frag[v] = nf
+                        dq.append(v)
+            nf += 1
+        self.frag = frag
+        fadj = [[] for _ in range(nf)]
+        for i, j in self.rot_bonds:
+            fadj[frag[i]].append((frag[j], i, j))
+            fadj[frag[j]].append((frag[i], j, i))
+
+        def bfs(root):
+            depth, par, order = {root: 0}, {root: None}, [root]
+            dq = deque([root])
+            while dq:
+                u = dq.popleft()
+                for v, a, b in fadj[u]:
+                    if v not in depth:
+                        depth[v] = depth[u] + 1
+                        par[v] = (u, a, b)
+                        order.append(v)
+                        dq.append(v)
+            return depth, par, order
+
+        nheavy = np.bincount(frag[self.heavy], minlength=nf)
+        keys = [(max(bfs(f)[0].values()), -int(nheavy[f]), f) for f in range(nf)]
+        root = min(keys)[2]
+        depth, par, order = bfs(root)
+        if len(depth) != nf:
+            raise ValueError("ligand graph is disconnected")
+        self.root = root
+        children = {f: [] for f in range(nf)}
+        for f in order:
+            if par[f] is not None:
+                children[par[f][0]].append(f)
+
+        def subtree(f):
+            out, st = [], [f]
+            while st:
+                g = st.pop()
+                out.extend(np.where(frag == g)[0].tolist())
+                st.extend(children[g])
+            return out
+
+        tors = []
+        for f in order:
+            if par[f] is None:
+                continue
+            _, a, b = par[f]
+            mask = np.zeros(n, dtype=bool)
+            mask[subtree(f)] = True
+            tors.append((depth[f], a, b, mask))
+        tors.sort(key=lambda x: -x[0])   # stable: leaf-to-root
+        self.tors_axis = np.array([[a, b] for _, a, b, _ in tors], dtype=np.int64).reshape(-1, 2)
+        self.tors_mask = np.array([m for *_, m in tors], dtype=bool).reshape(-1, n)
+        self.n_tors = len(tors)
+
+    def topo_distance(self):
+        adj = self._adjacency()
+        D = np.full((self.n, self.n), np.inf)
+        for s in range(self.n):
+            D[s, s] = 0
+            dq = deque([s])
+            while dq:
+                u = dq.popleft()
+                for v in adj[u]:
+                    if D[s, v] == np.inf:
+                        D[s, v] = D[s, u] + 1
+                        dq.append(v)
+        return D
+
+    def _build_pairs(self):
+        dm = self.topo_distance()
+        hv = np.where(self.heavy)[0]
+        pairs = [(i, j) for ii, i in enumerate(hv) for j in hv[ii + 1:] if self.frag[i] != self.frag[j] and dm[i, j] > 3]
+        self.pairs = np.array(pairs, dtype=np.int64).reshape(-1, 2)
+
+
+def bucket(x, step):
+    return max(step, step * math.ceil(x / step))
+
+
+def pack_ligands(preps, slot, rec_idx):
+    """Packed ligand arrays in the search-kernel layout (src/adock/mckernel.py:547-592).
+
+    slot [R, n_types] int (XS type -> grid slot of receptor r, -1 = not built), rec_idx [L].
+    Returns (AM, KM, dict of numpy arrays: ref, typ, tm, ax, axp, axu, pr, na, nt, np, rg, P, gslot)."""
+    L = len(preps)
+    nh = [int(p.heavy.sum()) for p in preps]
+    AM = 64 if max(nh) <= 64 else bucket(max(nh), 32)
+    KM = bucket(max(max(p.n_tors for p in preps), 1), 8)
+    P = max(1, max(len(p.pairs) for p in preps))
+    out = {
+        "ref": np.zeros((L, AM, 3), np.float32), "typ": np.zeros((L, AM), np.int32),
+        "tm": np.zeros((L, KM, AM), np.uint8), "ax": np.zeros((L, KM, 2), np.int32),
+        "axp": np.zeros((L, KM, 3), np.float32), "axu": np.zeros((L, KM, 3), np.float32),
+        "pr": np.zeros((L, P, 2), np.int32), "na": np.zeros(L, np.int32), "nt": np.zeros(L, np.int32),
+        "np": np.zeros(L, np.int32), "rg": np.zeros(L, np.float32), "gslot": np.zeros((L, AM), np.int32),
+    }
+    slot = np.asarray(slot)
+    rec_idx = np.asarray(rec_idx)
+    for l, p in enumerate(preps):
+        hv = np.where(p.heavy)[0]
+        remap = -np.ones(p.n, np.int64)
+        remap[hv] = np.arange(len(hv))
+        out["na"][l], out["nt"][l], out["np"][l], out["rg"][l] = len(hv), p.n_tors, len(p.pairs), p.rg
+        out["ref"][l, :len(hv)] = p.ref[hv]
+        out["typ"][l, :len(hv)] = p.types[hv]
+        if p.n_tors:
+            out["tm"][l, :p.n_tors, :len(hv)] = p.tors_mask[:, hv]
+            out["ax"][l, :p.n_tors] = remap[p.tors_axis]
+            if (out["ax"][l, :p.n_tors] < 0).any():
+                raise ValueError("torsion axis atom is not a heavy atom")
+            pa, pb = p.ref[p.tors_axis[:, 0]], p.ref[p.tors_axis[:, 1]]
+            u = pb - pa
+            out["axp"][l, :p.n_tors] = pa
+            out["axu"][l, :p.n_tors] = u / np.maximum(np.linalg.norm(u, axis=1, keepdims=True), 1e-6)
+        if len(p.pairs):
+            out["pr"][l, :len(p.pairs)] = remap[p.pairs]
+        out["gslot"][l, :len(hv)] = slot[rec_idx[l], out["typ"][l, :len(hv)]]
+        if (out["gslot"][l, :len(hv)] < 0).any():
+            raise ValueError("a ligand atom type has no grid")
+    out["P"] = np.int32(P)
+    return AM, KM, out
+
+
+def quat_to_mat(q):
+    q = np.asarray(q, np.float64)
+    q = q / np.linalg.norm(q, axis=-1, keepdims=True)
+    w, x, y, z = np.moveaxis(q, -1, 0)
+    return np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w),
+                     2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w),
+                     2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], -1).reshape(q.shape[:-1] + (3, 3))
+
+
+def random_quat(rng, n):
+    q = rng.standard_normal((n, 4))
+    return (q / np.linalg.norm(q, axis=1, keepdims=True)).astype(np.float32)

@@ -1,0 +1,224 @@
+"""Seeded synthetic docking workloads: receptors, ligands with torsion trees, boxes and chain start states.
+
+Receptor: heavy atoms on a jittered, randomly rotated FCC lattice inside a sphere, with an empty spherical cavity at the
+box centre. Random sequential placement saturates near 0.027 atoms/A^3 at 3.0 A separation, below the 0.046-0.056
+atoms/A^3 the workloads ask for, so a lattice whose nearest-neighbour spacing exceeds the minimum separation by twice
+the jitter amplitude is used instead (every pair stays at least `min_sep` apart by construction).
+XS type mix: C 63% (half C_H, half C_P), N 17%, O 19%, S 1%; N and O get donor / acceptor flags drawn per atom
+(N: donor 0.5, acceptor 0.3; O: donor 0.3, acceptor 0.8).
+
+Ligand: heavy-atom random tree, 1.5 A bonds, 109.47 deg bond angle to a random existing bond of the parent, at most 3
+bonds per atom, growth steps rejected when a new atom comes closer than 2.5 A to any atom that is not its parent or a
+neighbour of its parent. Elements C 70 / N 12 / O 15 / S + halogen 3 (halogens only on terminal atoms, O and S at most
+2 bonds). Rotatable bonds are drawn among non-terminal bonds (capped at their number).
+"""
+import math
+
+import numpy as np
+
+from .config import load_config
+from .ligand import LigandPrep, random_quat
+
+_CFG = load_config()
+IDX = _CFG["_xs_index"]
+
+
+def _fcc_sites(a_nn, radius):
+    a = a_nn * math.sqrt(2.0)
+    m = int(math.ceil(radius / a)) + 2
+    g = np.arange(-m, m + 1)
+    base = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3).astype(np.float64)
+    offs = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
+    return ((base[:, None, :] + offs[None]) * a).reshape(-1, 3)
+
+
+def _random_rotation(rng):
+    q = rng.standard_normal(4)
+    q /= np.linalg.norm(q)
+    w, x, y, z = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def receptor_types(rng, n):
+    u = rng.random(n)
+    t = np.empty(n, np.int32)
+    for i in range(n):
+        if u[i] < 0.315:
+            t[i] = IDX["C_H"]
+        elif u[i] < 0.63:
+            t[i] = IDX["C_P"]
+        elif u[i] < 0.80:
+            d, a = rng.random() < 0.5, rng.random() < 0.3
+            t[i] = IDX["N_DA" if d and a else "N_D" if d else "N_A" if a else "N_P"]
+        elif u[i] < 0.99:
+            d, a = rng.random() < 0.3, rng.random() < 0.8
+            t[i] = IDX["O_DA" if d and a else "O_D" if d else "O_A" if a else "O_P"]
+        else:
+            t[i] = IDX["S_P"]
+    return t
+
+
+def make_receptor(seed, n_atoms, radius, cavity, min_sep=3.0):
+    """Returns dict(rec_xyz [n,3] f32, rec_types [n] i32) centred on the origin."""
+    rng = np.random.default_rng(seed)
+    shell = 4.0 / 3.0 * math.pi * (radius ** 3 - cavity ** 3)
+    # FCC number density is sqrt(2)/a_nn^3: pick the largest spacing that still leaves 3% spare sites
+    a_nn = min(min_sep + 0.1, (math.sqrt(2.0) * shell / (1.03 * n_atoms)) ** (1.0 / 3.0))
+    if a_nn <= min_sep:
+        raise ValueError(f"{n_atoms} atoms at >= {min_sep} A do not fit a {radius} A sphere with a {cavity} A cavity")
+    jitter = 0.49 * (a_nn - min_sep)
+    sites = _fcc_sites(a_nn, radius) @ _random_rotation(rng).T + rng.uniform(-a_nn, a_nn, 3)
+    r = np.linalg.norm(sites, axis=1)
+    sites = sites[(r <= radius - jitter) & (r >= cavity + jitter)]
+    if len(sites) < n_atoms:
+        raise ValueError("lattice too small for the requested atom count")
+    sites = sites[rng.choice(len(sites), n_atoms, replace=False)]
+    v = rng.standard_normal((n_atoms, 3))
+    v *= (jitter * rng.random(n_atoms) ** (1 / 3) / np.linalg.norm(v, axis=1))[:, None]
+    xyz = (sites + v).astype(np.float32)
+    return {"rec_xyz": xyz, "rec_types": receptor_types(rng, n_atoms)}
+
+
+_MAXDEG = {"C": 3, "N": 3, "O": 2, "S": 2, "F": 1, "Cl": 1, "Br": 1}
+
+
+def _grow_tree(rng, n):
+    pos = np.zeros((n, 3))
+    nbr = [[] for _ in range(n)]
+    bonds = []
+    cos_t = math.cos(math.radians(109.47))
+    sin_t = math.sin(math.radians(109.47))
+    k = 1
+    fails = 0
+    while k < n:
+        cand = [i for i in range(k) if len(nbr[i]) < 3]
+        p = int(rng.choice(cand))
+        placed = False
+        for _ in range(60):
+            if not nbr[p]:
+                d = rng.standard_normal(3)
+                d /= np.linalg.norm(d)
+            else:
+                b = pos[nbr[p][rng.integers(len(nbr[p]))]] - pos[p]
+                b /= np.linalg.norm(b)
+                e1 = np.cross(b, [1.0, 0, 0] if abs(b[0]) < 0.9 else [0, 1.0, 0])
+                e1 /= np.linalg.norm(e1)
+                e2 = np.cross(b, e1)
+                phi = rng.uniform(0, 2 * math.pi)
+                d = cos_t * b + sin_t * (math.cos(phi) * e1 + math.sin(phi) * e2)
+                ok = True
+                for q in nbr[p]:
+                    c = pos[q] - pos[p]
+                    if np.dot(c, d) / np.linalg.norm(c) > math.cos(math.radians(100.0)):
+                        ok = False
+                        break
+                if not ok:
+                    continue
+            x = pos[p] + 1.5 * d
+            excl = set([p] + nbr[p])
+            dist = np.linalg.norm(pos[:k] - x, axis=1)
+            if all(dist[i] >= 2.5 for i in range(k) if i not in excl):
+                pos[k] = x
+                nbr[p].append(k)
+                nbr[k].append(p)
+                bonds.append((p, k))
+                k += 1
+                placed = True
+                break
+        if not placed:
+            fails += 1
+            if fails > 200:
+                return None
+    return pos, nbr, bonds
+
+
+def _ligand_types(rng, nbr):
+    n = len(nbr)
+    elem = []
+    for i in range(n):
+        deg = len(nbr[i])
+        for _ in range(20):
+            u = rng.random()
+            e = "C" if u < 0.70 else "N" if u < 0.82 else "O" if u < 0.97 else ("S" if rng.random() < 0.5 else
+                                                                                 rng.choice(["F", "Cl", "Br"]))
+            if deg <= _MAXDEG[e]:
+                break
+        else:
+            e = "C"
+        elem.append(str(e))
+    types = np.empty(n, np.int64)
+    for i in range(n):
+        e, deg = elem[i], len(nbr[i])
+        if e == "C":
+            hetero = any(elem[j] in ("N", "O") for j in nbr[i])
+            types[i] = IDX["C_P" if hetero else "C_H"]
+        elif e == "N":
+            u = rng.random()
+            if deg == 1:
+                types[i] = IDX["N_D" if u < 0.6 else "N_DA" if u < 0.8 else "N_A"]
+            elif deg == 2:
+                types[i] = IDX["N_A" if u < 0.5 else "N_D" if u < 0.8 else "N_P"]
+            else:
+                types[i] = IDX["N_P" if u < 0.7 else "N_D"]
+        elif e == "O":
+            types[i] = IDX["O_A" if deg == 2 or rng.random() < 0.6 else "O_DA"]
+        elif e == "S":
+            types[i] = IDX["S_P"]
+        else:
+            types[i] = IDX[{"F": "F_H", "Cl": "Cl_H", "Br": "Br_H"}[e]]
+    return types
+
+
+def make_ligand(seed, n_heavy, n_tors):
+    """Random tree ligand as a LigandPrep (heavy atoms only)."""
+    rng = np.random.default_rng(seed)
+    for _ in range(100):
+        g = _grow_tree(rng, n_heavy)
+        if g is not None:
+            break
+    else:
+        raise RuntimeError("ligand growth failed")
+    pos, nbr, bonds = g
+    types = _ligand_types(rng, nbr)
+    inner = [(min(i, j), max(i, j)) for i, j in bonds if len(nbr[i]) > 1 and len(nbr[j]) > 1]
+    k = min(n_tors, len(inner))
+    rot = [inner[i] for i in sorted(rng.choice(len(inner), k, replace=False))] if k else []
+    return LigandPrep(pos, types, bonds, rot)
+
+
+def grid_dims(size, spacing=0.375):
+    """Points per axis, the reference rule n = ceil(s / h) + 1 (src/adock/grid.py:272)."""
+    return tuple(int(math.ceil(s / spacing)) + 1 for s in np.broadcast_to(np.asarray(size, float), (3,)))
+
+
+def start_states(seed, n, center, size, n_tors, K):
+    """Uniform translation in the box shrunk by 1 A, uniform quaternion, theta ~ U(-pi, pi) on real torsions.
+    Returns (t [n,3], q [n,4], theta [n,K]) float32."""
+    rng = np.random.default_rng(seed)
+    c = np.asarray(center, np.float64)
+    h = np.asarray(size, np.float64) / 2 - 1.0
+    t = (c + rng.uniform(-h, h, (n, 3))).astype(np.float32)
+    q = random_quat(rng, n)
+    th = np.zeros((n, K), np.float32)
+    th[:, :n_tors] = rng.uniform(-math.pi, math.pi, (n, n_tors))
+    return t, q, th
+
+
+# Workload definitions. Receptor radius / cavity / atom counts, ligand sizes, box edges, chains and steps.
+CONFIGS = {
+    1: dict(rec=dict(n_atoms=1500, radius=20.0, cavity=6.0, min_sep=3.0), lig=(20, 4), box=20.0, chains=8, steps=200,
+            bfgs_iters=0),
+    2: dict(rec=dict(n_atoms=4000, radius=26.0, cavity=8.0, min_sep=2.8), lig=(35, 8), box=26.0, chains=256,
+            steps=1000, bfgs_iters=0),
+}
+
+
+def make_single(config, seed=0):
+    """A single-complex workload (configs 1 and 2): receptor, ligand prep, box centre and size."""
+    c = CONFIGS[config]
+    rec = make_receptor(seed, **c["rec"])
+    lig = make_ligand(seed + 1, *c["lig"])
+    return {"receptor": rec, "lig": lig, "center": np.zeros(3), "size": np.full(3, c["box"]), "chains": c["chains"],
+            "steps": c["steps"], "bfgs_iters": c["bfgs_iters"]}

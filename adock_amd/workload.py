@@ -1,0 +1,69 @@
+"""Synthetic single-complex and batched workloads assembled into the containers the kernels read.
+
+`host_arrays` gives the same inputs as plain numpy dicts (packed ligands, flat grids, scoring tables, receptors) so a CPU
+implementation can be driven with identical data.
+"""
+import numpy as np
+import torch
+
+from . import synth
+from .config import load_config, search_prm, sf_params
+from .grid import GridSet, make_gridset
+from .ligand import pack_ligands
+from .problem import LigandBatch, PairPotential, Problem, pad_receptors
+
+CFG = load_config()
+SPACING = CFG["grid"]["spacing"]
+PENALTY = CFG["grid"]["out_of_box_penalty"]
+
+
+def receptor_dict(rec, center, size):
+    return {"rec_xyz": rec["rec_xyz"], "rec_types": rec["rec_types"], "center": np.asarray(center, float),
+            "size": np.broadcast_to(np.asarray(size, float), (3,)).copy()}
+
+
+def grid_types(preps):
+    """Per receptor: ligand XS types present (C_H is added by make_gridset)."""
+    return [np.unique(p.types[p.types >= 0]) for p in preps]
+
+
+def gpu_problem(receptors, preps, rec_idx, sf="vina", device="cuda", with_rec=True):
+    """receptors: list of dicts (rec_xyz, rec_types, center, size); preps: one LigandPrep per complex; rec_idx [L].
+    Grids hold, per receptor, the union of the types of its ligands (+ C_H)."""
+    sfp = sf_params(CFG, sf)
+    rec_idx = np.asarray(rec_idx, np.int64)
+    lt = []
+    for r in range(len(receptors)):
+        ts = [p.types[p.types >= 0] for p, ri in zip(preps, rec_idx) if ri == r]
+        lt.append(np.unique(np.concatenate(ts)) if ts else np.zeros(0, np.int64))
+    gs = make_gridset(receptors, sfp, SPACING, PENALTY, device=device, lig_types=lt)
+    lb = LigandBatch(preps, device)
+    rec = pad_receptors(receptors, device) if with_rec else None
+    return Problem(gs, lb, torch.as_tensor(rec_idx, device=device), PairPotential(sfp, device), rec=rec)
+
+
+def host_arrays(prob):
+    """Numpy copies of everything the kernels read: (AM, KM, lig, grid, sf, rec)."""
+    gs = prob.gs
+    AM, KM, lig = pack_ligands(prob.lb.preps, gs.slot.cpu().numpy(), prob.rec_idx.cpu().numpy())
+    grid = dict(flat=gs.flat.float().cpu().numpy(), origin=gs.origin.float().cpu().numpy(),
+                lo=gs.lo.float().cpu().numpy(), hi=gs.hi.float().cpu().numpy(),
+                rec_idx=prob.rec_idx.cpu().numpy().astype(np.int32), offs=gs.offs.cpu().numpy().astype(np.int64),
+                dims=gs.dims.cpu().numpy().astype(np.int64), spacing=float(gs.spacing), penalty=float(gs.penalty))
+    sp = prob.pot.p
+    sf = dict(prm=search_prm(sp), rad=np.asarray(sp["radius"], np.float32), hyd=np.asarray(sp["hydrophobic"], np.int32),
+              don=np.asarray(sp["donor"], np.int32), acc=np.asarray(sp["acceptor"], np.int32))
+    rec = None
+    if prob.rec is not None:
+        rx, rt, rm = prob.rec
+        rec = dict(rxyz=rx.float().cpu().numpy(), rtyp=rt.cpu().numpy().astype(np.int32),
+                   rn=rm.sum(1).cpu().numpy().astype(np.int32))
+    return AM, KM, lig, grid, sf, rec
+
+
+def single(config, seed=0, sf="vina", device="cuda"):
+    """Configs 1 / 2: one receptor, one ligand. Returns (workload dict, Problem)."""
+    w = synth.make_single(config, seed)
+    rec = receptor_dict(w["receptor"], w["center"], w["size"])
+    prob = gpu_problem([rec], [w["lig"]], [0], sf=sf, device=device)
+    return w, prob

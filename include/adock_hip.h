@@ -1,0 +1,97 @@
+/* C ABI of libadock_hip.so: MI355X (gfx950) docking hot path.
+ *
+ * Every pointer is a device pointer (HIP / PyTorch-ROCm memory) unless stated otherwise; the library allocates
+ * nothing, copies nothing, launches on `stream` (NULL = the default stream) and returns without synchronising.
+ * Return value: 0 on success, a negative ADOCK_E_* code on invalid arguments, or a positive hipError_t from the
+ * launch. adock_last_error() gives a text for the last non-zero status of the calling thread.
+ *
+ * Argument groups follow the reference's kernel parameter order (src/adock/mckernel.py:328-345, bound by
+ * src/adock/cpukernel.py:97-101,263): ligand (13), grid (10), scoring function (5), receptor (4), exact, then the
+ * entry's own parameters. The reference compiles one module per (A_MAX, K_MAX) bucket; here AM and KM are the first
+ * two arguments and select a compiled instantiation (AM = 64; KM in {8, 16, 24, 32}).
+ */
+#ifndef ADOCK_HIP_H
+#define ADOCK_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ADOCK_E_ARG (-1)     /* null pointer or negative size */
+#define ADOCK_E_BUCKET (-2)  /* unsupported AM / KM bucket */
+#define ADOCK_E_LDS (-3)     /* per-wave pair list does not fit the LDS budget */
+#define ADOCK_E_TYPES (-4)   /* too many ligand types */
+
+typedef void* adock_stream_t; /* hipStream_t */
+
+/* Library version string and last error text of the calling thread. */
+const char* adock_version(void);
+const char* adock_last_error(void);
+
+/* Receptor scoring grids. Replaces the reference's build_grid kernel launch (src/adock/grid.py:132-166) and its host
+ * twin launch_build_grid_tiled (src/adock/cpukernel.py:104-108); same arguments minus the host tile size, plus the
+ * stream. prm: 15 floats in grid order (cutoff, g1_off, g1_w, g2_off, g2_w, rep_off, hyd_good, hyd_bad, hb_good,
+ * hb_bad, w_g1, w_g2, w_rep, w_hyd, w_hb). out: [n_lt, nx, ny, nz] (per_term = 0) or [5, n_lt, nx, ny, nz]. */
+int adock_build_grid(const float* rxyz, const int* rtype, int n_rec, const float* radius, const int* hyd,
+                     const int* don, const int* acc, const int* lig_types, int n_lt, float ox, float oy, float oz,
+                     float spacing, int nx, int ny, int nz, const float* prm, int per_term, float* out,
+                     adock_stream_t stream);
+
+/* Energy + analytic gradient for n_chains poses. Replaces launch_eg_kernel (eg_kernel, src/adock/mckernel.py:411-447;
+ * host binding src/adock/cpukernel.py:266-273). state [n, 7+KM], e_out [n], g_out [n, 6+KM] (entries 6+nt.. untouched).
+ * prm: 15 floats in search order (g1_off, g1_w, g2_off, g2_w, rep_off, hyd_good, hyd_bad, hb_good, hb_bad, cutoff,
+ * w_g1, ..., w_hb). */
+int adock_eg(int AM, int KM,
+             const float* ref, const int* typ, const unsigned char* tm, const int* ax, const float* axp,
+             const float* axu, const int* pr, const int* na, const int* nt, const int* np, const float* rg, int P_MAX,
+             const int* gslot,
+             const float* grid, const float* origin, const float* lo, const float* hi, const int* rec_idx,
+             const long long* goffs, const long long* gdims, int T, float spacing, float pen,
+             const float* prm, const float* rad, const int* hyd, const int* don, const int* acc,
+             const float* rxyz, const int* rtyp, const int* rn, int RN_MAX,
+             int exact,
+             const int* chain_lig, int n_chains, const float* state, float* e_out, float* g_out,
+             adock_stream_t stream);
+
+/* Monte Carlo chains with BFGS, in place on cur/cur_e/best/best_e. Replaces launch_mc_kernel (mc_kernel,
+ * src/adock/mckernel.py:328-408; host binding src/adock/cpukernel.py:289-306). nbr/nbr_n may be NULL (NB_MAX 0). */
+int adock_mc(int AM, int KM,
+             const float* ref, const int* typ, const unsigned char* tm, const int* ax, const float* axp,
+             const float* axu, const int* pr, const int* na, const int* nt, const int* np, const float* rg, int P_MAX,
+             const int* gslot,
+             const float* grid, const float* origin, const float* lo, const float* hi, const int* rec_idx,
+             const long long* goffs, const long long* gdims, int T, float spacing, float pen,
+             const float* prm, const float* rad, const int* hyd, const int* don, const int* acc,
+             const float* rxyz, const int* rtyp, const int* rn, int RN_MAX,
+             int exact,
+             const int* chain_lig, const int* chain_id, int n_chains, const int* nbr, const int* nbr_n, int NB_MAX,
+             float* cur, float* cur_e, float* best, float* best_e, int init_opt, int mc_steps, int bfgs_iters,
+             float temp, float amp_pos, float amp_rot, unsigned long long seed, int step_offset,
+             adock_stream_t stream);
+
+/* adock_mc plus a device counter: *eval_count (device unsigned long long, may be NULL) is incremented by the number of
+ * energy+gradient evaluations the launch performs. No reference counterpart: the reference does not count evaluations;
+ * this is the instrumented variant the benchmark uses to report score+grad evals/s. */
+int adock_mc_counted(int AM, int KM,
+                     const float* ref, const int* typ, const unsigned char* tm, const int* ax, const float* axp,
+                     const float* axu, const int* pr, const int* na, const int* nt, const int* np, const float* rg,
+                     int P_MAX, const int* gslot,
+                     const float* grid, const float* origin, const float* lo, const float* hi, const int* rec_idx,
+                     const long long* goffs, const long long* gdims, int T, float spacing, float pen,
+                     const float* prm, const float* rad, const int* hyd, const int* don, const int* acc,
+                     const float* rxyz, const int* rtyp, const int* rn, int RN_MAX,
+                     int exact,
+                     const int* chain_lig, const int* chain_id, int n_chains, const int* nbr, const int* nbr_n,
+                     int NB_MAX, float* cur, float* cur_e, float* best, float* best_e, int init_opt, int mc_steps,
+                     int bfgs_iters, float temp, float amp_pos, float amp_rot, unsigned long long seed,
+                     int step_offset, unsigned long long* eval_count, adock_stream_t stream);
+
+/* Launch shape of the chain kernels: chains (waves) per workgroup, 1..4. The reference's counterpart is the NW
+ * constant (src/adock/mckernel.py:525-540). Returns 0, or ADOCK_E_ARG. */
+int adock_set_chains_per_block(int chains_per_block);
+int adock_get_chains_per_block(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ADOCK_HIP_H */

@@ -1,0 +1,252 @@
+"""GPU parity through the C ABI: HIP kernels vs the FP64 CPU oracle on identical float32 inputs.
+
+Bars (written into each test): grids 1e-5 relative with a 1e-5 kcal/mol floor; per-pose energies 1e-4 relative with a
+1e-4 floor and gradients 1e-4 in vector norm (poses closer than 1e-4 A to a discontinuity of E or dE are excluded and
+counted: a cell face of the trilinear lookup, the 8 A cutoff, a slope-term kink); one BFGS iteration: >= 99% of
+chains within 1e-3; longer BFGS / MC runs: distributions (median, min, mean) within 0.5 kcal/mol, because float
+rounding flips Armijo branches (the reference's own CPU-vs-GPU bars, tests/test_cpukernel.py:118-148)."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from adock_amd import synth, workload
+from adock_amd.config import load_config, sf_params
+from adock_amd.grid import build_grid
+from adock_amd.kernel import KernelProblem
+from helpers import (EG_FLOOR, EG_REL, GRID_FLOOR, GRID_REL, IDX, SPACING, oracle_grid, random_receptor, rel_err,
+                     sf_tables, states, vec_rel)
+
+pytestmark = pytest.mark.gpu
+CFG = load_config()
+MARGIN = 1e-4
+
+
+def _oracle_of(prob):
+    AM, KM, lig, grid, sf, rec = workload.host_arrays(prob)
+    return oracle.OracleProblem(AM, KM, lig, grid, sf, rec)
+
+
+@pytest.fixture(scope="module")
+def cfg1():
+    w, prob = workload.single(1, seed=0)
+    return w, prob, KernelProblem(prob), _oracle_of(prob)
+
+
+@pytest.fixture(scope="module")
+def cfg2():
+    w, prob = workload.single(2, seed=0)
+    return w, prob, KernelProblem(prob), _oracle_of(prob)
+
+
+# ------------------------------------------------------------------------------------------------ grids
+@pytest.mark.parametrize("sf", ["vina", "vinardo"])
+def test_grid_random_receptor(sf):
+    """Reference recipe (tests/test_scoring_grid.py:54-75): 300 atoms in [0,14]^3, origin (3,3,3), n 21."""
+    print(f"[gpu] grid random receptor sf={sf}", flush=True)
+    sfp = sf_params(CFG, sf)
+    xyz, types = random_receptor()
+    origin, n = np.array([3.0, 3.0, 3.0]), (21, 21, 21)
+    lt = np.arange(18)
+    g = build_grid(xyz, types, sfp, origin, SPACING, n, lig_types=lt).cpu().numpy()
+    o = oracle_grid({"rec_xyz": xyz, "rec_types": types}, sfp, origin, n, lt)
+    assert rel_err(g, o, GRID_FLOOR, GRID_REL) < GRID_REL
+    gp = build_grid(xyz, types, sfp, origin, SPACING, n, lig_types=[0, 3, 7, 12], per_term=True).cpu().numpy()
+    op = oracle_grid({"rec_xyz": xyz, "rec_types": types}, sfp, origin, n, [0, 3, 7, 12], per_term=True)
+    assert rel_err(gp, op, GRID_FLOOR, GRID_REL) < GRID_REL
+
+
+def test_grid_config1_full(cfg1):
+    """Whole config-1 grid set (55^3 x types): 1,500 receptor atoms."""
+    print("[gpu] grid config 1", flush=True)
+    w, prob, kp, op = cfg1
+    sfp = sf_params(CFG, "vina")
+    rec = w["receptor"]
+    types = np.nonzero(prob.gs.slot[0].cpu().numpy() >= 0)[0]
+    o = oracle_grid(rec, sfp, -w["size"] / 2, (55, 55, 55), types)
+    g = prob.gs.flat.cpu().numpy().reshape(o.shape)
+    assert rel_err(g, o, GRID_FLOOR, GRID_REL) < GRID_REL
+
+
+def test_grid_config2_subbox(cfg2):
+    """Config 2 (71^3, 4,000 atoms): a 14^3 sub-box of the GPU grid vs the oracle built on that sub-box. The origin
+    -13 A and h = 0.375 make every point an exact multiple of 1/8, so both sides place the points identically."""
+    print("[gpu] grid config 2 sub-box", flush=True)
+    w, prob, kp, op = cfg2
+    sfp = sf_params(CFG, "vina")
+    types = np.nonzero(prob.gs.slot[0].cpu().numpy() >= 0)[0]
+    g = prob.gs.flat.cpu().numpy().reshape(len(types), 71, 71, 71)
+    for s0 in (0, 28, 57):
+        origin = -w["size"] / 2 + s0 * SPACING
+        o = oracle_grid(w["receptor"], sfp, origin, (14, 14, 14), types)
+        assert rel_err(g[:, s0:s0 + 14, s0:s0 + 14, s0:s0 + 14], o, GRID_FLOOR, GRID_REL) < GRID_REL
+
+
+# ------------------------------------------------------------------------------------------------ energy + gradient
+def _eg_parity(kp, op, w, st, exact=False):
+    N = len(st)
+    cl = np.zeros(N, np.int32)
+    e, g = kp.energy_grad(torch.as_tensor(cl).cuda(), torch.as_tensor(st).cuda(), exact=exact)
+    e, g = e.cpu().numpy(), g.cpu().numpy()
+    eo, go, _, m = op.energy_grad(cl, st, exact=exact)
+    keep = m >= MARGIN
+    D = 6 + w["lig"].n_tors
+    assert keep.mean() > 0.9, f"too many poses near a discontinuity: {(~keep).sum()}"
+    er = rel_err(e[keep], eo[keep], EG_FLOOR, EG_REL)
+    gr = vec_rel(g[keep, :D], go[keep, :D], EG_FLOOR)
+    assert er < EG_REL, er
+    assert gr < EG_REL, gr
+    return er, gr
+
+
+@pytest.mark.parametrize("which", ["near", "box"])
+def test_eg_config1(cfg1, which):
+    print(f"[gpu] eg config 1 {which}", flush=True)
+    w, prob, kp, op = cfg1
+    st = states(5, 256, w["center"], w["size"], w["lig"], kp.KM, sigma=1.5 if which == "near" else None)
+    _eg_parity(kp, op, w, st)
+
+
+def test_eg_config2(cfg2):
+    print("[gpu] eg config 2", flush=True)
+    w, prob, kp, op = cfg2
+    st = states(6, 256, w["center"], w["size"], w["lig"], kp.KM, sigma=2.0)
+    _eg_parity(kp, op, w, st)
+
+
+def test_eg_vinardo():
+    print("[gpu] eg vinardo", flush=True)
+    w, prob = workload.single(1, seed=3, sf="vinardo")
+    kp, op = KernelProblem(prob), _oracle_of(prob)
+    st = states(7, 256, w["center"], w["size"], w["lig"], kp.KM, sigma=1.5)
+    _eg_parity(kp, op, w, st)
+
+
+def test_eg_exact_mode(cfg1):
+    print("[gpu] eg exact", flush=True)
+    w, prob, kp, op = cfg1
+    st = states(8, 128, w["center"], w["size"], w["lig"], kp.KM, sigma=1.5)
+    _eg_parity(kp, op, w, st, exact=True)
+
+
+def test_eg_many_ligands_two_receptors():
+    """Multi-receptor flat grid set with uneven ligands (10-60 atoms, 0-15 torsions: KM bucket 16): chain_lig and
+    rec_idx indirection."""
+    print("[gpu] eg batch", flush=True)
+    rng = np.random.default_rng(0)
+    recs = [workload.receptor_dict(synth.make_receptor(s, 1500, 20.0, 6.0), np.zeros(3), 20.0 + 4 * s) for s in (0, 1)]
+    preps = [synth.make_ligand(100 + i, int(rng.integers(10, 61)), int(rng.integers(0, 16))) for i in range(6)]
+    prob = workload.gpu_problem(recs, preps, [0, 1, 0, 1, 1, 0])
+    kp, op = KernelProblem(prob), _oracle_of(prob)
+    assert kp.KM == 16
+    N = 240
+    cl = rng.integers(0, len(preps), N).astype(np.int32)
+    st = np.zeros((N, kp.SD), np.float32)
+    for i in range(N):
+        p = preps[cl[i]]
+        st[i] = states(1000 + i, 1, np.zeros(3), 18.0, p, kp.KM, sigma=2.0)[0]
+    e, g = kp.energy_grad(torch.as_tensor(cl).cuda(), torch.as_tensor(st).cuda())
+    e, g = e.cpu().numpy(), g.cpu().numpy()
+    eo, go, _, m = op.energy_grad(cl, st)
+    keep = m >= MARGIN
+    assert rel_err(e[keep], eo[keep], EG_FLOOR, EG_REL) < EG_REL
+    for i in np.nonzero(keep)[0]:
+        D = 6 + preps[cl[i]].n_tors
+        assert vec_rel(g[i:i + 1, :D], go[i:i + 1, :D], EG_FLOOR) < EG_REL, i
+
+
+# ------------------------------------------------------------------------------------------------ BFGS / MC
+def _run_both(kp, op, st, init_opt, mc_steps, iters, seed=0, step_offset=0, f64=True, cl=None):
+    N = len(st)
+    cl = np.zeros(N, np.int32) if cl is None else cl
+    ci = np.arange(N, dtype=np.int32)
+    out = {}
+    cur, best = torch.as_tensor(st).cuda(), torch.as_tensor(st).cuda()
+    ce, be = torch.zeros(N).cuda(), torch.full((N,), float("inf")).cuda()
+    kp.run_mc(torch.as_tensor(cl).cuda(), torch.as_tensor(ci).cuda(), cur, ce, best, be, init_opt, mc_steps, iters,
+              1.2, 2.0, 2.0, seed, step_offset)
+    out["gpu"] = (cur.cpu().numpy(), ce.cpu().numpy(), best.cpu().numpy(), be.cpu().numpy())
+    cur, best = st.copy(), st.copy()
+    ce, be = np.zeros(N, np.float32), np.full(N, np.inf, np.float32)
+    op.run_mc(cl, ci, cur, ce, best, be, init_opt, mc_steps, iters, 1.2, 2.0, 2.0, seed, step_offset, f64=f64)
+    out["cpu"] = (cur, ce, best, be)
+    return out
+
+
+def test_bfgs_one_iteration(cfg1):
+    print("[gpu] bfgs 1 iteration", flush=True)
+    w, prob, kp, op = cfg1
+    st = states(9, 512, w["center"], w["size"], w["lig"], kp.KM, sigma=1.0)
+    st[:, 7:] = 0
+    out = _run_both(kp, op, st, True, 0, 1)
+    eg, ec = out["gpu"][3], out["cpu"][3]
+    ok = np.abs(eg - ec) <= 1e-3 * np.maximum(np.abs(ec), 1)
+    assert ok.mean() >= 0.99, ok.mean()
+
+
+def test_bfgs_100_iterations(cfg1):
+    print("[gpu] bfgs 100 iterations", flush=True)
+    w, prob, kp, op = cfg1
+    st = states(10, 512, w["center"], w["size"], w["lig"], kp.KM, sigma=1.0)
+    out = _run_both(kp, op, st, True, 0, 100)
+    eg, ec = out["gpu"][3], out["cpu"][3]
+    assert abs(np.median(eg) - np.median(ec)) < 0.5
+    assert abs(eg.min() - ec.min()) < 0.5
+    assert (np.abs(eg - ec) <= 1e-3 * np.maximum(np.abs(ec), 1)).mean() > 0.2
+
+
+def test_mc_rng_and_mutation_match(cfg1):
+    """init_opt off, one MC step, one BFGS iteration: the move (entity, ball vector, torsion draw) comes from the
+    counter-based RNG, so matching energies show the GPU draws the same numbers as the oracle."""
+    print("[gpu] mc one step", flush=True)
+    w, prob, kp, op = cfg1
+    st = states(11, 512, w["center"], w["size"], w["lig"], kp.KM, sigma=1.0)
+    out = _run_both(kp, op, st, False, 1, 1, seed=777, step_offset=5)
+    xg, xc = out["gpu"][0], out["cpu"][0]
+    eg, ec = out["gpu"][1], out["cpu"][1]
+    ok = np.abs(eg - ec) <= 1e-3 * np.maximum(np.abs(ec), 1)
+    assert ok.mean() >= 0.97, ok.mean()
+
+
+def test_mc_config1_properties(cfg1):
+    """The config-1 workload itself: 8 chains x 200 MC steps, Vina-rule BFGS, seed 0."""
+    print("[gpu] mc config 1 (8 x 200)", flush=True)
+    w, prob, kp, op = cfg1
+    N = w["chains"]
+    st = states(0, N, w["center"], w["size"], w["lig"], kp.KM)
+    cl, ci = torch.zeros(N, dtype=torch.int32).cuda(), torch.arange(N, dtype=torch.int32).cuda()
+    e0, _ = kp.energy_grad(cl, torch.as_tensor(st).cuda())
+    runs = []
+    for split in (False, True, False):
+        cur, best = torch.as_tensor(st).cuda(), torch.as_tensor(st).cuda()
+        ce, be = torch.zeros(N).cuda(), torch.full((N,), float("inf")).cuda()
+        seed64 = 0 * 1_000_003 + 12345
+        if split:
+            kp.run_mc(cl, ci, cur, ce, best, be, True, 100, 0, 1.2, 2.0, 2.0, seed64, 0)
+            kp.run_mc(cl, ci, cur, ce, best, be, False, 100, 0, 1.2, 2.0, 2.0, seed64, 100)
+        else:
+            kp.run_mc(cl, ci, cur, ce, best, be, True, 200, 0, 1.2, 2.0, 2.0, seed64, 0)
+        runs.append([x.cpu().numpy() for x in (cur, ce, best, be)])
+    be = runs[0][3]
+    assert (be <= e0.cpu().numpy() + 1e-4).all()
+    e1, _ = kp.energy_grad(cl, torch.as_tensor(runs[0][2]).cuda())
+    np.testing.assert_allclose(e1.cpu().numpy(), be, rtol=1e-4, atol=1e-4)
+    for a, b in zip(runs[0], runs[2]):            # deterministic: same launch twice, same bits
+        np.testing.assert_array_equal(a, b)
+    for a, b in zip(runs[0], runs[1]):            # segment split == one launch
+        np.testing.assert_array_equal(a, b)
+    assert be.min() < -5.0
+
+
+def test_mc_best_score_distribution(cfg1):
+    """Same seeded starts, same seed: the per-chain best-score distributions of GPU and FP64 oracle agree."""
+    print("[gpu] mc distribution (64 x 30)", flush=True)
+    w, prob, kp, op = cfg1
+    st = states(12, 64, w["center"], w["size"], w["lig"], kp.KM)
+    out = _run_both(kp, op, st, True, 30, 0, seed=12345)
+    eg, ec = out["gpu"][3], out["cpu"][3]
+    assert abs(np.median(eg) - np.median(ec)) < 0.5, (np.median(eg), np.median(ec))
+    assert abs(eg.min() - ec.min()) < 0.5, (eg.min(), ec.min())
+    assert abs(eg.mean() - ec.mean()) < 0.5
