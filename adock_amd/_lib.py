@@ -37,9 +37,14 @@ def lib():
         L.adock_mc.argtypes = LIG + GRID + SF + REC + [I, P, P, I, P, P, I, P, P, P, P, I, I, I, F, F, F, U64, I, P]
         L.adock_mc_counted.argtypes = (LIG + GRID + SF + REC +
                                        [I, P, P, I, P, P, I, P, P, P, P, I, I, I, F, F, F, U64, I, P, P])
+        L.adock_mc_batch.argtypes = (LIG + GRID + SF + REC +
+                                     [I, P, P, I, P, P, I, P, P, P, P, I, I, I, F, F, F, U64, I, P, P, P, P, P, I, P,
+                                      P])
+        L.adock_batch_reset.argtypes = [P, P, I, P, P]
         L.adock_set_chains_per_block.argtypes = [I]
         L.adock_get_chains_per_block.restype = I
-        for name in ("adock_build_grid", "adock_eg", "adock_mc", "adock_mc_counted", "adock_set_chains_per_block"):
+        for name in ("adock_build_grid", "adock_eg", "adock_mc", "adock_mc_counted", "adock_mc_batch",
+                     "adock_batch_reset", "adock_set_chains_per_block"):
             getattr(L, name).restype = I
         _LIB = L
     return _LIB

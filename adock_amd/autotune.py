@@ -1,0 +1,83 @@
+"""Launch-shape autotuner for the chain kernels.
+
+Chooses chains (wavefronts) per workgroup W in {1, 2, 4} and the persistent grid size (resident capacity x {1, 2}, or
+one workgroup per W chains) for a given batch by timing short probes of the real workload (the same chains, a few MC
+steps, on scratch copies of the states) and keeps the fastest. Results are cached per (KM, chain-count bucket,
+pair-list size bucket, device name) for the process and optionally in a JSON file.
+"""
+import json
+import math
+import os
+import time
+
+import torch
+
+from . import _lib
+from .batch import BatchRunner
+
+
+def _key(kp, n):
+    name = torch.cuda.get_device_name(kp.device).replace(" ", "_")
+    return f"{name}|KM{kp.KM}|N{2 ** int(math.log2(max(n, 1)))}|P{(kp.P_MAX + 255) // 256}"
+
+
+class Autotuner:
+    def __init__(self, cache_file=None):
+        self.cache_file = cache_file or os.environ.get("ADOCK_TUNE_CACHE")
+        self.cache = {}
+        if self.cache_file and os.path.exists(self.cache_file):
+            self.cache = json.loads(open(self.cache_file).read())
+
+    def tune(self, runner, state, probe_steps=4, reps=2, candidates=None):
+        """runner: BatchRunner; state: start states [N, SD] (device). Returns (W, blocks) and applies it."""
+        kp = runner.kp
+        key = _key(kp, runner.N)
+        if key in self.cache:
+            W, blocks = self.cache[key]
+            self.apply(runner, W, blocks)
+            return W, blocks
+        cus = torch.cuda.get_device_properties(kp.device).multi_processor_count
+        candidates = candidates or [(W, b) for W in (4, 2, 1) for b in (0, -2, -1)]
+        best, best_t = None, float("inf")
+        dev = kp.device
+        ci = torch.arange(runner.N, dtype=torch.int32, device=dev)
+        for W, b in candidates:
+            self.apply(runner, W, b, cus)
+            times = []
+            for r in range(reps + 1):
+                cur, bst = state.clone(), state.clone()
+                ce = torch.zeros(runner.N, device=dev)
+                be = torch.full((runner.N,), float("inf"), device=dev)
+                torch.cuda.synchronize(dev)
+                t0 = time.perf_counter()
+                runner.launch(ci, cur, ce, bst, be, True, probe_steps, 0, 1.2, 2.0, 2.0, 1234, 0)
+                torch.cuda.synchronize(dev)
+                if r:
+                    times.append(time.perf_counter() - t0)
+            t = min(times)
+            if t < best_t:
+                best, best_t = (W, b), t
+        self.cache[key] = list(best)
+        if self.cache_file:
+            with open(self.cache_file, "w") as fh:
+                json.dump(self.cache, fh, indent=1)
+        self.apply(runner, *best, cus)
+        return tuple(best)
+
+    @staticmethod
+    def apply(runner, W, blocks, cus=None):
+        """blocks: 0 = resident capacity (CUs x occupancy); -2 = twice that; -1 = one workgroup per W chains."""
+        _lib.check(_lib.lib().adock_set_chains_per_block(int(W)))
+        if blocks == -1:
+            runner.blocks = (runner.N + W - 1) // W
+        elif blocks == -2:
+            cus = cus or torch.cuda.get_device_properties(runner.kp.device).multi_processor_count
+            runner.blocks = 2 * _resident_blocks(runner, W, cus)
+        else:
+            runner.blocks = int(blocks)
+
+
+def _resident_blocks(runner, W, cus):
+    """Resident-capacity estimate: the library's own occupancy query runs when blocks == 0; here CUs x (8 waves per
+    SIMD x 4 SIMDs / W) bounded by the chain count, a safe upper bound that the hardware trims."""
+    return min(cus * max(1, 16 // W), (runner.N + W - 1) // W)

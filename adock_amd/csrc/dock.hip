@@ -17,6 +17,9 @@
 //     registers, so H.g and the rank-2 update are register FMAs against readlane-broadcast operands.
 //   Gradient lane layout: 0-2 translation, 3-5 torque, 6 always zero, 7+k torsion k (state layout t(3) q(4) theta:
 //   torsion k sits on the same lane in the state and in the gradient; the zero lane is inert in BFGS).
+#include <map>
+#include <mutex>
+
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -316,16 +319,11 @@ DEVI float bfgs(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, int lane
     return f0;
 }
 
+// One MC chain on the calling wave (all 64 lanes active). w: this wave's LDS region.
 template <int KM>
-__global__ __launch_bounds__(256) void mc_kernel(LigArgs L, GridArgs G, SFArgs S, RecArgs R, int exact,
-                                                  const int* __restrict__ chain_lig, int n_chains, McArgs M) {
-    extern __shared__ __attribute__((aligned(16))) char lds[];
+DEVI int run_chain(int ch, int lane, const WaveLDS& w, const LigArgs& L, const GridArgs& G, const SFArgs& S,
+                   const RecArgs& R, int exact, const int* __restrict__ chain_lig, const McArgs& M) {
     constexpr int SD = 7 + KM;
-    const int lane = lane_id();
-    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
-    const int ch = blockIdx.x * (blockDim.x / WAVE) + wid;
-    if (ch >= n_chains) return;
-    const WaveLDS w = carve_lds(lds + wid * wave_lds_bytes(L.P_MAX), L.P_MAX);
     const SFConst sf = make_sf(S.prm);
     const int l = __builtin_amdgcn_readfirstlane(chain_lig[ch]);
     ChainCtx c = setup_chain<KM>(l, lane, L, G, S, R, exact, w);
@@ -384,6 +382,45 @@ __global__ __launch_bounds__(256) void mc_kernel(LigArgs L, GridArgs G, SFArgs S
         M.cur_e[ch] = ec; M.best_e[ch] = eb;
         if (M.eval_count) atomicAdd(M.eval_count, (unsigned long long)nev);
     }
+    return l;
+}
+
+template <int KM>
+__global__ __launch_bounds__(256) void mc_kernel(LigArgs L, GridArgs G, SFArgs S, RecArgs R, int exact,
+                                                  const int* __restrict__ chain_lig, int n_chains, McArgs M) {
+    extern __shared__ __attribute__((aligned(16))) char lds[];
+    const int lane = lane_id();
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+    const int ch = blockIdx.x * (blockDim.x / WAVE) + wid;
+    if (ch >= n_chains) return;
+    const WaveLDS w = carve_lds(lds + wid * wave_lds_bytes(L.P_MAX), L.P_MAX);
+    run_chain<KM>(ch, lane, w, L, G, S, R, exact, chain_lig, M);
+}
+
+// Persistent variant: a grid sized to the chip's resident capacity; every wave pulls chain indices from a global
+// counter (in `order`, e.g. longest chains first) until the queue is empty. Per complex it counts finished chains and
+// stamps the 100 MHz real-time clock when the last one ends (times[0] = earliest wave start).
+template <int KM>
+__global__ __launch_bounds__(256) void mc_persistent_kernel(LigArgs L, GridArgs G, SFArgs S, RecArgs R, int exact,
+                                                             const int* __restrict__ chain_lig, int n_chains, McArgs M,
+                                                             QueueArgs Q) {
+    extern __shared__ __attribute__((aligned(16))) char lds[];
+    const int lane = lane_id();
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+    const WaveLDS w = carve_lds(lds + wid * wave_lds_bytes(L.P_MAX), L.P_MAX);
+    if (Q.times && lane == 0) atomicMin(Q.times, (unsigned long long)__builtin_amdgcn_s_memrealtime());
+    while (true) {
+        int idx = 0;
+        if (lane == 0) idx = atomicAdd(Q.counter, 1);
+        idx = bcast_i(idx, 0);
+        if (idx >= n_chains) break;
+        const int ch = Q.order ? __builtin_amdgcn_readfirstlane(Q.order[idx]) : idx;
+        const int l = run_chain<KM>(ch, lane, w, L, G, S, R, exact, chain_lig, M);
+        if (Q.done && lane == 0) {
+            const int old = atomicAdd(Q.done + l, 1);
+            if (old + 1 == Q.lig_chains[l] && Q.times) Q.times[1 + l] = __builtin_amdgcn_s_memrealtime();
+        }
+    }
 }
 
 template <int KM>
@@ -424,6 +461,36 @@ int launch_mc_t(const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecA
 }
 
 template <int KM>
+int launch_mc_persistent_t(const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecArgs& R, int exact,
+                           const int* chain_lig, int n_chains, const McArgs& M, const QueueArgs& Q, int blocks,
+                           hipStream_t stream) {
+    const int W = g_waves_per_block;
+    const size_t shm = (size_t)W * lds_bytes_per_wave(L.P_MAX);
+    if (blocks <= 0) {
+        // resident capacity, queried once per (device, W, LDS size) so later launches (and graph captures) only launch
+        static std::mutex mu;
+        static std::map<long long, int> cap;
+        int dev = 0;
+        hipGetDevice(&dev);
+        const long long key = ((long long)dev << 40) | ((long long)W << 32) | (long long)shm;
+        std::lock_guard<std::mutex> lock(mu);
+        auto it = cap.find(key);
+        if (it == cap.end()) {
+            int cus = 0, per_cu = 0;
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, mc_persistent_kernel<KM>, W * WAVE, shm);
+            it = cap.emplace(key, cus * (per_cu > 0 ? per_cu : 1)).first;
+        }
+        blocks = it->second;
+    }
+    const int need = (n_chains + W - 1) / W;
+    if (blocks > need) blocks = need;
+    hipLaunchKernelGGL(mc_persistent_kernel<KM>, dim3(blocks), dim3(W * WAVE), shm, stream, L, G, S, R, exact,
+                       chain_lig, n_chains, M, Q);
+    return (int)hipGetLastError();
+}
+
+template <int KM>
 int launch_eg_t(const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecArgs& R, int exact,
                 const int* chain_lig, int n_chains, const float* state, float* e_out, float* g_out,
                 hipStream_t stream) {
@@ -454,6 +521,34 @@ int launch_mc(int KM, const LigArgs& L, const GridArgs& G, const SFArgs& S, cons
         case 32: return launch_mc_t<32>(L, G, S, R, exact, chain_lig, n_chains, M, stream);
         default: return -1;
     }
+}
+
+int launch_mc_persistent(int KM, const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecArgs& R, int exact,
+                         const int* chain_lig, int n_chains, const McArgs& M, const QueueArgs& Q, int blocks,
+                         hipStream_t stream) {
+    switch (KM) {
+        case 8: return launch_mc_persistent_t<8>(L, G, S, R, exact, chain_lig, n_chains, M, Q, blocks, stream);
+        case 16: return launch_mc_persistent_t<16>(L, G, S, R, exact, chain_lig, n_chains, M, Q, blocks, stream);
+        case 24: return launch_mc_persistent_t<24>(L, G, S, R, exact, chain_lig, n_chains, M, Q, blocks, stream);
+        case 32: return launch_mc_persistent_t<32>(L, G, S, R, exact, chain_lig, n_chains, M, Q, blocks, stream);
+        default: return -1;
+    }
+}
+
+__global__ void batch_reset_kernel(int* counter, int* done, int n_done, unsigned long long* times, int n_times) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0 && counter) counter[0] = 0;
+    if (done && i < n_done) done[i] = 0;
+    if (times && i < n_times) times[i] = i == 0 ? ~0ull : 0ull;
+}
+
+int launch_batch_reset(int* counter, int* done, int n_done, unsigned long long* times, int n_times,
+                       hipStream_t stream) {
+    int n = n_done > n_times ? n_done : n_times;
+    if (n < 1) n = 1;
+    hipLaunchKernelGGL(batch_reset_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, counter, done, n_done, times,
+                       n_times);
+    return (int)hipGetLastError();
 }
 
 int launch_eg(int KM, const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecArgs& R, int exact,

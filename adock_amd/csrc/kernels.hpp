@@ -19,12 +19,33 @@ struct McArgs {
     int init_opt, mc_steps, bfgs_iters; float temp, amp_pos, amp_rot; unsigned long long seed; int step_offset;
     unsigned long long* eval_count;
 };
+// work queue of the persistent chain kernel: order [n] (chain indices, may be NULL = identity), counter (1 int, 0 at
+// launch), done [L] (finished chains per complex, 0 at launch), lig_chains [L] (chains per complex), times [1 + L]
+// (times[0] = ~0 at launch; 100 MHz real-time stamps). done / lig_chains / times may be NULL.
+struct QueueArgs {
+    const int* order; int* counter; int* done; const int* lig_chains; unsigned long long* times;
+};
 int launch_build_grid(const float*, const int*, int, const float*, const int*, const int*, const int*, const int*,
                       int, float, float, float, float, int, int, int, const float*, int, float*, hipStream_t);
 int launch_mc(int KM, const LigArgs&, const GridArgs&, const SFArgs&, const RecArgs&, int, const int*, int,
               const McArgs&, hipStream_t);
 int launch_eg(int KM, const LigArgs&, const GridArgs&, const SFArgs&, const RecArgs&, int, const int*, int,
               const float*, float*, float*, hipStream_t);
+int launch_mc_persistent(int KM, const LigArgs&, const GridArgs&, const SFArgs&, const RecArgs&, int, const int*, int,
+                         const McArgs&, const QueueArgs&, int blocks, hipStream_t);
+int launch_batch_reset(int* counter, int* done, int n_done, unsigned long long* times, int n_times, hipStream_t);
+int launch_mlp_infer(int B, const float* X, const unsigned short* W1, const float* b1, const unsigned short* W2,
+                     const float* b2, const unsigned short* W3, const float* b3, float* Y, hipStream_t s);
+int launch_gemm_f32(int ta, int tb, int M, int N, int K, float alpha, const float* A, int lda, const float* Bm,
+                    int ldb, float beta, float* C, int ldc, hipStream_t s);
+int launch_bias_act(int M, int N, float* Y, const float* b, int relu, hipStream_t s);
+int launch_relu_bwd(long long n, float* dH, const float* H, hipStream_t s);
+int launch_colsum(int M, int N, const float* dY, float* db, hipStream_t s);
+int launch_mse(long long n, const float* Y, const float* T, const float* Mk, float inv_norm, float* dY, float* loss,
+               hipStream_t s);
+int launch_adam(long long n, float* w, const float* g, float* m, float* v, float lr, float b1, float b2, float eps,
+                int t, hipStream_t s);
+int launch_to_bf16(long long n, const float* x, unsigned short* y, hipStream_t s);
 int set_waves_per_block(int w);
 int get_waves_per_block();
 int lds_budget_ok(int P_MAX);

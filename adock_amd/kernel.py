@@ -59,6 +59,9 @@ class KernelProblem:
                             rn=torch.zeros(1, dtype=torch.int32, device=dev))
             self.RN_MAX = 0
         self.L, self.SD, self.DM = prob.lb.L, 7 + self.KM, 6 + self.KM
+        from .batch import chain_cost
+        self.preps = prob.lb.preps
+        self.cost = chain_cost(self.preps)
         self.eval_count = torch.zeros(1, dtype=torch.int64, device=dev)
 
     # ---- state packing ----

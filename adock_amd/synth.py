@@ -222,3 +222,33 @@ def make_single(config, seed=0):
     lig = make_ligand(seed + 1, *c["lig"])
     return {"receptor": rec, "lig": lig, "center": np.zeros(3), "size": np.full(3, c["box"]), "chains": c["chains"],
             "steps": c["steps"], "bfgs_iters": c["bfgs_iters"]}
+
+
+def make_screen(seed=0, n_ligands=512, chains=64, steps=500):
+    """Config 3: one config-1 receptor and box, n_ligands ligands with heavy atoms ~ U[10, 60] and torsions
+    ~ U[0, 15] (capped by the ligand's non-terminal bonds)."""
+    rng = np.random.default_rng(seed + 991)
+    rec = make_receptor(seed, **CONFIGS[1]["rec"])
+    ligs = [make_ligand(seed * 100003 + 17 + i, int(rng.integers(10, 61)), int(rng.integers(0, 16)))
+            for i in range(n_ligands)]
+    return {"receptor": rec, "ligs": ligs, "center": np.zeros(3), "size": np.full(3, CONFIGS[1]["box"]),
+            "chains": chains, "steps": steps, "bfgs_iters": 0}
+
+
+def make_multi(seed=0, n_receptors=64, ligands_per=32, chains=32, steps=500):
+    """Config 4: n_receptors receptors of 2,000-6,000 atoms (sphere radius for ~0.05 atoms/A^3 around a 6-8 A
+    cavity), box edges ~ U[16, 30] A, ligands_per ligands each from the U[10, 60] atoms / U[0, 15] torsions mix."""
+    rng = np.random.default_rng(seed + 4242)
+    recs, boxes, ligs, rec_idx = [], [], [], []
+    for r in range(n_receptors):
+        n = int(rng.integers(2000, 6001))
+        cav = float(rng.uniform(6.0, 8.0))
+        rad = (3.0 * n / (4.0 * math.pi * 0.05) + cav ** 3) ** (1.0 / 3.0)
+        recs.append(make_receptor(seed * 7919 + r, n, rad, cav, min_sep=2.8))
+        boxes.append(float(rng.uniform(16.0, 30.0)))
+        for k in range(ligands_per):
+            ligs.append(make_ligand(seed * 100003 + 50000 + r * ligands_per + k, int(rng.integers(10, 61)),
+                                    int(rng.integers(0, 16))))
+            rec_idx.append(r)
+    return {"receptors": recs, "boxes": boxes, "ligs": ligs, "rec_idx": np.asarray(rec_idx), "chains": chains,
+            "steps": steps, "bfgs_iters": 0}

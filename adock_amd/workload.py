@@ -67,3 +67,32 @@ def single(config, seed=0, sf="vina", device="cuda"):
     rec = receptor_dict(w["receptor"], w["center"], w["size"])
     prob = gpu_problem([rec], [w["lig"]], [0], sf=sf, device=device)
     return w, prob
+
+
+def screen(seed=0, n_ligands=512, chains=64, steps=500, sf="vina", device="cuda"):
+    """Config 3: one grid set (union of ligand types + C_H) shared by n_ligands ligands."""
+    w = synth.make_screen(seed, n_ligands, chains, steps)
+    rec = receptor_dict(w["receptor"], w["center"], w["size"])
+    prob = gpu_problem([rec], w["ligs"], np.zeros(len(w["ligs"]), np.int64), sf=sf, device=device, with_rec=False)
+    return w, prob
+
+
+def multi(seed=0, n_receptors=64, ligands_per=32, chains=32, steps=500, sf="vina", device="cuda"):
+    """Config 4: one grid set per receptor (its ligands' types + C_H), all resident."""
+    w = synth.make_multi(seed, n_receptors, ligands_per, chains, steps)
+    recs = [receptor_dict(r, np.zeros(3), b) for r, b in zip(w["receptors"], w["boxes"])]
+    prob = gpu_problem(recs, w["ligs"], w["rec_idx"], sf=sf, device=device, with_rec=False)
+    return w, prob
+
+
+def chain_starts(seed, preps, centers, sizes, chains, KM):
+    """Start states for `chains` chains per complex (translation uniform in the shrunk box, uniform quaternion,
+    theta ~ U(-pi, pi)); returns (state [L*chains, 7+KM] f32, chain_lig [L*chains] i32)."""
+    L = len(preps)
+    st = np.zeros((L * chains, 7 + KM), np.float32)
+    for l, p in enumerate(preps):
+        t, q, th = synth.start_states(seed * 7 + l, chains, centers[l], sizes[l], p.n_tors, KM)
+        st[l * chains:(l + 1) * chains, :3] = t
+        st[l * chains:(l + 1) * chains, 3:7] = q
+        st[l * chains:(l + 1) * chains, 7:] = th
+    return st, np.repeat(np.arange(L, dtype=np.int32), chains)

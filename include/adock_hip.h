@@ -86,6 +86,33 @@ int adock_mc_counted(int AM, int KM,
                      int bfgs_iters, float temp, float amp_pos, float amp_rot, unsigned long long seed,
                      int step_offset, unsigned long long* eval_count, adock_stream_t stream);
 
+/* Batched screening in one persistent launch: same chain arguments as adock_mc, plus a work queue. The grid is sized to
+ * the chip's resident capacity (blocks <= 0: CUs x occupancy) and every wavefront pulls chain indices from *counter
+ * through `order` (NULL = 0..n-1; pass longest-first for load balance) until the queue is empty. Optional per-complex
+ * bookkeeping: done [L] counts finished chains, lig_chains [L] gives chains per complex, times [1+L] receives the
+ * 100 MHz real-time stamp of the first wave start (times[0]) and of each complex's last chain (times[1+l]).
+ * counter, done and times must be reset before every launch (adock_batch_reset: a kernel, so the pair can be captured
+ * in a HIP graph). eval_count may be NULL. No reference counterpart: the reference launches one block per NW chains
+ * and compacts between segments on the host (src/adock/search.py:614-636). */
+int adock_mc_batch(int AM, int KM,
+                   const float* ref, const int* typ, const unsigned char* tm, const int* ax, const float* axp,
+                   const float* axu, const int* pr, const int* na, const int* nt, const int* np, const float* rg,
+                   int P_MAX, const int* gslot,
+                   const float* grid, const float* origin, const float* lo, const float* hi, const int* rec_idx,
+                   const long long* goffs, const long long* gdims, int T, float spacing, float pen,
+                   const float* prm, const float* rad, const int* hyd, const int* don, const int* acc,
+                   const float* rxyz, const int* rtyp, const int* rn, int RN_MAX,
+                   int exact,
+                   const int* chain_lig, const int* chain_id, int n_chains, const int* nbr, const int* nbr_n,
+                   int NB_MAX, float* cur, float* cur_e, float* best, float* best_e, int init_opt, int mc_steps,
+                   int bfgs_iters, float temp, float amp_pos, float amp_rot, unsigned long long seed,
+                   int step_offset, const int* order, int* counter, int* done, const int* lig_chains,
+                   unsigned long long* times, int blocks, unsigned long long* eval_count, adock_stream_t stream);
+
+/* Resets the work queue of adock_mc_batch with a kernel launch: *counter = 0, done[0..L) = 0, times[0] = ~0,
+ * times[1..L] = 0 (done / times may be NULL). */
+int adock_batch_reset(int* counter, int* done, int n_complexes, unsigned long long* times, adock_stream_t stream);
+
 /* Launch shape of the chain kernels: chains (waves) per workgroup, 1..4. The reference's counterpart is the NW
  * constant (src/adock/mckernel.py:525-540). Returns 0, or ADOCK_E_ARG. */
 int adock_set_chains_per_block(int chains_per_block);
