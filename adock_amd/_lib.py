@@ -41,10 +41,22 @@ def lib():
                                      [I, P, P, I, P, P, I, P, P, P, P, I, I, I, F, F, F, U64, I, P, P, P, P, P, I, P,
                                       P])
         L.adock_batch_reset.argtypes = [P, P, I, P, P]
+        LL = ctypes.c_longlong
+        L.adock_mc_step.argtypes = (LIG + GRID + SF + REC +
+                                    [P, P, I, P, P, P, P, I, I, I, F, F, F, U64, P, P, P, P, I, I, F, F, F, F, P, P])
+        L.adock_mlp_infer.argtypes = [I, P, P, P, P, P, P, P, P, P]
+        L.adock_gemm_f32.argtypes = [I, I, I, I, I, F, P, I, P, I, F, P, I, P]
+        L.adock_bias_act.argtypes = [I, I, P, P, I, P]
+        L.adock_relu_bwd.argtypes = [LL, P, P, P]
+        L.adock_colsum.argtypes = [I, I, P, P, P]
+        L.adock_mse.argtypes = [LL, P, P, P, F, P, P, P]
+        L.adock_adam.argtypes = [LL, P, P, P, P, F, F, F, F, I, P]
+        L.adock_to_bf16.argtypes = [LL, P, P, P]
         L.adock_set_chains_per_block.argtypes = [I]
         L.adock_get_chains_per_block.restype = I
         for name in ("adock_build_grid", "adock_eg", "adock_mc", "adock_mc_counted", "adock_mc_batch",
-                     "adock_batch_reset", "adock_set_chains_per_block"):
+                     "adock_batch_reset", "adock_set_chains_per_block", "adock_mc_step", "adock_mlp_infer", "adock_gemm_f32",
+                     "adock_bias_act", "adock_relu_bwd", "adock_colsum", "adock_mse", "adock_adam", "adock_to_bf16"):
             getattr(L, name).restype = I
         _LIB = L
     return _LIB

@@ -136,6 +136,69 @@ int adock_batch_reset(int* counter, int* done, int n_complexes, unsigned long lo
                                          (hipStream_t)stream));
 }
 
+int adock_mc_step(int AM, int KM, LIG_DECL, GRID_DECL, SF_DECL, REC_DECL, const int* chain_lig, const int* chain_id,
+                  int n_chains, float* cur, float* cur_e, float* best, float* best_e, int init, int step, int bfgs_iters,
+                  float temp, float amp_pos, float amp_rot, unsigned long long seed, const float* Y, float* feat,
+                  float* log_buf, int* log_count, int log_cap, int use_net, float p_plain, float sig_t, float sig_r,
+                  float sig_th, unsigned long long* eval_count, adock_stream_t stream) {
+    if (int st = common_checks(AM, KM, P_MAX, ref, grid, prm, 0, rxyz, rn, RN_MAX, chain_lig, n_chains)) return st;
+    if (n_chains == 0) return 0;
+    if (!chain_id || !cur || !cur_e || !best || !best_e) return fail(ADOCK_E_ARG, "null pointer");
+    if (log_buf && (!log_count || !feat || log_cap < 0)) return fail(ADOCK_E_ARG, "logging needs feat and a counter");
+    if (use_net && (!Y || !feat)) return fail(ADOCK_E_ARG, "network moves need Y and feat");
+    LigArgs L{ref, typ, tm, ax, axp, axu, pr, na, nt, np, rg, P_MAX, gslot};
+    GridArgs G{grid, origin, lo, hi, rec_idx, goffs, gdims, T, spacing, pen};
+    SFArgs S{prm, rad, hyd, don, acc};
+    RecArgs R{rxyz, rtyp, rn, RN_MAX};
+    McArgs M{chain_id, nullptr, nullptr, 0, cur, cur_e, best, best_e, init, 1, bfgs_iters, temp, amp_pos, amp_rot, seed,
+             step, eval_count};
+    StepArgs P{Y, feat, log_buf, log_count, log_cap, use_net, p_plain, sig_t, sig_r, sig_th, init};
+    return hip_status(launch_mc_step(KM, L, G, S, R, chain_lig, n_chains, M, P, (hipStream_t)stream));
+}
+
+// ---- learned-proposal network ----
+int adock_mlp_infer(int B, const float* X, const unsigned short* W1, const float* b1, const unsigned short* W2,
+                    const float* b2, const unsigned short* W3, const float* b3, float* Y, adock_stream_t stream) {
+    if (B < 0) return fail(ADOCK_E_ARG, "negative batch");
+    if (B == 0) return 0;
+    if (!X || !W1 || !b1 || !W2 || !b2 || !W3 || !b3 || !Y) return fail(ADOCK_E_ARG, "null pointer");
+    return hip_status(launch_mlp_infer(B, X, W1, b1, W2, b2, W3, b3, Y, (hipStream_t)stream));
+}
+int adock_gemm_f32(int trans_a, int trans_b, int M, int N, int K, float alpha, const float* A, int lda, const float* B,
+                   int ldb, float beta, float* C, int ldc, adock_stream_t stream) {
+    if (M < 0 || N < 0 || K < 0) return fail(ADOCK_E_ARG, "negative size");
+    if ((long long)M * N == 0) return 0;
+    if (!A || !B || !C) return fail(ADOCK_E_ARG, "null pointer");
+    return hip_status(launch_gemm_f32(trans_a, trans_b, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
+                                      (hipStream_t)stream));
+}
+int adock_bias_act(int M, int N, float* Y, const float* b, int relu, adock_stream_t stream) {
+    if ((long long)M * N <= 0) return M < 0 || N < 0 ? fail(ADOCK_E_ARG, "negative size") : 0;
+    return hip_status(launch_bias_act(M, N, Y, b, relu, (hipStream_t)stream));
+}
+int adock_relu_bwd(long long n, float* dH, const float* H, adock_stream_t stream) {
+    if (n <= 0) return n < 0 ? fail(ADOCK_E_ARG, "negative size") : 0;
+    return hip_status(launch_relu_bwd(n, dH, H, (hipStream_t)stream));
+}
+int adock_colsum(int M, int N, const float* dY, float* db, adock_stream_t stream) {
+    if (M < 0 || N <= 0) return N < 0 || M < 0 ? fail(ADOCK_E_ARG, "negative size") : 0;
+    return hip_status(launch_colsum(M, N, dY, db, (hipStream_t)stream));
+}
+int adock_mse(long long n, const float* Y, const float* T, const float* mask, float inv_norm, float* dY, float* loss,
+              adock_stream_t stream) {
+    if (n <= 0) return n < 0 ? fail(ADOCK_E_ARG, "negative size") : 0;
+    return hip_status(launch_mse(n, Y, T, mask, inv_norm, dY, loss, (hipStream_t)stream));
+}
+int adock_adam(long long n, float* w, const float* g, float* m, float* v, float lr, float beta1, float beta2,
+               float eps, int t, adock_stream_t stream) {
+    if (n <= 0 || t < 1) return n < 0 || t < 1 ? fail(ADOCK_E_ARG, "negative size or step < 1") : 0;
+    return hip_status(launch_adam(n, w, g, m, v, lr, beta1, beta2, eps, t, (hipStream_t)stream));
+}
+int adock_to_bf16(long long n, const float* x, unsigned short* y, adock_stream_t stream) {
+    if (n <= 0) return n < 0 ? fail(ADOCK_E_ARG, "negative size") : 0;
+    return hip_status(launch_to_bf16(n, x, y, (hipStream_t)stream));
+}
+
 int adock_set_chains_per_block(int w) {
     if (set_waves_per_block(w)) return fail(ADOCK_E_ARG, "chains_per_block must be 1..4");
     return 0;

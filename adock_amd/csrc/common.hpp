@@ -104,6 +104,22 @@ DEVI float pair_e(float d, int pf, const SFConst& s, float& de) {
     return ep;
 }
 
+// Branch-free form of pair_e for SIMT loops: same values (the slope terms clamp v to [0, 1]; their derivative is
+// non-zero only strictly inside the ramp). pf: bit 0 hydrophobic pair, bit 1 h-bond pair.
+DEVI float pair_e_nb(float d, int pf, const SFConst& s, float& de) {
+    const float u1 = (d - s.g1o) * s.g1s, g1 = fast_exp2(-u1 * u1);
+    const float u2 = (d - s.g2o) * s.g2s, g2 = fast_exp2(-u2 * u2);
+    const float dr = fminf(d - s.repo, 0.f);
+    float ep = s.w1 * g1 + s.w2 * g2 + s.w3 * dr * dr;
+    float dd = s.g1d * u1 * g1 + s.g2d * u2 * g2 + 2.f * s.w3 * dr;
+    const float vh = (s.hb_bad - d) * s.hinv, vb = (s.bb_bad - d) * s.binv;
+    const float hh = (pf & 1) ? 1.f : 0.f, hb = (pf & 2) ? 1.f : 0.f;
+    ep += hh * s.w4 * fminf(fmaxf(vh, 0.f), 1.f) + hb * s.w5 * fminf(fmaxf(vb, 0.f), 1.f);
+    dd -= hh * ((vh > 0.f && vh < 1.f) ? s.w4 * s.hinv : 0.f) + hb * ((vb > 0.f && vb < 1.f) ? s.w5 * s.binv : 0.f);
+    de = dd;
+    return ep;
+}
+
 // SplitMix64 finaliser and the per-(chain, step, slot) uniform draw in [0, 1) with 24 bits.
 DEVI uint64_t smix(uint64_t x) {
     x += 0x9E3779B97F4A7C15ULL;

@@ -45,8 +45,9 @@ struct ChainCtx {
     unsigned tmask;
     bool atom;
     const float* gp;
-    // lane = torsion
+    // lane = torsion: axis atoms and the reference axis (point, unit vector)
     int axa, axb;
+    float apx, apy, apz, aux, auy, auz;
     // uniform
     const float* axp; const float* axu;
     int na, nt, np;
@@ -103,13 +104,21 @@ DEVI ChainCtx setup_chain(int l, int lane, const LigArgs& L, const GridArgs& G, 
             c.tmask |= (L.tm[((long long)l * KM + k) * AMK + lane] ? 1u : 0u) << k;
     }
     c.axa = c.axb = 0;
+    c.apx = c.apy = c.apz = c.aux = c.auy = c.auz = 0.f;
     if (lane < c.nt) {
         c.axa = L.ax[((long long)l * KM + lane) * 2];
         c.axb = L.ax[((long long)l * KM + lane) * 2 + 1];
+        c.apx = c.axp[3 * lane]; c.apy = c.axp[3 * lane + 1]; c.apz = c.axp[3 * lane + 2];
+        c.aux = c.axu[3 * lane]; c.auy = c.axu[3 * lane + 1]; c.auz = c.axu[3 * lane + 2];
     }
     w.flg[lane] = c.flg;
+    wave_lds_sync();
+    // pair word: i | j << 6 | (hydrophobic pair) << 12 | (h-bond pair) << 13; 0x8000 marks a padding self-pair
     const int* pr = L.pr + (long long)l * L.P_MAX * 2;
-    for (int q = lane; q < c.np; q += WAVE) w.pl[q] = (unsigned short)(pr[2 * q] | (pr[2 * q + 1] << 8));
+    for (int q = lane; q < c.np; q += WAVE) {
+        const int i = pr[2 * q], j = pr[2 * q + 1];
+        w.pl[q] = (unsigned short)(i == j ? 0x8000 : (i | (j << 6) | (pair_flags(w.flg[i], w.flg[j]) << 12)));
+    }
     c.exact = exact;
     c.rrx = R.rxyz + (long long)ri * R.RN_MAX * 3;
     c.rrt = R.rtyp + (long long)ri * R.RN_MAX;
@@ -121,7 +130,8 @@ DEVI ChainCtx setup_chain(int l, int lane, const LigArgs& L, const GridArgs& G, 
 }
 
 // Energy and lane-distributed gradient of the lane-distributed state x. Returns E (uniform).
-DEVI float eval_eg(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, int lane, float x, float& gout) {
+DEVI float eval_eg(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, int lane, float x, float& gout, float& ea,
+                   float& Xx, float& Xy, float& Xz) {
     const float t0 = bcast(x, 0), t1 = bcast(x, 1), t2 = bcast(x, 2);
     float qw = bcast(x, 3), qx = bcast(x, 4), qy = bcast(x, 5), qz = bcast(x, 6);
     const float nq = fast_rsq(qw * qw + qx * qx + qy * qy + qz * qz);
@@ -135,8 +145,8 @@ DEVI float eval_eg(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, int l
     float px = c.rx, py = c.ry, pz = c.rz;
     for (int k = 0; k < c.nt; ++k) {
         const float sk = bcast(sn, 7 + k), ck = bcast(cs, 7 + k);
-        const float ax0 = c.axp[3 * k], ax1 = c.axp[3 * k + 1], ax2 = c.axp[3 * k + 2];
-        const float ux = c.axu[3 * k], uy = c.axu[3 * k + 1], uz = c.axu[3 * k + 2];
+        const float ax0 = bcast(c.apx, k), ax1 = bcast(c.apy, k), ax2 = bcast(c.apz, k);
+        const float ux = bcast(c.aux, k), uy = bcast(c.auy, k), uz = bcast(c.auz, k);
         if ((c.tmask >> k) & 1u) {
             const float vx = px - ax0, vy = py - ax1, vz = pz - ax2;
             const float d = (ux * vx + uy * vy + uz * vz) * (1.f - ck);
@@ -149,22 +159,49 @@ DEVI float eval_eg(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, int l
     const float X0 = R00 * px + R01 * py + R02 * pz + t0;
     const float X1 = R10 * px + R11 * py + R12 * pz + t1;
     const float X2 = R20 * px + R21 * py + R22 * pz + t2;
+    // grid corners: issue the 8 gathers now, consume them after the pair loop (latency overlaps the LDS work)
     float el = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
+    float v000 = 0.f, v001 = 0.f, v010 = 0.f, v011 = 0.f, v100 = 0.f, v101 = 0.f, v110 = 0.f, v111 = 0.f;
+    float wx = 0.f, wy = 0.f, wz = 0.f, in0 = 0.f, in1 = 0.f, in2 = 0.f;
+    const bool grid_atom = c.atom && !c.exact;
+    if (grid_atom) {
+        const float f0 = (X0 - c.o0) * c.inv_s, f1 = (X1 - c.o1) * c.inv_s, f2 = (X2 - c.o2) * c.inv_s;
+        const float fc0 = fminf(fmaxf(f0, 0.f), c.fmx0), fc1 = fminf(fmaxf(f1, 0.f), c.fmx1),
+                    fc2 = fminf(fmaxf(f2, 0.f), c.fmx2);
+        in0 = (f0 > 0.f && f0 < c.fmx0) ? c.inv_s : 0.f;
+        in1 = (f1 > 0.f && f1 < c.fmx1) ? c.inv_s : 0.f;
+        in2 = (f2 > 0.f && f2 < c.fmx2) ? c.inv_s : 0.f;
+        const int i0 = (int)floorf(fc0), i1 = (int)floorf(fc1), i2 = (int)floorf(fc2);
+        wx = fc0 - (float)i0; wy = fc1 - (float)i1; wz = fc2 - (float)i2;
+        const int b000 = (i0 * c.ny + i1) * c.nz + i2;
+        const int bx = b000 + c.ny * c.nz;
+        const float* G = c.gp;
+        v000 = G[b000]; v001 = G[b000 + 1]; v010 = G[b000 + c.nz]; v011 = G[b000 + c.nz + 1];
+        v100 = G[bx]; v101 = G[bx + 1]; v110 = G[bx + c.nz]; v111 = G[bx + c.nz + 1];
+    }
+    w.X4[lane] = make_float4(X0, X1, X2, c.rad);
+    w.Fx[lane] = 0.f; w.Fy[lane] = 0.f; w.Fz[lane] = 0.f;
+    wave_lds_sync();
+    // intra-ligand pairs, strided over lanes (a scheduled list has distinct first and second atoms per 64 pairs, so
+    // the force accumulations of one instruction never collide); branch-free pair terms
+    float ep_l = 0.f;
+    for (int q = lane; q < c.np; q += WAVE) {
+        const unsigned pw = w.pl[q];
+        if (pw & 0x8000u) continue;
+        const int i = pw & 63, j = (pw >> 6) & 63;
+        const float4 xi = w.X4[i], xj = w.X4[j];
+        const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+        const float r = fast_sqrt(rx * rx + ry * ry + rz * rz);
+        float de;
+        const float e = pair_e_nb(r - xi.w - xj.w, (int)(pw >> 12), sf, de);
+        const float in = r < sf.cut ? 1.f : 0.f;
+        ep_l += in * e;
+        const float sc = in * de * fast_rcp(fmaxf(r, 1e-6f));
+        atomicAdd(&w.Fx[i], sc * rx); atomicAdd(&w.Fy[i], sc * ry); atomicAdd(&w.Fz[i], sc * rz);
+        atomicAdd(&w.Fx[j], -sc * rx); atomicAdd(&w.Fy[j], -sc * ry); atomicAdd(&w.Fz[j], -sc * rz);
+    }
     if (c.atom) {
         if (!c.exact) {
-            const float f0 = (X0 - c.o0) * c.inv_s, f1 = (X1 - c.o1) * c.inv_s, f2 = (X2 - c.o2) * c.inv_s;
-            const float fc0 = fminf(fmaxf(f0, 0.f), c.fmx0), fc1 = fminf(fmaxf(f1, 0.f), c.fmx1),
-                        fc2 = fminf(fmaxf(f2, 0.f), c.fmx2);
-            const float in0 = (f0 > 0.f && f0 < c.fmx0) ? c.inv_s : 0.f;
-            const float in1 = (f1 > 0.f && f1 < c.fmx1) ? c.inv_s : 0.f;
-            const float in2 = (f2 > 0.f && f2 < c.fmx2) ? c.inv_s : 0.f;
-            const int i0 = (int)floorf(fc0), i1 = (int)floorf(fc1), i2 = (int)floorf(fc2);
-            const float wx = fc0 - (float)i0, wy = fc1 - (float)i1, wz = fc2 - (float)i2;
-            const int b000 = (i0 * c.ny + i1) * c.nz + i2;
-            const int bx = b000 + c.ny * c.nz;
-            const float* G = c.gp;
-            const float v000 = G[b000], v001 = G[b000 + 1], v010 = G[b000 + c.nz], v011 = G[b000 + c.nz + 1];
-            const float v100 = G[bx], v101 = G[bx + 1], v110 = G[bx + c.nz], v111 = G[bx + c.nz + 1];
             const float c00 = v000 + wz * (v001 - v000), c01 = v010 + wz * (v011 - v010);
             const float c10 = v100 + wz * (v101 - v100), c11 = v110 + wz * (v111 - v110);
             const float c0 = c00 + wy * (c01 - c00), c1 = c10 + wy * (c11 - c10);
@@ -187,8 +224,8 @@ DEVI float eval_eg(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, int l
                 const int fb = c.sfhyd[tb] | (c.sfdon[tb] << 1) | (c.sfacc[tb] << 2);
                 float de;
                 el += pair_e(r - c.rad - c.sfrad[tb], pair_flags(c.flg, fb), sf, de);
-                const float s = de * fast_rcp(fmaxf(r, 1e-6f));
-                gx += s * rx; gy += s * ry; gz += s * rz;
+                const float sc = de * fast_rcp(fmaxf(r, 1e-6f));
+                gx += sc * rx; gy += sc * ry; gz += sc * rz;
             }
         }
         const float l0 = fmaxf(c.lo0 - X0, 0.f), h0 = fmaxf(X0 - c.hi0, 0.f);
@@ -197,24 +234,9 @@ DEVI float eval_eg(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, int l
         el += c.pen * (l0 * l0 + h0 * h0 + l1 * l1 + h1 * h1 + l2 * l2 + h2 * h2);
         gx += 2.f * c.pen * (h0 - l0); gy += 2.f * c.pen * (h1 - l1); gz += 2.f * c.pen * (h2 - l2);
     }
-    w.X4[lane] = make_float4(X0, X1, X2, c.rad);
-    w.Fx[lane] = 0.f; w.Fy[lane] = 0.f; w.Fz[lane] = 0.f;
-    wave_lds_sync();
-    // intra-ligand pairs, strided over lanes; pair forces into LDS
-    for (int q = lane; q < c.np; q += WAVE) {
-        const unsigned pw = w.pl[q];
-        const int i = pw & 0xff, j = pw >> 8;
-        const float4 xi = w.X4[i], xj = w.X4[j];
-        const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-        const float r = fast_sqrt(rx * rx + ry * ry + rz * rz);
-        if (r < sf.cut) {
-            float de;
-            el += pair_e(r - xi.w - xj.w, pair_flags(w.flg[i], w.flg[j]), sf, de);
-            const float s = de * fast_rcp(fmaxf(r, 1e-6f));
-            atomicAdd(&w.Fx[i], s * rx); atomicAdd(&w.Fy[i], s * ry); atomicAdd(&w.Fz[i], s * rz);
-            atomicAdd(&w.Fx[j], -s * rx); atomicAdd(&w.Fy[j], -s * ry); atomicAdd(&w.Fz[j], -s * rz);
-        }
-    }
+    ea = el;            // this atom's grid (or exact inter) energy + out-of-box penalty
+    Xx = X0; Xy = X1; Xz = X2;
+    el += ep_l;
     wave_lds_sync();
     const float fx = gx + w.Fx[lane], fy = gy + w.Fy[lane], fz = gz + w.Fz[lane];
     // current torsion axes (lane k < nt) from the world coordinates of the axis atoms
@@ -254,6 +276,31 @@ DEVI float conf_inc(float x, float p, float alpha, int lane) {
     return lane == 3 ? qw : lane == 4 ? qx : lane == 5 ? qy : lane == 6 ? qz : lin;
 }
 
+// The reference's MC mutation (one entity: translation by amp_pos * ball, rotation by amp_rot / max(rg, 1) * ball, or
+// one torsion set uniformly in [-pi, pi)), drawn from the counter RNG slots 0, 1 and 10.. of (chain, step).
+DEVI float plain_move(float x, const ChainCtx& c, int lane, unsigned long long seed, int cid, int sg, float amp_pos,
+                      float amp_rot) {
+    const int nm = 2 + c.nt;
+    const int which = min((int)(urand(seed, cid, sg, 0) * (float)nm), nm - 1);
+    if (which == 0) {
+        float v0, v1, v2;
+        rand_ball(seed, cid, sg, 10, v0, v1, v2);
+        const float v = lane == 0 ? v0 : lane == 1 ? v1 : v2;
+        if (lane < 3) x += amp_pos * v;
+    } else if (which == 1) {
+        float v0, v1, v2;
+        rand_ball(seed, cid, sg, 10, v0, v1, v2);
+        const float sc = amp_rot / fmaxf(c.rg, 1.0f);
+        float qw = bcast(x, 3), qx = bcast(x, 4), qy = bcast(x, 5), qz = bcast(x, 6);
+        quat_inc(qw, qx, qy, qz, sc * v0, sc * v1, sc * v2);
+        x = lane == 3 ? qw : lane == 4 ? qx : lane == 5 ? qy : lane == 6 ? qz : x;
+    } else {
+        const float th = (2.f * urand(seed, cid, sg, 1) - 1.f) * 3.14159265f;
+        if (lane == 7 + which - 2) x = th;
+    }
+    return x;
+}
+
 // Vina-style BFGS on the lane-distributed state x (in/out); returns the final energy. One evaluation site: the first
 // pass evaluates x itself, every later pass evaluates one line-search trial.
 template <int KM>
@@ -267,8 +314,8 @@ DEVI float bfgs(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, int lane
     int step = 0, trial = -1;
     while (true) {
         const float xe = trial < 0 ? x : conf_inc(x, p, alpha, lane);
-        float ge;
-        const float fe = eval_eg(c, sf, w, lane, xe, ge);
+        float ge, ea_, x0_, x1_, x2_;
+        const float fe = eval_eg(c, sf, w, lane, xe, ge, ea_, x0_, x1_, x2_);
         ++nev;
         if (trial < 0) {
             f0 = fe; f_orig = fe; g = ge;
@@ -346,24 +393,7 @@ DEVI int run_chain(int ch, int lane, const WaveLDS& w, const LigArgs& L, const G
         int sg = 0;
         if (st >= 0) {
             sg = M.step_offset + st;
-            const int nm = 2 + c.nt;
-            const int which = min((int)(urand(seed, cid, sg, 0) * (float)nm), nm - 1);
-            if (which == 0) {
-                float v0, v1, v2;
-                rand_ball(seed, cid, sg, 10, v0, v1, v2);
-                const float v = lane == 0 ? v0 : lane == 1 ? v1 : v2;
-                if (lane < 3) x += M.amp_pos * v;
-            } else if (which == 1) {
-                float v0, v1, v2;
-                rand_ball(seed, cid, sg, 10, v0, v1, v2);
-                const float sc = M.amp_rot / fmaxf(c.rg, 1.0f);
-                float qw = bcast(x, 3), qx = bcast(x, 4), qy = bcast(x, 5), qz = bcast(x, 6);
-                quat_inc(qw, qx, qy, qz, sc * v0, sc * v1, sc * v2);
-                x = lane == 3 ? qw : lane == 4 ? qx : lane == 5 ? qy : lane == 6 ? qz : x;
-            } else {
-                const float th = (2.f * urand(seed, cid, sg, 1) - 1.f) * 3.14159265f;
-                if (lane == 7 + which - 2) x = th;
-            }
+            x = plain_move(x, c, lane, seed, cid, sg, M.amp_pos, M.amp_rot);
         }
         const float en = bfgs<KM>(c, sf, w, lane, x, iters, nev);
         if (st < 0) {
@@ -438,8 +468,8 @@ __global__ __launch_bounds__(256) void eg_kernel(LigArgs L, GridArgs G, SFArgs S
     const SFConst sf = make_sf(S.prm);
     const ChainCtx c = setup_chain<KM>(__builtin_amdgcn_readfirstlane(chain_lig[ch]), lane, L, G, S, R, exact, w);
     const float x = lane < SD ? state[(long long)ch * SD + lane] : 0.f;
-    float g;
-    const float e = eval_eg(c, sf, w, lane, x, g);
+    float g, ea_, x0_, x1_, x2_;
+    const float e = eval_eg(c, sf, w, lane, x, g, ea_, x0_, x1_, x2_);
     if (lane < 6) g_out[(long long)ch * DM + lane] = g;
     else if (lane >= 7 && lane < 7 + c.nt) g_out[(long long)ch * DM + lane - 1] = g;
     if (lane == 0) e_out[ch] = e;
@@ -471,14 +501,14 @@ int launch_mc_persistent_t(const LigArgs& L, const GridArgs& G, const SFArgs& S,
         static std::mutex mu;
         static std::map<long long, int> cap;
         int dev = 0;
-        hipGetDevice(&dev);
+        (void)hipGetDevice(&dev);
         const long long key = ((long long)dev << 40) | ((long long)W << 32) | (long long)shm;
         std::lock_guard<std::mutex> lock(mu);
         auto it = cap.find(key);
         if (it == cap.end()) {
             int cus = 0, per_cu = 0;
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, mc_persistent_kernel<KM>, W * WAVE, shm);
+            (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, mc_persistent_kernel<KM>, W * WAVE, shm);
             it = cap.emplace(key, cus * (per_cu > 0 ? per_cu : 1)).first;
         }
         blocks = it->second;
@@ -499,6 +529,234 @@ int launch_eg_t(const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecA
     const unsigned nblk = (unsigned)((n_chains + W - 1) / W);
     hipLaunchKernelGGL(eg_kernel<KM>, dim3(nblk), dim3(W * WAVE), shm, stream, L, G, S, R, exact, chain_lig,
                        n_chains, state, e_out, g_out);
+    return (int)hipGetLastError();
+}
+
+
+constexpr int NFEAT = 128, NOUT = 23, LOG_W = NFEAT + 2 * NOUT + 1;
+
+template <int CTRL, int ROW_MASK = 0xf>
+DEVI float dpp_max_step(float v) {
+    const float o = __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(-INFINITY), __float_as_int(v), CTRL,
+                                                               ROW_MASK, 0xf, false));
+    return fmaxf(v, o);
+}
+// maximum over the 64 lanes (all active), uniform result
+DEVI float wave_max(float v) {
+    v = dpp_max_step<0x111>(v);
+    v = dpp_max_step<0x112>(v);
+    v = dpp_max_step<0x114>(v);
+    v = dpp_max_step<0x118>(v);
+    v = dpp_max_step<0x142, 0xa>(v);
+    v = dpp_max_step<0x143, 0xc>(v);
+    return bcast(v, 63);
+}
+DEVI int wave_count(bool p) { return __popcll(__ballot(p)); }
+
+// standard normal from two counter-RNG draws (Box-Muller), per lane
+DEVI float rnorm(unsigned long long seed, int cid, int sg, int slot) {
+    const float u1 = fmaxf(urand(seed, cid, sg, slot), 1e-7f), u2 = urand(seed, cid, sg, slot + 1);
+    return sqrtf(-2.f * logf(u1)) * cosf(6.28318531f * u2);
+}
+
+// The 128 proposal-network features of the pose x (evaluated: energy E, lane-distributed gradient g, per-atom
+// energy ea): box-normalised translation, canonical quaternion, sin/cos of the torsions, squashed gradient, energy
+// summaries, a 16-bin histogram of atom energies, per-octant (ligand frame) mean / max atom energy, ligand size
+// descriptors and the mean energy of the atoms each torsion moves. Written as f[lane], f[64 + lane].
+DEVI void write_features(float* f, const ChainCtx& c, int lane, float x, float E, float g, float ea) {
+    float lo_v = 0.f, hi_v = 0.f;
+    const float cen[3] = {0.5f * (c.lo0 + c.hi0), 0.5f * (c.lo1 + c.hi1), 0.5f * (c.lo2 + c.hi2)};
+    const float hal[3] = {fmaxf(0.5f * (c.hi0 - c.lo0), 1e-3f), fmaxf(0.5f * (c.hi1 - c.lo1), 1e-3f),
+                          fmaxf(0.5f * (c.hi2 - c.lo2), 1e-3f)};
+    if (lane < 3) lo_v = (x - cen[lane == 0 ? 0 : lane == 1 ? 1 : 2]) / hal[lane == 0 ? 0 : lane == 1 ? 1 : 2];
+    float qw = bcast(x, 3), qx = bcast(x, 4), qy = bcast(x, 5), qz = bcast(x, 6);
+    const float qn = fast_rsq(qw * qw + qx * qx + qy * qy + qz * qz) * (qw < 0.f ? -1.f : 1.f);
+    if (lane >= 3 && lane < 7) lo_v = x * qn;
+    const float xc = __shfl(x, lane >= 23 && lane < 39 ? lane - 16 : 0);
+    if (lane >= 7 && lane < 23 && lane - 7 < c.nt) lo_v = sinf(x);
+    if (lane >= 23 && lane < 39 && lane - 23 < c.nt) lo_v = cosf(xc);
+    const int gsrc = lane >= 39 && lane < 45 ? lane - 39 : lane >= 45 && lane < 61 ? lane - 38 : 0;
+    const float gv = __shfl(g, gsrc);
+    if (lane >= 39 && lane < 61) lo_v = tanhf(0.1f * gv);
+    const float esum = wave_sum(c.atom ? ea : 0.f);
+    const float inv_na = 1.f / (float)max(c.na, 1);
+    const float X0 = 0.f;   // placeholder to keep lane-uniform code shape
+    (void)X0;
+    const int nout = wave_count(c.atom && ea > 1.f);
+    if (lane == 61) lo_v = tanhf(0.1f * E);
+    if (lane == 62) lo_v = tanhf(0.1f * esum);
+    if (lane == 63) lo_v = (float)nout * inv_na;
+    // 16-bin histogram of atom energies
+    const float edges[15] = {-1.2f, -0.9f, -0.7f, -0.5f, -0.35f, -0.2f, -0.1f, 0.f, 0.1f, 0.25f, 0.5f, 1.f, 2.f, 5.f, 20.f};
+    int bin = 0;
+#pragma unroll
+    for (int b = 0; b < 15; ++b) bin += ea > edges[b] ? 1 : 0;
+#pragma unroll
+    for (int b = 0; b < 16; ++b) {
+        const int cnt = wave_count(c.atom && bin == b);
+        if (lane == b) hi_v = (float)cnt * inv_na;
+    }
+    // ligand-frame octants of the reference coordinates
+    const int oct = (c.rx > 0.f ? 1 : 0) | (c.ry > 0.f ? 2 : 0) | (c.rz > 0.f ? 4 : 0);
+#pragma unroll
+    for (int o = 0; o < 8; ++o) {
+        const bool in = c.atom && oct == o;
+        const int cnt = wave_count(in);
+        const float sm = wave_sum(in ? ea : 0.f);
+        const float mx = wave_max(in ? ea : -INFINITY);
+        if (lane == 16 + o) hi_v = cnt ? tanhf(sm / (float)cnt) : 0.f;
+        if (lane == 24 + o) hi_v = cnt ? tanhf(mx) : 0.f;
+    }
+    if (lane == 32) hi_v = (float)c.na / 64.f;
+    if (lane == 33) hi_v = (float)c.nt / 16.f;
+    if (lane == 34) hi_v = c.rg / 5.f;
+    if (lane == 35) hi_v = (float)c.np / 1024.f;
+    for (int k = 0; k < c.nt && k < 16; ++k) {
+        const bool in = c.atom && ((c.tmask >> k) & 1u);
+        const int cnt = wave_count(in);
+        const float sm = wave_sum(in ? ea : 0.f);
+        if (lane == 48 + k) hi_v = cnt ? tanhf(sm / (float)cnt) : 0.f;
+    }
+    f[lane] = lo_v;
+    f[64 + lane] = hi_v;
+}
+
+// Move proposed by the network output y (lane-distributed like the state): translation + y_t + sig_t N(0,1),
+// rotation normalise(y_q) * q followed by a random rotation vector sig_r N(0, I), torsions + y_theta + sig_th N(0,1).
+DEVI float net_move(float x, float y, const ChainCtx& c, int lane, unsigned long long seed, int cid, int sg,
+                    float sig_t, float sig_r, float sig_th) {
+    const float n = rnorm(seed, cid, sg, 300 + 2 * lane);
+    float out = x;
+    if (lane < 3) out = x + y + sig_t * n;
+    if (lane >= 7 && lane < 7 + c.nt) out = x + y + sig_th * n;
+    float aw = bcast(y, 3), ax = bcast(y, 4), ay = bcast(y, 5), az = bcast(y, 6);
+    const float an = aw * aw + ax * ax + ay * ay + az * az;
+    if (an > 1e-6f) {
+        const float r = fast_rsq(an);
+        aw *= r; ax *= r; ay *= r; az *= r;
+    } else { aw = 1.f; ax = ay = az = 0.f; }
+    const float bw = bcast(x, 3), bx = bcast(x, 4), by = bcast(x, 5), bz = bcast(x, 6);
+    float qw = aw * bw - ax * bx - ay * by - az * bz;
+    float qx = aw * bx + ax * bw + ay * bz - az * by;
+    float qy = aw * by - ax * bz + ay * bw + az * bx;
+    float qz = aw * bz + ax * by - ay * bx + az * bw;
+    quat_inc(qw, qx, qy, qz, sig_r * bcast(n, 3), sig_r * bcast(n, 4), sig_r * bcast(n, 5));
+    out = lane == 3 ? qw : lane == 4 ? qx : lane == 5 ? qy : lane == 6 ? qz : out;
+    return out;
+}
+
+// target of an accepted move cur -> x: translation delta, rotation dq = q_x conj(q_cur) (w >= 0), wrapped torsion
+// deltas; mask 1 on the components that exist
+DEVI void move_target(float cur, float x, const ChainCtx& c, int lane, float& t, float& m) {
+    float aw = bcast(x, 3), ax = bcast(x, 4), ay = bcast(x, 5), az = bcast(x, 6);
+    float bw = bcast(cur, 3), bx = -bcast(cur, 4), by = -bcast(cur, 5), bz = -bcast(cur, 6);
+    const float na = fast_rsq(aw * aw + ax * ax + ay * ay + az * az), nb = fast_rsq(bw * bw + bx * bx + by * by + bz * bz);
+    aw *= na; ax *= na; ay *= na; az *= na; bw *= nb; bx *= nb; by *= nb; bz *= nb;
+    float qw = aw * bw - ax * bx - ay * by - az * bz;
+    float qx = aw * bx + ax * bw + ay * bz - az * by;
+    float qy = aw * by - ax * bz + ay * bw + az * bx;
+    float qz = aw * bz + ax * by - ay * bx + az * bw;
+    const float sg = qw < 0.f ? -1.f : 1.f;
+    t = 0.f; m = 0.f;
+    if (lane < 3) { t = x - cur; m = 1.f; }
+    if (lane == 3) { t = qw * sg; m = 1.f; }
+    if (lane == 4) { t = qx * sg; m = 1.f; }
+    if (lane == 5) { t = qy * sg; m = 1.f; }
+    if (lane == 6) { t = qz * sg; m = 1.f; }
+    if (lane >= 7 && lane < 7 + c.nt) {
+        float d = x - cur;
+        d = d - 6.28318531f * rintf(d * 0.159154943f);
+        t = d; m = 1.f;
+    }
+}
+
+// One MC step of every chain (launched once per step): optional features at the current pose (feat), a move from the
+// network (Y) or the reference's random mutation (fallback with probability p_plain, or when Y is not finite), BFGS,
+// Metropolis, logging of accepted moves (features before the move, target, mask, energy gain) and the features of the
+// new current pose. init = 1: BFGS of the start pose only (the reference's init_opt). With use_net = 0 and init_opt
+// followed by steps 0..S-1 this reproduces the fused plain kernel bit for bit.
+template <int KM>
+__global__ __launch_bounds__(256) void mc_step_kernel(LigArgs L, GridArgs G, SFArgs S, RecArgs R,
+                                                       const int* __restrict__ chain_lig, int n_chains, McArgs M,
+                                                       StepArgs P) {
+    extern __shared__ __attribute__((aligned(16))) char lds[];
+    constexpr int SD = 7 + KM;
+    const int lane = lane_id();
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+    const int ch = blockIdx.x * (blockDim.x / WAVE) + wid;
+    if (ch >= n_chains) return;
+    const WaveLDS w = carve_lds(lds + wid * wave_lds_bytes(L.P_MAX), L.P_MAX);
+    const SFConst sf = make_sf(S.prm);
+    const int l = __builtin_amdgcn_readfirstlane(chain_lig[ch]);
+    const ChainCtx c = setup_chain<KM>(l, lane, L, G, S, R, 0, w);
+    const int cid = __builtin_amdgcn_readfirstlane(M.chain_id[ch]);
+    const int S_ = 7 + c.nt;
+    const int iters = M.bfgs_iters > 0 ? M.bfgs_iters : (25 + c.na) / 3;
+    float* gcur = M.cur + (long long)ch * SD;
+    float* gbest = M.best + (long long)ch * SD;
+    float cur = lane < S_ ? gcur[lane] : 0.f;
+    float best = lane < S_ ? gbest[lane] : 0.f;
+    float eb = M.best_e[ch], ec = M.cur_e[ch];
+    int nev = 0;
+    const unsigned long long seed = M.seed;
+    const int sg = M.step_offset;
+    if (P.init) {
+        float x = cur;
+        ec = bfgs<KM>(c, sf, w, lane, x, iters, nev);
+        cur = x;
+        if (ec < eb) { eb = ec; best = cur; }
+    } else {
+        bool plain = !P.use_net || !P.Y || urand(seed, cid, sg, 40) < P.p_plain;
+        float y = 0.f;
+        if (!plain) {
+            y = lane < NOUT ? P.Y[(long long)ch * NOUT + lane] : 0.f;
+            plain = wave_count(!isfinite(y)) > 0;
+        }
+        float x = plain ? plain_move(cur, c, lane, seed, cid, sg, M.amp_pos, M.amp_rot)
+                        : net_move(cur, y, c, lane, seed, cid, sg, P.sig_t, P.sig_r, P.sig_th);
+        const float en = bfgs<KM>(c, sf, w, lane, x, iters, nev);
+        const bool accept = (en < ec) || (urand(seed, cid, sg, 2) < expf((ec - en) / M.temp));
+        if (accept) {
+            if (P.log_buf && P.feat) {
+                int slot = 0;
+                if (lane == 0) slot = atomicAdd(P.log_count, 1);
+                slot = bcast_i(slot, 0);
+                if (slot < P.log_cap) {
+                    float* rec = P.log_buf + (long long)slot * LOG_W;
+                    const float* fin = P.feat + (long long)ch * NFEAT;
+                    rec[lane] = fin[lane];
+                    rec[64 + lane] = fin[64 + lane];
+                    float t, m;
+                    move_target(cur, x, c, lane, t, m);
+                    if (lane < NOUT) { rec[NFEAT + lane] = t; rec[NFEAT + NOUT + lane] = m; }
+                    if (lane == 0) rec[NFEAT + 2 * NOUT] = ec - en;
+                }
+            }
+            ec = en; cur = x;
+            if (en < eb) { eb = en; best = x; }
+        }
+    }
+    if (P.feat) {
+        float g, ea, a0, a1, a2;
+        const float e = eval_eg(c, sf, w, lane, cur, g, ea, a0, a1, a2);
+        ++nev;
+        write_features(P.feat + (long long)ch * NFEAT, c, lane, cur, e, g, ea);
+    }
+    if (lane < S_) { gcur[lane] = cur; gbest[lane] = best; }
+    if (lane == 0) {
+        M.cur_e[ch] = ec; M.best_e[ch] = eb;
+        if (M.eval_count) atomicAdd(M.eval_count, (unsigned long long)nev);
+    }
+}
+
+template <int KM>
+int launch_mc_step_t(const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecArgs& R, const int* chain_lig,
+                     int n_chains, const McArgs& M, const StepArgs& P, hipStream_t stream) {
+    const int W = g_waves_per_block;
+    const size_t shm = (size_t)W * lds_bytes_per_wave(L.P_MAX);
+    const unsigned nblk = (unsigned)((n_chains + W - 1) / W);
+    hipLaunchKernelGGL(mc_step_kernel<KM>, dim3(nblk), dim3(W * WAVE), shm, stream, L, G, S, R, chain_lig, n_chains,
+                       M, P);
     return (int)hipGetLastError();
 }
 
@@ -549,6 +807,17 @@ int launch_batch_reset(int* counter, int* done, int n_done, unsigned long long* 
     hipLaunchKernelGGL(batch_reset_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, counter, done, n_done, times,
                        n_times);
     return (int)hipGetLastError();
+}
+
+int launch_mc_step(int KM, const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecArgs& R,
+                   const int* chain_lig, int n_chains, const McArgs& M, const StepArgs& P, hipStream_t stream) {
+    switch (KM) {
+        case 8: return launch_mc_step_t<8>(L, G, S, R, chain_lig, n_chains, M, P, stream);
+        case 16: return launch_mc_step_t<16>(L, G, S, R, chain_lig, n_chains, M, P, stream);
+        case 24: return launch_mc_step_t<24>(L, G, S, R, chain_lig, n_chains, M, P, stream);
+        case 32: return launch_mc_step_t<32>(L, G, S, R, chain_lig, n_chains, M, P, stream);
+        default: return -1;
+    }
 }
 
 int launch_eg(int KM, const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecArgs& R, int exact,

@@ -25,6 +25,12 @@ struct McArgs {
 struct QueueArgs {
     const int* order; int* counter; int* done; const int* lig_chains; unsigned long long* times;
 };
+// learned-proposal MC step: Y [N,23] network output (NULL: plain moves), feat [N,128] features (in: at the current
+// pose; out: at the new current pose; NULL: not computed), log_buf [cap, 128+23+23+1] accepted-move records.
+struct StepArgs {
+    const float* Y; float* feat; float* log_buf; int* log_count; int log_cap;
+    int use_net; float p_plain, sig_t, sig_r, sig_th; int init;
+};
 int launch_build_grid(const float*, const int*, int, const float*, const int*, const int*, const int*, const int*,
                       int, float, float, float, float, int, int, int, const float*, int, float*, hipStream_t);
 int launch_mc(int KM, const LigArgs&, const GridArgs&, const SFArgs&, const RecArgs&, int, const int*, int,
@@ -46,6 +52,8 @@ int launch_mse(long long n, const float* Y, const float* T, const float* Mk, flo
 int launch_adam(long long n, float* w, const float* g, float* m, float* v, float lr, float b1, float b2, float eps,
                 int t, hipStream_t s);
 int launch_to_bf16(long long n, const float* x, unsigned short* y, hipStream_t s);
+int launch_mc_step(int KM, const LigArgs&, const GridArgs&, const SFArgs&, const RecArgs&, const int*, int,
+                   const McArgs&, const StepArgs&, hipStream_t);
 int set_waves_per_block(int w);
 int get_waves_per_block();
 int lds_budget_ok(int P_MAX);

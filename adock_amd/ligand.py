@@ -136,20 +136,61 @@ class LigandPrep:
         self.pairs = np.array(pairs, dtype=np.int64).reshape(-1, 2)
 
 
+def schedule_pairs(pairs, width=64):
+    """Orders (and orients) intra pairs into rounds of `width` in which every first atom and every second atom is
+    distinct, padding each round with (0, 0) self-pairs that the kernels skip. A kernel that strides pairs over 64
+    lanes then never has two lanes of one instruction accumulate into the same atom. Greedy, highest-degree pairs
+    first; each pair tries both orientations in the earliest round that accepts it. Energy and forces do not depend on
+    the order or orientation of the pair list."""
+    pairs = np.asarray(pairs, np.int64).reshape(-1, 2)
+    if len(pairs) == 0:
+        return pairs
+    n = int(pairs.max()) + 1
+    deg = np.bincount(pairs.reshape(-1), minlength=n)
+    order = np.argsort(-(deg[pairs[:, 0]] + deg[pairs[:, 1]]), kind="stable")
+    rounds = []
+    for q in order:
+        a, b = int(pairs[q, 0]), int(pairs[q, 1])
+        for first, second, lst in rounds:
+            if len(lst) >= width:
+                continue
+            if a not in first and b not in second:
+                first.add(a); second.add(b); lst.append((a, b))
+                break
+            if b not in first and a not in second:
+                first.add(b); second.add(a); lst.append((b, a))
+                break
+        else:
+            rounds.append(({a}, {b}, [(a, b)]))
+    out = np.zeros((len(rounds) * width, 2), np.int64)
+    for r, (_, _, lst) in enumerate(rounds):
+        out[r * width:r * width + len(lst)] = lst
+    return out
+
+
 def bucket(x, step):
     return max(step, step * math.ceil(x / step))
 
 
-def pack_ligands(preps, slot, rec_idx):
+def pack_ligands(preps, slot, rec_idx, schedule=True):
     """Packed ligand arrays in the search-kernel layout (src/adock/mckernel.py:547-592).
 
     slot [R, n_types] int (XS type -> grid slot of receptor r, -1 = not built), rec_idx [L].
-    Returns (AM, KM, dict of numpy arrays: ref, typ, tm, ax, axp, axu, pr, na, nt, np, rg, P, gslot)."""
+    schedule: pair list in conflict-free rounds of 64 (schedule_pairs; np counts the padding self-pairs) instead of
+    the reference's sorted order. Returns (AM, KM, dict of numpy arrays: ref, typ, tm, ax, axp, axu, pr, na, nt, np,
+    rg, P, gslot)."""
     L = len(preps)
     nh = [int(p.heavy.sum()) for p in preps]
     AM = 64 if max(nh) <= 64 else bucket(max(nh), 32)
     KM = bucket(max(max(p.n_tors for p in preps), 1), 8)
-    P = max(1, max(len(p.pairs) for p in preps))
+    pair_lists = []
+    for p in preps:
+        hv = np.where(p.heavy)[0]
+        remap = -np.ones(p.n, np.int64)
+        remap[hv] = np.arange(len(hv))
+        pl = remap[p.pairs] if len(p.pairs) else np.zeros((0, 2), np.int64)
+        pair_lists.append(schedule_pairs(pl) if schedule else pl)
+    P = max(1, max(len(x) for x in pair_lists))
     out = {
         "ref": np.zeros((L, AM, 3), np.float32), "typ": np.zeros((L, AM), np.int32),
         "tm": np.zeros((L, KM, AM), np.uint8), "ax": np.zeros((L, KM, 2), np.int32),
@@ -163,7 +204,7 @@ def pack_ligands(preps, slot, rec_idx):
         hv = np.where(p.heavy)[0]
         remap = -np.ones(p.n, np.int64)
         remap[hv] = np.arange(len(hv))
-        out["na"][l], out["nt"][l], out["np"][l], out["rg"][l] = len(hv), p.n_tors, len(p.pairs), p.rg
+        out["na"][l], out["nt"][l], out["np"][l], out["rg"][l] = len(hv), p.n_tors, len(pair_lists[l]), p.rg
         out["ref"][l, :len(hv)] = p.ref[hv]
         out["typ"][l, :len(hv)] = p.types[hv]
         if p.n_tors:
@@ -175,8 +216,8 @@ def pack_ligands(preps, slot, rec_idx):
             u = pb - pa
             out["axp"][l, :p.n_tors] = pa
             out["axu"][l, :p.n_tors] = u / np.maximum(np.linalg.norm(u, axis=1, keepdims=True), 1e-6)
-        if len(p.pairs):
-            out["pr"][l, :len(p.pairs)] = remap[p.pairs]
+        if len(pair_lists[l]):
+            out["pr"][l, :len(pair_lists[l])] = pair_lists[l]
         out["gslot"][l, :len(hv)] = slot[rec_idx[l], out["typ"][l, :len(hv)]]
         if (out["gslot"][l, :len(hv)] < 0).any():
             raise ValueError("a ligand atom type has no grid")

@@ -113,6 +113,50 @@ int adock_mc_batch(int AM, int KM,
  * times[1..L] = 0 (done / times may be NULL). */
 int adock_batch_reset(int* counter, int* done, int n_complexes, unsigned long long* times, adock_stream_t stream);
 
+/* ---- learned MC proposal network (128 -> 256 -> 256 -> 23, ReLU). No reference counterpart: the reference proposes
+ * moves at random only (src/adock/mckernel.py:381-398). ---- */
+
+/* One Monte Carlo step of every chain, for step-by-step search (one launch per step; capture a sequence in a HIP graph).
+ * The move comes from the network output Y [n,23] (use_net = 1; with probability p_plain, or when Y is not finite, the
+ * reference's random mutation is used instead) or is the reference's mutation (use_net = 0); then BFGS and Metropolis
+ * with the reference's RNG slots for step `step`. feat [n,128] (may be NULL) holds the features of each chain's current
+ * pose: read for logging, rewritten after the step. log_buf [log_cap, 175] (may be NULL) receives accepted moves
+ * (features before the move, 23 targets, 23 masks, energy gain) at *log_count (atomic). init = 1 runs the start
+ * BFGS only (the reference's init_opt). With use_net = 0, init followed by steps 0..S-1 equals adock_mc(init_opt = 1,
+ * mc_steps = S) bit for bit. */
+int adock_mc_step(int AM, int KM,
+                  const float* ref, const int* typ, const unsigned char* tm, const int* ax, const float* axp,
+                  const float* axu, const int* pr, const int* na, const int* nt, const int* np, const float* rg,
+                  int P_MAX, const int* gslot,
+                  const float* grid, const float* origin, const float* lo, const float* hi, const int* rec_idx,
+                  const long long* goffs, const long long* gdims, int T, float spacing, float pen,
+                  const float* prm, const float* rad, const int* hyd, const int* don, const int* acc,
+                  const float* rxyz, const int* rtyp, const int* rn, int RN_MAX,
+                  const int* chain_lig, const int* chain_id, int n_chains, float* cur, float* cur_e, float* best,
+                  float* best_e, int init, int step, int bfgs_iters, float temp, float amp_pos, float amp_rot,
+                  unsigned long long seed, const float* Y, float* feat, float* log_buf, int* log_count, int log_cap,
+                  int use_net, float p_plain, float sig_t, float sig_r, float sig_th, unsigned long long* eval_count,
+                  adock_stream_t stream);
+
+/* Inference on MFMA: Y[B,23] = W3 relu(W2 relu(W1 X + b1) + b2) + b3 with BF16 weights [out, in] (W1 [256,128],
+ * W2 [256,256], W3 [23,256]), BF16 activations and FP32 accumulation; X [B,128] and biases FP32. */
+int adock_mlp_infer(int B, const float* X, const unsigned short* W1, const float* b1, const unsigned short* W2,
+                    const float* b2, const unsigned short* W3, const float* b3, float* Y, adock_stream_t stream);
+
+/* Training primitives (FP32): C = alpha op(A) op(B) + beta C (row-major, op = transpose when trans_x != 0);
+ * Y += b row-broadcast (+ ReLU); dH *= (H > 0); db = column sums of dY [M,N]; masked MSE with its gradient
+ * (dY = 2 mask (Y - T) inv_norm, *loss += sum mask (Y - T)^2 inv_norm); bias-corrected Adam step t; FP32 -> BF16. */
+int adock_gemm_f32(int trans_a, int trans_b, int M, int N, int K, float alpha, const float* A, int lda, const float* B,
+                   int ldb, float beta, float* C, int ldc, adock_stream_t stream);
+int adock_bias_act(int M, int N, float* Y, const float* b, int relu, adock_stream_t stream);
+int adock_relu_bwd(long long n, float* dH, const float* H, adock_stream_t stream);
+int adock_colsum(int M, int N, const float* dY, float* db, adock_stream_t stream);
+int adock_mse(long long n, const float* Y, const float* T, const float* mask, float inv_norm, float* dY, float* loss,
+              adock_stream_t stream);
+int adock_adam(long long n, float* w, const float* g, float* m, float* v, float lr, float beta1, float beta2,
+               float eps, int t, adock_stream_t stream);
+int adock_to_bf16(long long n, const float* x, unsigned short* y, adock_stream_t stream);
+
 /* Launch shape of the chain kernels: chains (waves) per workgroup, 1..4. The reference's counterpart is the NW
  * constant (src/adock/mckernel.py:525-540). Returns 0, or ADOCK_E_ARG. */
 int adock_set_chains_per_block(int chains_per_block);

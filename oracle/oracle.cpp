@@ -311,6 +311,7 @@ R eval_eg(const R* s, R* g, const Ctx<R>& c, const SF<R>& sf, R* Xout = nullptr,
     }
     for (int q = 0; q < c.np; ++q) {
         const int i = c.pr[2 * q], j = c.pr[2 * q + 1];
+        if (i == j) continue;   // padding entry of a scheduled pair list
         R rx = X[3 * i] - X[3 * j], ry = X[3 * i + 1] - X[3 * j + 1], rz = X[3 * i + 2] - X[3 * j + 2];
         R r = std::sqrt(rx * rx + ry * ry + rz * rz);
         if (margin) {

@@ -1,7 +1,7 @@
 """GPU parity through the C ABI: HIP kernels vs the FP64 CPU oracle on identical float32 inputs.
 
 Bars (written into each test): grids 1e-5 relative with a 1e-5 kcal/mol floor; per-pose energies 1e-4 relative with a
-1e-4 floor and gradients 1e-4 in vector norm (poses closer than 1e-4 A to a discontinuity of E or dE are excluded and
+1e-4 floor and gradients 1e-4 in vector norm (poses closer than 2e-5 A to a discontinuity of E or dE are excluded and
 counted: a cell face of the trilinear lookup, the 8 A cutoff, a slope-term kink: within 2e-5 A the float32 and
 float64 poses can fall on different sides); one BFGS iteration: >= 99% of
 chains within 1e-3; longer BFGS / MC runs: distributions (median, min, mean) within 0.5 kcal/mol, because float
