@@ -200,7 +200,7 @@ int adock_to_bf16(long long n, const float* x, unsigned short* y, adock_stream_t
 }
 
 int adock_set_chains_per_block(int w) {
-    if (set_waves_per_block(w)) return fail(ADOCK_E_ARG, "chains_per_block must be 1..4");
+    if (set_waves_per_block(w)) return fail(ADOCK_E_ARG, "chains_per_block must be 0 (adaptive) or 1..4");
     return 0;
 }
 int adock_get_chains_per_block(void) { return get_waves_per_block(); }

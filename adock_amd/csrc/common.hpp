@@ -37,6 +37,25 @@ DEVI float wave_sum(float v) {
     return bcast(v, 63);
 }
 
+// Sums of N independent per-lane values over the wave, all results uniform; the DPP steps of the N sums interleave.
+template <int N>
+DEVI void wave_sum_multi(float (&v)[N]) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] += dpp_f<0x111>(v[i]);
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] += dpp_f<0x112>(v[i]);
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] += dpp_f<0x114>(v[i]);
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] += dpp_f<0x118>(v[i]);
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] += dpp_f<0x142, 0xa>(v[i]);
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] += dpp_f<0x143, 0xc>(v[i]);
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = bcast(v[i], 63);
+}
+
 // Orders this wave's LDS writes before its later LDS reads (one wave owns the region).
 DEVI void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

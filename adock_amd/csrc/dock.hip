@@ -29,13 +29,17 @@ namespace {
 
 constexpr int AMK = 64;   // heavy-atom bucket handled by these kernels (one atom per lane)
 
-DEVI int wave_lds_bytes(int P_MAX) { return 16 * AMK + 12 * AMK + 4 * AMK + 2 * ((P_MAX + 7) & ~7); }
+// per-wave LDS: X4 [64] float4, F [3][64], flags [64], pair words [P64] u16, inverse lane table [P64] u16
+// (P64 = P_MAX rounded up to a multiple of 64)
+DEVI int pairs64(int P_MAX) { return (P_MAX + 63) & ~63; }
+DEVI int wave_lds_bytes(int P_MAX) { return 16 * AMK + 12 * AMK + 4 * AMK + 4 * pairs64(P_MAX); }
 
 struct WaveLDS {
     float4* X4;            // world xyz + radius
     float* Fx; float* Fy; float* Fz;
     int* flg;
     unsigned short* pl;
+    unsigned short* inv;   // [round][atom]: lane holding the atom as first (low byte) / second (high byte), 0xff = none
 };
 
 struct ChainCtx {
@@ -55,6 +59,7 @@ struct ChainCtx {
     int ny, nz;
     float o0, o1, o2, lo0, lo1, lo2, hi0, hi1, hi2, inv_s, pen, fmx0, fmx1, fmx2;
     int exact;
+    int sched;            // 1: pair list in conflict-free rounds of 64 (force pull through ds_bpermute, no atomics)
     const float* rrx; const int* rrt; int rn; const int* nb; int nbn;
     const float* sfrad; const int* sfhyd; const int* sfdon; const int* sfacc;
 };
@@ -67,7 +72,7 @@ DEVI WaveLDS carve_lds(char* base, int P_MAX) {
     w.Fz = w.Fy + AMK;
     w.flg = reinterpret_cast<int*>(base + 28 * AMK);
     w.pl = reinterpret_cast<unsigned short*>(base + 32 * AMK);
-    (void)P_MAX;
+    w.inv = w.pl + pairs64(P_MAX);
     return w;
 }
 
@@ -119,6 +124,29 @@ DEVI ChainCtx setup_chain(int l, int lane, const LigArgs& L, const GridArgs& G, 
         const int i = pr[2 * q], j = pr[2 * q + 1];
         w.pl[q] = (unsigned short)(i == j ? 0x8000 : (i | (j << 6) | (pair_flags(w.flg[i], w.flg[j]) << 12)));
     }
+    // inverse lane tables of a scheduled pair list, and the check that makes the atomic-free path safe: within every
+    // round each atom may hold at most one first and one second slot
+    const int rounds = (c.np + 63) >> 6;
+    unsigned char* inv8 = reinterpret_cast<unsigned char*>(w.inv);
+    for (int r = 0; r < rounds; ++r) w.inv[r * 64 + lane] = 0xffff;
+    wave_lds_sync();
+    for (int r = 0; r < rounds; ++r) {
+        const int q = r * 64 + lane;
+        const unsigned pw = q < c.np ? w.pl[q] : 0x8000u;
+        if (!(pw & 0x8000u)) {
+            inv8[(r * 64 + (pw & 63)) * 2] = (unsigned char)lane;
+            inv8[(r * 64 + ((pw >> 6) & 63)) * 2 + 1] = (unsigned char)lane;
+        }
+    }
+    wave_lds_sync();
+    bool mine = true;
+    for (int r = 0; r < rounds; ++r) {
+        const int q = r * 64 + lane;
+        const unsigned pw = q < c.np ? w.pl[q] : 0x8000u;
+        if (!(pw & 0x8000u))
+            mine = mine && inv8[(r * 64 + (pw & 63)) * 2] == lane && inv8[(r * 64 + ((pw >> 6) & 63)) * 2 + 1] == lane;
+    }
+    c.sched = __ballot(!mine) == 0ull ? 1 : 0;
     c.exact = exact;
     c.rrx = R.rxyz + (long long)ri * R.RN_MAX * 3;
     c.rrt = R.rtyp + (long long)ri * R.RN_MAX;
@@ -130,6 +158,7 @@ DEVI ChainCtx setup_chain(int l, int lane, const LigArgs& L, const GridArgs& G, 
 }
 
 // Energy and lane-distributed gradient of the lane-distributed state x. Returns E (uniform).
+template <int KM>
 DEVI float eval_eg(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, int lane, float x, float& gout, float& ea,
                    float& Xx, float& Xy, float& Xz) {
     const float t0 = bcast(x, 0), t1 = bcast(x, 1), t2 = bcast(x, 2);
@@ -182,23 +211,54 @@ DEVI float eval_eg(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, int l
     w.X4[lane] = make_float4(X0, X1, X2, c.rad);
     w.Fx[lane] = 0.f; w.Fy[lane] = 0.f; w.Fz[lane] = 0.f;
     wave_lds_sync();
-    // intra-ligand pairs, strided over lanes (a scheduled list has distinct first and second atoms per 64 pairs, so
-    // the force accumulations of one instruction never collide); branch-free pair terms
-    float ep_l = 0.f;
-    for (int q = lane; q < c.np; q += WAVE) {
-        const unsigned pw = w.pl[q];
-        if (pw & 0x8000u) continue;
-        const int i = pw & 63, j = (pw >> 6) & 63;
-        const float4 xi = w.X4[i], xj = w.X4[j];
-        const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-        const float r = fast_sqrt(rx * rx + ry * ry + rz * rz);
-        float de;
-        const float e = pair_e_nb(r - xi.w - xj.w, (int)(pw >> 12), sf, de);
-        const float in = r < sf.cut ? 1.f : 0.f;
-        ep_l += in * e;
-        const float sc = in * de * fast_rcp(fmaxf(r, 1e-6f));
-        atomicAdd(&w.Fx[i], sc * rx); atomicAdd(&w.Fy[i], sc * ry); atomicAdd(&w.Fz[i], sc * rz);
-        atomicAdd(&w.Fx[j], -sc * rx); atomicAdd(&w.Fy[j], -sc * ry); atomicAdd(&w.Fz[j], -sc * rz);
+    // intra-ligand pairs. Scheduled list: round r puts pair 64 r + lane on this lane; the atom on lane a pulls the
+    // forces of the (at most two) lanes that hold it through ds_bpermute and accumulates them in registers. Any other
+    // list: pairs strided over lanes with LDS float atomics into F. Pair terms are branch-free.
+    float ep_l = 0.f, pfx = 0.f, pfy = 0.f, pfz = 0.f;
+    if (c.sched) {
+        const int rounds = (c.np + 63) >> 6;
+        for (int r = 0; r < rounds; ++r) {
+            const int q = r * 64 + lane;
+            const unsigned pw = q < c.np ? w.pl[q] : 0x8000u;
+            const unsigned iv = w.inv[q];
+            float sx = 0.f, sy = 0.f, sz = 0.f;
+            if (!(pw & 0x8000u)) {
+                const int i = pw & 63, j = (pw >> 6) & 63;
+                const float4 xi = w.X4[i], xj = w.X4[j];
+                const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+                const float rr = fast_sqrt(rx * rx + ry * ry + rz * rz);
+                float de;
+                const float e = pair_e_nb(rr - xi.w - xj.w, (int)(pw >> 12), sf, de);
+                const float in = rr < sf.cut ? 1.f : 0.f;
+                ep_l += in * e;
+                const float sc = in * de * fast_rcp(fmaxf(rr, 1e-6f));
+                sx = sc * rx; sy = sc * ry; sz = sc * rz;
+            }
+            const int li = iv & 0xff, lj = iv >> 8;
+            const float mi = li != 0xff ? 1.f : 0.f, mj = lj != 0xff ? 1.f : 0.f;
+            const int si = li & 63, sj = lj & 63;
+            pfx += mi * __shfl(sx, si) - mj * __shfl(sx, sj);
+            pfy += mi * __shfl(sy, si) - mj * __shfl(sy, sj);
+            pfz += mi * __shfl(sz, si) - mj * __shfl(sz, sj);
+        }
+    } else {
+        for (int q = lane; q < c.np; q += WAVE) {
+            const unsigned pw = w.pl[q];
+            if (pw & 0x8000u) continue;
+            const int i = pw & 63, j = (pw >> 6) & 63;
+            const float4 xi = w.X4[i], xj = w.X4[j];
+            const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+            const float rr = fast_sqrt(rx * rx + ry * ry + rz * rz);
+            float de;
+            const float e = pair_e_nb(rr - xi.w - xj.w, (int)(pw >> 12), sf, de);
+            const float in = rr < sf.cut ? 1.f : 0.f;
+            ep_l += in * e;
+            const float sc = in * de * fast_rcp(fmaxf(rr, 1e-6f));
+            atomicAdd(&w.Fx[i], sc * rx); atomicAdd(&w.Fy[i], sc * ry); atomicAdd(&w.Fz[i], sc * rz);
+            atomicAdd(&w.Fx[j], -sc * rx); atomicAdd(&w.Fy[j], -sc * ry); atomicAdd(&w.Fz[j], -sc * rz);
+        }
+        wave_lds_sync();
+        pfx = w.Fx[lane]; pfy = w.Fy[lane]; pfz = w.Fz[lane];
     }
     if (c.atom) {
         if (!c.exact) {
@@ -237,8 +297,7 @@ DEVI float eval_eg(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, int l
     ea = el;            // this atom's grid (or exact inter) energy + out-of-box penalty
     Xx = X0; Xy = X1; Xz = X2;
     el += ep_l;
-    wave_lds_sync();
-    const float fx = gx + w.Fx[lane], fy = gy + w.Fy[lane], fz = gz + w.Fz[lane];
+    const float fx = gx + pfx, fy = gy + pfy, fz = gz + pfz;
     // current torsion axes (lane k < nt) from the world coordinates of the axis atoms
     float pwx = 0.f, pwy = 0.f, pwz = 0.f, uwx = 0.f, uwy = 0.f, uwz = 0.f;
     if (lane < c.nt) {
@@ -248,24 +307,34 @@ DEVI float eval_eg(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, int l
         uwx = ux * nn; uwy = uy * nn; uwz = uz * nn;
         pwx = A.x; pwy = A.y; pwz = A.z;
     }
-    float g = 0.f;
+    // every gradient component and the energy are reduced over the wave together, so the DPP steps of the 8 + KM
+    // independent sums interleave instead of running one dependent chain after another
     const float dX = X0 - t0, dY = X1 - t1, dZ = X2 - t2;
-    const float s0 = wave_sum(fx), s1 = wave_sum(fy), s2 = wave_sum(fz);
-    const float s3 = wave_sum(dY * fz - dZ * fy), s4 = wave_sum(dZ * fx - dX * fz), s5 = wave_sum(dX * fy - dY * fx);
-    g = lane == 0 ? s0 : lane == 1 ? s1 : lane == 2 ? s2 : lane == 3 ? s3 : lane == 4 ? s4 : lane == 5 ? s5 : 0.f;
-    for (int k = 0; k < c.nt; ++k) {
-        const float pxk = bcast(pwx, k), pyk = bcast(pwy, k), pzk = bcast(pwz, k);
-        const float uxk = bcast(uwx, k), uyk = bcast(uwy, k), uzk = bcast(uwz, k);
+    float v[7 + KM];
+    v[0] = fx; v[1] = fy; v[2] = fz;
+    v[3] = dY * fz - dZ * fy; v[4] = dZ * fx - dX * fz; v[5] = dX * fy - dY * fx;
+    v[6] = el;
+#pragma unroll
+    for (int k = 0; k < KM; ++k) {
         float tv = 0.f;
-        if ((c.tmask >> k) & 1u) {
-            const float vx = X0 - pxk, vy = X1 - pyk, vz = X2 - pzk;
-            tv = (vy * fz - vz * fy) * uxk + (vz * fx - vx * fz) * uyk + (vx * fy - vy * fx) * uzk;
+        if (k < c.nt) {
+            const float pxk = bcast(pwx, k), pyk = bcast(pwy, k), pzk = bcast(pwz, k);
+            const float uxk = bcast(uwx, k), uyk = bcast(uwy, k), uzk = bcast(uwz, k);
+            if ((c.tmask >> k) & 1u) {
+                const float vx = X0 - pxk, vy = X1 - pyk, vz = X2 - pzk;
+                tv = (vy * fz - vz * fy) * uxk + (vz * fx - vx * fz) * uyk + (vx * fy - vy * fx) * uzk;
+            }
         }
-        const float gk = wave_sum(tv);
-        if (lane == 7 + k) g = gk;
+        v[7 + k] = tv;
     }
+    wave_sum_multi<7 + KM>(v);
+    float g = 0.f;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) g = lane == i ? v[i] : g;
+#pragma unroll
+    for (int k = 0; k < KM; ++k) g = (lane == 7 + k && k < c.nt) ? v[7 + k] : g;
     gout = g;
-    return wave_sum(el);
+    return v[6];
 }
 
 // x (+) alpha p on the lane-distributed state
@@ -315,7 +384,7 @@ DEVI float bfgs(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, int lane
     while (true) {
         const float xe = trial < 0 ? x : conf_inc(x, p, alpha, lane);
         float ge, ea_, x0_, x1_, x2_;
-        const float fe = eval_eg(c, sf, w, lane, xe, ge, ea_, x0_, x1_, x2_);
+        const float fe = eval_eg<KM>(c, sf, w, lane, xe, ge, ea_, x0_, x1_, x2_);
         ++nev;
         if (trial < 0) {
             f0 = fe; f_orig = fe; g = ge;
@@ -469,20 +538,35 @@ __global__ __launch_bounds__(256) void eg_kernel(LigArgs L, GridArgs G, SFArgs S
     const ChainCtx c = setup_chain<KM>(__builtin_amdgcn_readfirstlane(chain_lig[ch]), lane, L, G, S, R, exact, w);
     const float x = lane < SD ? state[(long long)ch * SD + lane] : 0.f;
     float g, ea_, x0_, x1_, x2_;
-    const float e = eval_eg(c, sf, w, lane, x, g, ea_, x0_, x1_, x2_);
+    const float e = eval_eg<KM>(c, sf, w, lane, x, g, ea_, x0_, x1_, x2_);
     if (lane < 6) g_out[(long long)ch * DM + lane] = g;
     else if (lane >= 7 && lane < 7 + c.nt) g_out[(long long)ch * DM + lane - 1] = g;
     if (lane == 0) e_out[ch] = e;
 }
 
-int g_waves_per_block = 4;
+int g_waves_per_block = 0;   // 0: adaptive (auto_waves)
 
-inline int lds_bytes_per_wave(int P_MAX) { return 16 * AMK + 12 * AMK + 4 * AMK + 2 * ((P_MAX + 7) & ~7); }
+// chains per workgroup when not set explicitly: spread small launches one wave per CU (latency), pack large ones 4
+// per workgroup (a workgroup's waves share the CU's LDS and instruction cache)
+inline int auto_waves(int n_chains) {
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (cus <= 0) cus = 256;
+    }
+    const int w = n_chains / (4 * cus);
+    return w >= 4 ? 4 : w >= 2 ? 2 : 1;
+}
+inline int waves_for(int n_chains) { return g_waves_per_block > 0 ? g_waves_per_block : auto_waves(n_chains); }
+
+inline int lds_bytes_per_wave(int P_MAX) { return 16 * AMK + 12 * AMK + 4 * AMK + 4 * ((P_MAX + 63) & ~63); }
 
 template <int KM>
 int launch_mc_t(const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecArgs& R, int exact,
                 const int* chain_lig, int n_chains, const McArgs& M, hipStream_t stream) {
-    const int W = g_waves_per_block;
+    const int W = waves_for(n_chains);
     const size_t shm = (size_t)W * lds_bytes_per_wave(L.P_MAX);
     const unsigned nblk = (unsigned)((n_chains + W - 1) / W);
     hipLaunchKernelGGL(mc_kernel<KM>, dim3(nblk), dim3(W * WAVE), shm, stream, L, G, S, R, exact, chain_lig,
@@ -494,7 +578,7 @@ template <int KM>
 int launch_mc_persistent_t(const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecArgs& R, int exact,
                            const int* chain_lig, int n_chains, const McArgs& M, const QueueArgs& Q, int blocks,
                            hipStream_t stream) {
-    const int W = g_waves_per_block;
+    const int W = g_waves_per_block > 0 ? g_waves_per_block : 4;
     const size_t shm = (size_t)W * lds_bytes_per_wave(L.P_MAX);
     if (blocks <= 0) {
         // resident capacity, queried once per (device, W, LDS size) so later launches (and graph captures) only launch
@@ -524,7 +608,7 @@ template <int KM>
 int launch_eg_t(const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecArgs& R, int exact,
                 const int* chain_lig, int n_chains, const float* state, float* e_out, float* g_out,
                 hipStream_t stream) {
-    const int W = g_waves_per_block;
+    const int W = waves_for(n_chains);
     const size_t shm = (size_t)W * lds_bytes_per_wave(L.P_MAX);
     const unsigned nblk = (unsigned)((n_chains + W - 1) / W);
     hipLaunchKernelGGL(eg_kernel<KM>, dim3(nblk), dim3(W * WAVE), shm, stream, L, G, S, R, exact, chain_lig,
@@ -738,7 +822,7 @@ __global__ __launch_bounds__(256) void mc_step_kernel(LigArgs L, GridArgs G, SFA
     }
     if (P.feat) {
         float g, ea, a0, a1, a2;
-        const float e = eval_eg(c, sf, w, lane, cur, g, ea, a0, a1, a2);
+        const float e = eval_eg<KM>(c, sf, w, lane, cur, g, ea, a0, a1, a2);
         ++nev;
         write_features(P.feat + (long long)ch * NFEAT, c, lane, cur, e, g, ea);
     }
@@ -752,7 +836,7 @@ __global__ __launch_bounds__(256) void mc_step_kernel(LigArgs L, GridArgs G, SFA
 template <int KM>
 int launch_mc_step_t(const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecArgs& R, const int* chain_lig,
                      int n_chains, const McArgs& M, const StepArgs& P, hipStream_t stream) {
-    const int W = g_waves_per_block;
+    const int W = waves_for(n_chains);
     const size_t shm = (size_t)W * lds_bytes_per_wave(L.P_MAX);
     const unsigned nblk = (unsigned)((n_chains + W - 1) / W);
     hipLaunchKernelGGL(mc_step_kernel<KM>, dim3(nblk), dim3(W * WAVE), shm, stream, L, G, S, R, chain_lig, n_chains,
@@ -763,7 +847,7 @@ int launch_mc_step_t(const LigArgs& L, const GridArgs& G, const SFArgs& S, const
 }  // namespace
 
 int set_waves_per_block(int w) {
-    if (w < 1 || w > 4) return -1;
+    if (w < 0 || w > 4) return -1;
     g_waves_per_block = w;
     return 0;
 }

@@ -1,0 +1,105 @@
+"""Learned Monte Carlo proposal: step-by-step search with the proposal network between steps.
+
+Each MC step is one adock_mc_step launch over all chains; with the network on, adock_mlp_infer (BF16 MFMA) runs on the
+features of every chain's current pose right before it. The whole step sequence can be captured once in a HIP graph
+(torch.cuda.CUDAGraph) and replayed. The exact Vina score and the reference's Metropolis test judge every move; the
+reference's random mutation is the fallback (probability p_plain, or whenever the network output is not finite).
+The network proposal is not symmetric, so the chain no longer satisfies detailed balance: it is an optimiser, not a
+sampler, which is all the docking search needs (the best score is what is reported).
+
+Training data: accepted moves logged by the plain method (features before the move, the move as target, a mask of the
+components that exist, the energy gain), on generator seeds disjoint from the evaluation seeds.
+"""
+import numpy as np
+import torch
+
+from . import _lib
+from .mlp import IN, OUT, ProposalNet
+
+LOG_W = IN + 2 * OUT + 1
+
+
+class StepSearch:
+    def __init__(self, kp, chain_lig, chain_id=None, net=None, log_cap=0):
+        self.kp = kp
+        dev = kp.device
+        self.N = int(chain_lig.numel())
+        self.chain_lig = chain_lig.to(device=dev, dtype=torch.int32).contiguous()
+        self.chain_id = (torch.arange(self.N, dtype=torch.int32, device=dev) if chain_id is None
+                         else chain_id.to(device=dev, dtype=torch.int32).contiguous())
+        self.net = net
+        self.feat = torch.zeros(self.N, IN, device=dev)
+        self.Y = torch.zeros(self.N, OUT, device=dev)
+        self.log_cap = int(log_cap)
+        self.log_buf = torch.zeros(max(self.log_cap, 1), LOG_W, device=dev) if self.log_cap else None
+        self.log_count = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.graphs = {}
+
+    def _step(self, cur, cur_e, best, best_e, init, step, bfgs_iters, temp, amp_pos, amp_rot, seed, use_net, p_plain,
+              sig, log, feat_on, count_evals):
+        kp = self.kp
+        if use_net and not init:
+            self.net.infer(self.feat, self.Y)
+        args = (kp._args(False)[:-1] +
+                [_lib.ptr(self.chain_lig), _lib.ptr(self.chain_id), self.N, _lib.ptr(cur), _lib.ptr(cur_e),
+                 _lib.ptr(best), _lib.ptr(best_e), int(init), int(step), int(bfgs_iters), float(temp), float(amp_pos),
+                 float(amp_rot), int(seed) & (2 ** 64 - 1), _lib.ptr(self.Y) if use_net else None,
+                 _lib.ptr(self.feat) if feat_on else None,
+                 _lib.ptr(self.log_buf) if log and self.log_buf is not None else None, _lib.ptr(self.log_count),
+                 self.log_cap, int(bool(use_net)), float(p_plain), float(sig[0]), float(sig[1]), float(sig[2]),
+                 _lib.ptr(kp.eval_count) if count_evals else None, _lib.stream_of(kp.device)])
+        _lib.check(_lib.lib().adock_mc_step(*args))
+
+    def run(self, cur, cur_e, best, best_e, steps, bfgs_iters=0, temp=1.2, amp_pos=2.0, amp_rot=2.0, seed=0,
+            step_offset=0, init_opt=True, use_net=False, p_plain=0.25, sig=(0.5, 0.15, 0.3), log=False,
+            count_evals=False):
+        """In place on cur / cur_e / best / best_e (float32 contiguous device tensors), like KernelProblem.run_mc.
+        Features are kept current whenever the network or the move log needs them."""
+        if use_net and self.net is None:
+            raise _lib.AdockError("use_net needs a ProposalNet")
+        feat_on = bool(use_net or log)
+        if init_opt:
+            self._step(cur, cur_e, best, best_e, 1, step_offset, bfgs_iters, temp, amp_pos, amp_rot, seed, False,
+                       p_plain, sig, log, feat_on, count_evals)
+        elif feat_on:
+            raise _lib.AdockError("features of the start poses come from the initial step: use init_opt=True")
+        for st in range(steps):
+            self._step(cur, cur_e, best, best_e, 0, step_offset + st, bfgs_iters, temp, amp_pos, amp_rot, seed,
+                       use_net, p_plain, sig, log, feat_on, count_evals)
+
+    def capture(self, cur, cur_e, best, best_e, steps, **kw):
+        """Captures run(...) on these buffers in a HIP graph; returns the graph (replay() re-runs the search; reset the
+        buffers between replays)."""
+        dev = self.kp.device
+        s = torch.cuda.Stream(device=dev)
+        s.wait_stream(torch.cuda.current_stream(dev))
+        with torch.cuda.stream(s):
+            self._step(cur, cur_e, best, best_e, 1, 0, kw.get("bfgs_iters", 0), kw.get("temp", 1.2),
+                       kw.get("amp_pos", 2.0), kw.get("amp_rot", 2.0), kw.get("seed", 0), False, 0.0,
+                       kw.get("sig", (0.5, 0.15, 0.3)), False, False, False)
+        torch.cuda.current_stream(dev).wait_stream(s)
+        torch.cuda.synchronize(dev)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.run(cur, cur_e, best, best_e, steps, **kw)
+        return g
+
+    def logged(self):
+        """(features [n,128], targets [n,23], masks [n,23], energy gain [n]) of the accepted moves logged so far."""
+        n = min(int(self.log_count.item()), self.log_cap)
+        L = self.log_buf[:n]
+        return L[:, :IN], L[:, IN:IN + OUT], L[:, IN + OUT:IN + 2 * OUT], L[:, IN + 2 * OUT]
+
+
+def move_weights(mask, gain):
+    """Per-component regression weights: moves that lowered the energy count 1 + min(gain, 5), uphill (Metropolis)
+    moves 0.25, missing components 0."""
+    w = torch.where(gain > 0, 1.0 + gain.clamp(max=5.0), torch.full_like(gain, 0.25))
+    return (mask * w[:, None]).contiguous()
+
+
+def train_proposal(X, T, M, gain, epochs=8, batch=4096, lr=1e-3, seed=0, log=None, device="cuda"):
+    net = ProposalNet(device, seed=seed)
+    W = move_weights(M, gain)
+    losses = net.fit(X.contiguous(), T.contiguous(), W, epochs=epochs, batch=batch, lr=lr, seed=seed, log=log)
+    return net, losses

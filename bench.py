@@ -67,11 +67,27 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--chains-per-block", type=int, default=0)
+    ap.add_argument("--workload", choices=["single", "screen", "multi", "learned"], default="single",
+                    help="single = the headline line (default); the others print their own JSON line")
+    ap.add_argument("--small", action="store_true", help="scaled-down sizes for the opt-in workloads")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build runs the docking path on a single MI355X "
               "(multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "single":
+        from adock_amd import experiments as ex
+        if args.workload == "screen":
+            out = ex.screen(n_ligands=64 if args.small else 512, chains=64, steps=50 if args.small else 500)
+        elif args.workload == "multi":
+            out = ex.multi(n_receptors=8 if args.small else 64, ligands_per=8 if args.small else 32, chains=32,
+                           steps=50 if args.small else 500)
+        else:
+            out = ex.learned(train_seeds=range(1000, 1004 if args.small else 1032), eval_seeds=range(2 if args.small else 8),
+                             train_steps=40 if args.small else 200, budgets=(20, 40) if args.small else (50, 100, 200),
+                             full_steps=100 if args.small else 1000, epochs=2 if args.small else 6)
+        print(json.dumps(out), flush=True)
+        return
     dev = torch.device("cuda:0")
     w, prob = workload.single(2, seed=0, device=dev)
     kp = KernelProblem(prob)

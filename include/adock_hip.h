@@ -157,8 +157,9 @@ int adock_adam(long long n, float* w, const float* g, float* m, float* v, float 
                float eps, int t, adock_stream_t stream);
 int adock_to_bf16(long long n, const float* x, unsigned short* y, adock_stream_t stream);
 
-/* Launch shape of the chain kernels: chains (waves) per workgroup, 1..4. The reference's counterpart is the NW
- * constant (src/adock/mckernel.py:525-540). Returns 0, or ADOCK_E_ARG. */
+/* Launch shape of the chain kernels: chains (waves) per workgroup, 1..4, or 0 = adaptive (default: one chain per
+ * workgroup while chains <= 8 x CUs, 2 up to 16 x CUs, else 4; the persistent batch kernel uses 4). The reference's
+ * counterpart is the NW constant (src/adock/mckernel.py:525-540). Returns 0, or ADOCK_E_ARG. */
 int adock_set_chains_per_block(int chains_per_block);
 int adock_get_chains_per_block(void);
 

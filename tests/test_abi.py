@@ -44,6 +44,6 @@ def test_bucket_validation_without_gpu():
     assert b"AM" in L.adock_last_error()
     args[0], args[1] = 64, 12
     assert L.adock_eg(*args) == -2
-    assert L.adock_set_chains_per_block(0) == -1
+    assert L.adock_set_chains_per_block(-1) == -1
     assert L.adock_set_chains_per_block(5) == -1
     assert L.adock_version().startswith(b"adock_hip")

@@ -83,4 +83,4 @@ def test_autotuner_picks_a_shape(small_screen):
     torch.cuda.synchronize()
     np.testing.assert_array_equal(ref[3], be.cpu().numpy())
     from adock_amd import _lib
-    _lib.check(_lib.lib().adock_set_chains_per_block(4))
+    _lib.check(_lib.lib().adock_set_chains_per_block(0))
