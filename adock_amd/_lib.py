@@ -11,7 +11,7 @@ import torch
 LIB_PATH = Path(__file__).resolve().parent / "libadock_hip.so"
 
 P, I, F, U64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_ulonglong
-LIG = [I, I] + [P] * 11 + [I, P]            # AM, KM, ref..rg, P_MAX, gslot
+LIG = [I, I] + [P] * 11 + [I, P, P, P]      # AM, KM, ref..rg, P_MAX, gslot, inc, ist
 GRID = [P] * 7 + [I, F, F]                  # grid..gdims, T, spacing, pen
 SF = [P] * 5
 REC = [P, P, P, I]
@@ -54,8 +54,11 @@ def lib():
         L.adock_to_bf16.argtypes = [LL, P, P, P]
         L.adock_set_chains_per_block.argtypes = [I]
         L.adock_get_chains_per_block.restype = I
+        L.adock_set_line_search_width.argtypes = [I]
+        L.adock_get_line_search_width.restype = I
         for name in ("adock_build_grid", "adock_eg", "adock_mc", "adock_mc_counted", "adock_mc_batch",
-                     "adock_batch_reset", "adock_set_chains_per_block", "adock_mc_step", "adock_mlp_infer", "adock_gemm_f32",
+                     "adock_batch_reset", "adock_set_chains_per_block", "adock_set_line_search_width", "adock_mc_step",
+                     "adock_mlp_infer", "adock_gemm_f32",
                      "adock_bias_act", "adock_relu_bwd", "adock_colsum", "adock_mse", "adock_adam", "adock_to_bf16"):
             getattr(L, name).restype = I
         _LIB = L

@@ -24,7 +24,8 @@ static int hip_status(int st) {
 static int check_buckets(int AM, int KM, int P_MAX) {
     if (AM != 64) return fail(ADOCK_E_BUCKET, "AM must be 64 (at most 64 heavy atoms per ligand)");
     if (KM != 8 && KM != 16 && KM != 24 && KM != 32) return fail(ADOCK_E_BUCKET, "KM must be one of 8, 16, 24, 32");
-    if (P_MAX < 1 || !lds_budget_ok(P_MAX)) return fail(ADOCK_E_LDS, "P_MAX out of range for the per-wave LDS budget");
+    if (P_MAX < 1 || !lds_budget_ok(KM, P_MAX))
+        return fail(ADOCK_E_LDS, "P_MAX out of range for the per-wave LDS budget");
     return 0;
 }
 
@@ -49,28 +50,31 @@ int adock_build_grid(const float* rxyz, const int* rtype, int n_rec, const float
 
 #define LIG_DECL                                                                                                     \
     const float *ref, const int *typ, const unsigned char *tm, const int *ax, const float *axp, const float *axu,    \
-        const int *pr, const int *na, const int *nt, const int *np, const float *rg, int P_MAX, const int *gslot
+        const int *pr, const int *na, const int *nt, const int *np, const float *rg, int P_MAX, const int *gslot, \
+        const unsigned short *inc, const int *ist
 #define GRID_DECL                                                                                                    \
     const float *grid, const float *origin, const float *lo, const float *hi, const int *rec_idx,                    \
         const long long *goffs, const long long *gdims, int T, float spacing, float pen
 #define SF_DECL const float *prm, const float *rad, const int *hyd, const int *don, const int *acc
 #define REC_DECL const float *rxyz, const int *rtyp, const int *rn, int RN_MAX
 
-static int common_checks(int AM, int KM, int P_MAX, const void* ref, const void* grid, const void* prm, int exact,
-                         const void* rxyz, const void* rn, int RN_MAX, const void* chain_lig, int n_chains) {
+static int common_checks(int AM, int KM, int P_MAX, const void* ref, const void* inc, const void* ist, const void* grid,
+                         const void* prm, int exact, const void* rxyz, const void* rn, int RN_MAX,
+                         const void* chain_lig, int n_chains) {
     if (int st = check_buckets(AM, KM, P_MAX)) return st;
     if (n_chains < 0) return fail(ADOCK_E_ARG, "negative n_chains");
-    if (!ref || !grid || !prm || !chain_lig) return fail(ADOCK_E_ARG, "null pointer");
+    if (!ref || !inc || !ist || !grid || !prm || !chain_lig) return fail(ADOCK_E_ARG, "null pointer");
     if (exact && (!rxyz || !rn || RN_MAX <= 0)) return fail(ADOCK_E_ARG, "exact mode needs receptor atoms");
     return 0;
 }
 
 int adock_eg(int AM, int KM, LIG_DECL, GRID_DECL, SF_DECL, REC_DECL, int exact, const int* chain_lig, int n_chains,
              const float* state, float* e_out, float* g_out, adock_stream_t stream) {
-    if (int st = common_checks(AM, KM, P_MAX, ref, grid, prm, exact, rxyz, rn, RN_MAX, chain_lig, n_chains)) return st;
+    if (int st = common_checks(AM, KM, P_MAX, ref, inc, ist, grid, prm, exact, rxyz, rn, RN_MAX, chain_lig, n_chains))
+        return st;
     if (n_chains == 0) return 0;
     if (!state || !e_out || !g_out) return fail(ADOCK_E_ARG, "null pointer");
-    LigArgs L{ref, typ, tm, ax, axp, axu, pr, na, nt, np, rg, P_MAX, gslot};
+    LigArgs L{ref, typ, tm, ax, axp, axu, pr, na, nt, np, rg, P_MAX, gslot, inc, ist};
     GridArgs G{grid, origin, lo, hi, rec_idx, goffs, gdims, T, spacing, pen};
     SFArgs S{prm, rad, hyd, don, acc};
     RecArgs R{rxyz, rtyp, rn, RN_MAX};
@@ -82,12 +86,13 @@ int adock_mc_counted(int AM, int KM, LIG_DECL, GRID_DECL, SF_DECL, REC_DECL, int
                      float* cur_e, float* best, float* best_e, int init_opt, int mc_steps, int bfgs_iters, float temp,
                      float amp_pos, float amp_rot, unsigned long long seed, int step_offset,
                      unsigned long long* eval_count, adock_stream_t stream) {
-    if (int st = common_checks(AM, KM, P_MAX, ref, grid, prm, exact, rxyz, rn, RN_MAX, chain_lig, n_chains)) return st;
+    if (int st = common_checks(AM, KM, P_MAX, ref, inc, ist, grid, prm, exact, rxyz, rn, RN_MAX, chain_lig, n_chains))
+        return st;
     if (n_chains == 0) return 0;
     if (!chain_id || !cur || !cur_e || !best || !best_e) return fail(ADOCK_E_ARG, "null pointer");
     if (mc_steps < 0) return fail(ADOCK_E_ARG, "negative mc_steps");
     if (nbr && (!nbr_n || NB_MAX <= 0)) return fail(ADOCK_E_ARG, "nbr needs nbr_n and NB_MAX > 0");
-    LigArgs L{ref, typ, tm, ax, axp, axu, pr, na, nt, np, rg, P_MAX, gslot};
+    LigArgs L{ref, typ, tm, ax, axp, axu, pr, na, nt, np, rg, P_MAX, gslot, inc, ist};
     GridArgs G{grid, origin, lo, hi, rec_idx, goffs, gdims, T, spacing, pen};
     SFArgs S{prm, rad, hyd, don, acc};
     RecArgs R{rxyz, rtyp, rn, RN_MAX};
@@ -100,7 +105,8 @@ int adock_mc(int AM, int KM, LIG_DECL, GRID_DECL, SF_DECL, REC_DECL, int exact, 
              const int* chain_id, int n_chains, const int* nbr, const int* nbr_n, int NB_MAX, float* cur, float* cur_e,
              float* best, float* best_e, int init_opt, int mc_steps, int bfgs_iters, float temp, float amp_pos,
              float amp_rot, unsigned long long seed, int step_offset, adock_stream_t stream) {
-    return adock_mc_counted(AM, KM, ref, typ, tm, ax, axp, axu, pr, na, nt, np, rg, P_MAX, gslot, grid, origin, lo, hi,
+    return adock_mc_counted(AM, KM, ref, typ, tm, ax, axp, axu, pr, na, nt, np, rg, P_MAX, gslot, inc, ist, grid,
+                            origin, lo, hi,
                             rec_idx, goffs, gdims, T, spacing, pen, prm, rad, hyd, don, acc, rxyz, rtyp, rn, RN_MAX,
                             exact, chain_lig, chain_id, n_chains, nbr, nbr_n, NB_MAX, cur, cur_e, best, best_e,
                             init_opt, mc_steps, bfgs_iters, temp, amp_pos, amp_rot, seed, step_offset, nullptr,
@@ -113,13 +119,14 @@ int adock_mc_batch(int AM, int KM, LIG_DECL, GRID_DECL, SF_DECL, REC_DECL, int e
                    float amp_pos, float amp_rot, unsigned long long seed, int step_offset, const int* order,
                    int* counter, int* done, const int* lig_chains, unsigned long long* times, int blocks,
                    unsigned long long* eval_count, adock_stream_t stream) {
-    if (int st = common_checks(AM, KM, P_MAX, ref, grid, prm, exact, rxyz, rn, RN_MAX, chain_lig, n_chains)) return st;
+    if (int st = common_checks(AM, KM, P_MAX, ref, inc, ist, grid, prm, exact, rxyz, rn, RN_MAX, chain_lig, n_chains))
+        return st;
     if (n_chains == 0) return 0;
     if (!chain_id || !cur || !cur_e || !best || !best_e || !counter) return fail(ADOCK_E_ARG, "null pointer");
     if (done && !lig_chains) return fail(ADOCK_E_ARG, "done needs lig_chains");
     if (mc_steps < 0) return fail(ADOCK_E_ARG, "negative mc_steps");
     if (nbr && (!nbr_n || NB_MAX <= 0)) return fail(ADOCK_E_ARG, "nbr needs nbr_n and NB_MAX > 0");
-    LigArgs L{ref, typ, tm, ax, axp, axu, pr, na, nt, np, rg, P_MAX, gslot};
+    LigArgs L{ref, typ, tm, ax, axp, axu, pr, na, nt, np, rg, P_MAX, gslot, inc, ist};
     GridArgs G{grid, origin, lo, hi, rec_idx, goffs, gdims, T, spacing, pen};
     SFArgs S{prm, rad, hyd, don, acc};
     RecArgs R{rxyz, rtyp, rn, RN_MAX};
@@ -141,12 +148,13 @@ int adock_mc_step(int AM, int KM, LIG_DECL, GRID_DECL, SF_DECL, REC_DECL, const 
                   float temp, float amp_pos, float amp_rot, unsigned long long seed, const float* Y, float* feat,
                   float* log_buf, int* log_count, int log_cap, int use_net, float p_plain, float sig_t, float sig_r,
                   float sig_th, unsigned long long* eval_count, adock_stream_t stream) {
-    if (int st = common_checks(AM, KM, P_MAX, ref, grid, prm, 0, rxyz, rn, RN_MAX, chain_lig, n_chains)) return st;
+    if (int st = common_checks(AM, KM, P_MAX, ref, inc, ist, grid, prm, 0, rxyz, rn, RN_MAX, chain_lig, n_chains))
+        return st;
     if (n_chains == 0) return 0;
     if (!chain_id || !cur || !cur_e || !best || !best_e) return fail(ADOCK_E_ARG, "null pointer");
     if (log_buf && (!log_count || !feat || log_cap < 0)) return fail(ADOCK_E_ARG, "logging needs feat and a counter");
     if (use_net && (!Y || !feat)) return fail(ADOCK_E_ARG, "network moves need Y and feat");
-    LigArgs L{ref, typ, tm, ax, axp, axu, pr, na, nt, np, rg, P_MAX, gslot};
+    LigArgs L{ref, typ, tm, ax, axp, axu, pr, na, nt, np, rg, P_MAX, gslot, inc, ist};
     GridArgs G{grid, origin, lo, hi, rec_idx, goffs, gdims, T, spacing, pen};
     SFArgs S{prm, rad, hyd, don, acc};
     RecArgs R{rxyz, rtyp, rn, RN_MAX};
@@ -204,5 +212,11 @@ int adock_set_chains_per_block(int w) {
     return 0;
 }
 int adock_get_chains_per_block(void) { return get_waves_per_block(); }
+
+int adock_set_line_search_width(int ns) {
+    if (set_line_search_width(ns)) return fail(ADOCK_E_ARG, "line_search_width must be 0 (adaptive), 1, 2 or 4");
+    return 0;
+}
+int adock_get_line_search_width(void) { return get_line_search_width(); }
 
 }  // extern "C"

@@ -56,6 +56,23 @@ DEVI void wave_sum_multi(float (&v)[N]) {
     for (int i = 0; i < N; ++i) v[i] = bcast(v[i], 63);
 }
 
+template <int CTRL, int ROW_MASK = 0xf>
+DEVI float dpp_max_step(float v) {
+    const float o = __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(-INFINITY), __float_as_int(v), CTRL,
+                                                               ROW_MASK, 0xf, false));
+    return fmaxf(v, o);
+}
+// maximum over the 64 lanes (all active), uniform result
+DEVI float wave_max_f(float v) {
+    v = dpp_max_step<0x111>(v);
+    v = dpp_max_step<0x112>(v);
+    v = dpp_max_step<0x114>(v);
+    v = dpp_max_step<0x118>(v);
+    v = dpp_max_step<0x142, 0xa>(v);
+    v = dpp_max_step<0x143, 0xc>(v);
+    return bcast(v, 63);
+}
+
 // Orders this wave's LDS writes before its later LDS reads (one wave owns the region).
 DEVI void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

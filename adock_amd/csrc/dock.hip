@@ -7,10 +7,12 @@
 // MI355X mapping (one wave64 = one chain; W chains per workgroup, independent):
 //   * lane a holds heavy atom a (<= 64 atoms): its reference coordinates, radius, type flags, grid-slot pointer and
 //     the bit mask of torsions that move it live in registers for the whole chain;
-//   * lane 7+k computes sin/cos of torsion k once per evaluation; the forward-kinematics loop broadcasts them with
-//     v_readlane (scalar registers), and the reference axes come from scalar loads;
-//   * per-wave LDS holds world coordinates + radius (float4 per atom), type flags, the pair list (u16 i | j << 8) and
-//     the pair-force accumulators (ds_add_f32); pairs are strided over lanes;
+//   * lane 7+k computes sin/cos of torsion k once per evaluation and stores them in the torsion's LDS record next to
+//     its reference axis; forward kinematics and the torque projection read the records as LDS broadcasts (record
+//     k + 1 in flight while k is applied);
+//   * intra-ligand pairs are strided over lanes, two rounds of 64 per iteration, branch-free; each pair writes its
+//     gradient to its own LDS slot and every atom sums the slots of its pairs from a per-atom list built once per
+//     chain (no atomics, no cross-lane traffic in the pair loop);
 //   * every reduction (energy, 3 forces, 3 torques, one per torsion, BFGS dot products) is a DPP row scan + row
 //     broadcast, no LDS round trip;
 //   * BFGS vectors are lane-distributed (lane i = component i) and lane i keeps row i of the inverse Hessian in
@@ -29,17 +31,33 @@ namespace {
 
 constexpr int AMK = 64;   // heavy-atom bucket handled by these kernels (one atom per lane)
 
-// per-wave LDS: X4 [64] float4, F [3][64], flags [64], pair words [P64] u16, inverse lane table [P64] u16
-// (P64 = P_MAX rounded up to a multiple of 64)
-DEVI int pairs64(int P_MAX) { return (P_MAX + 63) & ~63; }
-DEVI int wave_lds_bytes(int P_MAX) { return 16 * AMK + 12 * AMK + 4 * AMK + 4 * pairs64(P_MAX); }
+// Per-wave LDS layout, in bytes (P128 = P_MAX rounded up to a multiple of 128, KM + 1 torsion records):
+//   X4  [64]        float4  world xyz + radius of each atom
+//   TA  [KM + 1]    float4  reference axis point of torsion k; .w = sin(theta_k) of the current evaluation
+//   TB  [KM + 1]    float4  reference axis unit vector;        .w = cos(theta_k)
+//   TC  [KM + 1]    float4  current (world) axis point
+//   TD  [KM + 1]    float4  current axis unit vector
+//   FS  [P128 + 1]  float4  gradient slot of pair q (dE/dX of its first atom); slot P128 stays zero
+//   flg [64]        int     atom type flags
+//   pl  [P128]      u16     pair words, strided over lanes (pair q on lane q % 64 in round q / 64)
+//   inc [2 P128 + 512] u16  per-atom pair-slot lists, copied from the host-built packed ligand (each atom's list
+//                           padded to 8 entries; start and padded count live in the atom's lane); bit 15 = negate
+__host__ __device__ inline int pad128(int P_MAX) { return (P_MAX + 127) & ~127; }
+__host__ __device__ inline int inc_stride(int P_MAX) { return 2 * pad128(P_MAX) + 512; }
+__host__ __device__ inline int wave_lds_bytes(int KM, int P_MAX) {
+    const int P = pad128(P_MAX);
+    return 16 * AMK + 4 * 16 * (KM + 1) + 16 * (P + 1) + 4 * AMK + 2 * P + 2 * inc_stride(P_MAX);
+}
+constexpr int LDS_PER_WORKGROUP = 64 * 1024;
 
 struct WaveLDS {
-    float4* X4;            // world xyz + radius
-    float* Fx; float* Fy; float* Fz;
+    float4* X4;
+    float4* TA; float4* TB; float4* TC; float4* TD;
+    float4* FS;
     int* flg;
     unsigned short* pl;
-    unsigned short* inv;   // [round][atom]: lane holding the atom as first (low byte) / second (high byte), 0xff = none
+    unsigned short* inc;
+    int P128;
 };
 
 struct ChainCtx {
@@ -49,30 +67,33 @@ struct ChainCtx {
     unsigned tmask;
     bool atom;
     const float* gp;
-    // lane = torsion: axis atoms and the reference axis (point, unit vector)
+    int is, ic8;          // this atom's slot list: inc[is .. is + ic8), padded to a multiple of 8
+    // lane = torsion: axis atoms
     int axa, axb;
-    float apx, apy, apz, aux, auy, auz;
     // uniform
-    const float* axp; const float* axu;
-    int na, nt, np;
+    int na, nt, np, np2, D8;
     float rg;
     int ny, nz;
     float o0, o1, o2, lo0, lo1, lo2, hi0, hi1, hi2, inv_s, pen, fmx0, fmx1, fmx2;
     int exact;
-    int sched;            // 1: pair list in conflict-free rounds of 64 (force pull through ds_bpermute, no atomics)
     const float* rrx; const int* rrt; int rn; const int* nb; int nbn;
     const float* sfrad; const int* sfhyd; const int* sfdon; const int* sfacc;
 };
 
+template <int KM>
 DEVI WaveLDS carve_lds(char* base, int P_MAX) {
     WaveLDS w;
-    w.X4 = reinterpret_cast<float4*>(base);
-    w.Fx = reinterpret_cast<float*>(base + 16 * AMK);
-    w.Fy = w.Fx + AMK;
-    w.Fz = w.Fy + AMK;
-    w.flg = reinterpret_cast<int*>(base + 28 * AMK);
-    w.pl = reinterpret_cast<unsigned short*>(base + 32 * AMK);
-    w.inv = w.pl + pairs64(P_MAX);
+    w.P128 = pad128(P_MAX);
+    float4* f = reinterpret_cast<float4*>(base);
+    w.X4 = f; f += AMK;
+    w.TA = f; f += KM + 1;
+    w.TB = f; f += KM + 1;
+    w.TC = f; f += KM + 1;
+    w.TD = f; f += KM + 1;
+    w.FS = f; f += w.P128 + 1;
+    w.flg = reinterpret_cast<int*>(f);
+    w.pl = reinterpret_cast<unsigned short*>(w.flg + AMK);
+    w.inc = w.pl + w.P128;
     return w;
 }
 
@@ -81,8 +102,9 @@ DEVI ChainCtx setup_chain(int l, int lane, const LigArgs& L, const GridArgs& G, 
                           int exact, const WaveLDS& w) {
     ChainCtx c;
     c.na = __builtin_amdgcn_readfirstlane(L.na[l]);
-    c.nt = __builtin_amdgcn_readfirstlane(L.nt[l]);
-    c.np = __builtin_amdgcn_readfirstlane(L.np[l]);
+    c.nt = min(__builtin_amdgcn_readfirstlane(L.nt[l]), KM);
+    c.np = min(__builtin_amdgcn_readfirstlane(L.np[l]), L.P_MAX);
+    c.np2 = (c.np + 127) & ~127;
     c.rg = L.rg[l];
     const int ri = __builtin_amdgcn_readfirstlane(G.rec_idx[l]);
     const int nx = (int)G.gdims[3 * ri];
@@ -93,8 +115,6 @@ DEVI ChainCtx setup_chain(int l, int lane, const LigArgs& L, const GridArgs& G, 
     c.inv_s = 1.f / G.spacing; c.pen = G.pen;
     c.fmx0 = (float)(nx - 1) - 1e-4f; c.fmx1 = (float)(c.ny - 1) - 1e-4f; c.fmx2 = (float)(c.nz - 1) - 1e-4f;
     const long long n3 = (long long)nx * c.ny * c.nz;
-    c.axp = L.axp + (long long)l * KM * 3;
-    c.axu = L.axu + (long long)l * KM * 3;
     c.atom = lane < c.na;
     c.rx = c.ry = c.rz = 0.f; c.rad = 0.f; c.flg = 0; c.tmask = 0u;
     c.gp = G.grid + G.goffs[ri];
@@ -108,45 +128,45 @@ DEVI ChainCtx setup_chain(int l, int lane, const LigArgs& L, const GridArgs& G, 
         for (int k = 0; k < c.nt; ++k)
             c.tmask |= (L.tm[((long long)l * KM + k) * AMK + lane] ? 1u : 0u) << k;
     }
+    // torsion records: reference axes (constant for the chain); current axes are rewritten every evaluation
     c.axa = c.axb = 0;
-    c.apx = c.apy = c.apz = c.aux = c.auy = c.auz = 0.f;
-    if (lane < c.nt) {
-        c.axa = L.ax[((long long)l * KM + lane) * 2];
-        c.axb = L.ax[((long long)l * KM + lane) * 2 + 1];
-        c.apx = c.axp[3 * lane]; c.apy = c.axp[3 * lane + 1]; c.apz = c.axp[3 * lane + 2];
-        c.aux = c.axu[3 * lane]; c.auy = c.axu[3 * lane + 1]; c.auz = c.axu[3 * lane + 2];
+    if (lane <= KM) {
+        float4 ta = make_float4(0.f, 0.f, 0.f, 0.f), tb = make_float4(0.f, 0.f, 0.f, 1.f);
+        if (lane < c.nt) {
+            const float* axp = L.axp + ((long long)l * KM + lane) * 3;
+            const float* axu = L.axu + ((long long)l * KM + lane) * 3;
+            c.axa = L.ax[((long long)l * KM + lane) * 2];
+            c.axb = L.ax[((long long)l * KM + lane) * 2 + 1];
+            ta = make_float4(axp[0], axp[1], axp[2], 0.f);
+            tb = make_float4(axu[0], axu[1], axu[2], 1.f);
+        }
+        w.TA[lane] = ta; w.TB[lane] = tb;
+        w.TC[lane] = make_float4(0.f, 0.f, 0.f, 0.f); w.TD[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
     w.flg[lane] = c.flg;
+    if (lane == 0) w.FS[w.P128] = make_float4(0.f, 0.f, 0.f, 0.f);
     wave_lds_sync();
-    // pair word: i | j << 6 | (hydrophobic pair) << 12 | (h-bond pair) << 13; 0x8000 marks a padding self-pair
+    // pair word: i | j << 6 | (hydrophobic pair) << 12 | (h-bond pair) << 13; 0x8000 marks padding (and self-pairs)
     const int* pr = L.pr + (long long)l * L.P_MAX * 2;
-    for (int q = lane; q < c.np; q += WAVE) {
-        const int i = pr[2 * q], j = pr[2 * q + 1];
-        w.pl[q] = (unsigned short)(i == j ? 0x8000 : (i | (j << 6) | (pair_flags(w.flg[i], w.flg[j]) << 12)));
-    }
-    // inverse lane tables of a scheduled pair list, and the check that makes the atomic-free path safe: within every
-    // round each atom may hold at most one first and one second slot
-    const int rounds = (c.np + 63) >> 6;
-    unsigned char* inv8 = reinterpret_cast<unsigned char*>(w.inv);
-    for (int r = 0; r < rounds; ++r) w.inv[r * 64 + lane] = 0xffff;
-    wave_lds_sync();
-    for (int r = 0; r < rounds; ++r) {
-        const int q = r * 64 + lane;
-        const unsigned pw = q < c.np ? w.pl[q] : 0x8000u;
-        if (!(pw & 0x8000u)) {
-            inv8[(r * 64 + (pw & 63)) * 2] = (unsigned char)lane;
-            inv8[(r * 64 + ((pw >> 6) & 63)) * 2 + 1] = (unsigned char)lane;
+    for (int q = lane; q < w.P128; q += WAVE) {
+        unsigned short word = 0x8000;
+        if (q < c.np) {
+            const int i = pr[2 * q], j = pr[2 * q + 1];
+            if (i != j) word = (unsigned short)(i | (j << 6) | (pair_flags(w.flg[i], w.flg[j]) << 12));
         }
+        w.pl[q] = word;
     }
     wave_lds_sync();
-    bool mine = true;
-    for (int r = 0; r < rounds; ++r) {
-        const int q = r * 64 + lane;
-        const unsigned pw = q < c.np ? w.pl[q] : 0x8000u;
-        if (!(pw & 0x8000u))
-            mine = mine && inv8[(r * 64 + (pw & 63)) * 2] == lane && inv8[(r * 64 + ((pw >> 6) & 63)) * 2 + 1] == lane;
+    // this atom's pair-slot list (host-built, 8 entries per 16-byte copy)
+    {
+        const int e = c.atom ? L.ist[(long long)l * AMK + lane] : 0;
+        c.is = e & 0xffff;
+        c.ic8 = (e >> 16) & 0xffff;
+        const uint4* src = reinterpret_cast<const uint4*>(L.inc + (long long)l * inc_stride(L.P_MAX) + c.is);
+        uint4* dst = reinterpret_cast<uint4*>(w.inc + c.is);
+        for (int g = 0; g < (c.ic8 >> 3); ++g) dst[g] = src[g];
+        c.D8 = (int)wave_max_f((float)c.ic8);
     }
-    c.sched = __ballot(!mine) == 0ull ? 1 : 0;
     c.exact = exact;
     c.rrx = R.rxyz + (long long)ri * R.RN_MAX * 3;
     c.rrt = R.rtyp + (long long)ri * R.RN_MAX;
@@ -155,6 +175,21 @@ DEVI ChainCtx setup_chain(int l, int lane, const LigArgs& L, const GridArgs& G, 
     c.sfrad = S.rad; c.sfhyd = S.hyd; c.sfdon = S.don; c.sfacc = S.acc;
     wave_lds_sync();
     return c;
+}
+
+// energy of pair word pw (0 for padding) and its gradient slot (dE/dX of the first atom; the second gets minus it)
+DEVI float pair_slot(const WaveLDS& w, unsigned pw, const SFConst& sf, float4& s) {
+    const float valid = (pw & 0x8000u) ? 0.f : 1.f;
+    const int i = pw & 63, j = (pw >> 6) & 63;
+    const float4 xi = w.X4[i], xj = w.X4[j];
+    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+    const float rr = fast_sqrt(rx * rx + ry * ry + rz * rz);
+    float de;
+    const float e = pair_e_nb(rr - xi.w - xj.w, (int)((pw >> 12) & 3u), sf, de);
+    const float in = rr < sf.cut ? valid : 0.f;
+    const float sc = in * de * fast_rcp(fmaxf(rr, 1e-6f));
+    s = make_float4(sc * rx, sc * ry, sc * rz, 0.f);
+    return in * e;
 }
 
 // Energy and lane-distributed gradient of the lane-distributed state x. Returns E (uniform).
@@ -168,27 +203,40 @@ DEVI float eval_eg(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, int l
     const float R00 = 1.f - 2.f * (qy * qy + qz * qz), R01 = 2.f * (qx * qy - qz * qw), R02 = 2.f * (qx * qz + qy * qw);
     const float R10 = 2.f * (qx * qy + qz * qw), R11 = 1.f - 2.f * (qx * qx + qz * qz), R12 = 2.f * (qy * qz - qx * qw);
     const float R20 = 2.f * (qx * qz - qy * qw), R21 = 2.f * (qy * qz + qx * qw), R22 = 1.f - 2.f * (qx * qx + qy * qy);
-    float sn, cs;
-    sincosf(x, &sn, &cs);   // meaningful on lanes 7 .. 6+nt
-    // forward kinematics, leaf-to-root about the reference axes
+    // lane 7 + k: sin / cos of torsion k into its record
+    {
+        float sn, cs;
+        sincosf(x, &sn, &cs);
+        const int tk = lane - 7;
+        if (tk >= 0 && tk < c.nt) {
+            reinterpret_cast<float*>(w.TA + tk)[3] = sn;
+            reinterpret_cast<float*>(w.TB + tk)[3] = cs;
+        }
+    }
+    wave_lds_sync();
+    // forward kinematics, leaf-to-root about the reference axes: records read 4 at a time, the rotation applied where
+    // the atom's mask bit is set (records past nt are identities)
     float px = c.rx, py = c.ry, pz = c.rz;
-    for (int k = 0; k < c.nt; ++k) {
-        const float sk = bcast(sn, 7 + k), ck = bcast(cs, 7 + k);
-        const float ax0 = bcast(c.apx, k), ax1 = bcast(c.apy, k), ax2 = bcast(c.apz, k);
-        const float ux = bcast(c.aux, k), uy = bcast(c.auy, k), uz = bcast(c.auz, k);
-        if ((c.tmask >> k) & 1u) {
-            const float vx = px - ax0, vy = py - ax1, vz = pz - ax2;
-            const float d = (ux * vx + uy * vy + uz * vz) * (1.f - ck);
-            const float cx = uy * vz - uz * vy, cy = uz * vx - ux * vz, cz = ux * vy - uy * vx;
-            px = ax0 + vx * ck + cx * sk + ux * d;
-            py = ax1 + vy * ck + cy * sk + uy * d;
-            pz = ax2 + vz * ck + cz * sk + uz * d;
+    for (int kb = 0; kb < c.nt; kb += 4) {
+        float4 a[4], b[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) { a[u] = w.TA[kb + u]; b[u] = w.TB[kb + u]; }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const bool m = (c.tmask >> (kb + u)) & 1u;
+            const float vx = px - a[u].x, vy = py - a[u].y, vz = pz - a[u].z;
+            const float d = (b[u].x * vx + b[u].y * vy + b[u].z * vz) * (1.f - b[u].w);
+            const float cx = b[u].y * vz - b[u].z * vy, cy = b[u].z * vx - b[u].x * vz, cz = b[u].x * vy - b[u].y * vx;
+            const float nx = a[u].x + vx * b[u].w + cx * a[u].w + b[u].x * d;
+            const float ny = a[u].y + vy * b[u].w + cy * a[u].w + b[u].y * d;
+            const float nz = a[u].z + vz * b[u].w + cz * a[u].w + b[u].z * d;
+            px = m ? nx : px; py = m ? ny : py; pz = m ? nz : pz;
         }
     }
     const float X0 = R00 * px + R01 * py + R02 * pz + t0;
     const float X1 = R10 * px + R11 * py + R12 * pz + t1;
     const float X2 = R20 * px + R21 * py + R22 * pz + t2;
-    // grid corners: issue the 8 gathers now, consume them after the pair loop (latency overlaps the LDS work)
+    // grid corners: issue the 8 gathers now, consume them after the pair work (latency overlaps the LDS work)
     float el = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
     float v000 = 0.f, v001 = 0.f, v010 = 0.f, v011 = 0.f, v100 = 0.f, v101 = 0.f, v110 = 0.f, v111 = 0.f;
     float wx = 0.f, wy = 0.f, wz = 0.f, in0 = 0.f, in1 = 0.f, in2 = 0.f;
@@ -209,56 +257,45 @@ DEVI float eval_eg(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, int l
         v100 = G[bx]; v101 = G[bx + 1]; v110 = G[bx + c.nz]; v111 = G[bx + c.nz + 1];
     }
     w.X4[lane] = make_float4(X0, X1, X2, c.rad);
-    w.Fx[lane] = 0.f; w.Fy[lane] = 0.f; w.Fz[lane] = 0.f;
     wave_lds_sync();
-    // intra-ligand pairs. Scheduled list: round r puts pair 64 r + lane on this lane; the atom on lane a pulls the
-    // forces of the (at most two) lanes that hold it through ds_bpermute and accumulates them in registers. Any other
-    // list: pairs strided over lanes with LDS float atomics into F. Pair terms are branch-free.
-    float ep_l = 0.f, pfx = 0.f, pfy = 0.f, pfz = 0.f;
-    if (c.sched) {
-        const int rounds = (c.np + 63) >> 6;
-        for (int r = 0; r < rounds; ++r) {
-            const int q = r * 64 + lane;
-            const unsigned pw = q < c.np ? w.pl[q] : 0x8000u;
-            const unsigned iv = w.inv[q];
-            float sx = 0.f, sy = 0.f, sz = 0.f;
-            if (!(pw & 0x8000u)) {
-                const int i = pw & 63, j = (pw >> 6) & 63;
-                const float4 xi = w.X4[i], xj = w.X4[j];
-                const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-                const float rr = fast_sqrt(rx * rx + ry * ry + rz * rz);
-                float de;
-                const float e = pair_e_nb(rr - xi.w - xj.w, (int)(pw >> 12), sf, de);
-                const float in = rr < sf.cut ? 1.f : 0.f;
-                ep_l += in * e;
-                const float sc = in * de * fast_rcp(fmaxf(rr, 1e-6f));
-                sx = sc * rx; sy = sc * ry; sz = sc * rz;
+    // current torsion axes (lane k < nt) from the world coordinates of the axis atoms
+    if (lane < c.nt) {
+        const float4 A = w.X4[c.axa], B = w.X4[c.axb];
+        const float ux = B.x - A.x, uy = B.y - A.y, uz = B.z - A.z;
+        const float nn = fast_rsq(fmaxf(ux * ux + uy * uy + uz * uz, 1e-12f));
+        w.TC[lane] = make_float4(A.x, A.y, A.z, 0.f);
+        w.TD[lane] = make_float4(ux * nn, uy * nn, uz * nn, 0.f);
+    }
+    // intra-ligand pairs, two rounds of 64 per iteration, branch-free: each pair writes its gradient slot
+    float ep_l = 0.f;
+    for (int q0 = lane; q0 < c.np2; q0 += 2 * WAVE) {
+        const unsigned pw0 = w.pl[q0], pw1 = w.pl[q0 + WAVE];
+        float4 s0, s1;
+        ep_l += pair_slot(w, pw0, sf, s0);
+        ep_l += pair_slot(w, pw1, sf, s1);
+        w.FS[q0] = s0;
+        w.FS[q0 + WAVE] = s1;
+    }
+    wave_lds_sync();
+    // each atom sums its slots: 8 list entries per 16-byte read, then 8 independent slot reads in flight (padding
+    // entries and lanes past their list read the zero slot)
+    float pfx = 0.f, pfy = 0.f, pfz = 0.f;
+    {
+        const uint4* inc4 = reinterpret_cast<const uint4*>(w.inc);
+        for (int k = 0; k < c.D8; k += 8) {
+            const bool ok = k < c.ic8;
+            const uint4 v = inc4[ok ? (c.is + k) >> 3 : 0];
+            const unsigned zs = (unsigned)w.P128;
+            const unsigned ws[8] = {ok ? v.x & 0xffffu : zs, ok ? v.x >> 16 : zs, ok ? v.y & 0xffffu : zs,
+                                    ok ? v.y >> 16 : zs,     ok ? v.z & 0xffffu : zs, ok ? v.z >> 16 : zs,
+                                    ok ? v.w & 0xffffu : zs, ok ? v.w >> 16 : zs};
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const float4 f = w.FS[ws[u] & 0x7fffu];
+                const float sg = (ws[u] & 0x8000u) ? -1.f : 1.f;
+                pfx += sg * f.x; pfy += sg * f.y; pfz += sg * f.z;
             }
-            const int li = iv & 0xff, lj = iv >> 8;
-            const float mi = li != 0xff ? 1.f : 0.f, mj = lj != 0xff ? 1.f : 0.f;
-            const int si = li & 63, sj = lj & 63;
-            pfx += mi * __shfl(sx, si) - mj * __shfl(sx, sj);
-            pfy += mi * __shfl(sy, si) - mj * __shfl(sy, sj);
-            pfz += mi * __shfl(sz, si) - mj * __shfl(sz, sj);
         }
-    } else {
-        for (int q = lane; q < c.np; q += WAVE) {
-            const unsigned pw = w.pl[q];
-            if (pw & 0x8000u) continue;
-            const int i = pw & 63, j = (pw >> 6) & 63;
-            const float4 xi = w.X4[i], xj = w.X4[j];
-            const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-            const float rr = fast_sqrt(rx * rx + ry * ry + rz * rz);
-            float de;
-            const float e = pair_e_nb(rr - xi.w - xj.w, (int)(pw >> 12), sf, de);
-            const float in = rr < sf.cut ? 1.f : 0.f;
-            ep_l += in * e;
-            const float sc = in * de * fast_rcp(fmaxf(rr, 1e-6f));
-            atomicAdd(&w.Fx[i], sc * rx); atomicAdd(&w.Fy[i], sc * ry); atomicAdd(&w.Fz[i], sc * rz);
-            atomicAdd(&w.Fx[j], -sc * rx); atomicAdd(&w.Fy[j], -sc * ry); atomicAdd(&w.Fz[j], -sc * rz);
-        }
-        wave_lds_sync();
-        pfx = w.Fx[lane]; pfy = w.Fy[lane]; pfz = w.Fz[lane];
     }
     if (c.atom) {
         if (!c.exact) {
@@ -298,34 +335,28 @@ DEVI float eval_eg(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, int l
     Xx = X0; Xy = X1; Xz = X2;
     el += ep_l;
     const float fx = gx + pfx, fy = gy + pfy, fz = gz + pfz;
-    // current torsion axes (lane k < nt) from the world coordinates of the axis atoms
-    float pwx = 0.f, pwy = 0.f, pwz = 0.f, uwx = 0.f, uwy = 0.f, uwz = 0.f;
-    if (lane < c.nt) {
-        const float4 A = w.X4[c.axa], B = w.X4[c.axb];
-        const float ux = B.x - A.x, uy = B.y - A.y, uz = B.z - A.z;
-        const float nn = fast_rsq(fmaxf(ux * ux + uy * uy + uz * uz, 1e-12f));
-        uwx = ux * nn; uwy = uy * nn; uwz = uz * nn;
-        pwx = A.x; pwy = A.y; pwz = A.z;
-    }
-    // every gradient component and the energy are reduced over the wave together, so the DPP steps of the 8 + KM
+    // every gradient component and the energy are reduced over the wave together, so the DPP steps of the 7 + KM
     // independent sums interleave instead of running one dependent chain after another
     const float dX = X0 - t0, dY = X1 - t1, dZ = X2 - t2;
     float v[7 + KM];
     v[0] = fx; v[1] = fy; v[2] = fz;
     v[3] = dY * fz - dZ * fy; v[4] = dZ * fx - dX * fz; v[5] = dX * fy - dY * fx;
     v[6] = el;
+    // torque about each current axis from the atoms it moves; records in groups of 8 (records past nt are zero)
 #pragma unroll
-    for (int k = 0; k < KM; ++k) {
-        float tv = 0.f;
-        if (k < c.nt) {
-            const float pxk = bcast(pwx, k), pyk = bcast(pwy, k), pzk = bcast(pwz, k);
-            const float uxk = bcast(uwx, k), uyk = bcast(uwy, k), uzk = bcast(uwz, k);
-            if ((c.tmask >> k) & 1u) {
-                const float vx = X0 - pxk, vy = X1 - pyk, vz = X2 - pzk;
-                tv = (vy * fz - vz * fy) * uxk + (vz * fx - vx * fz) * uyk + (vx * fy - vy * fx) * uzk;
+    for (int kb = 0; kb < KM; kb += 8) {
+        if (kb < c.nt) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const float4 cp = w.TC[kb + u], cu = w.TD[kb + u];
+                const float mk = ((c.tmask >> (kb + u)) & 1u) ? 1.f : 0.f;
+                const float vx = X0 - cp.x, vy = X1 - cp.y, vz = X2 - cp.z;
+                v[7 + kb + u] = mk * ((vy * fz - vz * fy) * cu.x + (vz * fx - vx * fz) * cu.y + (vx * fy - vy * fx) * cu.z);
             }
+        } else {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[7 + kb + u] = 0.f;
         }
-        v[7 + k] = tv;
     }
     wave_sum_multi<7 + KM>(v);
     float g = 0.f;
@@ -435,10 +466,114 @@ DEVI float bfgs(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, int lane
     return f0;
 }
 
-// One MC chain on the calling wave (all 64 lanes active). w: this wave's LDS region.
-template <int KM>
+// ---- speculative line search: the NS waves of one workgroup share one chain ----
+// Every wave holds the same chain state and runs the same BFGS bookkeeping (identical results). At each line search
+// the waves evaluate the trials t = tb + wave (step 2^-t) of one batch in parallel and all take the first one that
+// passes the Armijo test (after ten failures: trial 9 with the step halved once more), so the accepted points,
+// energies and evaluation counts equal the one-wave search bit for bit while a step costs about one evaluation of
+// latency instead of two. Worth it when the chains leave SIMDs idle (a few hundred chains on 256 CUs).
+struct SpecLDS { float* x; float* g; float* fe; int* ok; };   // x [64], g [64], fe [4], ok [4]
+constexpr int SPEC_LDS_BYTES = 4 * (64 + 64 + 4 + 4);
+
+DEVI SpecLDS carve_spec(char* base) {
+    float* f = reinterpret_cast<float*>(base);
+    return SpecLDS{f, f + 64, f + 128, reinterpret_cast<int*>(f + 132)};
+}
+
+template <int KM, int NS>
+DEVI float bfgs_spec(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, const SpecLDS& xs, int lane, int wid,
+                     float& x, int iters, int& nev) {
+    constexpr int DV = 7 + KM;
+    float h[DV];
+#pragma unroll
+    for (int j = 0; j < DV; ++j) h[j] = (j == lane) ? 1.f : 0.f;
+    const float xo = x;
+    float g, ea_, x0_, x1_, x2_;
+    float f0 = eval_eg<KM>(c, sf, w, lane, x, g, ea_, x0_, x1_, x2_);
+    ++nev;
+    const float f_orig = f0;
+    int step = 0;
+    while (iters > 0) {
+        float s = 0.f;
+#pragma unroll
+        for (int j = 0; j < DV; ++j) s += h[j] * bcast(g, j);
+        const float p = -s;
+        const float pg = wave_sum(p * g);
+        float xa = x, ga = 0.f, fa = 0.f, alpha = 1.f;
+        for (int tb = 0; tb < 10; tb += NS) {
+            const int t = tb + wid;
+            bool ok = false;
+            float xe = x, fe = 0.f, ge = 0.f;
+            if (t < 10) {
+                const float at = ldexpf(1.f, -t);
+                xe = conf_inc(x, p, at, lane);
+                fe = eval_eg<KM>(c, sf, w, lane, xe, ge, ea_, x0_, x1_, x2_);
+                ok = fe - f0 < 1e-4f * at * pg;
+            }
+            if (lane == 0) xs.ok[wid] = ok ? 1 : 0;
+            __syncthreads();
+            int first = -1;
+#pragma unroll
+            for (int u = NS - 1; u >= 0; --u) first = xs.ok[u] ? u : first;
+            first = __builtin_amdgcn_readfirstlane(first);
+            const int win = first >= 0 ? first : (tb + NS >= 10 ? 9 - tb : -1);
+            if (win < 0) {
+                nev += NS;
+                __syncthreads();
+                continue;
+            }
+            nev += win + 1;
+            if (wid == win) {
+                xs.x[lane] = xe; xs.g[lane] = ge;
+                if (lane == 0) xs.fe[0] = fe;
+            }
+            __syncthreads();
+            xa = xs.x[lane]; ga = xs.g[lane]; fa = xs.fe[0];
+            alpha = ldexpf(1.f, -(tb + win));
+            if (first < 0) alpha *= 0.5f;
+            __syncthreads();
+            break;
+        }
+        const float y = ga - g;
+        f0 = fa; x = xa; g = ga;
+        const float gg = wave_sum(g * g);
+        if (!(sqrtf(gg) >= 1e-5f)) break;
+        const float yp = wave_sum(y * p);
+        if (step == 0) {
+            const float yy = wave_sum(y * y);
+            if (fabsf(yy) > EPSF) {
+                const float dg = alpha * yp / yy;
+#pragma unroll
+                for (int j = 0; j < DV; ++j) h[j] = (j == lane) ? dg : 0.f;
+            }
+        }
+        if (!(alpha * yp < EPSF)) {
+            float mhy = 0.f;
+#pragma unroll
+            for (int j = 0; j < DV; ++j) mhy += h[j] * bcast(y, j);
+            mhy = -mhy;
+            const float yhy = -wave_sum(y * mhy);
+            const float r = 1.f / (alpha * yp);
+            const float c2 = alpha * alpha * (r * r * yhy + r);
+            const float ar = alpha * r;
+#pragma unroll
+            for (int j = 0; j < DV; ++j) {
+                const float pj = bcast(p, j), mj = bcast(mhy, j);
+                h[j] += ar * (mhy * pj + mj * p) + c2 * p * pj;
+            }
+        }
+        if (++step >= iters) break;
+    }
+    if (!(f0 <= f_orig)) { f0 = f_orig; x = xo; }
+    return f0;
+}
+
+// One MC chain on the calling wave (all 64 lanes active), or on the NS waves of a workgroup with the speculative line
+// search (wave wid; wave 0 writes the results). w: this wave's LDS region.
+template <int KM, int NS = 1>
 DEVI int run_chain(int ch, int lane, const WaveLDS& w, const LigArgs& L, const GridArgs& G, const SFArgs& S,
-                   const RecArgs& R, int exact, const int* __restrict__ chain_lig, const McArgs& M) {
+                   const RecArgs& R, int exact, const int* __restrict__ chain_lig, const McArgs& M, int wid = 0,
+                   SpecLDS xs = SpecLDS{}) {
     constexpr int SD = 7 + KM;
     const SFConst sf = make_sf(S.prm);
     const int l = __builtin_amdgcn_readfirstlane(chain_lig[ch]);
@@ -464,7 +599,9 @@ DEVI int run_chain(int ch, int lane, const WaveLDS& w, const LigArgs& L, const G
             sg = M.step_offset + st;
             x = plain_move(x, c, lane, seed, cid, sg, M.amp_pos, M.amp_rot);
         }
-        const float en = bfgs<KM>(c, sf, w, lane, x, iters, nev);
+        float en;
+        if constexpr (NS == 1) en = bfgs<KM>(c, sf, w, lane, x, iters, nev);
+        else en = bfgs_spec<KM, NS>(c, sf, w, xs, lane, wid, x, iters, nev);
         if (st < 0) {
             ec = en; cur = x;
             if (ec < eb) { eb = ec; best = cur; }
@@ -476,10 +613,12 @@ DEVI int run_chain(int ch, int lane, const WaveLDS& w, const LigArgs& L, const G
             }
         }
     }
-    if (lane < S_) { gcur[lane] = cur; gbest[lane] = best; }
-    if (lane == 0) {
-        M.cur_e[ch] = ec; M.best_e[ch] = eb;
-        if (M.eval_count) atomicAdd(M.eval_count, (unsigned long long)nev);
+    if (wid == 0) {
+        if (lane < S_) { gcur[lane] = cur; gbest[lane] = best; }
+        if (lane == 0) {
+            M.cur_e[ch] = ec; M.best_e[ch] = eb;
+            if (M.eval_count) atomicAdd(M.eval_count, (unsigned long long)nev);
+        }
     }
     return l;
 }
@@ -492,8 +631,22 @@ __global__ __launch_bounds__(256) void mc_kernel(LigArgs L, GridArgs G, SFArgs S
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
     const int ch = blockIdx.x * (blockDim.x / WAVE) + wid;
     if (ch >= n_chains) return;
-    const WaveLDS w = carve_lds(lds + wid * wave_lds_bytes(L.P_MAX), L.P_MAX);
+    const WaveLDS w = carve_lds<KM>(lds + wid * wave_lds_bytes(KM, L.P_MAX), L.P_MAX);
     run_chain<KM>(ch, lane, w, L, G, S, R, exact, chain_lig, M);
+}
+
+// One chain per workgroup of NS waves (speculative line search); grid = n_chains.
+template <int KM, int NS>
+__global__ __launch_bounds__(256) void mc_spec_kernel(LigArgs L, GridArgs G, SFArgs S, RecArgs R, int exact,
+                                                       const int* __restrict__ chain_lig, int n_chains, McArgs M) {
+    extern __shared__ __attribute__((aligned(16))) char lds[];
+    const int lane = lane_id();
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+    const int ch = blockIdx.x;
+    if (ch >= n_chains) return;
+    const int wb = wave_lds_bytes(KM, L.P_MAX);
+    const WaveLDS w = carve_lds<KM>(lds + wid * wb, L.P_MAX);
+    run_chain<KM, NS>(ch, lane, w, L, G, S, R, exact, chain_lig, M, wid, carve_spec(lds + NS * wb));
 }
 
 // Persistent variant: a grid sized to the chip's resident capacity; every wave pulls chain indices from a global
@@ -506,7 +659,7 @@ __global__ __launch_bounds__(256) void mc_persistent_kernel(LigArgs L, GridArgs 
     extern __shared__ __attribute__((aligned(16))) char lds[];
     const int lane = lane_id();
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
-    const WaveLDS w = carve_lds(lds + wid * wave_lds_bytes(L.P_MAX), L.P_MAX);
+    const WaveLDS w = carve_lds<KM>(lds + wid * wave_lds_bytes(KM, L.P_MAX), L.P_MAX);
     if (Q.times && lane == 0) atomicMin(Q.times, (unsigned long long)__builtin_amdgcn_s_memrealtime());
     while (true) {
         int idx = 0;
@@ -533,7 +686,7 @@ __global__ __launch_bounds__(256) void eg_kernel(LigArgs L, GridArgs G, SFArgs S
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
     const int ch = blockIdx.x * (blockDim.x / WAVE) + wid;
     if (ch >= n_chains) return;
-    const WaveLDS w = carve_lds(lds + wid * wave_lds_bytes(L.P_MAX), L.P_MAX);
+    const WaveLDS w = carve_lds<KM>(lds + wid * wave_lds_bytes(KM, L.P_MAX), L.P_MAX);
     const SFConst sf = make_sf(S.prm);
     const ChainCtx c = setup_chain<KM>(__builtin_amdgcn_readfirstlane(chain_lig[ch]), lane, L, G, S, R, exact, w);
     const float x = lane < SD ? state[(long long)ch * SD + lane] : 0.f;
@@ -544,11 +697,7 @@ __global__ __launch_bounds__(256) void eg_kernel(LigArgs L, GridArgs G, SFArgs S
     if (lane == 0) e_out[ch] = e;
 }
 
-int g_waves_per_block = 0;   // 0: adaptive (auto_waves)
-
-// chains per workgroup when not set explicitly: spread small launches one wave per CU (latency), pack large ones 4
-// per workgroup (a workgroup's waves share the CU's LDS and instruction cache)
-inline int auto_waves(int n_chains) {
+inline int cu_count() {
     static int cus = 0;
     if (!cus) {
         int dev = 0;
@@ -556,18 +705,53 @@ inline int auto_waves(int n_chains) {
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         if (cus <= 0) cus = 256;
     }
+    return cus;
+}
+
+int g_waves_per_block = 0;   // 0: adaptive (auto_waves)
+
+// chains per workgroup when not set explicitly: spread small launches one wave per CU (latency), pack large ones 4
+// per workgroup (a workgroup's waves share the CU's LDS and instruction cache)
+inline int auto_waves(int n_chains) {
+    const int cus = cu_count();
     const int w = n_chains / (4 * cus);
     return w >= 4 ? 4 : w >= 2 ? 2 : 1;
 }
 inline int waves_for(int n_chains) { return g_waves_per_block > 0 ? g_waves_per_block : auto_waves(n_chains); }
 
-inline int lds_bytes_per_wave(int P_MAX) { return 16 * AMK + 12 * AMK + 4 * AMK + 4 * ((P_MAX + 63) & ~63); }
+// chains per workgroup actually launched: the requested count, lowered until the workgroup's LDS fits
+inline int fit_waves(int W, int KM, int P_MAX) {
+    while (W > 1 && (size_t)W * wave_lds_bytes(KM, P_MAX) > (size_t)LDS_PER_WORKGROUP) --W;
+    return W;
+}
+
+int g_line_search_width = 0;   // 0: adaptive (line_search_width)
+
+// waves per chain of the speculative line search: 4 while the chains fit one per CU, 2 up to two per CU, else 1;
+// lowered until the workgroup's LDS fits
+inline int line_search_width(int n_chains, int KM, int P_MAX) {
+    int ns = g_line_search_width;
+    if (ns <= 0) ns = n_chains <= cu_count() ? 4 : n_chains <= 2 * cu_count() ? 2 : 1;
+    while (ns > 1 && (size_t)ns * wave_lds_bytes(KM, P_MAX) + SPEC_LDS_BYTES > (size_t)LDS_PER_WORKGROUP) ns >>= 1;
+    return ns;
+}
 
 template <int KM>
 int launch_mc_t(const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecArgs& R, int exact,
                 const int* chain_lig, int n_chains, const McArgs& M, hipStream_t stream) {
-    const int W = waves_for(n_chains);
-    const size_t shm = (size_t)W * lds_bytes_per_wave(L.P_MAX);
+    const int ns = line_search_width(n_chains, KM, L.P_MAX);
+    if (ns > 1) {
+        const size_t shm = (size_t)ns * wave_lds_bytes(KM, L.P_MAX) + SPEC_LDS_BYTES;
+        if (ns == 4)
+            hipLaunchKernelGGL((mc_spec_kernel<KM, 4>), dim3(n_chains), dim3(4 * WAVE), shm, stream, L, G, S, R, exact,
+                               chain_lig, n_chains, M);
+        else
+            hipLaunchKernelGGL((mc_spec_kernel<KM, 2>), dim3(n_chains), dim3(2 * WAVE), shm, stream, L, G, S, R, exact,
+                               chain_lig, n_chains, M);
+        return (int)hipGetLastError();
+    }
+    const int W = fit_waves(waves_for(n_chains), KM, L.P_MAX);
+    const size_t shm = (size_t)W * wave_lds_bytes(KM, L.P_MAX);
     const unsigned nblk = (unsigned)((n_chains + W - 1) / W);
     hipLaunchKernelGGL(mc_kernel<KM>, dim3(nblk), dim3(W * WAVE), shm, stream, L, G, S, R, exact, chain_lig,
                        n_chains, M);
@@ -578,8 +762,8 @@ template <int KM>
 int launch_mc_persistent_t(const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecArgs& R, int exact,
                            const int* chain_lig, int n_chains, const McArgs& M, const QueueArgs& Q, int blocks,
                            hipStream_t stream) {
-    const int W = g_waves_per_block > 0 ? g_waves_per_block : 4;
-    const size_t shm = (size_t)W * lds_bytes_per_wave(L.P_MAX);
+    const int W = fit_waves(g_waves_per_block > 0 ? g_waves_per_block : 4, KM, L.P_MAX);
+    const size_t shm = (size_t)W * wave_lds_bytes(KM, L.P_MAX);
     if (blocks <= 0) {
         // resident capacity, queried once per (device, W, LDS size) so later launches (and graph captures) only launch
         static std::mutex mu;
@@ -608,8 +792,8 @@ template <int KM>
 int launch_eg_t(const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecArgs& R, int exact,
                 const int* chain_lig, int n_chains, const float* state, float* e_out, float* g_out,
                 hipStream_t stream) {
-    const int W = waves_for(n_chains);
-    const size_t shm = (size_t)W * lds_bytes_per_wave(L.P_MAX);
+    const int W = fit_waves(waves_for(n_chains), KM, L.P_MAX);
+    const size_t shm = (size_t)W * wave_lds_bytes(KM, L.P_MAX);
     const unsigned nblk = (unsigned)((n_chains + W - 1) / W);
     hipLaunchKernelGGL(eg_kernel<KM>, dim3(nblk), dim3(W * WAVE), shm, stream, L, G, S, R, exact, chain_lig,
                        n_chains, state, e_out, g_out);
@@ -619,22 +803,6 @@ int launch_eg_t(const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecA
 
 constexpr int NFEAT = 128, NOUT = 23, LOG_W = NFEAT + 2 * NOUT + 1;
 
-template <int CTRL, int ROW_MASK = 0xf>
-DEVI float dpp_max_step(float v) {
-    const float o = __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(-INFINITY), __float_as_int(v), CTRL,
-                                                               ROW_MASK, 0xf, false));
-    return fmaxf(v, o);
-}
-// maximum over the 64 lanes (all active), uniform result
-DEVI float wave_max(float v) {
-    v = dpp_max_step<0x111>(v);
-    v = dpp_max_step<0x112>(v);
-    v = dpp_max_step<0x114>(v);
-    v = dpp_max_step<0x118>(v);
-    v = dpp_max_step<0x142, 0xa>(v);
-    v = dpp_max_step<0x143, 0xc>(v);
-    return bcast(v, 63);
-}
 DEVI int wave_count(bool p) { return __popcll(__ballot(p)); }
 
 // standard normal from two counter-RNG draws (Box-Muller), per lane
@@ -687,7 +855,7 @@ DEVI void write_features(float* f, const ChainCtx& c, int lane, float x, float E
         const bool in = c.atom && oct == o;
         const int cnt = wave_count(in);
         const float sm = wave_sum(in ? ea : 0.f);
-        const float mx = wave_max(in ? ea : -INFINITY);
+        const float mx = wave_max_f(in ? ea : -INFINITY);
         if (lane == 16 + o) hi_v = cnt ? tanhf(sm / (float)cnt) : 0.f;
         if (lane == 24 + o) hi_v = cnt ? tanhf(mx) : 0.f;
     }
@@ -769,7 +937,7 @@ __global__ __launch_bounds__(256) void mc_step_kernel(LigArgs L, GridArgs G, SFA
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
     const int ch = blockIdx.x * (blockDim.x / WAVE) + wid;
     if (ch >= n_chains) return;
-    const WaveLDS w = carve_lds(lds + wid * wave_lds_bytes(L.P_MAX), L.P_MAX);
+    const WaveLDS w = carve_lds<KM>(lds + wid * wave_lds_bytes(KM, L.P_MAX), L.P_MAX);
     const SFConst sf = make_sf(S.prm);
     const int l = __builtin_amdgcn_readfirstlane(chain_lig[ch]);
     const ChainCtx c = setup_chain<KM>(l, lane, L, G, S, R, 0, w);
@@ -836,8 +1004,8 @@ __global__ __launch_bounds__(256) void mc_step_kernel(LigArgs L, GridArgs G, SFA
 template <int KM>
 int launch_mc_step_t(const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecArgs& R, const int* chain_lig,
                      int n_chains, const McArgs& M, const StepArgs& P, hipStream_t stream) {
-    const int W = waves_for(n_chains);
-    const size_t shm = (size_t)W * lds_bytes_per_wave(L.P_MAX);
+    const int W = fit_waves(waves_for(n_chains), KM, L.P_MAX);
+    const size_t shm = (size_t)W * wave_lds_bytes(KM, L.P_MAX);
     const unsigned nblk = (unsigned)((n_chains + W - 1) / W);
     hipLaunchKernelGGL(mc_step_kernel<KM>, dim3(nblk), dim3(W * WAVE), shm, stream, L, G, S, R, chain_lig, n_chains,
                        M, P);
@@ -852,7 +1020,13 @@ int set_waves_per_block(int w) {
     return 0;
 }
 int get_waves_per_block() { return g_waves_per_block; }
-int lds_budget_ok(int P_MAX) { return (size_t)4 * lds_bytes_per_wave(P_MAX) <= 64 * 1024; }
+int set_line_search_width(int ns) {
+    if (ns != 0 && ns != 1 && ns != 2 && ns != 4) return -1;
+    g_line_search_width = ns;
+    return 0;
+}
+int get_line_search_width() { return g_line_search_width; }
+int lds_budget_ok(int KM, int P_MAX) { return wave_lds_bytes(KM, P_MAX) <= LDS_PER_WORKGROUP; }
 
 int launch_mc(int KM, const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecArgs& R, int exact,
               const int* chain_lig, int n_chains, const McArgs& M, hipStream_t stream) {

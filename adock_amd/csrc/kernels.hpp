@@ -6,6 +6,7 @@ namespace adock {
 struct LigArgs {
     const float* ref; const int* typ; const unsigned char* tm; const int* ax; const float* axp; const float* axu;
     const int* pr; const int* na; const int* nt; const int* np; const float* rg; int P_MAX; const int* gslot;
+    const unsigned short* inc; const int* ist;
 };
 struct GridArgs {
     const float* grid; const float* origin; const float* lo; const float* hi; const int* rec_idx;
@@ -56,5 +57,7 @@ int launch_mc_step(int KM, const LigArgs&, const GridArgs&, const SFArgs&, const
                    const McArgs&, const StepArgs&, hipStream_t);
 int set_waves_per_block(int w);
 int get_waves_per_block();
-int lds_budget_ok(int P_MAX);
+int set_line_search_width(int ns);
+int get_line_search_width();
+int lds_budget_ok(int KM, int P_MAX);
 }  // namespace adock

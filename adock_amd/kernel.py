@@ -85,7 +85,7 @@ class KernelProblem:
         p = _lib.ptr
         return ([self.AM, self.KM] + [p(L[k]) for k in ("ref", "typ", "tm", "ax", "axp", "axu", "pr", "na", "nt",
                                                          "np", "rg")]
-                + [self.P_MAX, p(L["gslot"])]
+                + [self.P_MAX, p(L["gslot"]), p(L["inc"]), p(L["ist"])]
                 + [p(G[k]) for k in ("flat", "origin", "lo", "hi", "rec_idx", "offs", "dims")]
                 + [G["T"], G["spacing"], G["penalty"]]
                 + [p(S[k]) for k in ("prm", "rad", "hyd", "don", "acc")]

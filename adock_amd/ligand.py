@@ -136,49 +136,52 @@ class LigandPrep:
         self.pairs = np.array(pairs, dtype=np.int64).reshape(-1, 2)
 
 
-def schedule_pairs(pairs, width=64):
-    """Orders (and orients) intra pairs into rounds of `width` in which every first atom and every second atom is
-    distinct, padding each round with (0, 0) self-pairs that the kernels skip. A kernel that strides pairs over 64
-    lanes then never has two lanes of one instruction accumulate into the same atom. Greedy, highest-degree pairs
-    first; each pair tries both orientations in the earliest round that accepts it. Energy and forces do not depend on
-    the order or orientation of the pair list."""
+def pad128(P):
+    return (int(P) + 127) // 128 * 128
+
+
+def inc_stride(P_MAX):
+    """Length of one ligand's row of slot lists: two entries per pair plus padding of each atom's list to 8."""
+    return 2 * pad128(P_MAX) + 512
+
+
+def slot_lists(pairs, P_MAX, AM=64):
+    """Per-atom pair-slot lists of one ligand for the kernels' atomic-free pair gradient: atom a's list holds q for
+    every pair q = (a, j) and q | 0x8000 (negate) for every pair q = (i, a), ascending in q, padded to a multiple of 8
+    entries with the always-zero slot pad128(P_MAX). Returns (inc [inc_stride] uint16, ist [AM] int32 =
+    start | padded_count << 16)."""
+    z = pad128(P_MAX)
+    inc = np.full(inc_stride(P_MAX), z, np.uint16)
+    ist = np.zeros(AM, np.int32)
     pairs = np.asarray(pairs, np.int64).reshape(-1, 2)
-    if len(pairs) == 0:
-        return pairs
-    n = int(pairs.max()) + 1
-    deg = np.bincount(pairs.reshape(-1), minlength=n)
-    order = np.argsort(-(deg[pairs[:, 0]] + deg[pairs[:, 1]]), kind="stable")
-    rounds = []
-    for q in order:
-        a, b = int(pairs[q, 0]), int(pairs[q, 1])
-        for first, second, lst in rounds:
-            if len(lst) >= width:
-                continue
-            if a not in first and b not in second:
-                first.add(a); second.add(b); lst.append((a, b))
-                break
-            if b not in first and a not in second:
-                first.add(b); second.add(a); lst.append((b, a))
-                break
-        else:
-            rounds.append(({a}, {b}, [(a, b)]))
-    out = np.zeros((len(rounds) * width, 2), np.int64)
-    for r, (_, _, lst) in enumerate(rounds):
-        out[r * width:r * width + len(lst)] = lst
-    return out
+    q = np.arange(len(pairs))
+    real = pairs[:, 0] != pairs[:, 1]
+    atom = np.concatenate([pairs[real, 0], pairs[real, 1]])
+    val = np.concatenate([q[real], q[real] | 0x8000])
+    order = np.lexsort((val & 0x7fff, atom))
+    atom, val = atom[order], val[order]
+    cnt = np.bincount(atom, minlength=AM)[:AM]
+    pc = (cnt + 7) // 8 * 8
+    start = np.concatenate([[0], np.cumsum(pc)[:-1]])
+    if start[-1] + pc[-1] > len(inc):
+        raise ValueError("slot lists exceed their row")
+    first = np.concatenate([[0], np.cumsum(cnt)[:-1]])
+    for a in np.nonzero(cnt)[0]:
+        inc[start[a]:start[a] + cnt[a]] = val[first[a]:first[a] + cnt[a]]
+    ist[:] = start | (pc << 16)
+    return inc, ist
 
 
 def bucket(x, step):
     return max(step, step * math.ceil(x / step))
 
 
-def pack_ligands(preps, slot, rec_idx, schedule=True):
+def pack_ligands(preps, slot, rec_idx):
     """Packed ligand arrays in the search-kernel layout (src/adock/mckernel.py:547-592).
 
-    slot [R, n_types] int (XS type -> grid slot of receptor r, -1 = not built), rec_idx [L].
-    schedule: pair list in conflict-free rounds of 64 (schedule_pairs; np counts the padding self-pairs) instead of
-    the reference's sorted order. Returns (AM, KM, dict of numpy arrays: ref, typ, tm, ax, axp, axu, pr, na, nt, np,
-    rg, P, gslot)."""
+    slot [R, n_types] int (XS type -> grid slot of receptor r, -1 = not built), rec_idx [L]. Returns (AM, KM, dict of
+    numpy arrays: ref, typ, tm, ax, axp, axu, pr, na, nt, np, rg, P, gslot, inc, ist); inc / ist: slot_lists of each
+    ligand."""
     L = len(preps)
     nh = [int(p.heavy.sum()) for p in preps]
     AM = 64 if max(nh) <= 64 else bucket(max(nh), 32)
@@ -189,7 +192,7 @@ def pack_ligands(preps, slot, rec_idx, schedule=True):
         remap = -np.ones(p.n, np.int64)
         remap[hv] = np.arange(len(hv))
         pl = remap[p.pairs] if len(p.pairs) else np.zeros((0, 2), np.int64)
-        pair_lists.append(schedule_pairs(pl) if schedule else pl)
+        pair_lists.append(pl)
     P = max(1, max(len(x) for x in pair_lists))
     out = {
         "ref": np.zeros((L, AM, 3), np.float32), "typ": np.zeros((L, AM), np.int32),
@@ -197,6 +200,7 @@ def pack_ligands(preps, slot, rec_idx, schedule=True):
         "axp": np.zeros((L, KM, 3), np.float32), "axu": np.zeros((L, KM, 3), np.float32),
         "pr": np.zeros((L, P, 2), np.int32), "na": np.zeros(L, np.int32), "nt": np.zeros(L, np.int32),
         "np": np.zeros(L, np.int32), "rg": np.zeros(L, np.float32), "gslot": np.zeros((L, AM), np.int32),
+        "inc": np.zeros((L, inc_stride(P)), np.uint16), "ist": np.zeros((L, AM), np.int32),
     }
     slot = np.asarray(slot)
     rec_idx = np.asarray(rec_idx)
@@ -218,6 +222,7 @@ def pack_ligands(preps, slot, rec_idx, schedule=True):
             out["axu"][l, :p.n_tors] = u / np.maximum(np.linalg.norm(u, axis=1, keepdims=True), 1e-6)
         if len(pair_lists[l]):
             out["pr"][l, :len(pair_lists[l])] = pair_lists[l]
+        out["inc"][l], out["ist"][l] = slot_lists(pair_lists[l], P, AM)
         out["gslot"][l, :len(hv)] = slot[rec_idx[l], out["typ"][l, :len(hv)]]
         if (out["gslot"][l, :len(hv)] < 0).any():
             raise ValueError("a ligand atom type has no grid")

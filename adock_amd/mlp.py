@@ -105,6 +105,21 @@ class ProposalNet:
         _lib.check(L.adock_colsum(B, HID, _lib.ptr(dH1), _lib.ptr(g["b1"]), s))
         return self.loss
 
+    def eval_loss(self, X, T, mask=None, batch=4096):
+        """Masked MSE of the FP32 forward pass over all rows (no parameter update)."""
+        L = _lib.lib()
+        s = _s(self.device)
+        n_valid = float(mask.sum().item()) if mask is not None else float(T.numel())
+        acc = torch.zeros(1, device=self.device)
+        for b0 in range(0, X.shape[0], batch):
+            xb, tb = X[b0:b0 + batch].contiguous(), T[b0:b0 + batch].contiguous()
+            mb = mask[b0:b0 + batch].contiguous() if mask is not None else None
+            y = self.forward(xb)
+            dY = torch.empty_like(y)
+            _lib.check(L.adock_mse(y.numel(), _lib.ptr(y), _lib.ptr(tb), _lib.ptr(mb), 1.0 / max(n_valid, 1.0),
+                                   _lib.ptr(dY), _lib.ptr(acc), s))
+        return float(acc.item())
+
     def step(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
         self.t += 1
         L = _lib.lib()

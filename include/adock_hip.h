@@ -6,9 +6,16 @@
  * launch. adock_last_error() gives a text for the last non-zero status of the calling thread.
  *
  * Argument groups follow the reference's kernel parameter order (src/adock/mckernel.py:328-345, bound by
- * src/adock/cpukernel.py:97-101,263): ligand (13), grid (10), scoring function (5), receptor (4), exact, then the
+ * src/adock/cpukernel.py:97-101,263): ligand (13 + 2), grid (10), scoring function (5), receptor (4), exact, then the
  * entry's own parameters. The reference compiles one module per (A_MAX, K_MAX) bucket; here AM and KM are the first
  * two arguments and select a compiled instantiation (AM = 64; KM in {8, 16, 24, 32}).
+ *
+ * The ligand group is the reference's packed layout (src/adock/mckernel.py:547-592: ref, typ, tm, ax, axp, axu, pr,
+ * na, nt, np, rg, P_MAX, gslot) plus two arrays of this library's own, built once per ligand set on the host
+ * (adock_amd.ligand.slot_lists): inc [L, 2 * P128 + 512] uint16 and ist [L, AM] int32, P128 = P_MAX rounded up to a
+ * multiple of 128. Atom a's list inc[l, s .. s + c) (ist = s | c << 16, s and c multiples of 8) names the intra pairs
+ * of atom a: q for a pair q = (a, j), q | 0x8000 for a pair q = (i, a), ascending, padded with P128. The kernels use
+ * it to sum the pair gradient of every atom without atomics.
  */
 #ifndef ADOCK_HIP_H
 #define ADOCK_HIP_H
@@ -44,7 +51,7 @@ int adock_build_grid(const float* rxyz, const int* rtype, int n_rec, const float
 int adock_eg(int AM, int KM,
              const float* ref, const int* typ, const unsigned char* tm, const int* ax, const float* axp,
              const float* axu, const int* pr, const int* na, const int* nt, const int* np, const float* rg, int P_MAX,
-             const int* gslot,
+             const int* gslot, const unsigned short* inc, const int* ist,
              const float* grid, const float* origin, const float* lo, const float* hi, const int* rec_idx,
              const long long* goffs, const long long* gdims, int T, float spacing, float pen,
              const float* prm, const float* rad, const int* hyd, const int* don, const int* acc,
@@ -58,7 +65,7 @@ int adock_eg(int AM, int KM,
 int adock_mc(int AM, int KM,
              const float* ref, const int* typ, const unsigned char* tm, const int* ax, const float* axp,
              const float* axu, const int* pr, const int* na, const int* nt, const int* np, const float* rg, int P_MAX,
-             const int* gslot,
+             const int* gslot, const unsigned short* inc, const int* ist,
              const float* grid, const float* origin, const float* lo, const float* hi, const int* rec_idx,
              const long long* goffs, const long long* gdims, int T, float spacing, float pen,
              const float* prm, const float* rad, const int* hyd, const int* don, const int* acc,
@@ -75,7 +82,7 @@ int adock_mc(int AM, int KM,
 int adock_mc_counted(int AM, int KM,
                      const float* ref, const int* typ, const unsigned char* tm, const int* ax, const float* axp,
                      const float* axu, const int* pr, const int* na, const int* nt, const int* np, const float* rg,
-                     int P_MAX, const int* gslot,
+                     int P_MAX, const int* gslot, const unsigned short* inc, const int* ist,
                      const float* grid, const float* origin, const float* lo, const float* hi, const int* rec_idx,
                      const long long* goffs, const long long* gdims, int T, float spacing, float pen,
                      const float* prm, const float* rad, const int* hyd, const int* don, const int* acc,
@@ -97,7 +104,7 @@ int adock_mc_counted(int AM, int KM,
 int adock_mc_batch(int AM, int KM,
                    const float* ref, const int* typ, const unsigned char* tm, const int* ax, const float* axp,
                    const float* axu, const int* pr, const int* na, const int* nt, const int* np, const float* rg,
-                   int P_MAX, const int* gslot,
+                   int P_MAX, const int* gslot, const unsigned short* inc, const int* ist,
                    const float* grid, const float* origin, const float* lo, const float* hi, const int* rec_idx,
                    const long long* goffs, const long long* gdims, int T, float spacing, float pen,
                    const float* prm, const float* rad, const int* hyd, const int* don, const int* acc,
@@ -127,7 +134,7 @@ int adock_batch_reset(int* counter, int* done, int n_complexes, unsigned long lo
 int adock_mc_step(int AM, int KM,
                   const float* ref, const int* typ, const unsigned char* tm, const int* ax, const float* axp,
                   const float* axu, const int* pr, const int* na, const int* nt, const int* np, const float* rg,
-                  int P_MAX, const int* gslot,
+                  int P_MAX, const int* gslot, const unsigned short* inc, const int* ist,
                   const float* grid, const float* origin, const float* lo, const float* hi, const int* rec_idx,
                   const long long* goffs, const long long* gdims, int T, float spacing, float pen,
                   const float* prm, const float* rad, const int* hyd, const int* don, const int* acc,
@@ -162,6 +169,13 @@ int adock_to_bf16(long long n, const float* x, unsigned short* y, adock_stream_t
  * counterpart is the NW constant (src/adock/mckernel.py:525-540). Returns 0, or ADOCK_E_ARG. */
 int adock_set_chains_per_block(int chains_per_block);
 int adock_get_chains_per_block(void);
+
+/* Waves per chain of adock_mc / adock_mc_counted's speculative line search: 1 (off), 2 or 4, or 0 = adaptive (4 while
+ * chains <= CUs, 2 up to 2 x CUs, else 1; lowered until the workgroup's LDS fits). The waves of one workgroup evaluate
+ * consecutive line-search trials of one chain in parallel; results, energies and evaluation counts are the same bit
+ * for bit as with 1. No counterpart in the reference (its line search is sequential). Returns 0, or ADOCK_E_ARG. */
+int adock_set_line_search_width(int waves);
+int adock_get_line_search_width(void);
 
 #ifdef __cplusplus
 }

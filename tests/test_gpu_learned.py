@@ -7,7 +7,7 @@ import torch
 
 from adock_amd import synth, workload
 from adock_amd.kernel import KernelProblem
-from adock_amd.learned import StepSearch, train_proposal
+from adock_amd.learned import StepSearch, move_weights
 from adock_amd.mlp import ProposalNet
 
 pytestmark = pytest.mark.gpu
@@ -86,5 +86,9 @@ def test_training_on_logged_moves_reduces_loss(cfg1):
     ss.run(*_bufs(st), 40, seed=11, log=True)
     X, T, M, gain = ss.logged()
     assert X.shape[0] > 200
-    net, losses = train_proposal(X, T, M, gain, epochs=6, batch=256, lr=2e-3)
-    assert losses[-1] < losses[0]
+    W = move_weights(M, gain)
+    net = ProposalNet("cuda", seed=0)
+    before = net.eval_loss(X.contiguous(), T.contiguous(), W)
+    net.fit(X.contiguous(), T.contiguous(), W, epochs=6, batch=256, lr=2e-3)
+    after = net.eval_loss(X.contiguous(), T.contiguous(), W)
+    assert after < 0.9 * before, (before, after)

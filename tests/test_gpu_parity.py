@@ -159,6 +159,31 @@ def test_eg_many_ligands_two_receptors():
         assert vec_rel(g[i:i + 1, :D], go[i:i + 1, :D], EG_FLOOR) < EG_REL, i
 
 
+def test_eg_edge_ligands():
+    """A rigid ligand (no torsions, no intra pairs), a 64-atom ligand with 15 torsions (the largest pair list, one
+    chain per workgroup by LDS) and a one-torsion ligand in one batch."""
+    print("[gpu] eg edge ligands", flush=True)
+    rec = workload.receptor_dict(synth.make_receptor(3, 1500, 20.0, 6.0), np.zeros(3), 24.0)
+    preps = [synth.make_ligand(7, 12, 0), synth.make_ligand(8, 64, 15), synth.make_ligand(9, 33, 1)]
+    assert len(preps[0].pairs) == 0 and int(preps[1].heavy.sum()) == 64
+    prob = workload.gpu_problem([rec], preps, [0, 0, 0])
+    kp, op = KernelProblem(prob), _oracle_of(prob)
+    N = 96
+    cl = (np.arange(N) % 3).astype(np.int32)
+    st = np.zeros((N, kp.SD), np.float32)
+    for i in range(N):
+        st[i] = states(2000 + i, 1, np.zeros(3), 20.0, preps[cl[i]], kp.KM, sigma=2.0)[0]
+    e, g = kp.energy_grad(torch.as_tensor(cl).cuda(), torch.as_tensor(st).cuda())
+    e, g = e.cpu().numpy(), g.cpu().numpy()
+    eo, go, _, m = op.energy_grad(cl, st)
+    keep = m >= MARGIN
+    assert keep.sum() > N // 2
+    assert rel_err(e[keep], eo[keep], EG_FLOOR, EG_REL) < EG_REL
+    for i in np.nonzero(keep)[0]:
+        D = 6 + preps[cl[i]].n_tors
+        assert vec_rel(g[i:i + 1, :D], go[i:i + 1, :D], EG_FLOOR) < EG_REL, i
+
+
 # ------------------------------------------------------------------------------------------------ BFGS / MC
 def _run_both(kp, op, st, init_opt, mc_steps, iters, seed=0, step_offset=0, f64=True, cl=None):
     N = len(st)

@@ -365,25 +365,3 @@ def test_exact_mode_matches_direct_pairwise(config1):
         lo, hi = -10.0, 10.0
         pen = PENALTY * (np.maximum(lo - X[i, :na], 0) ** 2 + np.maximum(X[i, :na] - hi, 0) ** 2).sum()
         assert abs(e[i] - (sum(terms) + pen)) <= 2e-7 * sum(abs(x) for x in terms) + 1e-9
-
-
-def test_scheduled_pair_list_is_a_permutation_with_conflict_free_rounds():
-    """pack_ligands' scheduled pair list holds exactly the reference pairs (up to orientation), every 64-entry round
-    has distinct first and distinct second atoms, and the energy is the same as with the reference's sorted list."""
-    from adock_amd.ligand import pack_ligands
-    rec, lig, prob, meta = _small_problem(n_heavy=30, n_tors=6)
-    _, _, packed = pack_ligands([lig], meta["slot"], [0], schedule=True)
-    _, _, plain = pack_ligands([lig], meta["slot"], [0], schedule=False)
-    pr = packed["pr"][0, :packed["np"][0]]
-    real = pr[pr[:, 0] != pr[:, 1]]
-    assert sorted(map(tuple, np.sort(real, 1).tolist())) == sorted(map(tuple, plain["pr"][0, :plain["np"][0]].tolist()))
-    for r in range(0, len(pr), 64):
-        blk = pr[r:r + 64]
-        blk = blk[blk[:, 0] != blk[:, 1]]
-        assert len(set(blk[:, 0])) == len(blk) and len(set(blk[:, 1])) == len(blk)
-    st = states(21, 16, np.zeros(3), 10.0, lig, prob.KM)
-    e1, g1, _, _ = prob.energy_grad(np.zeros(len(st), np.int32), st)
-    prob.lig.update({k: np.ascontiguousarray(v) if isinstance(v, np.ndarray) else v for k, v in plain.items()})
-    e2, g2, _, _ = prob.energy_grad(np.zeros(len(st), np.int32), st)
-    np.testing.assert_allclose(e1, e2, rtol=1e-12, atol=1e-12)
-    np.testing.assert_allclose(g1, g2, rtol=1e-10, atol=1e-10)

@@ -1,0 +1,49 @@
+"""Host-built pair-slot lists (adock_amd.ligand.slot_lists): every intra pair appears once in the list of each of its
+atoms (first atom plain, second atom with the negate bit), lists ascend, start and length are multiples of 8 and the
+padding names the zero slot."""
+import numpy as np
+import pytest
+
+from adock_amd import synth
+from adock_amd.ligand import inc_stride, pack_ligands, pad128, slot_lists
+
+
+def _heavy_pairs(lig):
+    hv = np.where(lig.heavy)[0]
+    remap = -np.ones(lig.n, np.int64)
+    remap[hv] = np.arange(len(hv))
+    return remap[lig.pairs] if len(lig.pairs) else np.zeros((0, 2), np.int64)
+
+
+@pytest.mark.parametrize("args", [(7, 12, 0), (9, 33, 1), (8, 64, 15), (3, 35, 8)])
+def test_slot_lists_cover_every_pair_twice(args):
+    pl = _heavy_pairs(synth.make_ligand(*args))
+    P_MAX = max(1, len(pl)) + 5
+    inc, ist = slot_lists(pl, P_MAX)
+    Z = pad128(P_MAX)
+    assert inc.shape == (inc_stride(P_MAX),) and inc.dtype == np.uint16
+    seen = set()
+    for a in range(64):
+        s, c = int(ist[a]) & 0xffff, int(ist[a]) >> 16
+        assert s % 8 == 0 and c % 8 == 0 and s + c <= len(inc)
+        ent = inc[s:s + c].astype(np.int64)
+        real = ent[ent != Z]
+        assert (ent[len(real):] == Z).all()
+        assert (np.diff(real & 0x7fff) > 0).all()
+        for e in real:
+            q, side = e & 0x7fff, 1 if e & 0x8000 else 0
+            assert pl[q, side] == a
+            seen.add((q, side))
+    assert len(seen) == 2 * len(pl)
+
+
+def test_pack_ligands_rows_match_per_ligand_lists():
+    preps = [synth.make_ligand(7, 12, 0), synth.make_ligand(9, 33, 1)]
+    slot = np.arange(32)[None, :]
+    AM, KM, out = pack_ligands(preps, slot, [0, 0])
+    P = int(out["P"])
+    assert out["inc"].shape == (2, inc_stride(P))
+    for l, p in enumerate(preps):
+        inc, ist = slot_lists(_heavy_pairs(p), P, AM)
+        np.testing.assert_array_equal(out["inc"][l], inc)
+        np.testing.assert_array_equal(out["ist"][l], ist)
