@@ -70,6 +70,8 @@ def main():
     ap.add_argument("--workload", choices=["single", "screen", "multi", "learned"], default="single",
                     help="single = the headline line (default); the others print their own JSON line")
     ap.add_argument("--small", action="store_true", help="scaled-down sizes for the opt-in workloads")
+    ap.add_argument("--copy-calib-mib", type=int, default=0,
+                    help="after the timed steps, one device-to-device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build runs the docking path on a single MI355X "
@@ -151,13 +153,44 @@ def main():
         "mc_kernel_ms": mc_ms,
         "best_score": {"min": float(best_e.min()), "per_step": [float(x) for x in best_e]},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                     "kernel": "mc_kernel<8>", "flops_per_eval": fl},
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": measured_traffic(mc_kernel_name(kp.KM, N)),
+                     "kernel": mc_kernel_name(kp.KM, N), "flops_per_eval": fl,
+                     "traffic_source": "profiles/mc_spec_kernel_config2_traffic.json (rocprofv3 FETCH_SIZE / WRITE_SIZE, "
+                                       "calibrated on a 1 GiB device copy)"},
     }
+    if args.copy_calib_mib > 0:
+        src = torch.empty(args.copy_calib_mib << 18, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        out["copy_calib_bytes"] = 2 * src.numel() * 4
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(w, prob, threads)
     print(json.dumps(out), flush=True)
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of this workload's search kernel from the committed counter summary, or None when the
+    summary is absent or was measured on another kernel."""
+    f = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "mc_spec_kernel_config2_traffic.json")
+    try:
+        with open(f) as fh:
+            d = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    return float(d["traffic_bytes"]) if d.get("kernel") == kernel else None
+
+
+def mc_kernel_name(KM, n_chains):
+    """Kernel adock_mc launches for this many chains with the adaptive settings: the speculative line search on 4
+    waves per chain while the chains fit one per CU, 2 up to two per CU, else one wave per chain."""
+    from adock_amd import _lib
+    ns = int(_lib.lib().adock_get_line_search_width())
+    if ns == 0:
+        cus = torch.cuda.get_device_properties(0).multi_processor_count
+        ns = 4 if n_chains <= cus else 2 if n_chains <= 2 * cus else 1
+    return f"mc_spec_kernel<{KM}, {ns}>" if ns > 1 else f"mc_kernel<{KM}>"
 
 
 def kp_chains_per_block():
