@@ -10,7 +10,7 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libadock_hip.so"
-SOURCES = ["grid.hip", "dock.hip", "mlp.hip", "capi.hip"]
+SOURCES = ["grid.hip", "dock.hip", "stagea.hip", "mlp.hip", "capi.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
          # packed FP32 (v_pk_*_f32) code generation is switched off for every kernel

@@ -207,6 +207,24 @@ int adock_to_bf16(long long n, const float* x, unsigned short* y, adock_stream_t
     return hip_status(launch_to_bf16(n, x, y, (hipStream_t)stream));
 }
 
+int adock_stage_a(int AM, const float* conf, const int* gslot, const int* na, int L, int C, const float* quats,
+                  int Q, const float* trans, const int* ntrans, int NT_MAX, const float* grid, const float* origin,
+                  const float* lo, const float* hi, const int* rec_idx, const long long* goffs,
+                  const long long* gdims, float spacing, float pen, float cap, float* best_e, int* best_i,
+                  adock_stream_t stream) {
+    if (AM != 64) return fail(ADOCK_E_BUCKET, "AM must be 64 (at most 64 heavy atoms per ligand)");
+    if (L < 0 || C < 0 || Q < 0 || NT_MAX < 0) return fail(ADOCK_E_ARG, "negative size");
+    if ((long long)L * C * Q == 0) return 0;
+    if ((long long)L * C * Q > 0x7fffffffll) return fail(ADOCK_E_ARG, "L * C * Q too large");
+    if (NT_MAX < 1) return fail(ADOCK_E_ARG, "NT_MAX must be at least 1");
+    if (!conf || !gslot || !na || !quats || !trans || !ntrans || !grid || !origin || !lo || !hi || !rec_idx ||
+        !goffs || !gdims || !best_e || !best_i)
+        return fail(ADOCK_E_ARG, "null pointer");
+    GridArgs G{grid, origin, lo, hi, rec_idx, goffs, gdims, 0, spacing, pen};
+    return hip_status(launch_stage_a(conf, gslot, na, AM, L, C, quats, Q, trans, ntrans, NT_MAX, G, cap, best_e, best_i,
+                                     (hipStream_t)stream));
+}
+
 int adock_set_chains_per_block(int w) {
     if (set_waves_per_block(w)) return fail(ADOCK_E_ARG, "chains_per_block must be 0 (adaptive) or 1..4");
     return 0;

@@ -105,6 +105,59 @@ class KernelProblem:
                                        _lib.stream_of(self.device)))
         return e, g
 
+    def stage_a(self, conf_h, quats, trans, ntrans, cap):
+        """conf_h [L, C, AM, 3] heavy-atom conformer coordinates (ligand frame), quats [Q, 4], trans [L, NT, 3]
+        (padded lattices), ntrans [L] -> (best_e [L, C*Q], best_i [L, C*Q] int64 index into trans)
+        (src/adock/mckernel.py:690-705)."""
+        L, C = conf_h.shape[:2]
+        Q, NT = quats.shape[0], trans.shape[1]
+        if L != self.L:
+            raise _lib.AdockError("conf_h must hold one row per complex")
+        be = torch.empty(L * C * Q, device=self.device)
+        bi = torch.empty(L * C * Q, dtype=torch.int32, device=self.device)
+        cf = conf_h.to(self.device).float().contiguous()
+        qs = quats.to(self.device).float().contiguous()
+        tr = trans.to(self.device).float().contiguous()
+        nt = ntrans.to(device=self.device, dtype=torch.int32).contiguous()
+        if cf.shape[2] != self.AM:
+            raise _lib.AdockError(f"conf_h must have AM = {self.AM} atom slots")
+        G, L_ = self.grid, self.lig
+        self._keep = (cf, qs, tr, nt)
+        _lib.check(_lib.lib().adock_stage_a(
+            self.AM, _lib.ptr(cf), _lib.ptr(L_["gslot"]), _lib.ptr(L_["na"]), L, C, _lib.ptr(qs), Q, _lib.ptr(tr),
+            _lib.ptr(nt), max(int(NT), 1), *[_lib.ptr(G[k]) for k in ("flat", "origin", "lo", "hi", "rec_idx", "offs",
+                                                                       "dims")],
+            G["spacing"], G["penalty"], float(cap), _lib.ptr(be), _lib.ptr(bi), _lib.stream_of(self.device)))
+        return be.view(L, C * Q), bi.view(L, C * Q).long()
+
+    def receptor_neighbours(self, prob, chain_lig, states, K, margin):
+        """Per chain: the receptor atoms within cutoff + margin of any heavy atom of the chain's pose, nearest list
+        first in receptor order -> (nbr [N, NB] int32, nbr_n [N] int32) (src/adock/mckernel.py:662-688)."""
+        from .pose import pose_coords
+        rx, rt, rm = prob.rec
+        lb = prob.lb
+        cut = float(prob.pot.p["cutoff"]) + margin
+        N = states.shape[0]
+        lists = []
+        cl = chain_lig.long()
+        for l in torch.unique(cl).tolist():
+            idx = (cl == l).nonzero().squeeze(1)
+            t, q, th = self.unpack(states[idx], K)
+            sub = lb.subset(torch.tensor([l], device=states.device))
+            X = pose_coords(sub, t[None], q[None], torch.zeros_like(t)[None], th[None])[0][:, lb.heavy[l]]
+            ri = int(prob.rec_idx[l])
+            xr = rx[ri][rm[ri]].to(X.device).float()
+            near = (torch.cdist(X.reshape(-1, 3), xr).view(len(idx), -1, len(xr)) < cut).any(1)
+            lists.append((idx, near))
+        NB = max(1, max(int(n.sum(1).max()) for _, n in lists))
+        nbr = torch.zeros(N, NB, dtype=torch.int32, device=states.device)
+        nbr_n = torch.zeros(N, dtype=torch.int32, device=states.device)
+        for idx, near in lists:
+            order = torch.argsort((~near).to(torch.int8), dim=1, stable=True)[:, :NB]
+            nbr[idx, :order.shape[1]] = order.to(torch.int32)
+            nbr_n[idx] = near.sum(1).to(torch.int32)
+        return nbr, nbr_n
+
     def run_mc(self, chain_lig, chain_id, cur, cur_e, best, best_e, init_opt, mc_steps, bfgs_iters, temp,
                amp_pos, amp_rot, seed, step_offset, exact=False, nbr=None, nbr_n=None, count_evals=False):
         """In place on cur/cur_e/best/best_e (float32 contiguous device tensors). exact=True with nbr [N, NB] /

@@ -9,35 +9,115 @@ import math
 import numpy as np
 import torch
 
+from .scoring import pair_terms
+
 K_BUCKET = 8
 
 
 class LigandBatch:
-    """L ligand preps (one per complex) with the torsion count bucketed to a multiple of 8."""
+    """Padded tensors of L ligand preps (one per complex), the torsion count bucketed to a multiple of 8
+    (src/adock/ligand.py:125-182): ref [L,A,3], types [L,A] (-1 = hydrogen / padding), heavy, amask, axis [L,K,2],
+    tmask [L,K,A], tvalid [L,K], pairs [L,P,2], pmask [L,P], rg, n_tors_vina, axis_p / axis_u [L,K,3]."""
 
     def __init__(self, preps, device):
         self.preps = preps
-        self.L = len(preps)
+        L = len(preps)
+        A = max(p.n for p in preps)
         K = max(max(p.n_tors for p in preps), 1)
-        self.K = K_BUCKET * math.ceil(K / K_BUCKET)
-        tv = np.zeros((self.L, self.K), bool)
+        K = K_BUCKET * math.ceil(K / K_BUCKET)
+        P = max(max(len(p.pairs) for p in preps), 1)
+        ref = np.zeros((L, A, 3), np.float32)
+        types = -np.ones((L, A), np.int64)
+        amask = np.zeros((L, A), bool)
+        axis = np.zeros((L, K, 2), np.int64)
+        tmask = np.zeros((L, K, A), bool)
+        tvalid = np.zeros((L, K), bool)
+        pairs = np.zeros((L, P, 2), np.int64)
+        pmask = np.zeros((L, P), bool)
         for l, p in enumerate(preps):
-            tv[l, :p.n_tors] = True
-        self.tvalid = torch.as_tensor(tv, device=device)
-        self.n_tors_vina = torch.tensor([float(p.n_tors_vina) for p in preps], device=device)
-        self.heavy_count = torch.tensor([int(p.heavy.sum()) for p in preps], device=device)
+            ref[l, :p.n] = p.ref
+            types[l, :p.n] = p.types
+            amask[l, :p.n] = True
+            axis[l, :p.n_tors] = p.tors_axis
+            tmask[l, :p.n_tors, :p.n] = p.tors_mask
+            tvalid[l, :p.n_tors] = True
+            pairs[l, :len(p.pairs)] = p.pairs
+            pmask[l, :len(p.pairs)] = True
+        T = lambda a: torch.as_tensor(a, device=device)
+        self.L, self.A, self.K, self.P = L, A, K, P
+        self.ref, self.types, self.amask = T(ref), T(types), T(amask)
+        self.heavy = self.types >= 0
+        self.axis, self.tmask, self.tvalid = T(axis), T(tmask), T(tvalid)
+        self.pairs, self.pmask = T(pairs), T(pmask)
+        self.rg = T(np.array([p.rg for p in preps], np.float32))
+        self.n_tors_vina = T(np.array([p.n_tors_vina for p in preps], np.float32))
+        self.heavy_count = self.heavy.sum(1)
+        li = torch.arange(L, device=device)[:, None]
+        pa, pb = self.ref[li, self.axis[..., 0]], self.ref[li, self.axis[..., 1]]
+        u = pb - pa
+        self.axis_p = pa
+        self.axis_u = u / u.norm(dim=-1, keepdim=True).clamp(min=1e-6)
+
+    def subset(self, idx):
+        new = LigandBatch.__new__(LigandBatch)
+        new.preps = [self.preps[i] for i in idx.tolist()]
+        new.A, new.K, new.P = self.A, self.K, self.P
+        new.L = len(idx)
+        for k in ("ref", "types", "heavy", "amask", "axis", "tmask", "tvalid", "pairs", "pmask", "rg", "n_tors_vina",
+                  "heavy_count", "axis_p", "axis_u"):
+            setattr(new, k, getattr(self, k)[idx])
+        return new
 
 
 class PairPotential:
+    """Scoring-function parameters (sf_params dict) and the per-type tables as tensors
+    (src/adock/scoring.py:34-61)."""
+
     def __init__(self, sfp, device="cpu"):
         self.p = sfp
         self.device = device
+        self.radius = torch.tensor(sfp["radius"], dtype=torch.float32, device=device)
+        self.hyd = torch.tensor(sfp["hydrophobic"], device=device).bool()
+        self.don = torch.tensor(sfp["donor"], device=device).bool()
+        self.acc = torch.tensor(sfp["acceptor"], device=device).bool()
+
+    def prm(self):
+        p = self.p
+        return (p["g1_off"], p["g1_w"], p["g2_off"], p["g2_w"], p["rep_off"], p["hyd_good"], p["hyd_bad"],
+                p["hb_good"], p["hb_bad"], p["cutoff"])
+
+    def terms(self, r, ti, tj):
+        return pair_terms(r, ti, tj, self.radius, self.hyd, self.don, self.acc, self.prm())
+
+    def weights(self, device=None):
+        p = self.p
+        return torch.tensor([p["w_g1"], p["w_g2"], p["w_rep"], p["w_hyd"], p["w_hb"]],
+                            device=device or self.radius.device)
+
+    def energy(self, r, ti, tj):
+        return (self.terms(r, ti, tj) * self.weights(r.device)).sum(-1)
 
 
 class Problem:
     def __init__(self, gridset, lb, rec_idx, pot, rec=None):
         self.gs, self.lb, self.rec_idx, self.pot = gridset, lb, rec_idx, pot
         self.rec = rec   # optional (xyz [R,N,3], types [R,N], mask [R,N]) for the exact (grid-free) energy
+
+    def subset(self, idx):
+        return Problem(self.gs, self.lb.subset(idx), self.rec_idx[idx], self.pot, self.rec)
+
+    def intra(self, X):
+        """Intra-ligand energy [L, M] of coordinates X [L, M, A, 3] (src/adock/search.py:108-119)."""
+        lb = self.lb
+        L, M, A, _ = X.shape
+        pi = lb.pairs[:, None, :, :].expand(L, M, lb.P, 2)
+        xi = torch.gather(X, 2, pi[..., 0:1].expand(L, M, lb.P, 3))
+        xj = torch.gather(X, 2, pi[..., 1:2].expand(L, M, lb.P, 3))
+        r = (xi - xj).norm(dim=-1)
+        ti = torch.gather(lb.types, 1, lb.pairs[..., 0])[:, None].expand(L, M, lb.P)
+        tj = torch.gather(lb.types, 1, lb.pairs[..., 1])[:, None].expand(L, M, lb.P)
+        e = self.pot.energy(r, ti.clamp(min=0), tj.clamp(min=0))
+        return torch.where(lb.pmask[:, None, :], e, torch.zeros_like(e)).sum(-1)
 
 
 def pad_receptors(receptors, device):

@@ -1,0 +1,211 @@
+"""The search driver around the kernels: Stage A start poses, Monte Carlo + BFGS in segments, refinement, exact
+polish and finalize (exact rescoring, clustering, affinities).
+
+dock_problem_kernel reproduces the reference's engine-v2 control flow (src/adock/search.py:544-671): chain widening
+for small batches, Stage A (stage_a_kernel, src/adock/search.py:322-366), MC segments of mc_segment steps with
+init_opt on segment 0 and step offsets sg * mc_segment (so a split run equals an unsplit one), compaction of
+finished / early-stopped complexes, the size-scaled segment budget (mc_size_ref), the longer BFGS refinement of every
+chain's best state, the exact-pairwise polish of the polish_top best chains per complex with receptor neighbour lists,
+and finalize_kernel (src/adock/search.py:506-537). The kernels come from a KernelProblem (libadock_hip.so, MI355X);
+tests may pass another object with the same methods as `kernel_problem`.
+"""
+import math
+import time
+
+import numpy as np
+import torch
+
+from .pose import internal_coords, pose_coords, random_quat
+
+
+@torch.no_grad()
+def stage_a_kernel(prob, kp, boxes_c, boxes_s, cfg, gen):
+    """Rigid enumeration: NT random torsion draws per ligand, the C lowest-intra-energy conformers x Q random rotations
+    x the complex's translation lattice (step stageA_translation_step inside the box shrunk by 1 A, points whose
+    nearest-grid C_H probe energy exceeds stageA_probe_max dropped), scored by kp.stage_a with the per-atom cap
+    stageA_atom_cap; the n_chains best (conformer, rotation, translation) triples per complex become the chains' start
+    poses. Returns (t [L,M,3], q [L,M,4], theta [L,M,K], e [L,M])."""
+    lb, gs = prob.lb, prob.gs
+    dev = lb.ref.device
+    L, K = lb.L, lb.K
+    C, NT, Q, M = cfg["stageA_conformers"], cfg["stageA_conformer_trials"], cfg["stageA_rotations"], cfg["n_chains"]
+    th = (torch.rand((L, NT, K), generator=gen, device=dev) * 2 - 1) * math.pi
+    th[:, 0] = 0.0
+    th = th * lb.tvalid[:, None, :]
+    e_intra = prob.intra(internal_coords(lb, th))
+    ci = torch.topk(-e_intra, min(C, NT), dim=1).indices
+    th_c = torch.gather(th, 1, ci[..., None].expand(L, ci.shape[1], K))
+    e_c = torch.gather(e_intra, 1, ci)
+    Xc = internal_coords(lb, th_c)
+    C = Xc.shape[1]
+    hidx = torch.zeros(L, kp.AM, dtype=torch.long, device=dev)
+    for l, p in enumerate(lb.preps):
+        hv = torch.as_tensor(np.where(p.heavy)[0], device=dev)
+        hidx[l, :len(hv)] = hv
+    conf_h = torch.gather(Xc, 2, hidx[:, None, :, None].expand(L, C, kp.AM, 3))
+    quats = random_quat((Q,), gen, dev)
+    step = cfg["stageA_translation_step"]
+    Ts = []
+    for l in range(L):
+        c, s = boxes_c[l], boxes_s[l]
+        axes = [torch.arange(-float(s[d]) / 2 + 1.0, float(s[d]) / 2 - 1.0 + 1e-6, step, device=dev) for d in range(3)]
+        T = torch.stack(torch.meshgrid(*axes, indexing="ij"), -1).reshape(-1, 3) + torch.as_tensor(c, device=dev)
+        ep = gs.lookup_nearest(T, torch.zeros(len(T), dtype=torch.long, device=dev), prob.rec_idx[l].expand(len(T)))
+        T = T[ep <= cfg.get("stageA_probe_max", 0.0)]
+        Ts.append(T if len(T) else torch.as_tensor(c, device=dev, dtype=torch.float32)[None])
+    nT = torch.tensor([len(T) for T in Ts], device=dev)
+    trans = torch.zeros(L, int(nT.max()), 3, device=dev)
+    for l, T in enumerate(Ts):
+        trans[l, :len(T)] = T
+    best_e, best_i = kp.stage_a(conf_h, quats, trans, nT, cfg.get("stageA_atom_cap", 1e9))
+    best_e = best_e + e_c.repeat_interleave(Q, dim=1)
+    k = min(M, C * Q)
+    top = torch.topk(-best_e, k, dim=1).indices
+    if k < M:
+        top = top[:, torch.arange(M, device=dev) % k]
+    li = torch.arange(L, device=dev)[:, None]
+    t0 = trans[li, best_i[li, top]]
+    return t0, quats[top % Q], th_c[li, top // Q], best_e[li, top]
+
+
+@torch.no_grad()
+def finalize_kernel(prob, kp, states, chain_lig, L, P, cfg):
+    """Exact total energies (pairwise inter + penalty + intra, grid total without receptor atoms) of the L x P final
+    states, intra from the torch pair potential, heavy-atom RMSD, greedy clustering at cluster_rmsd (lowest energy
+    first, at most n_out representatives) and Vina-style affinities (src/adock/search.py:506-537)."""
+    lb = prob.lb
+    bt, bq, bth = kp.unpack(states, lb.K)
+    X = pose_coords(lb, bt.view(L, P, 3), bq.view(L, P, 4), torch.zeros_like(bt).view(L, P, 3),
+                    bth.view(L, P, lb.K))
+    etot = kp.energy_grad(chain_lig, states, exact=prob.rec is not None)[0].view(L, P)
+    eintra = prob.intra(X)
+    hv = lb.heavy[:, None, None, :].float()
+    D = ((((X[:, :, None] - X[:, None, :]) ** 2).sum(-1) * hv).sum(-1) / hv.sum(-1).clamp(min=1)).sqrt()
+    Xc, Ec, Ic, Dc = X.cpu().numpy(), etot.cpu().numpy(), eintra.cpu().numpy(), D.cpu().numpy()
+    Nt = lb.n_tors_vina.cpu().numpy()
+    rw = prob.pot.p["rot_weight"]
+    cut, n_out = cfg["cluster_rmsd"], cfg["n_out"]
+    results = []
+    for l in range(L):
+        reps = []
+        for i in np.argsort(Ec[l], kind="stable"):
+            if all(Dc[l, i, j] >= cut for j in reps):
+                reps.append(i)
+                if len(reps) == n_out:
+                    break
+        ri = np.array(reps)
+        inter = Ec[l, ri] - Ic[l, ri]
+        aff = (inter + Ic[l, ri] - Ic[l, ri[0]]) / (1 + rw * Nt[l])
+        results.append({"coords": Xc[l, ri], "affinity": aff, "inter": inter, "intra": Ic[l, ri], "total": Ec[l, ri]})
+    return results
+
+
+def dock_problem_kernel(prob, boxes_c, boxes_s, cfg, seed=0, log=None, stats=None, kernel_problem=None):
+    """Full search of every complex of `prob` (boxes_c / boxes_s: per-complex box centre and edge lengths), settings
+    from a search preset (config.load_config()["search"]["v2_default"] etc.). Returns one dict per complex (coords
+    [n_out, A, 3], affinity, inter, intra, total). `stats` (dict) receives the widening factor, chain-steps and phase
+    times."""
+    lb = prob.lb
+    dev = lb.ref.device
+    if kernel_problem is None:
+        from .kernel import KernelProblem as kernel_problem
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    L = lb.L
+    # small batches leave the GPU idle and latency is set by chain depth: widen (more, shorter chains) at constant
+    # chain-steps until L * n_chains >= min_total_chains, keeping at least min_chain_steps steps per chain
+    f = 1
+    while (L * cfg["n_chains"] * f * 2 <= cfg.get("min_total_chains", 0)
+           and cfg["mc_steps"] // (f * 2) >= cfg.get("min_chain_steps", 32)):
+        f *= 2
+    if f > 1:
+        cfg = dict(cfg, n_chains=cfg["n_chains"] * f, mc_steps=cfg["mc_steps"] // f,
+                   mc_segment=max(1, cfg.get("mc_segment", cfg["mc_steps"]) // f))
+    M = cfg["n_chains"]
+    tick = {}
+
+    def mark(k):
+        if dev.type == "cuda":
+            torch.cuda.synchronize()
+        tick[k] = time.perf_counter()
+
+    mark("t0")
+    kp = kernel_problem(prob)
+    t, q0, th, _ = stage_a_kernel(prob, kp, boxes_c, boxes_s, cfg, gen)
+    mark("stageA")
+    N = L * M
+    cur = kp.pack(t.reshape(N, 3), q0.reshape(N, 4), th.reshape(N, -1))
+    best = cur.clone()
+    cur_e = torch.zeros(N, device=dev)
+    best_e = torch.full((N,), float("inf"), device=dev)
+    chain_lig = torch.arange(L, device=dev, dtype=torch.int32).repeat_interleave(M)
+    chain_id = torch.arange(N, device=dev, dtype=torch.int32)
+    seg = max(1, cfg.get("mc_segment", cfg["mc_steps"]))
+    n_seg = max(1, math.ceil(cfg["mc_steps"] / seg)) if cfg["mc_steps"] > 0 else 0
+    active = torch.ones(L, dtype=torch.bool, device=dev)
+    stall = torch.zeros(L, dtype=torch.long, device=dev)
+    glob = torch.full((L,), float("inf"), device=dev)
+    es_rounds, es_tol = cfg.get("early_stop_rounds", 0), cfg.get("early_stop_tol", 0.01)
+    # size-scaled budget (Vina scales its MC steps with 50 + heavy atoms): complex l runs
+    # n_seg * (50 + nh_l) / (50 + mc_size_ref) segments; the loop runs to the largest of them
+    if cfg.get("mc_size_ref", 0) > 0:
+        nh = lb.heavy.sum(1).float()
+        seg_lim = torch.ceil(n_seg * (50 + nh) / (50 + cfg["mc_size_ref"])).long().clamp(min=1)
+        n_seg = int(seg_lim.max()) if n_seg > 0 else 0
+    else:
+        seg_lim = torch.full((L,), n_seg, dtype=torch.long, device=dev)
+    chain_steps = 0
+    seed64 = int(seed) * 1_000_003 + 12345
+    mc_args = (cfg["bfgs_iters"], cfg["mc_temperature"], cfg["mc_amp_pos"], cfg["mc_amp_rot"], seed64)
+    if n_seg == 0:
+        kp.run_mc(chain_lig, chain_id, cur, cur_e, best, best_e, True, 0, *mc_args, 0)
+    for sg in range(n_seg):
+        active = active & (seg_lim > sg)
+        idx = active.repeat_interleave(M).nonzero().squeeze(1)
+        if len(idx) == 0:
+            break
+        if len(idx) == N:
+            kp.run_mc(chain_lig, chain_id, cur, cur_e, best, best_e, sg == 0, seg, *mc_args, sg * seg)
+        else:   # compaction: only the chains of still-active complexes are launched
+            c_, ce_ = cur[idx].contiguous(), cur_e[idx].contiguous()
+            b_, be_ = best[idx].contiguous(), best_e[idx].contiguous()
+            kp.run_mc(chain_lig[idx], chain_id[idx], c_, ce_, b_, be_, sg == 0, seg, *mc_args, sg * seg)
+            cur[idx], cur_e[idx], best[idx], best_e[idx] = c_, ce_, b_, be_
+        chain_steps += len(idx) * seg
+        gb = best_e.view(L, M).min(1).values
+        improved = gb < glob - es_tol
+        glob = torch.minimum(glob, gb)
+        stall = torch.where(improved, torch.zeros_like(stall), stall + 1)
+        if es_rounds > 0:
+            active = active & (stall < es_rounds)
+        if log is not None:
+            log(f"segment {sg}: active {int(active.sum())}/{L}, best E mean {glob.mean().item():.3f}")
+    mark("mc")
+    # refine every chain's best state with a longer BFGS
+    if cfg.get("final_bfgs_iters", 0) > 0:
+        fe = torch.zeros(N, device=dev)
+        bb = best.clone()
+        kp.run_mc(chain_lig, chain_id, best, fe, bb, best_e, True, 0, cfg["final_bfgs_iters"], *mc_args[1:], 0)
+        best = bb
+    mark("refine")
+    # exact polish: the polish_top lowest-grid-energy chain bests per complex get polish_iters BFGS iterations on the
+    # exact pairwise energy, each over its receptor neighbour list
+    P = min(cfg.get("polish_top", 0), M)
+    if P > 0 and cfg.get("polish_iters", 0) > 0 and prob.rec is not None:
+        top = torch.topk(-best_e.view(L, M), P, dim=1).indices
+        sel = (top + torch.arange(L, device=dev)[:, None] * M).reshape(-1)
+        pc, pb = best[sel].contiguous(), best[sel].clone()
+        pe, pbe = torch.zeros(L * P, device=dev), torch.full((L * P,), float("inf"), device=dev)
+        margin = cfg.get("polish_nbr_margin", 2.0)
+        nb = kp.receptor_neighbours(prob, chain_lig[sel], pc, lb.K, margin) if margin > 0 else (None, None)
+        kp.run_mc(chain_lig[sel], chain_id[sel], pc, pe, pb, pbe, True, 0, cfg["polish_iters"], *mc_args[1:], 0,
+                  exact=True, nbr=nb[0], nbr_n=nb[1])
+        best, M, chain_lig = pb, P, chain_lig[sel]
+    mark("polish")
+    res = finalize_kernel(prob, kp, best, chain_lig, L, M, cfg)
+    mark("final")
+    if stats is not None:
+        ks = list(tick)
+        stats.update(widen=f, chain_steps=chain_steps, chain_steps_full=int(seg_lim.sum()) * cfg["n_chains"] * seg,
+                     **{f"t_{b}": round(tick[b] - tick[a], 3) for a, b in zip(ks[:-1], ks[1:])})
+    return res

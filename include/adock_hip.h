@@ -164,6 +164,18 @@ int adock_adam(long long n, float* w, const float* g, float* m, float* v, float 
                float eps, int t, adock_stream_t stream);
 int adock_to_bf16(long long n, const float* x, unsigned short* y, adock_stream_t stream);
 
+/* Stage A rigid enumeration. Replaces stagea_kernel (src/adock/mckernel.py:451-521; Python seam KernelProblem.stage_a,
+ * mckernel.py:690-705; host twin cpukernel.py:276-287). conf [L, C, AM, 3] heavy-atom conformer coordinates in the
+ * ligand frame, gslot / na from the packed ligand, quats [Q, 4], trans [L, NT_MAX, 3] translation lattices (padded),
+ * ntrans [L]; grid arguments as in the chain kernels. Out, per block b = (l * C + c) * Q + q: best_e[b] = min over
+ * the lattice of sum_a min(nearest-grid energy + out-of-box penalty, cap), best_i[b] = its lattice index (ties: the
+ * lowest). */
+int adock_stage_a(int AM, const float* conf, const int* gslot, const int* na, int L, int C, const float* quats, int Q,
+                  const float* trans, const int* ntrans, int NT_MAX,
+                  const float* grid, const float* origin, const float* lo, const float* hi, const int* rec_idx,
+                  const long long* goffs, const long long* gdims, float spacing, float pen, float cap,
+                  float* best_e, int* best_i, adock_stream_t stream);
+
 /* Launch shape of the chain kernels: chains (waves) per workgroup, 1..4, or 0 = adaptive (default: one chain per
  * workgroup while chains <= 8 x CUs, 2 up to 16 x CUs, else 4; the persistent batch kernel uses 4). The reference's
  * counterpart is the NW constant (src/adock/mckernel.py:525-540). Returns 0, or ADOCK_E_ARG. */

@@ -104,3 +104,68 @@ def vec_rel(a, b, floor):
     """max over rows of ||a - b|| / max(||b||, floor) (tests/test_cpukernel.py:203-207)."""
     a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
     return float((np.linalg.norm(a - b, axis=1) / np.maximum(np.linalg.norm(b, axis=1), floor)).max()) if a.size else 0.0
+
+
+class OracleKernel:
+    """CPU twin of adock_amd.kernel.KernelProblem for tests: same constructor and methods, every computation done by
+    the FP64 oracle on host tensors (the role the reference's CpuKernelProblem plays, src/adock/cpukernel.py:231-306).
+    Lets the search driver run on CPU for control-flow checks and as the comparison run for the GPU. The MC chains run
+    in the FP32 restatement, which (like the kernels) keeps the state in float32 between launches, so a run split
+    into segments equals the unsplit run."""
+
+    def __init__(self, prob):
+        import torch
+        from adock_amd import workload
+        self.AM, self.KM, lig, grid, sf, rec = workload.host_arrays(prob)
+        self.op = oracle.OracleProblem(self.AM, self.KM, lig, grid, sf, rec)
+        self.L, self.SD, self.DM = prob.lb.L, 7 + self.KM, 6 + self.KM
+        self.device = torch.device("cpu")
+
+    def pack(self, t, q, theta):
+        from adock_amd.kernel import KernelProblem
+        return KernelProblem.pack(self, t.cpu(), q.cpu(), theta.cpu())
+
+    def unpack(self, s, K):
+        from adock_amd.kernel import KernelProblem
+        return KernelProblem.unpack(self, s, K)
+
+    def energy_grad(self, chain_lig, state, exact=False):
+        import torch
+        e, g, _, _ = self.op.energy_grad(chain_lig.cpu().numpy(), state.cpu().numpy(), exact=exact)
+        return torch.as_tensor(e, dtype=torch.float32), torch.as_tensor(g, dtype=torch.float32)
+
+    def stage_a(self, conf_h, quats, trans, ntrans, cap):
+        import torch
+        be, bi = self.op.stage_a(conf_h.cpu().numpy(), quats.cpu().numpy(), trans.cpu().numpy(),
+                                 ntrans.cpu().numpy(), cap)
+        return torch.as_tensor(be, dtype=torch.float32), torch.as_tensor(bi, dtype=torch.int64)
+
+    def receptor_neighbours(self, prob, chain_lig, states, K, margin):
+        from adock_amd.kernel import KernelProblem
+        return KernelProblem.receptor_neighbours(self, prob, chain_lig, states, K, margin)
+
+    def run_mc(self, chain_lig, chain_id, cur, cur_e, best, best_e, init_opt, mc_steps, bfgs_iters, temp, amp_pos,
+               amp_rot, seed, step_offset, exact=False, nbr=None, nbr_n=None):
+        self.op.run_mc(chain_lig.cpu().numpy(), chain_id.cpu().numpy(), cur.numpy(), cur_e.numpy(), best.numpy(),
+                       best_e.numpy(), init_opt, mc_steps, bfgs_iters, temp, amp_pos, amp_rot, seed, step_offset,
+                       exact=exact, nbr=None if nbr is None else nbr.cpu().numpy(),
+                       nbr_n=None if nbr_n is None else nbr_n.cpu().numpy(), f64=False)
+
+
+def cpu_search_problem(rec, preps, center, size, sf="vina", rec_idx=None):
+    """A search Problem on host tensors: oracle-built grids in a GridSet, the ligand batch, the pair potential and the
+    padded receptor (for the exact polish)."""
+    import torch
+    from adock_amd.grid import GridSet
+    from adock_amd.problem import LigandBatch, PairPotential, Problem, pad_receptors
+    _, meta = cpu_problem(rec, preps, center, size, sf=sf, rec_idx=rec_idx)
+    size = np.broadcast_to(np.asarray(size, float), (3,))
+    center = np.asarray(center, float)
+    G = torch.as_tensor(meta["G"])
+    gs = GridSet([G], torch.as_tensor(meta["origin"], dtype=torch.float32)[None],
+                 torch.as_tensor(center - size / 2, dtype=torch.float32)[None],
+                 torch.as_tensor(center + size / 2, dtype=torch.float32)[None], SPACING, PENALTY,
+                 slot=torch.as_tensor(meta["slot"]), n_types=G.shape[0])
+    ri = torch.zeros(len(preps), dtype=torch.long) if rec_idx is None else torch.as_tensor(rec_idx)
+    return Problem(gs, LigandBatch(preps, "cpu"), ri, PairPotential(sf_params(CFG, sf), "cpu"),
+                   rec=pad_receptors([rec], "cpu"))
