@@ -11,7 +11,7 @@ import torch
 LIB_PATH = Path(__file__).resolve().parent / "libadock_hip.so"
 
 P, I, F, U64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_ulonglong
-LIG = [I, I] + [P] * 11 + [I, P, P, P]      # AM, KM, ref..rg, P_MAX, gslot, inc, ist
+LIG = [I, I] + [P] * 11 + [I, P, P, P, I]   # AM, KM, ref..rg, P_MAX, gslot, inc, ist, DEG_MAX
 GRID = [P] * 7 + [I, F, F]                  # grid..gdims, T, spacing, pen
 SF = [P] * 5
 REC = [P, P, P, I]

@@ -51,17 +51,19 @@ int adock_build_grid(const float* rxyz, const int* rtype, int n_rec, const float
 #define LIG_DECL                                                                                                     \
     const float *ref, const int *typ, const unsigned char *tm, const int *ax, const float *axp, const float *axu,    \
         const int *pr, const int *na, const int *nt, const int *np, const float *rg, int P_MAX, const int *gslot, \
-        const unsigned short *inc, const int *ist
+        const unsigned short *inc, const int *ist, int DEG_MAX
 #define GRID_DECL                                                                                                    \
     const float *grid, const float *origin, const float *lo, const float *hi, const int *rec_idx,                    \
         const long long *goffs, const long long *gdims, int T, float spacing, float pen
 #define SF_DECL const float *prm, const float *rad, const int *hyd, const int *don, const int *acc
 #define REC_DECL const float *rxyz, const int *rtyp, const int *rn, int RN_MAX
 
-static int common_checks(int AM, int KM, int P_MAX, const void* ref, const void* inc, const void* ist, const void* grid,
-                         const void* prm, int exact, const void* rxyz, const void* rn, int RN_MAX,
+static int common_checks(int AM, int KM, int P_MAX, const void* ref, const void* inc, const void* ist, int DEG_MAX,
+                         const void* grid, const void* prm, int exact, const void* rxyz, const void* rn, int RN_MAX,
                          const void* chain_lig, int n_chains) {
     if (int st = check_buckets(AM, KM, P_MAX)) return st;
+    if (DEG_MAX < 0 || DEG_MAX > 64 || (DEG_MAX & 7))
+        return fail(ADOCK_E_ARG, "DEG_MAX must be a multiple of 8 in 0..64");
     if (n_chains < 0) return fail(ADOCK_E_ARG, "negative n_chains");
     if (!ref || !inc || !ist || !grid || !prm || !chain_lig) return fail(ADOCK_E_ARG, "null pointer");
     if (exact && (!rxyz || !rn || RN_MAX <= 0)) return fail(ADOCK_E_ARG, "exact mode needs receptor atoms");
@@ -70,11 +72,12 @@ static int common_checks(int AM, int KM, int P_MAX, const void* ref, const void*
 
 int adock_eg(int AM, int KM, LIG_DECL, GRID_DECL, SF_DECL, REC_DECL, int exact, const int* chain_lig, int n_chains,
              const float* state, float* e_out, float* g_out, adock_stream_t stream) {
-    if (int st = common_checks(AM, KM, P_MAX, ref, inc, ist, grid, prm, exact, rxyz, rn, RN_MAX, chain_lig, n_chains))
+    if (int st = common_checks(AM, KM, P_MAX, ref, inc, ist, DEG_MAX, grid, prm, exact, rxyz, rn, RN_MAX, chain_lig,
+                               n_chains))
         return st;
     if (n_chains == 0) return 0;
     if (!state || !e_out || !g_out) return fail(ADOCK_E_ARG, "null pointer");
-    LigArgs L{ref, typ, tm, ax, axp, axu, pr, na, nt, np, rg, P_MAX, gslot, inc, ist};
+    LigArgs L{ref, typ, tm, ax, axp, axu, pr, na, nt, np, rg, P_MAX, gslot, inc, ist, DEG_MAX};
     GridArgs G{grid, origin, lo, hi, rec_idx, goffs, gdims, T, spacing, pen};
     SFArgs S{prm, rad, hyd, don, acc};
     RecArgs R{rxyz, rtyp, rn, RN_MAX};
@@ -86,13 +89,14 @@ int adock_mc_counted(int AM, int KM, LIG_DECL, GRID_DECL, SF_DECL, REC_DECL, int
                      float* cur_e, float* best, float* best_e, int init_opt, int mc_steps, int bfgs_iters, float temp,
                      float amp_pos, float amp_rot, unsigned long long seed, int step_offset,
                      unsigned long long* eval_count, adock_stream_t stream) {
-    if (int st = common_checks(AM, KM, P_MAX, ref, inc, ist, grid, prm, exact, rxyz, rn, RN_MAX, chain_lig, n_chains))
+    if (int st = common_checks(AM, KM, P_MAX, ref, inc, ist, DEG_MAX, grid, prm, exact, rxyz, rn, RN_MAX, chain_lig,
+                               n_chains))
         return st;
     if (n_chains == 0) return 0;
     if (!chain_id || !cur || !cur_e || !best || !best_e) return fail(ADOCK_E_ARG, "null pointer");
     if (mc_steps < 0) return fail(ADOCK_E_ARG, "negative mc_steps");
     if (nbr && (!nbr_n || NB_MAX <= 0)) return fail(ADOCK_E_ARG, "nbr needs nbr_n and NB_MAX > 0");
-    LigArgs L{ref, typ, tm, ax, axp, axu, pr, na, nt, np, rg, P_MAX, gslot, inc, ist};
+    LigArgs L{ref, typ, tm, ax, axp, axu, pr, na, nt, np, rg, P_MAX, gslot, inc, ist, DEG_MAX};
     GridArgs G{grid, origin, lo, hi, rec_idx, goffs, gdims, T, spacing, pen};
     SFArgs S{prm, rad, hyd, don, acc};
     RecArgs R{rxyz, rtyp, rn, RN_MAX};
@@ -105,8 +109,8 @@ int adock_mc(int AM, int KM, LIG_DECL, GRID_DECL, SF_DECL, REC_DECL, int exact, 
              const int* chain_id, int n_chains, const int* nbr, const int* nbr_n, int NB_MAX, float* cur, float* cur_e,
              float* best, float* best_e, int init_opt, int mc_steps, int bfgs_iters, float temp, float amp_pos,
              float amp_rot, unsigned long long seed, int step_offset, adock_stream_t stream) {
-    return adock_mc_counted(AM, KM, ref, typ, tm, ax, axp, axu, pr, na, nt, np, rg, P_MAX, gslot, inc, ist, grid,
-                            origin, lo, hi,
+    return adock_mc_counted(AM, KM, ref, typ, tm, ax, axp, axu, pr, na, nt, np, rg, P_MAX, gslot, inc, ist, DEG_MAX,
+                            grid, origin, lo, hi,
                             rec_idx, goffs, gdims, T, spacing, pen, prm, rad, hyd, don, acc, rxyz, rtyp, rn, RN_MAX,
                             exact, chain_lig, chain_id, n_chains, nbr, nbr_n, NB_MAX, cur, cur_e, best, best_e,
                             init_opt, mc_steps, bfgs_iters, temp, amp_pos, amp_rot, seed, step_offset, nullptr,
@@ -119,14 +123,15 @@ int adock_mc_batch(int AM, int KM, LIG_DECL, GRID_DECL, SF_DECL, REC_DECL, int e
                    float amp_pos, float amp_rot, unsigned long long seed, int step_offset, const int* order,
                    int* counter, int* done, const int* lig_chains, unsigned long long* times, int blocks,
                    unsigned long long* eval_count, adock_stream_t stream) {
-    if (int st = common_checks(AM, KM, P_MAX, ref, inc, ist, grid, prm, exact, rxyz, rn, RN_MAX, chain_lig, n_chains))
+    if (int st = common_checks(AM, KM, P_MAX, ref, inc, ist, DEG_MAX, grid, prm, exact, rxyz, rn, RN_MAX, chain_lig,
+                               n_chains))
         return st;
     if (n_chains == 0) return 0;
     if (!chain_id || !cur || !cur_e || !best || !best_e || !counter) return fail(ADOCK_E_ARG, "null pointer");
     if (done && !lig_chains) return fail(ADOCK_E_ARG, "done needs lig_chains");
     if (mc_steps < 0) return fail(ADOCK_E_ARG, "negative mc_steps");
     if (nbr && (!nbr_n || NB_MAX <= 0)) return fail(ADOCK_E_ARG, "nbr needs nbr_n and NB_MAX > 0");
-    LigArgs L{ref, typ, tm, ax, axp, axu, pr, na, nt, np, rg, P_MAX, gslot, inc, ist};
+    LigArgs L{ref, typ, tm, ax, axp, axu, pr, na, nt, np, rg, P_MAX, gslot, inc, ist, DEG_MAX};
     GridArgs G{grid, origin, lo, hi, rec_idx, goffs, gdims, T, spacing, pen};
     SFArgs S{prm, rad, hyd, don, acc};
     RecArgs R{rxyz, rtyp, rn, RN_MAX};
@@ -148,13 +153,14 @@ int adock_mc_step(int AM, int KM, LIG_DECL, GRID_DECL, SF_DECL, REC_DECL, const 
                   float temp, float amp_pos, float amp_rot, unsigned long long seed, const float* Y, float* feat,
                   float* log_buf, int* log_count, int log_cap, int use_net, float p_plain, float sig_t, float sig_r,
                   float sig_th, unsigned long long* eval_count, adock_stream_t stream) {
-    if (int st = common_checks(AM, KM, P_MAX, ref, inc, ist, grid, prm, 0, rxyz, rn, RN_MAX, chain_lig, n_chains))
+    if (int st = common_checks(AM, KM, P_MAX, ref, inc, ist, DEG_MAX, grid, prm, 0, rxyz, rn, RN_MAX, chain_lig,
+                               n_chains))
         return st;
     if (n_chains == 0) return 0;
     if (!chain_id || !cur || !cur_e || !best || !best_e) return fail(ADOCK_E_ARG, "null pointer");
     if (log_buf && (!log_count || !feat || log_cap < 0)) return fail(ADOCK_E_ARG, "logging needs feat and a counter");
     if (use_net && (!Y || !feat)) return fail(ADOCK_E_ARG, "network moves need Y and feat");
-    LigArgs L{ref, typ, tm, ax, axp, axu, pr, na, nt, np, rg, P_MAX, gslot, inc, ist};
+    LigArgs L{ref, typ, tm, ax, axp, axu, pr, na, nt, np, rg, P_MAX, gslot, inc, ist, DEG_MAX};
     GridArgs G{grid, origin, lo, hi, rec_idx, goffs, gdims, T, spacing, pen};
     SFArgs S{prm, rad, hyd, don, acc};
     RecArgs R{rxyz, rtyp, rn, RN_MAX};

@@ -42,13 +42,26 @@ constexpr int AMK = 64;   // heavy-atom bucket handled by these kernels (one ato
 //   pl  [P128]      u16     pair words, strided over lanes (pair q on lane q % 64 in round q / 64)
 //   inc [2 P128 + 512] u16  per-atom pair-slot lists, copied from the host-built packed ligand (each atom's list
 //                           padded to 8 entries; start and padded count live in the atom's lane); bit 15 = negate
+// Dense variant (the latency-bound speculative kernel): FS is replaced by FD [64 DEG_MAX + 1] float4, row k column
+// a ^ k = the gradient of atom a's k-th pair (sign applied by the writer; unused slots stay zero, the last one takes
+// the padding pairs' writes), and inc by pdst [P128] u32 (destination slot of pair q's first atom | second atom << 16).
+// The per-atom sum reads one permuted row per entry (no list indirection, no bank conflicts); the XOR skew spreads
+// the writes of one atom's pairs over the banks.
 __host__ __device__ inline int pad128(int P_MAX) { return (P_MAX + 127) & ~127; }
 __host__ __device__ inline int inc_stride(int P_MAX) { return 2 * pad128(P_MAX) + 512; }
 __host__ __device__ inline int wave_lds_bytes(int KM, int P_MAX) {
     const int P = pad128(P_MAX);
     return 16 * AMK + 4 * 16 * (KM + 1) + 16 * (P + 1) + 4 * AMK + 2 * P + 2 * inc_stride(P_MAX);
 }
-constexpr int LDS_PER_WORKGROUP = 64 * 1024;
+__host__ __device__ inline int wave_lds_bytes_dense(int KM, int P_MAX, int DEG_MAX) {
+    const int P = pad128(P_MAX);
+    return 16 * AMK + 4 * 16 * (KM + 1) + 16 * (AMK * DEG_MAX + 1) + 4 * AMK + 2 * P + 4 * P;
+}
+template <bool DENSE>
+__host__ __device__ inline int lds_bytes(int KM, int P_MAX, int DEG_MAX) {
+    return DENSE ? wave_lds_bytes_dense(KM, P_MAX, DEG_MAX) : wave_lds_bytes(KM, P_MAX);
+}
+constexpr int LDS_PER_WORKGROUP = 160 * 1024;   // gfx950: 160 KB of LDS per CU, all of it addressable by one workgroup
 
 struct WaveLDS {
     float4* X4;
@@ -57,7 +70,9 @@ struct WaveLDS {
     int* flg;
     unsigned short* pl;
     unsigned short* inc;
-    int P128;
+    float4* FD;           // dense variant
+    unsigned* pdst;
+    int P128, trash;
 };
 
 struct ChainCtx {
@@ -80,8 +95,8 @@ struct ChainCtx {
     const float* sfrad; const int* sfhyd; const int* sfdon; const int* sfacc;
 };
 
-template <int KM>
-DEVI WaveLDS carve_lds(char* base, int P_MAX) {
+template <int KM, bool DENSE = false>
+DEVI WaveLDS carve_lds(char* base, int P_MAX, int DEG_MAX = 0) {
     WaveLDS w;
     w.P128 = pad128(P_MAX);
     float4* f = reinterpret_cast<float4*>(base);
@@ -90,14 +105,20 @@ DEVI WaveLDS carve_lds(char* base, int P_MAX) {
     w.TB = f; f += KM + 1;
     w.TC = f; f += KM + 1;
     w.TD = f; f += KM + 1;
-    w.FS = f; f += w.P128 + 1;
+    w.FS = w.FD = nullptr;
+    w.inc = nullptr;
+    w.pdst = nullptr;
+    w.trash = AMK * DEG_MAX;
+    if constexpr (DENSE) { w.FD = f; f += AMK * DEG_MAX + 1; }
+    else { w.FS = f; f += w.P128 + 1; }
     w.flg = reinterpret_cast<int*>(f);
     w.pl = reinterpret_cast<unsigned short*>(w.flg + AMK);
-    w.inc = w.pl + w.P128;
+    if constexpr (DENSE) w.pdst = reinterpret_cast<unsigned*>(w.pl + w.P128);
+    else w.inc = w.pl + w.P128;
     return w;
 }
 
-template <int KM>
+template <int KM, bool DENSE = false>
 DEVI ChainCtx setup_chain(int l, int lane, const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecArgs& R,
                           int exact, const WaveLDS& w) {
     ChainCtx c;
@@ -144,7 +165,12 @@ DEVI ChainCtx setup_chain(int l, int lane, const LigArgs& L, const GridArgs& G, 
         w.TC[lane] = make_float4(0.f, 0.f, 0.f, 0.f); w.TD[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
     w.flg[lane] = c.flg;
-    if (lane == 0) w.FS[w.P128] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if constexpr (DENSE) {
+        for (int i = lane; i <= w.trash; i += WAVE) w.FD[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int q = lane; q < w.P128; q += WAVE) w.pdst[q] = (unsigned)w.trash | ((unsigned)w.trash << 16);
+    } else if (lane == 0) {
+        w.FS[w.P128] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
     wave_lds_sync();
     // pair word: i | j << 6 | (hydrophobic pair) << 12 | (h-bond pair) << 13; 0x8000 marks padding (and self-pairs)
     const int* pr = L.pr + (long long)l * L.P_MAX * 2;
@@ -157,14 +183,25 @@ DEVI ChainCtx setup_chain(int l, int lane, const LigArgs& L, const GridArgs& G, 
         w.pl[q] = word;
     }
     wave_lds_sync();
-    // this atom's pair-slot list (host-built, 8 entries per 16-byte copy)
+    // this atom's pair-slot list (host-built): copied to LDS 8 entries per 16-byte copy, or (dense variant) turned
+    // into the destination slots of its pairs
     {
         const int e = c.atom ? L.ist[(long long)l * AMK + lane] : 0;
         c.is = e & 0xffff;
-        c.ic8 = (e >> 16) & 0xffff;
-        const uint4* src = reinterpret_cast<const uint4*>(L.inc + (long long)l * inc_stride(L.P_MAX) + c.is);
-        uint4* dst = reinterpret_cast<uint4*>(w.inc + c.is);
-        for (int g = 0; g < (c.ic8 >> 3); ++g) dst[g] = src[g];
+        c.ic8 = min((e >> 16) & 0xffff, DENSE ? L.DEG_MAX : 0xffff);
+        const unsigned short* src = L.inc + (long long)l * inc_stride(L.P_MAX) + c.is;
+        if constexpr (DENSE) {
+            unsigned short* p16 = reinterpret_cast<unsigned short*>(w.pdst);
+            for (int k = 0; k < c.ic8; ++k) {
+                const unsigned v = src[k];
+                if ((v & 0x7fffu) < (unsigned)c.np)
+                    p16[2 * (v & 0x7fffu) + (v >> 15)] = (unsigned short)(k * AMK + (lane ^ k));
+            }
+        } else {
+            const uint4* src4 = reinterpret_cast<const uint4*>(src);
+            uint4* dst = reinterpret_cast<uint4*>(w.inc + c.is);
+            for (int g = 0; g < (c.ic8 >> 3); ++g) dst[g] = src4[g];
+        }
         c.D8 = (int)wave_max_f((float)c.ic8);
     }
     c.exact = exact;
@@ -193,7 +230,7 @@ DEVI float pair_slot(const WaveLDS& w, unsigned pw, const SFConst& sf, float4& s
 }
 
 // Energy and lane-distributed gradient of the lane-distributed state x. Returns E (uniform).
-template <int KM>
+template <int KM, bool DENSE = false>
 DEVI float eval_eg(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, int lane, float x, float& gout, float& ea,
                    float& Xx, float& Xy, float& Xz) {
     const float t0 = bcast(x, 0), t1 = bcast(x, 1), t2 = bcast(x, 2);
@@ -266,21 +303,39 @@ DEVI float eval_eg(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, int l
         w.TC[lane] = make_float4(A.x, A.y, A.z, 0.f);
         w.TD[lane] = make_float4(ux * nn, uy * nn, uz * nn, 0.f);
     }
-    // intra-ligand pairs, two rounds of 64 per iteration, branch-free: each pair writes its gradient slot
+    // intra-ligand pairs, two rounds of 64 per iteration, branch-free: each pair writes its gradient slot (dense
+    // variant: +slot into its first atom's row, -slot into its second atom's row)
     float ep_l = 0.f;
     for (int q0 = lane; q0 < c.np2; q0 += 2 * WAVE) {
         const unsigned pw0 = w.pl[q0], pw1 = w.pl[q0 + WAVE];
         float4 s0, s1;
         ep_l += pair_slot(w, pw0, sf, s0);
         ep_l += pair_slot(w, pw1, sf, s1);
-        w.FS[q0] = s0;
-        w.FS[q0 + WAVE] = s1;
+        if constexpr (DENSE) {
+            const unsigned d0 = w.pdst[q0], d1 = w.pdst[q0 + WAVE];
+            w.FD[d0 & 0xffffu] = s0;
+            w.FD[d0 >> 16] = make_float4(-s0.x, -s0.y, -s0.z, 0.f);
+            w.FD[d1 & 0xffffu] = s1;
+            w.FD[d1 >> 16] = make_float4(-s1.x, -s1.y, -s1.z, 0.f);
+        } else {
+            w.FS[q0] = s0;
+            w.FS[q0 + WAVE] = s1;
+        }
     }
     wave_lds_sync();
     // each atom sums its slots: 8 list entries per 16-byte read, then 8 independent slot reads in flight (padding
     // entries and lanes past their list read the zero slot)
     float pfx = 0.f, pfy = 0.f, pfz = 0.f;
-    {
+    if constexpr (DENSE) {
+        for (int k = 0; k < c.D8; k += 8) {
+            const int xk = lane ^ k;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const float4 f = w.FD[(k + u) * AMK + (xk ^ u)];
+                pfx += f.x; pfy += f.y; pfz += f.z;
+            }
+        }
+    } else {
         const uint4* inc4 = reinterpret_cast<const uint4*>(w.inc);
         for (int k = 0; k < c.D8; k += 8) {
             const bool ok = k < c.ic8;
@@ -403,7 +458,7 @@ DEVI float plain_move(float x, const ChainCtx& c, int lane, unsigned long long s
 
 // Vina-style BFGS on the lane-distributed state x (in/out); returns the final energy. One evaluation site: the first
 // pass evaluates x itself, every later pass evaluates one line-search trial.
-template <int KM>
+template <int KM, bool DENSE = false>
 DEVI float bfgs(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, int lane, float& x, int iters, int& nev) {
     constexpr int DV = 7 + KM;
     float h[DV];
@@ -415,7 +470,7 @@ DEVI float bfgs(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, int lane
     while (true) {
         const float xe = trial < 0 ? x : conf_inc(x, p, alpha, lane);
         float ge, ea_, x0_, x1_, x2_;
-        const float fe = eval_eg<KM>(c, sf, w, lane, xe, ge, ea_, x0_, x1_, x2_);
+        const float fe = eval_eg<KM, DENSE>(c, sf, w, lane, xe, ge, ea_, x0_, x1_, x2_);
         ++nev;
         if (trial < 0) {
             f0 = fe; f_orig = fe; g = ge;
@@ -480,7 +535,7 @@ DEVI SpecLDS carve_spec(char* base) {
     return SpecLDS{f, f + 64, f + 128, reinterpret_cast<int*>(f + 132)};
 }
 
-template <int KM, int NS>
+template <int KM, int NS, bool DENSE>
 DEVI float bfgs_spec(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, const SpecLDS& xs, int lane, int wid,
                      float& x, int iters, int& nev) {
     constexpr int DV = 7 + KM;
@@ -489,7 +544,7 @@ DEVI float bfgs_spec(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, con
     for (int j = 0; j < DV; ++j) h[j] = (j == lane) ? 1.f : 0.f;
     const float xo = x;
     float g, ea_, x0_, x1_, x2_;
-    float f0 = eval_eg<KM>(c, sf, w, lane, x, g, ea_, x0_, x1_, x2_);
+    float f0 = eval_eg<KM, DENSE>(c, sf, w, lane, x, g, ea_, x0_, x1_, x2_);
     ++nev;
     const float f_orig = f0;
     int step = 0;
@@ -507,7 +562,7 @@ DEVI float bfgs_spec(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, con
             if (t < 10) {
                 const float at = ldexpf(1.f, -t);
                 xe = conf_inc(x, p, at, lane);
-                fe = eval_eg<KM>(c, sf, w, lane, xe, ge, ea_, x0_, x1_, x2_);
+                fe = eval_eg<KM, DENSE>(c, sf, w, lane, xe, ge, ea_, x0_, x1_, x2_);
                 ok = fe - f0 < 1e-4f * at * pg;
             }
             if (lane == 0) xs.ok[wid] = ok ? 1 : 0;
@@ -570,14 +625,14 @@ DEVI float bfgs_spec(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, con
 
 // One MC chain on the calling wave (all 64 lanes active), or on the NS waves of a workgroup with the speculative line
 // search (wave wid; wave 0 writes the results). w: this wave's LDS region.
-template <int KM, int NS = 1>
+template <int KM, int NS = 1, bool DENSE = false>
 DEVI int run_chain(int ch, int lane, const WaveLDS& w, const LigArgs& L, const GridArgs& G, const SFArgs& S,
                    const RecArgs& R, int exact, const int* __restrict__ chain_lig, const McArgs& M, int wid = 0,
                    SpecLDS xs = SpecLDS{}) {
     constexpr int SD = 7 + KM;
     const SFConst sf = make_sf(S.prm);
     const int l = __builtin_amdgcn_readfirstlane(chain_lig[ch]);
-    ChainCtx c = setup_chain<KM>(l, lane, L, G, S, R, exact, w);
+    ChainCtx c = setup_chain<KM, DENSE>(l, lane, L, G, S, R, exact, w);
     if (M.nbr) { c.nb = M.nbr + (long long)ch * M.NB_MAX; c.nbn = M.nbr_n[ch]; }
     const int cid = __builtin_amdgcn_readfirstlane(M.chain_id[ch]);
     const int S_ = 7 + c.nt;
@@ -600,8 +655,8 @@ DEVI int run_chain(int ch, int lane, const WaveLDS& w, const LigArgs& L, const G
             x = plain_move(x, c, lane, seed, cid, sg, M.amp_pos, M.amp_rot);
         }
         float en;
-        if constexpr (NS == 1) en = bfgs<KM>(c, sf, w, lane, x, iters, nev);
-        else en = bfgs_spec<KM, NS>(c, sf, w, xs, lane, wid, x, iters, nev);
+        if constexpr (NS == 1) en = bfgs<KM, DENSE>(c, sf, w, lane, x, iters, nev);
+        else en = bfgs_spec<KM, NS, DENSE>(c, sf, w, xs, lane, wid, x, iters, nev);
         if (st < 0) {
             ec = en; cur = x;
             if (ec < eb) { eb = ec; best = cur; }
@@ -644,9 +699,9 @@ __global__ __launch_bounds__(256) void mc_spec_kernel(LigArgs L, GridArgs G, SFA
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
     const int ch = blockIdx.x;
     if (ch >= n_chains) return;
-    const int wb = wave_lds_bytes(KM, L.P_MAX);
-    const WaveLDS w = carve_lds<KM>(lds + wid * wb, L.P_MAX);
-    run_chain<KM, NS>(ch, lane, w, L, G, S, R, exact, chain_lig, M, wid, carve_spec(lds + NS * wb));
+    const int wb = wave_lds_bytes_dense(KM, L.P_MAX, L.DEG_MAX);
+    const WaveLDS w = carve_lds<KM, true>(lds + wid * wb, L.P_MAX, L.DEG_MAX);
+    run_chain<KM, NS, true>(ch, lane, w, L, G, S, R, exact, chain_lig, M, wid, carve_spec(lds + NS * wb));
 }
 
 // Persistent variant: a grid sized to the chip's resident capacity; every wave pulls chain indices from a global
@@ -729,30 +784,49 @@ int g_line_search_width = 0;   // 0: adaptive (line_search_width)
 
 // waves per chain of the speculative line search: 4 while the chains fit one per CU, 2 up to two per CU, else 1;
 // lowered until the workgroup's LDS fits
-inline int line_search_width(int n_chains, int KM, int P_MAX) {
+inline int line_search_width(int n_chains, int KM, int P_MAX, int DEG_MAX) {
     int ns = g_line_search_width;
     if (ns <= 0) ns = n_chains <= cu_count() ? 4 : n_chains <= 2 * cu_count() ? 2 : 1;
-    while (ns > 1 && (size_t)ns * wave_lds_bytes(KM, P_MAX) + SPEC_LDS_BYTES > (size_t)LDS_PER_WORKGROUP) ns >>= 1;
+    while (ns > 1 && (size_t)ns * wave_lds_bytes_dense(KM, P_MAX, DEG_MAX) + SPEC_LDS_BYTES >
+                         (size_t)LDS_PER_WORKGROUP)
+        ns >>= 1;
     return ns;
+}
+
+// dynamic LDS above 64 KB is requested from the runtime once per kernel and size
+inline void allow_lds(const void* fn, size_t bytes) {
+    if (bytes <= 64 * 1024) return;
+    static std::mutex mu;
+    static std::map<const void*, size_t> granted;
+    std::lock_guard<std::mutex> lock(mu);
+    size_t& g = granted[fn];
+    if (bytes > g) {
+        (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+        g = bytes;
+    }
 }
 
 template <int KM>
 int launch_mc_t(const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecArgs& R, int exact,
                 const int* chain_lig, int n_chains, const McArgs& M, hipStream_t stream) {
-    const int ns = line_search_width(n_chains, KM, L.P_MAX);
+    const int ns = line_search_width(n_chains, KM, L.P_MAX, L.DEG_MAX);
     if (ns > 1) {
-        const size_t shm = (size_t)ns * wave_lds_bytes(KM, L.P_MAX) + SPEC_LDS_BYTES;
-        if (ns == 4)
+        const size_t shm = (size_t)ns * wave_lds_bytes_dense(KM, L.P_MAX, L.DEG_MAX) + SPEC_LDS_BYTES;
+        if (ns == 4) {
+            allow_lds(reinterpret_cast<const void*>(mc_spec_kernel<KM, 4>), shm);
             hipLaunchKernelGGL((mc_spec_kernel<KM, 4>), dim3(n_chains), dim3(4 * WAVE), shm, stream, L, G, S, R, exact,
                                chain_lig, n_chains, M);
-        else
+        } else {
+            allow_lds(reinterpret_cast<const void*>(mc_spec_kernel<KM, 2>), shm);
             hipLaunchKernelGGL((mc_spec_kernel<KM, 2>), dim3(n_chains), dim3(2 * WAVE), shm, stream, L, G, S, R, exact,
                                chain_lig, n_chains, M);
+        }
         return (int)hipGetLastError();
     }
     const int W = fit_waves(waves_for(n_chains), KM, L.P_MAX);
     const size_t shm = (size_t)W * wave_lds_bytes(KM, L.P_MAX);
     const unsigned nblk = (unsigned)((n_chains + W - 1) / W);
+    allow_lds(reinterpret_cast<const void*>(mc_kernel<KM>), shm);
     hipLaunchKernelGGL(mc_kernel<KM>, dim3(nblk), dim3(W * WAVE), shm, stream, L, G, S, R, exact, chain_lig,
                        n_chains, M);
     return (int)hipGetLastError();
@@ -783,6 +857,7 @@ int launch_mc_persistent_t(const LigArgs& L, const GridArgs& G, const SFArgs& S,
     }
     const int need = (n_chains + W - 1) / W;
     if (blocks > need) blocks = need;
+    allow_lds(reinterpret_cast<const void*>(mc_persistent_kernel<KM>), shm);
     hipLaunchKernelGGL(mc_persistent_kernel<KM>, dim3(blocks), dim3(W * WAVE), shm, stream, L, G, S, R, exact,
                        chain_lig, n_chains, M, Q);
     return (int)hipGetLastError();
@@ -795,6 +870,7 @@ int launch_eg_t(const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecA
     const int W = fit_waves(waves_for(n_chains), KM, L.P_MAX);
     const size_t shm = (size_t)W * wave_lds_bytes(KM, L.P_MAX);
     const unsigned nblk = (unsigned)((n_chains + W - 1) / W);
+    allow_lds(reinterpret_cast<const void*>(eg_kernel<KM>), shm);
     hipLaunchKernelGGL(eg_kernel<KM>, dim3(nblk), dim3(W * WAVE), shm, stream, L, G, S, R, exact, chain_lig,
                        n_chains, state, e_out, g_out);
     return (int)hipGetLastError();
@@ -1007,6 +1083,7 @@ int launch_mc_step_t(const LigArgs& L, const GridArgs& G, const SFArgs& S, const
     const int W = fit_waves(waves_for(n_chains), KM, L.P_MAX);
     const size_t shm = (size_t)W * wave_lds_bytes(KM, L.P_MAX);
     const unsigned nblk = (unsigned)((n_chains + W - 1) / W);
+    allow_lds(reinterpret_cast<const void*>(mc_step_kernel<KM>), shm);
     hipLaunchKernelGGL(mc_step_kernel<KM>, dim3(nblk), dim3(W * WAVE), shm, stream, L, G, S, R, chain_lig, n_chains,
                        M, P);
     return (int)hipGetLastError();

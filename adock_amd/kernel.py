@@ -39,6 +39,7 @@ class KernelProblem:
         T = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=dev)
         self.lig = {k: (T(v) if isinstance(v, np.ndarray) else v) for k, v in lig.items()}
         self.P_MAX = int(lig["P"])
+        self.DEG_MAX = int(lig["DEG"])
         self.grid = dict(flat=gs.flat.float().contiguous(), origin=gs.origin.float().contiguous(),
                          lo=gs.lo.float().contiguous(), hi=gs.hi.float().contiguous(),
                          rec_idx=prob.rec_idx.to(device=dev, dtype=torch.int32).contiguous(),
@@ -85,7 +86,7 @@ class KernelProblem:
         p = _lib.ptr
         return ([self.AM, self.KM] + [p(L[k]) for k in ("ref", "typ", "tm", "ax", "axp", "axu", "pr", "na", "nt",
                                                          "np", "rg")]
-                + [self.P_MAX, p(L["gslot"]), p(L["inc"]), p(L["ist"])]
+                + [self.P_MAX, p(L["gslot"]), p(L["inc"]), p(L["ist"]), self.DEG_MAX]
                 + [p(G[k]) for k in ("flat", "origin", "lo", "hi", "rec_idx", "offs", "dims")]
                 + [G["T"], G["spacing"], G["penalty"]]
                 + [p(S[k]) for k in ("prm", "rad", "hyd", "don", "acc")]

@@ -180,8 +180,8 @@ def pack_ligands(preps, slot, rec_idx):
     """Packed ligand arrays in the search-kernel layout (src/adock/mckernel.py:547-592).
 
     slot [R, n_types] int (XS type -> grid slot of receptor r, -1 = not built), rec_idx [L]. Returns (AM, KM, dict of
-    numpy arrays: ref, typ, tm, ax, axp, axu, pr, na, nt, np, rg, P, gslot, inc, ist); inc / ist: slot_lists of each
-    ligand."""
+    numpy arrays: ref, typ, tm, ax, axp, axu, pr, na, nt, np, rg, P, gslot, inc, ist, DEG); inc / ist: slot_lists of
+    each ligand, DEG: the longest (padded) list."""
     L = len(preps)
     nh = [int(p.heavy.sum()) for p in preps]
     AM = 64 if max(nh) <= 64 else bucket(max(nh), 32)
@@ -227,6 +227,7 @@ def pack_ligands(preps, slot, rec_idx):
         if (out["gslot"][l, :len(hv)] < 0).any():
             raise ValueError("a ligand atom type has no grid")
     out["P"] = np.int32(P)
+    out["DEG"] = np.int32(int((out["ist"] >> 16).max()) if L else 0)
     return AM, KM, out
 
 

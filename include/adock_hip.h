@@ -6,7 +6,7 @@
  * launch. adock_last_error() gives a text for the last non-zero status of the calling thread.
  *
  * Argument groups follow the reference's kernel parameter order (src/adock/mckernel.py:328-345, bound by
- * src/adock/cpukernel.py:97-101,263): ligand (13 + 2), grid (10), scoring function (5), receptor (4), exact, then the
+ * src/adock/cpukernel.py:97-101,263): ligand (13 + 3), grid (10), scoring function (5), receptor (4), exact, then the
  * entry's own parameters. The reference compiles one module per (A_MAX, K_MAX) bucket; here AM and KM are the first
  * two arguments and select a compiled instantiation (AM = 64; KM in {8, 16, 24, 32}).
  *
@@ -14,8 +14,9 @@
  * na, nt, np, rg, P_MAX, gslot) plus two arrays of this library's own, built once per ligand set on the host
  * (adock_amd.ligand.slot_lists): inc [L, 2 * P128 + 512] uint16 and ist [L, AM] int32, P128 = P_MAX rounded up to a
  * multiple of 128. Atom a's list inc[l, s .. s + c) (ist = s | c << 16, s and c multiples of 8) names the intra pairs
- * of atom a: q for a pair q = (a, j), q | 0x8000 for a pair q = (i, a), ascending, padded with P128. The kernels use
- * it to sum the pair gradient of every atom without atomics.
+ * of atom a: q for a pair q = (a, j), q | 0x8000 for a pair q = (i, a), ascending, padded with P128; DEG_MAX is the
+ * largest c over all atoms and ligands (a multiple of 8, at most 64). The kernels use the lists to sum the pair
+ * gradient of every atom without atomics; DEG_MAX sizes the per-atom gradient rows of the single-complex kernel.
  */
 #ifndef ADOCK_HIP_H
 #define ADOCK_HIP_H
@@ -51,7 +52,7 @@ int adock_build_grid(const float* rxyz, const int* rtype, int n_rec, const float
 int adock_eg(int AM, int KM,
              const float* ref, const int* typ, const unsigned char* tm, const int* ax, const float* axp,
              const float* axu, const int* pr, const int* na, const int* nt, const int* np, const float* rg, int P_MAX,
-             const int* gslot, const unsigned short* inc, const int* ist,
+             const int* gslot, const unsigned short* inc, const int* ist, int DEG_MAX,
              const float* grid, const float* origin, const float* lo, const float* hi, const int* rec_idx,
              const long long* goffs, const long long* gdims, int T, float spacing, float pen,
              const float* prm, const float* rad, const int* hyd, const int* don, const int* acc,
@@ -65,7 +66,7 @@ int adock_eg(int AM, int KM,
 int adock_mc(int AM, int KM,
              const float* ref, const int* typ, const unsigned char* tm, const int* ax, const float* axp,
              const float* axu, const int* pr, const int* na, const int* nt, const int* np, const float* rg, int P_MAX,
-             const int* gslot, const unsigned short* inc, const int* ist,
+             const int* gslot, const unsigned short* inc, const int* ist, int DEG_MAX,
              const float* grid, const float* origin, const float* lo, const float* hi, const int* rec_idx,
              const long long* goffs, const long long* gdims, int T, float spacing, float pen,
              const float* prm, const float* rad, const int* hyd, const int* don, const int* acc,
@@ -82,7 +83,7 @@ int adock_mc(int AM, int KM,
 int adock_mc_counted(int AM, int KM,
                      const float* ref, const int* typ, const unsigned char* tm, const int* ax, const float* axp,
                      const float* axu, const int* pr, const int* na, const int* nt, const int* np, const float* rg,
-                     int P_MAX, const int* gslot, const unsigned short* inc, const int* ist,
+                     int P_MAX, const int* gslot, const unsigned short* inc, const int* ist, int DEG_MAX,
                      const float* grid, const float* origin, const float* lo, const float* hi, const int* rec_idx,
                      const long long* goffs, const long long* gdims, int T, float spacing, float pen,
                      const float* prm, const float* rad, const int* hyd, const int* don, const int* acc,
@@ -104,7 +105,7 @@ int adock_mc_counted(int AM, int KM,
 int adock_mc_batch(int AM, int KM,
                    const float* ref, const int* typ, const unsigned char* tm, const int* ax, const float* axp,
                    const float* axu, const int* pr, const int* na, const int* nt, const int* np, const float* rg,
-                   int P_MAX, const int* gslot, const unsigned short* inc, const int* ist,
+                   int P_MAX, const int* gslot, const unsigned short* inc, const int* ist, int DEG_MAX,
                    const float* grid, const float* origin, const float* lo, const float* hi, const int* rec_idx,
                    const long long* goffs, const long long* gdims, int T, float spacing, float pen,
                    const float* prm, const float* rad, const int* hyd, const int* don, const int* acc,
@@ -134,7 +135,7 @@ int adock_batch_reset(int* counter, int* done, int n_complexes, unsigned long lo
 int adock_mc_step(int AM, int KM,
                   const float* ref, const int* typ, const unsigned char* tm, const int* ax, const float* axp,
                   const float* axu, const int* pr, const int* na, const int* nt, const int* np, const float* rg,
-                  int P_MAX, const int* gslot, const unsigned short* inc, const int* ist,
+                  int P_MAX, const int* gslot, const unsigned short* inc, const int* ist, int DEG_MAX,
                   const float* grid, const float* origin, const float* lo, const float* hi, const int* rec_idx,
                   const long long* goffs, const long long* gdims, int T, float spacing, float pen,
                   const float* prm, const float* rad, const int* hyd, const int* don, const int* acc,

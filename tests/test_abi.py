@@ -37,7 +37,7 @@ def test_bucket_validation_without_gpu():
     L = _lib.lib()
     null = [None] * 200
     # AM = 32 is not a compiled bucket: rejected before any HIP call
-    args = [32, 8] + null[:11] + [1, None, None, None] + null[:7] + [1, 0.375, 10.0] + null[:5] + [None, None, None, 0, 0,
+    args = [32, 8] + null[:11] + [1, None, None, None, 0] + null[:7] + [1, 0.375, 10.0] + null[:5] + [None, None, None, 0, 0,
                                                                                        None, 0, None, None, None, None]
     st = L.adock_eg(*args)
     assert st == -2
