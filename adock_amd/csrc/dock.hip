@@ -33,10 +33,10 @@ constexpr int AMK = 64;   // heavy-atom bucket handled by these kernels (one ato
 
 // Per-wave LDS layout, in bytes (P128 = P_MAX rounded up to a multiple of 128, KM + 1 torsion records):
 //   X4  [64]        float4  world xyz + radius of each atom
-//   TA  [KM + 1]    float4  reference axis point of torsion k; .w = sin(theta_k) of the current evaluation
-//   TB  [KM + 1]    float4  reference axis unit vector;        .w = cos(theta_k)
-//   TC  [KM + 1]    float4  current (world) axis point
-//   TD  [KM + 1]    float4  current axis unit vector
+//   TR0, TR1, TR2 [KM + 1] float4  torsion k's rotation about its reference axis as an affine map (row i: R_i0,
+//                                  R_i1, R_i2, b_i; p -> R p + b), rebuilt every evaluation (identity past nt)
+//   TU  [KM + 1]    float4  current (world) axis unit vector u_k
+//   TM  [KM + 1]    float4  current axis moment m_k = p_k x u_k (torque about the axis: u.(X x F) + m.F)
 //   FS  [P128 + 1]  float4  gradient slot of pair q (dE/dX of its first atom); slot P128 stays zero
 //   flg [64]        int     atom type flags
 //   pl  [P128]      u16     pair words, strided over lanes (pair q on lane q % 64 in round q / 64)
@@ -51,11 +51,11 @@ __host__ __device__ inline int pad128(int P_MAX) { return (P_MAX + 127) & ~127; 
 __host__ __device__ inline int inc_stride(int P_MAX) { return 2 * pad128(P_MAX) + 512; }
 __host__ __device__ inline int wave_lds_bytes(int KM, int P_MAX) {
     const int P = pad128(P_MAX);
-    return 16 * AMK + 4 * 16 * (KM + 1) + 16 * (P + 1) + 4 * AMK + 2 * P + 2 * inc_stride(P_MAX);
+    return 16 * AMK + 5 * 16 * (KM + 1) + 16 * (P + 1) + 4 * AMK + 2 * P + 2 * inc_stride(P_MAX);
 }
 __host__ __device__ inline int wave_lds_bytes_dense(int KM, int P_MAX, int DEG_MAX) {
     const int P = pad128(P_MAX);
-    return 16 * AMK + 4 * 16 * (KM + 1) + 16 * (AMK * DEG_MAX + 1) + 4 * AMK + 2 * P + 4 * P;
+    return 16 * AMK + 5 * 16 * (KM + 1) + 16 * (AMK * DEG_MAX + 1) + 4 * AMK + 2 * P + 4 * P;
 }
 template <bool DENSE>
 __host__ __device__ inline int lds_bytes(int KM, int P_MAX, int DEG_MAX) {
@@ -65,7 +65,7 @@ constexpr int LDS_PER_WORKGROUP = 160 * 1024;   // gfx950: 160 KB of LDS per CU,
 
 struct WaveLDS {
     float4* X4;
-    float4* TA; float4* TB; float4* TC; float4* TD;
+    float4* TR0; float4* TR1; float4* TR2; float4* TU; float4* TM;
     float4* FS;
     int* flg;
     unsigned short* pl;
@@ -83,8 +83,9 @@ struct ChainCtx {
     bool atom;
     const float* gp;
     int is, ic8;          // this atom's slot list: inc[is .. is + ic8), padded to a multiple of 8
-    // lane = torsion: axis atoms
+    // lane = torsion k: its axis atoms; lane 7 + k: torsion k's reference axis (point, unit vector)
     int axa, axb;
+    float rpx, rpy, rpz, rux, ruy, ruz;
     // uniform
     int na, nt, np, np2, D8;
     float rg;
@@ -101,10 +102,11 @@ DEVI WaveLDS carve_lds(char* base, int P_MAX, int DEG_MAX = 0) {
     w.P128 = pad128(P_MAX);
     float4* f = reinterpret_cast<float4*>(base);
     w.X4 = f; f += AMK;
-    w.TA = f; f += KM + 1;
-    w.TB = f; f += KM + 1;
-    w.TC = f; f += KM + 1;
-    w.TD = f; f += KM + 1;
+    w.TR0 = f; f += KM + 1;
+    w.TR1 = f; f += KM + 1;
+    w.TR2 = f; f += KM + 1;
+    w.TU = f; f += KM + 1;
+    w.TM = f; f += KM + 1;
     w.FS = w.FD = nullptr;
     w.inc = nullptr;
     w.pdst = nullptr;
@@ -149,20 +151,29 @@ DEVI ChainCtx setup_chain(int l, int lane, const LigArgs& L, const GridArgs& G, 
         for (int k = 0; k < c.nt; ++k)
             c.tmask |= (L.tm[((long long)l * KM + k) * AMK + lane] ? 1u : 0u) << k;
     }
-    // torsion records: reference axes (constant for the chain); current axes are rewritten every evaluation
+    // torsion records start as identities / zeros; the maps of torsions < nt and the current axes are rewritten every
+    // evaluation. Lane k holds torsion k's axis atoms, lane 7 + k its reference axis.
     c.axa = c.axb = 0;
-    if (lane <= KM) {
-        float4 ta = make_float4(0.f, 0.f, 0.f, 0.f), tb = make_float4(0.f, 0.f, 0.f, 1.f);
-        if (lane < c.nt) {
-            const float* axp = L.axp + ((long long)l * KM + lane) * 3;
-            const float* axu = L.axu + ((long long)l * KM + lane) * 3;
-            c.axa = L.ax[((long long)l * KM + lane) * 2];
-            c.axb = L.ax[((long long)l * KM + lane) * 2 + 1];
-            ta = make_float4(axp[0], axp[1], axp[2], 0.f);
-            tb = make_float4(axu[0], axu[1], axu[2], 1.f);
+    if (lane < c.nt) {
+        c.axa = L.ax[((long long)l * KM + lane) * 2];
+        c.axb = L.ax[((long long)l * KM + lane) * 2 + 1];
+    }
+    c.rpx = c.rpy = c.rpz = c.rux = c.ruy = c.ruz = 0.f;
+    {
+        const int tk = lane - 7;
+        if (tk >= 0 && tk < c.nt) {
+            const float* axp = L.axp + ((long long)l * KM + tk) * 3;
+            const float* axu = L.axu + ((long long)l * KM + tk) * 3;
+            c.rpx = axp[0]; c.rpy = axp[1]; c.rpz = axp[2];
+            c.rux = axu[0]; c.ruy = axu[1]; c.ruz = axu[2];
         }
-        w.TA[lane] = ta; w.TB[lane] = tb;
-        w.TC[lane] = make_float4(0.f, 0.f, 0.f, 0.f); w.TD[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    if (lane <= KM) {
+        w.TR0[lane] = make_float4(1.f, 0.f, 0.f, 0.f);
+        w.TR1[lane] = make_float4(0.f, 1.f, 0.f, 0.f);
+        w.TR2[lane] = make_float4(0.f, 0.f, 1.f, 0.f);
+        w.TU[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+        w.TM[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
     w.flg[lane] = c.flg;
     if constexpr (DENSE) {
@@ -240,33 +251,37 @@ DEVI float eval_eg(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, int l
     const float R00 = 1.f - 2.f * (qy * qy + qz * qz), R01 = 2.f * (qx * qy - qz * qw), R02 = 2.f * (qx * qz + qy * qw);
     const float R10 = 2.f * (qx * qy + qz * qw), R11 = 1.f - 2.f * (qx * qx + qz * qz), R12 = 2.f * (qy * qz - qx * qw);
     const float R20 = 2.f * (qx * qz - qy * qw), R21 = 2.f * (qy * qz + qx * qw), R22 = 1.f - 2.f * (qx * qx + qy * qy);
-    // lane 7 + k: sin / cos of torsion k into its record
+    // lane 7 + k: torsion k's rotation about its reference axis (Rodrigues: R = c I + s [u]x + (1 - c) u u^T) as the
+    // affine map p -> R p + (a - R a)
     {
         float sn, cs;
         sincosf(x, &sn, &cs);
         const int tk = lane - 7;
         if (tk >= 0 && tk < c.nt) {
-            reinterpret_cast<float*>(w.TA + tk)[3] = sn;
-            reinterpret_cast<float*>(w.TB + tk)[3] = cs;
+            const float ux = c.rux, uy = c.ruy, uz = c.ruz, t = 1.f - cs;
+            const float r00 = cs + t * ux * ux, r01 = t * ux * uy - sn * uz, r02 = t * ux * uz + sn * uy;
+            const float r10 = t * ux * uy + sn * uz, r11 = cs + t * uy * uy, r12 = t * uy * uz - sn * ux;
+            const float r20 = t * ux * uz - sn * uy, r21 = t * uy * uz + sn * ux, r22 = cs + t * uz * uz;
+            const float ax = c.rpx, ay = c.rpy, az = c.rpz;
+            w.TR0[tk] = make_float4(r00, r01, r02, ax - (r00 * ax + r01 * ay + r02 * az));
+            w.TR1[tk] = make_float4(r10, r11, r12, ay - (r10 * ax + r11 * ay + r12 * az));
+            w.TR2[tk] = make_float4(r20, r21, r22, az - (r20 * ax + r21 * ay + r22 * az));
         }
     }
     wave_lds_sync();
-    // forward kinematics, leaf-to-root about the reference axes: records read 4 at a time, the rotation applied where
-    // the atom's mask bit is set (records past nt are identities)
+    // forward kinematics, leaf-to-root: maps read 4 at a time, applied where the atom's mask bit is set (maps past nt
+    // are identities)
     float px = c.rx, py = c.ry, pz = c.rz;
     for (int kb = 0; kb < c.nt; kb += 4) {
-        float4 a[4], b[4];
+        float4 a[4], b[4], d[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) { a[u] = w.TA[kb + u]; b[u] = w.TB[kb + u]; }
+        for (int u = 0; u < 4; ++u) { a[u] = w.TR0[kb + u]; b[u] = w.TR1[kb + u]; d[u] = w.TR2[kb + u]; }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const bool m = (c.tmask >> (kb + u)) & 1u;
-            const float vx = px - a[u].x, vy = py - a[u].y, vz = pz - a[u].z;
-            const float d = (b[u].x * vx + b[u].y * vy + b[u].z * vz) * (1.f - b[u].w);
-            const float cx = b[u].y * vz - b[u].z * vy, cy = b[u].z * vx - b[u].x * vz, cz = b[u].x * vy - b[u].y * vx;
-            const float nx = a[u].x + vx * b[u].w + cx * a[u].w + b[u].x * d;
-            const float ny = a[u].y + vy * b[u].w + cy * a[u].w + b[u].y * d;
-            const float nz = a[u].z + vz * b[u].w + cz * a[u].w + b[u].z * d;
+            const float nx = a[u].x * px + a[u].y * py + a[u].z * pz + a[u].w;
+            const float ny = b[u].x * px + b[u].y * py + b[u].z * pz + b[u].w;
+            const float nz = d[u].x * px + d[u].y * py + d[u].z * pz + d[u].w;
             px = m ? nx : px; py = m ? ny : py; pz = m ? nz : pz;
         }
     }
@@ -295,13 +310,14 @@ DEVI float eval_eg(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, int l
     }
     w.X4[lane] = make_float4(X0, X1, X2, c.rad);
     wave_lds_sync();
-    // current torsion axes (lane k < nt) from the world coordinates of the axis atoms
+    // current torsion axes (lane k < nt) from the world coordinates of the axis atoms: unit vector and moment
     if (lane < c.nt) {
         const float4 A = w.X4[c.axa], B = w.X4[c.axb];
-        const float ux = B.x - A.x, uy = B.y - A.y, uz = B.z - A.z;
+        float ux = B.x - A.x, uy = B.y - A.y, uz = B.z - A.z;
         const float nn = fast_rsq(fmaxf(ux * ux + uy * uy + uz * uz, 1e-12f));
-        w.TC[lane] = make_float4(A.x, A.y, A.z, 0.f);
-        w.TD[lane] = make_float4(ux * nn, uy * nn, uz * nn, 0.f);
+        ux *= nn; uy *= nn; uz *= nn;
+        w.TU[lane] = make_float4(ux, uy, uz, 0.f);
+        w.TM[lane] = make_float4(A.y * uz - A.z * uy, A.z * ux - A.x * uz, A.x * uy - A.y * ux, 0.f);
     }
     // intra-ligand pairs, two rounds of 64 per iteration, branch-free: each pair writes its gradient slot (dense
     // variant: +slot into its first atom's row, -slot into its second atom's row)
@@ -397,16 +413,17 @@ DEVI float eval_eg(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, int l
     v[0] = fx; v[1] = fy; v[2] = fz;
     v[3] = dY * fz - dZ * fy; v[4] = dZ * fx - dX * fz; v[5] = dX * fy - dY * fx;
     v[6] = el;
-    // torque about each current axis from the atoms it moves; records in groups of 8 (records past nt are zero)
+    // torque about each current axis from the atoms it moves: u.((X - p) x F) = u.(X x F) + m.F with m = p x u;
+    // records in groups of 8 (records past nt are zero)
+    const float Tx = X1 * fz - X2 * fy, Ty = X2 * fx - X0 * fz, Tz = X0 * fy - X1 * fx;
 #pragma unroll
     for (int kb = 0; kb < KM; kb += 8) {
         if (kb < c.nt) {
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
-                const float4 cp = w.TC[kb + u], cu = w.TD[kb + u];
+                const float4 cu = w.TU[kb + u], cm = w.TM[kb + u];
                 const float mk = ((c.tmask >> (kb + u)) & 1u) ? 1.f : 0.f;
-                const float vx = X0 - cp.x, vy = X1 - cp.y, vz = X2 - cp.z;
-                v[7 + kb + u] = mk * ((vy * fz - vz * fy) * cu.x + (vz * fx - vx * fz) * cu.y + (vx * fy - vy * fx) * cu.z);
+                v[7 + kb + u] = mk * (cu.x * Tx + cu.y * Ty + cu.z * Tz + cm.x * fx + cm.y * fy + cm.z * fz);
             }
         } else {
 #pragma unroll
@@ -481,11 +498,13 @@ DEVI float bfgs(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, int lane
             if (!ok && trial < 9) { ++trial; continue; }
             const float y = ge - g;
             f0 = fe; x = xe; g = ge;
-            const float gg = wave_sum(ge * ge);
+            // |g|^2, y.p and y.y reduced together (the same DPP sequence per sum as three separate reductions)
+            float red[3] = {ge * ge, y * p, y * y};
+            wave_sum_multi<3>(red);
+            const float gg = red[0], yp = red[1];
             if (!(sqrtf(gg) >= 1e-5f)) break;
-            const float yp = wave_sum(y * p);
             if (step == 0) {
-                const float yy = wave_sum(y * y);
+                const float yy = red[2];
                 if (fabsf(yy) > EPSF) {
                     const float dg = alpha * yp / yy;
 #pragma unroll
@@ -591,11 +610,12 @@ DEVI float bfgs_spec(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, con
         }
         const float y = ga - g;
         f0 = fa; x = xa; g = ga;
-        const float gg = wave_sum(g * g);
+        float red[3] = {g * g, y * p, y * y};
+        wave_sum_multi<3>(red);
+        const float gg = red[0], yp = red[1];
         if (!(sqrtf(gg) >= 1e-5f)) break;
-        const float yp = wave_sum(y * p);
         if (step == 0) {
-            const float yy = wave_sum(y * y);
+            const float yy = red[2];
             if (fabsf(yy) > EPSF) {
                 const float dg = alpha * yp / yy;
 #pragma unroll
