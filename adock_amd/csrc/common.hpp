@@ -84,12 +84,22 @@ DEVI float fast_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
 DEVI float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 DEVI float fast_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 DEVI float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+// sin / cos through the hardware units, which take revolutions: the argument is reduced to [-1/2, 1/2] revolution
+// first (v_sin_f32 / v_cos_f32 are accurate there; error ~1e-7, far below the 1e-4 parity bar)
+DEVI void fast_sincos(float x, float& s, float& c) {
+    float r = x * 0.15915494309189535f;
+    r -= rintf(r);
+    s = __builtin_amdgcn_sinf(r);
+    c = __builtin_amdgcn_cosf(r);
+}
 
 // Scoring-function constants in evaluation-friendly form. Built from the 15-float search-order vector
 // (g1_off, g1_w, g2_off, g2_w, rep_off, hyd_good, hyd_bad, hb_good, hb_bad, cutoff, w_g1, w_g2, w_rep, w_hyd, w_hb).
 struct SFConst {
     float g1o, g1s, g1d;   // offset, sqrt(log2 e)/width, dE factor -2 w / (width * sqrt(log2 e))
     float g2o, g2s, g2d;
+    float g1b, g2b;        // -offset * sqrt(log2 e)/width: u = d * g1s + g1b in one FMA
+    float hbh, bbh;        // bad * inv: ramp v = bad * inv - d * inv in one FMA
     float w1, w2, w3, w4, w5;
     float repo;
     float hb_bad, hinv;    // hydrophobic: bad, 1/(bad - good)
@@ -108,6 +118,8 @@ DEVI SFConst make_sf(const float* __restrict__ prm) {
     s.w1 = prm[10]; s.w2 = prm[11]; s.w3 = prm[12]; s.w4 = prm[13]; s.w5 = prm[14];
     s.g1d = -2.f * s.w1 / (prm[1] * rl);
     s.g2d = -2.f * s.w2 / (prm[3] * rl);
+    s.g1b = -s.g1o * s.g1s; s.g2b = -s.g2o * s.g2s;
+    s.hbh = s.hb_bad * s.hinv; s.bbh = s.bb_bad * s.binv;
     return s;
 }
 
@@ -140,18 +152,27 @@ DEVI float pair_e(float d, int pf, const SFConst& s, float& de) {
     return ep;
 }
 
-// Branch-free form of pair_e for SIMT loops: same values (the slope terms clamp v to [0, 1]; their derivative is
-// non-zero only strictly inside the ramp). pf: bit 0 hydrophobic pair, bit 1 h-bond pair.
-DEVI float pair_e_nb(float d, int pf, const SFConst& s, float& de) {
-    const float u1 = (d - s.g1o) * s.g1s, g1 = fast_exp2(-u1 * u1);
-    const float u2 = (d - s.g2o) * s.g2s, g2 = fast_exp2(-u2 * u2);
+// Branch-free form of pair_e for SIMT loops: same terms (the slope terms clamp v to [0, 1]; their derivative is
+// non-zero only strictly inside the ramp). pf: bit 0 hydrophobic pair, bit 1 h-bond pair. HYD / HB = false leave the
+// term out for every lane (callers pass false when no lane holds such a pair).
+DEVI float pair_e_nb(float d, int pf, const SFConst& s, float& de, bool HYD = true, bool HB = true) {
+    const float u1 = fmaf(d, s.g1s, s.g1b), g1 = fast_exp2(-u1 * u1);
+    const float u2 = fmaf(d, s.g2s, s.g2b), g2 = fast_exp2(-u2 * u2);
     const float dr = fminf(d - s.repo, 0.f);
     float ep = s.w1 * g1 + s.w2 * g2 + s.w3 * dr * dr;
     float dd = s.g1d * u1 * g1 + s.g2d * u2 * g2 + 2.f * s.w3 * dr;
-    const float vh = (s.hb_bad - d) * s.hinv, vb = (s.bb_bad - d) * s.binv;
-    const float hh = (pf & 1) ? 1.f : 0.f, hb = (pf & 2) ? 1.f : 0.f;
-    ep += hh * s.w4 * fminf(fmaxf(vh, 0.f), 1.f) + hb * s.w5 * fminf(fmaxf(vb, 0.f), 1.f);
-    dd -= hh * ((vh > 0.f && vh < 1.f) ? s.w4 * s.hinv : 0.f) + hb * ((vb > 0.f && vb < 1.f) ? s.w5 * s.binv : 0.f);
+    if (HYD) {
+        const float vh = fmaf(-d, s.hinv, s.hbh);
+        const float hh = (pf & 1) ? 1.f : 0.f;
+        ep += hh * s.w4 * fminf(fmaxf(vh, 0.f), 1.f);
+        dd -= hh * ((vh > 0.f && vh < 1.f) ? s.w4 * s.hinv : 0.f);
+    }
+    if (HB) {
+        const float vb = fmaf(-d, s.binv, s.bbh);
+        const float hb = (pf & 2) ? 1.f : 0.f;
+        ep += hb * s.w5 * fminf(fmaxf(vb, 0.f), 1.f);
+        dd -= hb * ((vb > 0.f && vb < 1.f) ? s.w5 * s.binv : 0.f);
+    }
     de = dd;
     return ep;
 }
@@ -185,7 +206,7 @@ DEVI void quat_inc(float& w, float& x, float& y, float& z, float rx, float ry, f
     float hw, k;
     if (ang > 1e-6f) {
         float sn, cs;
-        sincosf(0.5f * ang, &sn, &cs);
+        fast_sincos(0.5f * ang, sn, cs);
         hw = cs; k = sn / ang;
     } else { hw = 1.f; k = 0.5f; }
     const float aw = hw, ax = rx * k, ay = ry * k, az = rz * k;

@@ -53,8 +53,9 @@ __host__ __device__ inline int wave_lds_bytes(int KM, int P_MAX) {
     const int P = pad128(P_MAX);
     return 16 * AMK + 5 * 16 * (KM + 1) + 16 * (P + 1) + 4 * AMK + 2 * P + 2 * inc_stride(P_MAX);
 }
+__host__ __device__ inline int pad512(int P_MAX) { return (P_MAX + 511) & ~511; }
 __host__ __device__ inline int wave_lds_bytes_dense(int KM, int P_MAX, int DEG_MAX) {
-    const int P = pad128(P_MAX);
+    const int P = pad512(P_MAX);   // pair words and destinations padded to whole 8-round chunks
     return 16 * AMK + 5 * 16 * (KM + 1) + 16 * (AMK * DEG_MAX + 1) + 4 * AMK + 2 * P + 4 * P;
 }
 template <bool DENSE>
@@ -99,7 +100,7 @@ struct ChainCtx {
 template <int KM, bool DENSE = false>
 DEVI WaveLDS carve_lds(char* base, int P_MAX, int DEG_MAX = 0) {
     WaveLDS w;
-    w.P128 = pad128(P_MAX);
+    w.P128 = DENSE ? pad512(P_MAX) : pad128(P_MAX);
     float4* f = reinterpret_cast<float4*>(base);
     w.X4 = f; f += AMK;
     w.TR0 = f; f += KM + 1;
@@ -225,19 +226,22 @@ DEVI ChainCtx setup_chain(int l, int lane, const LigArgs& L, const GridArgs& G, 
     return c;
 }
 
-// energy of pair word pw (0 for padding) and its gradient slot (dE/dX of the first atom; the second gets minus it)
-DEVI float pair_slot(const WaveLDS& w, unsigned pw, const SFConst& sf, float4& s) {
+// energy of pair word pw (0 for padding) at atom coordinates + radii xi, xj, and its gradient slot (dE/dX of the
+// first atom; the second gets minus it)
+DEVI float pair_math(float4 xi, float4 xj, unsigned pw, const SFConst& sf, float4& s, bool hyd, bool hb) {
     const float valid = (pw & 0x8000u) ? 0.f : 1.f;
-    const int i = pw & 63, j = (pw >> 6) & 63;
-    const float4 xi = w.X4[i], xj = w.X4[j];
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
     const float rr = fast_sqrt(rx * rx + ry * ry + rz * rz);
     float de;
-    const float e = pair_e_nb(rr - xi.w - xj.w, (int)((pw >> 12) & 3u), sf, de);
+    const float e = pair_e_nb(rr - xi.w - xj.w, (int)((pw >> 12) & 3u), sf, de, hyd, hb);
     const float in = rr < sf.cut ? valid : 0.f;
     const float sc = in * de * fast_rcp(fmaxf(rr, 1e-6f));
     s = make_float4(sc * rx, sc * ry, sc * rz, 0.f);
     return in * e;
+}
+
+DEVI float pair_slot(const WaveLDS& w, unsigned pw, const SFConst& sf, float4& s, bool hyd, bool hb) {
+    return pair_math(w.X4[pw & 63], w.X4[(pw >> 6) & 63], pw, sf, s, hyd, hb);
 }
 
 // Energy and lane-distributed gradient of the lane-distributed state x. Returns E (uniform).
@@ -255,7 +259,7 @@ DEVI float eval_eg(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, int l
     // affine map p -> R p + (a - R a)
     {
         float sn, cs;
-        sincosf(x, &sn, &cs);
+        fast_sincos(x, sn, cs);
         const int tk = lane - 7;
         if (tk >= 0 && tk < c.nt) {
             const float ux = c.rux, uy = c.ruy, uz = c.ruz, t = 1.f - cs;
@@ -319,21 +323,35 @@ DEVI float eval_eg(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, int l
         w.TU[lane] = make_float4(ux, uy, uz, 0.f);
         w.TM[lane] = make_float4(A.y * uz - A.z * uy, A.z * ux - A.x * uz, A.x * uy - A.y * ux, 0.f);
     }
-    // intra-ligand pairs, two rounds of 64 per iteration, branch-free: each pair writes its gradient slot (dense
-    // variant: +slot into its first atom's row, -slot into its second atom's row)
+    // intra-ligand pairs, branch-free: each pair writes its gradient slot. Dense variant: chunks of 8 rounds of 64
+    // with every pair word, destination and coordinate read issued before the math (one LDS round trip per chunk),
+    // +slot into the first atom's row, -slot into the second's. Compact variant: two rounds per iteration, pair lists
+    // h-bond pairs first, then hydrophobic ones, so most iterations skip those terms (uniform test over the wave).
     float ep_l = 0.f;
-    for (int q0 = lane; q0 < c.np2; q0 += 2 * WAVE) {
-        const unsigned pw0 = w.pl[q0], pw1 = w.pl[q0 + WAVE];
-        float4 s0, s1;
-        ep_l += pair_slot(w, pw0, sf, s0);
-        ep_l += pair_slot(w, pw1, sf, s1);
-        if constexpr (DENSE) {
-            const unsigned d0 = w.pdst[q0], d1 = w.pdst[q0 + WAVE];
-            w.FD[d0 & 0xffffu] = s0;
-            w.FD[d0 >> 16] = make_float4(-s0.x, -s0.y, -s0.z, 0.f);
-            w.FD[d1 & 0xffffu] = s1;
-            w.FD[d1 >> 16] = make_float4(-s1.x, -s1.y, -s1.z, 0.f);
-        } else {
+    if constexpr (DENSE) {
+        for (int cb = 0; cb < c.np; cb += 8 * WAVE) {
+            unsigned pw[8], dd[8];
+            float4 xi[8], xj[8];
+#pragma unroll
+            for (int r = 0; r < 8; ++r) { pw[r] = w.pl[cb + r * WAVE + lane]; dd[r] = w.pdst[cb + r * WAVE + lane]; }
+#pragma unroll
+            for (int r = 0; r < 8; ++r) { xi[r] = w.X4[pw[r] & 63]; xj[r] = w.X4[(pw[r] >> 6) & 63]; }
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+                float4 s0;
+                ep_l += pair_math(xi[r], xj[r], pw[r], sf, s0, true, true);
+                w.FD[dd[r] & 0xffffu] = s0;
+                w.FD[dd[r] >> 16] = make_float4(-s0.x, -s0.y, -s0.z, 0.f);
+            }
+        }
+    } else {
+        for (int q0 = lane; q0 < c.np2; q0 += 2 * WAVE) {
+            const unsigned pw0 = w.pl[q0], pw1 = w.pl[q0 + WAVE];
+            const bool hyd = __ballot(((pw0 | pw1) & 0x1000u) != 0u) != 0ull;
+            const bool hb = __ballot(((pw0 | pw1) & 0x2000u) != 0u) != 0ull;
+            float4 s0, s1;
+            ep_l += pair_slot(w, pw0, sf, s0, hyd, hb);
+            ep_l += pair_slot(w, pw1, sf, s1, hyd, hb);
             w.FS[q0] = s0;
             w.FS[q0 + WAVE] = s1;
         }

@@ -15,7 +15,7 @@ from . import synth, workload
 from .autotune import Autotuner
 from .batch import BatchRunner, GraphedBatch, admit
 from .kernel import KernelProblem
-from .learned import StepSearch, train_proposal
+from .learned import StepSearch, jump_dataset, train_proposal
 
 
 def _bufs(st):
@@ -100,64 +100,99 @@ def complexes(seeds, config=2):
     return recs, ligs, prob
 
 
-def learned(train_seeds=range(1000, 1032), eval_seeds=range(8), chains=256, train_steps=200, budgets=(50, 100, 200),
-            full_steps=1000, epochs=6, p_plain=0.25, seed=0, log=print):
-    # 1. training data from the plain method on disjoint seeds
+def _timed(fn):
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    fn()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t
+
+
+def _budget_to_gap(rows, key, target):
+    """Smallest step budget whose mean gap is <= target, its time, or (None, None)."""
+    for r in rows:
+        if r[key + "_gap"] <= target:
+            return r["steps"], r[key + "_s"]
+    return None, None
+
+
+def learned(n_train=512, train_chains=64, train_steps=400, n_eval=512, eval_chains=64,
+            budgets=(10, 15, 20, 30, 40, 60, 80, 120, 160, 240, 320, 480), full_steps=1000, epochs=5, p_plain=0.25,
+            gap_target=0.1, seed=0, target="move", every=2, log=print):
+    """Config 5: the proposal network trained on the plain method's runs over n_train complexes, then plain (fused
+    kernel, one launch) and learned (step kernel + MFMA inference, the whole search captured in one HIP graph)
+    searches on n_eval disjoint complexes at a range of MC-step budgets. Gap = mean over complexes of
+    best(budget) - best(plain, full_steps). Reports, per method, the smallest budget reaching gap_target and the
+    docked ligands/s there. target "move": regress the accepted moves (features before the move -> the move);
+    "jump": regress, from the pose every `every` steps, the jump to the best pose its chain ends with."""
     t_all = time.perf_counter()
-    _, tligs, tprob = complexes(list(train_seeds))
+    # 1. accepted moves of the plain method on the training complexes
+    _, tligs, tprob = complexes(range(10_000, 10_000 + n_train))
     tkp = KernelProblem(tprob)
     L = len(tligs)
-    st, cl = workload.chain_starts(seed + 1, tligs, [np.zeros(3)] * L, [np.full(3, 26.0)] * L, chains, tkp.KM)
+    st, cl = workload.chain_starts(seed + 1, tligs, [np.zeros(3)] * L, [np.full(3, 26.0)] * L, train_chains, tkp.KM)
     st, cl = torch.as_tensor(st, device=tkp.device), torch.as_tensor(cl, device=tkp.device)
-    cap = int(len(cl) * train_steps * 0.6) + 1024
-    ss = StepSearch(tkp, cl, log_cap=cap)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ss.run(*_bufs(st), train_steps, seed=seed + 777, log=True)
-    torch.cuda.synchronize()
-    t_log = time.perf_counter() - t0
-    X, T, M, gain = ss.logged()
+    holder = {}
+    if target == "move":
+        cap = int(len(cl) * train_steps * 0.85) + 1024
+        ss = StepSearch(tkp, cl, log_cap=cap)
+        log(f"training set: {L} complexes x {train_chains} chains x {train_steps} steps (log capacity {cap} moves)")
+        t_log = _timed(lambda: ss.run(*_bufs(st), train_steps, seed=seed + 777, log=True))
+        X, T, M, gain = ss.logged()
+        W = None
+    else:
+        ss = StepSearch(tkp, cl)
+        log(f"training set: {L} complexes x {train_chains} chains x {train_steps} steps, jumps every {every}")
+        t_log = _timed(lambda: holder.update(d=jump_dataset(ss, st, train_steps, seed=seed + 777, every=every)))
+        X, T, W = holder["d"]
+        M = gain = None
     n_moves = int(X.shape[0])
-    log(f"logged {n_moves} accepted moves in {t_log:.1f} s")
-    t0 = time.perf_counter()
-    net, losses = train_proposal(X, T, M, gain, epochs=epochs, batch=4096, lr=1e-3, seed=seed, log=log)
-    torch.cuda.synchronize()
-    t_train = time.perf_counter() - t0
-    del ss, X, T, M, gain
-    # 2. evaluation on disjoint complexes
-    _, eligs, eprob = complexes(list(eval_seeds))
+    log(f"{n_moves} training samples in {t_log:.1f} s")
+    t_train = _timed(lambda: holder.update(r=train_proposal(X, T, M, gain, epochs=epochs, batch=4096, lr=1e-3,
+                                                            seed=seed, log=log, weights=W)))
+    net, losses = holder["r"]
+    del ss, X, T, M, gain, W, tkp, tprob, holder
+    torch.cuda.empty_cache()
+    # 2. disjoint evaluation complexes
+    _, eligs, eprob = complexes(range(20_000, 20_000 + n_eval))
     ekp = KernelProblem(eprob)
     E = len(eligs)
-    st, cl = workload.chain_starts(seed + 2, eligs, [np.zeros(3)] * E, [np.full(3, 26.0)] * E, chains, ekp.KM)
+    st, cl = workload.chain_starts(seed + 2, eligs, [np.zeros(3)] * E, [np.full(3, 26.0)] * E, eval_chains, ekp.KM)
     st, cl = torch.as_tensor(st, device=ekp.device), torch.as_tensor(cl, device=ekp.device)
     ci = torch.arange(len(cl), dtype=torch.int32, device=ekp.device)
     s64 = seed * 1_000_003 + 12345
 
     def plain(steps):
         b = _bufs(st)
-        torch.cuda.synchronize()
-        t = time.perf_counter()
-        ekp.run_mc(cl, ci, *b, True, steps, 0, 1.2, 2.0, 2.0, s64, 0)
-        torch.cuda.synchronize()
-        return b[3].view(E, chains).min(1).values.cpu().numpy(), time.perf_counter() - t
+        dt = _timed(lambda: ekp.run_mc(cl, ci, *b, True, steps, 0, 1.2, 2.0, 2.0, s64, 0))
+        return b[3].view(E, eval_chains).min(1).values.cpu().numpy(), dt
 
+    plain(10)   # warm-up (module load, first-launch costs)
     ref, t_full = plain(full_steps)
-    rows = []
+    log(f"plain reference: {full_steps} steps in {t_full:.2f} s, mean best {ref.mean():.3f}")
     ss = StepSearch(ekp, cl, net=net)
+    rows = []
     for S in budgets:
         bp, tp = plain(S)
         b = _bufs(st)
-        torch.cuda.synchronize()
-        t = time.perf_counter()
-        ss.run(*b, S, seed=s64, use_net=True, p_plain=p_plain)
-        torch.cuda.synchronize()
-        tl = time.perf_counter() - t
-        bl = b[3].view(E, chains).min(1).values.cpu().numpy()
+        g = ss.capture(*b, S, seed=s64, use_net=True, p_plain=p_plain)
+        for x, y in zip(b, _bufs(st)):
+            x.copy_(y)
+        tl = _timed(g.replay)
+        bl = b[3].view(E, eval_chains).min(1).values.cpu().numpy()
+        del g
         rows.append({"steps": S, "plain_gap": float((bp - ref).mean()), "learned_gap": float((bl - ref).mean()),
                      "plain_s": tp, "learned_s": tl})
         log(f"S={S}: plain gap {rows[-1]['plain_gap']:.3f} ({tp:.2f} s), learned gap {rows[-1]['learned_gap']:.3f} "
             f"({tl:.2f} s)")
-    return {"workload": f"learned proposal: trained on {n_moves} accepted moves from {L} complexes (seeds "
-                        f"{min(train_seeds)}-{max(train_seeds)}), evaluated on {E} disjoint complexes x {chains} chains",
-            "full_plain_steps": full_steps, "full_plain_s": t_full, "log_s": t_log, "train_s": t_train,
-            "train_loss": losses, "rows": rows, "total_s": time.perf_counter() - t_all}
+    sp, tp = _budget_to_gap(rows, "plain", gap_target)
+    sl, tl = _budget_to_gap(rows, "learned", gap_target)
+    out = {"workload": f"learned proposal ({target} target): trained on {n_moves} samples from {L} config-2 complexes, "
+                       f"evaluated on {E} disjoint complexes x {eval_chains} chains ({E * eval_chains} chains per step)",
+           "gap_target_kcal": gap_target, "full_plain_steps": full_steps, "full_plain_s": t_full, "log_s": t_log,
+           "train_s": t_train, "train_loss": losses, "rows": rows, "total_s": time.perf_counter() - t_all,
+           "plain_steps_to_gap": sp, "learned_steps_to_gap": sl,
+           "plain_docked_ligands_per_s": E / tp if tp else None,
+           "learned_docked_ligands_per_s": E / tl if tl else None}
+    out["speedup_at_matched_gap"] = (tp / tl) if (tp and tl) else None
+    return out

@@ -10,6 +10,8 @@ sampler, which is all the docking search needs (the best score is what is report
 Training data: accepted moves logged by the plain method (features before the move, the move as target, a mask of the
 components that exist, the energy gain), on generator seeds disjoint from the evaluation seeds.
 """
+import math
+
 import numpy as np
 import torch
 
@@ -52,17 +54,17 @@ class StepSearch:
 
     def run(self, cur, cur_e, best, best_e, steps, bfgs_iters=0, temp=1.2, amp_pos=2.0, amp_rot=2.0, seed=0,
             step_offset=0, init_opt=True, use_net=False, p_plain=0.25, sig=(0.5, 0.15, 0.3), log=False,
-            count_evals=False):
+            count_evals=False, features=False):
         """In place on cur / cur_e / best / best_e (float32 contiguous device tensors), like KernelProblem.run_mc.
-        Features are kept current whenever the network or the move log needs them."""
+        Features (self.feat) are kept current whenever the network, the move log or `features` needs them; every step
+        reads the features of its current pose, so a search continued with init_opt=False relies on self.feat being
+        the one left by the previous call on the same chains."""
         if use_net and self.net is None:
             raise _lib.AdockError("use_net needs a ProposalNet")
-        feat_on = bool(use_net or log)
+        feat_on = bool(use_net or log or features)
         if init_opt:
             self._step(cur, cur_e, best, best_e, 1, step_offset, bfgs_iters, temp, amp_pos, amp_rot, seed, False,
                        p_plain, sig, log, feat_on, count_evals)
-        elif feat_on:
-            raise _lib.AdockError("features of the start poses come from the initial step: use init_opt=True")
         for st in range(steps):
             self._step(cur, cur_e, best, best_e, 0, step_offset + st, bfgs_iters, temp, amp_pos, amp_rot, seed,
                        use_net, p_plain, sig, log, feat_on, count_evals)
@@ -91,6 +93,58 @@ class StepSearch:
         return L[:, :IN], L[:, IN:IN + OUT], L[:, IN + OUT:IN + 2 * OUT], L[:, IN + 2 * OUT]
 
 
+def pose_delta(best, cur, n_tors):
+    """Jump from the poses cur to the poses best [N, 7+KM], in the network's output layout: translation difference,
+    rotation q_best conj(q_cur) with w >= 0, wrapped torsion differences; mask 1 on components that exist."""
+    N, SD = cur.shape
+    out = torch.zeros(N, OUT, device=cur.device)
+    mask = torch.zeros(N, OUT, device=cur.device)
+    out[:, :3] = best[:, :3] - cur[:, :3]
+    a = best[:, 3:7] / best[:, 3:7].norm(dim=1, keepdim=True)
+    b = cur[:, 3:7] / cur[:, 3:7].norm(dim=1, keepdim=True)
+    bw, bx, by, bz = b[:, 0], -b[:, 1], -b[:, 2], -b[:, 3]
+    aw, ax, ay, az = a.unbind(1)
+    q = torch.stack([aw * bw - ax * bx - ay * by - az * bz, aw * bx + ax * bw + ay * bz - az * by,
+                     aw * by - ax * bz + ay * bw + az * bx, aw * bz + ax * by - ay * bx + az * bw], 1)
+    out[:, 3:7] = q * torch.where(q[:, :1] < 0, -1.0, 1.0)
+    mask[:, :7] = 1.0
+    K = min(SD - 7, OUT - 7)
+    d = best[:, 7:7 + K] - cur[:, 7:7 + K]
+    d = d - 2 * math.pi * torch.round(d / (2 * math.pi))
+    valid = (torch.arange(K, device=cur.device)[None, :] < n_tors[:, None]).float()
+    out[:, 7:7 + K] = d * valid
+    mask[:, 7:7 + K] = valid
+    return out, mask
+
+
+def jump_dataset(search, start, steps, seed=0, every=4, max_gain=5.0):
+    """Training set of the 'jump' target: run the plain method step by step from `start`, keep the features and the
+    current pose of every chain every `every` steps, and after the run take as target the jump from each kept pose to
+    the best pose its chain found (pose_delta), weighted 1 + min(E(kept) - E(best), max_gain).
+    Returns (X [n,128], T [n,23], W [n,23])."""
+    kp = search.kp
+    cur, best = start.clone(), start.clone()
+    cur_e = torch.zeros(start.shape[0], device=start.device)
+    best_e = torch.full((start.shape[0],), float("inf"), device=start.device)
+    search.run(cur, cur_e, best, best_e, 0, seed=seed, init_opt=True, features=True)
+    feats, poses, energies = [], [], []
+    for s in range(steps):
+        search.run(cur, cur_e, best, best_e, 1, seed=seed, step_offset=s, init_opt=False, features=True)
+        if s % every == 0:
+            feats.append(search.feat.clone())
+            poses.append(cur.clone())
+            energies.append(cur_e.clone())
+    nt = torch.as_tensor([kp.preps[l].n_tors for l in search.chain_lig.tolist()], device=start.device)
+    X, T, W = [], [], []
+    for f, c, e in zip(feats, poses, energies):
+        t, m = pose_delta(best, c, nt)
+        gap = (e - best_e).clamp(min=0.0, max=max_gain)
+        X.append(f)
+        T.append(t)
+        W.append(m * (1.0 + gap)[:, None])
+    return torch.cat(X), torch.cat(T), torch.cat(W)
+
+
 def move_weights(mask, gain):
     """Per-component regression weights: moves that lowered the energy count 1 + min(gain, 5), uphill (Metropolis)
     moves 0.25, missing components 0."""
@@ -98,8 +152,10 @@ def move_weights(mask, gain):
     return (mask * w[:, None]).contiguous()
 
 
-def train_proposal(X, T, M, gain, epochs=8, batch=4096, lr=1e-3, seed=0, log=None, device="cuda"):
+def train_proposal(X, T, M, gain, epochs=8, batch=4096, lr=1e-3, seed=0, log=None, device="cuda", weights=None):
+    """Trains a ProposalNet on (features, targets) with move_weights(M, gain), or with explicit per-component
+    weights. Returns (net, per-epoch losses)."""
     net = ProposalNet(device, seed=seed)
-    W = move_weights(M, gain)
+    W = move_weights(M, gain) if weights is None else weights.contiguous()
     losses = net.fit(X.contiguous(), T.contiguous(), W, epochs=epochs, batch=batch, lr=lr, seed=seed, log=log)
     return net, losses

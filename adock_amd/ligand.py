@@ -176,6 +176,17 @@ def bucket(x, step):
     return max(step, step * math.ceil(x / step))
 
 
+def pair_classes(pairs, types):
+    """Term class of each heavy-atom pair: 2 = h-bond pair (donor-acceptor either way), 1 = hydrophobic pair,
+    0 = neither (XS type flags of config/adock.json)."""
+    from .config import load_config
+    t = load_config()["_xs_table"]
+    hyd, don, acc = (np.asarray(t[k], bool) for k in ("hydrophobic", "donor", "acceptor"))
+    ti, tj = types[pairs[:, 0]], types[pairs[:, 1]]
+    hb = (don[ti] & acc[tj]) | (acc[ti] & don[tj])
+    return np.where(hb, 2, np.where(hyd[ti] & hyd[tj], 1, 0))
+
+
 def pack_ligands(preps, slot, rec_idx):
     """Packed ligand arrays in the search-kernel layout (src/adock/mckernel.py:547-592).
 
@@ -192,6 +203,10 @@ def pack_ligands(preps, slot, rec_idx):
         remap = -np.ones(p.n, np.int64)
         remap[hv] = np.arange(len(hv))
         pl = remap[p.pairs] if len(p.pairs) else np.zeros((0, 2), np.int64)
+        if len(pl):
+            # h-bond pairs first, then hydrophobic, then the rest: the kernels skip those terms in rounds without them
+            cls = pair_classes(p.pairs, p.types)
+            pl = pl[np.argsort(-cls, kind="stable")]
         pair_lists.append(pl)
     P = max(1, max(len(x) for x in pair_lists))
     out = {

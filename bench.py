@@ -70,6 +70,8 @@ def main():
     ap.add_argument("--workload", choices=["single", "screen", "multi", "learned"], default="single",
                     help="single = the headline line (default); the others print their own JSON line")
     ap.add_argument("--small", action="store_true", help="scaled-down sizes for the opt-in workloads")
+    ap.add_argument("--target", choices=["move", "jump"], default="move",
+                    help="training target of the learned proposal (--workload learned)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="after the timed steps, one device-to-device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -85,9 +87,9 @@ def main():
             out = ex.multi(n_receptors=8 if args.small else 64, ligands_per=8 if args.small else 32, chains=32,
                            steps=50 if args.small else 500)
         else:
-            out = ex.learned(train_seeds=range(1000, 1004 if args.small else 1032), eval_seeds=range(2 if args.small else 8),
-                             train_steps=40 if args.small else 200, budgets=(20, 40) if args.small else (50, 100, 200),
-                             full_steps=100 if args.small else 1000, epochs=2 if args.small else 6)
+            out = (ex.learned(n_train=8, train_steps=40, n_eval=8, budgets=(10, 20, 40), full_steps=100, epochs=2,
+                              target=args.target)
+                   if args.small else ex.learned(target=args.target))
         print(json.dumps(out), flush=True)
         return
     dev = torch.device("cuda:0")

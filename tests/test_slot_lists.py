@@ -44,6 +44,10 @@ def test_pack_ligands_rows_match_per_ligand_lists():
     P = int(out["P"])
     assert out["inc"].shape == (2, inc_stride(P))
     for l, p in enumerate(preps):
-        inc, ist = slot_lists(_heavy_pairs(p), P, AM)
+        pr = out["pr"][l, :out["np"][l]]
+        # the packed pair list is the ligand's (heavy-atom) pair set, reordered by term class
+        assert sorted(map(tuple, pr.tolist())) == sorted(map(tuple, _heavy_pairs(p).tolist()))
+        inc, ist = slot_lists(pr, P, AM)
         np.testing.assert_array_equal(out["inc"][l], inc)
         np.testing.assert_array_equal(out["ist"][l], ist)
+    assert out["DEG"] == max(int((out["ist"][l] >> 16).max()) for l in range(2))
