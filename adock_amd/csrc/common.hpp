@@ -202,12 +202,12 @@ DEVI void rand_ball(uint64_t seed, int chain, int step, int slot, float& a, floa
 
 // (w,x,y,z) <- quat(rotation vector r) * (w,x,y,z), normalised
 DEVI void quat_inc(float& w, float& x, float& y, float& z, float rx, float ry, float rz) {
-    const float ang = sqrtf(rx * rx + ry * ry + rz * rz);
+    const float ang = fast_sqrt(rx * rx + ry * ry + rz * rz);
     float hw, k;
     if (ang > 1e-6f) {
         float sn, cs;
         fast_sincos(0.5f * ang, sn, cs);
-        hw = cs; k = sn / ang;
+        hw = cs; k = sn * fast_rcp(ang);
     } else { hw = 1.f; k = 0.5f; }
     const float aw = hw, ax = rx * k, ay = ry * k, az = rz * k;
     const float nw = aw * w - ax * x - ay * y - az * z;

@@ -564,17 +564,20 @@ DEVI float bfgs(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, int lane
 // passes the Armijo test (after ten failures: trial 9 with the step halved once more), so the accepted points,
 // energies and evaluation counts equal the one-wave search bit for bit while a step costs about one evaluation of
 // latency instead of two. Worth it when the chains leave SIMDs idle (a few hundred chains on 256 CUs).
-struct SpecLDS { float* x; float* g; float* fe; int* ok; };   // x [64], g [64], fe [4], ok [4]
-constexpr int SPEC_LDS_BYTES = 4 * (64 + 64 + 4 + 4);
+// Exchange area, two buffers x (waves <= 4): each wave publishes its trial (pose, gradient, energy, Armijo outcome)
+// in its own slot; one barrier per batch, and alternating buffers make a slot safe to rewrite two batches later (every
+// wave has passed the intervening barrier, so it has finished reading).
+struct SpecLDS { float* x; float* g; float* fe; int* ok; };   // x [2][4][64], g [2][4][64], fe [2][4], ok [2][4]
+constexpr int SPEC_LDS_BYTES = 4 * (2 * 4 * 64 * 2 + 2 * 4 * 2);
 
 DEVI SpecLDS carve_spec(char* base) {
     float* f = reinterpret_cast<float*>(base);
-    return SpecLDS{f, f + 64, f + 128, reinterpret_cast<int*>(f + 132)};
+    return SpecLDS{f, f + 512, f + 1024, reinterpret_cast<int*>(f + 1032)};
 }
 
 template <int KM, int NS, bool DENSE>
 DEVI float bfgs_spec(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, const SpecLDS& xs, int lane, int wid,
-                     float& x, int iters, int& nev) {
+                     float& x, int iters, int& nev, int& buf) {
     constexpr int DV = 7 + KM;
     float h[DV];
 #pragma unroll
@@ -602,28 +605,26 @@ DEVI float bfgs_spec(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, con
                 fe = eval_eg<KM, DENSE>(c, sf, w, lane, xe, ge, ea_, x0_, x1_, x2_);
                 ok = fe - f0 < 1e-4f * at * pg;
             }
-            if (lane == 0) xs.ok[wid] = ok ? 1 : 0;
+            const int slot = buf * 4 + wid;
+            xs.x[slot * 64 + lane] = xe;
+            xs.g[slot * 64 + lane] = ge;
+            if (lane == 0) { xs.fe[slot] = fe; xs.ok[slot] = ok ? 1 : 0; }
             __syncthreads();
             int first = -1;
 #pragma unroll
-            for (int u = NS - 1; u >= 0; --u) first = xs.ok[u] ? u : first;
+            for (int u = NS - 1; u >= 0; --u) first = xs.ok[buf * 4 + u] ? u : first;
             first = __builtin_amdgcn_readfirstlane(first);
             const int win = first >= 0 ? first : (tb + NS >= 10 ? 9 - tb : -1);
+            const int rd = buf * 4 + (win < 0 ? 0 : win);
+            buf ^= 1;
             if (win < 0) {
                 nev += NS;
-                __syncthreads();
                 continue;
             }
             nev += win + 1;
-            if (wid == win) {
-                xs.x[lane] = xe; xs.g[lane] = ge;
-                if (lane == 0) xs.fe[0] = fe;
-            }
-            __syncthreads();
-            xa = xs.x[lane]; ga = xs.g[lane]; fa = xs.fe[0];
+            xa = xs.x[rd * 64 + lane]; ga = xs.g[rd * 64 + lane]; fa = xs.fe[rd];
             alpha = ldexpf(1.f, -(tb + win));
             if (first < 0) alpha *= 0.5f;
-            __syncthreads();
             break;
         }
         const float y = ga - g;
@@ -684,6 +685,7 @@ DEVI int run_chain(int ch, int lane, const WaveLDS& w, const LigArgs& L, const G
     int nev = 0;
     const unsigned long long seed = M.seed;
     if (!M.init_opt && ec < eb) { eb = ec; best = cur; }
+    int xbuf = 0;   // exchange buffer of the speculative line search; alternates over the whole chain
     // st = -1 is the optional initial optimisation of cur; st >= 0 are MC steps
     for (int st = M.init_opt ? -1 : 0; st < M.mc_steps; ++st) {
         float x = cur;
@@ -694,7 +696,7 @@ DEVI int run_chain(int ch, int lane, const WaveLDS& w, const LigArgs& L, const G
         }
         float en;
         if constexpr (NS == 1) en = bfgs<KM, DENSE>(c, sf, w, lane, x, iters, nev);
-        else en = bfgs_spec<KM, NS, DENSE>(c, sf, w, xs, lane, wid, x, iters, nev);
+        else en = bfgs_spec<KM, NS, DENSE>(c, sf, w, xs, lane, wid, x, iters, nev, xbuf);
         if (st < 0) {
             ec = en; cur = x;
             if (ec < eb) { eb = ec; best = cur; }
