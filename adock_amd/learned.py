@@ -2,7 +2,7 @@
 
 Each MC step is one adock_mc_step launch over all chains; with the network on, adock_mlp_infer (BF16 MFMA) runs on the
 features of every chain's current pose right before it. The whole step sequence can be captured once in a HIP graph
-(torch.cuda.CUDAGraph) and replayed. The exact Vina score and the reference's Metropolis test judge every move; the
+(torch.cuda.CUDAGraph) and replayed. The exact Vina score and the reference's Metropolis test decide every move; the
 reference's random mutation is the fallback (probability p_plain, or whenever the network output is not finite).
 The network proposal is not symmetric, so the chain no longer satisfies detailed balance: it is an optimiser, not a
 sampler, which is all the docking search needs (the best score is what is reported).

@@ -78,7 +78,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build runs the docking path on a single MI355X "
-              "(multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
+              "(the path shards by complex; no multi-GPU run is part of this benchmark)", file=sys.stderr)
         sys.exit(2)
     if args.workload != "single":
         from adock_amd import experiments as ex
