@@ -121,9 +121,9 @@ DEVI WaveLDS carve_lds(char* base, int P_MAX, int DEG_MAX = 0) {
     return w;
 }
 
-template <int KM, bool DENSE = false>
+template <int KM, bool DENSE = false, bool EX = false>
 DEVI ChainCtx setup_chain(int l, int lane, const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecArgs& R,
-                          int exact, const WaveLDS& w) {
+                          const WaveLDS& w) {
     ChainCtx c;
     c.na = __builtin_amdgcn_readfirstlane(L.na[l]);
     c.nt = min(__builtin_amdgcn_readfirstlane(L.nt[l]), KM);
@@ -216,12 +216,14 @@ DEVI ChainCtx setup_chain(int l, int lane, const LigArgs& L, const GridArgs& G, 
         }
         c.D8 = (int)wave_max_f((float)c.ic8);
     }
-    c.exact = exact;
-    c.rrx = R.rxyz + (long long)ri * R.RN_MAX * 3;
-    c.rrt = R.rtyp + (long long)ri * R.RN_MAX;
-    c.rn = exact ? R.rn[ri] : 0;
+    c.exact = EX ? 1 : 0;
     c.nb = nullptr; c.nbn = 0;
-    c.sfrad = S.rad; c.sfhyd = S.hyd; c.sfdon = S.don; c.sfacc = S.acc;
+    if constexpr (EX) {
+        c.rrx = R.rxyz + (long long)ri * R.RN_MAX * 3;
+        c.rrt = R.rtyp + (long long)ri * R.RN_MAX;
+        c.rn = R.rn[ri];
+        c.sfrad = S.rad; c.sfhyd = S.hyd; c.sfdon = S.don; c.sfacc = S.acc;
+    }
     wave_lds_sync();
     return c;
 }
@@ -244,8 +246,10 @@ DEVI float pair_slot(const WaveLDS& w, unsigned pw, const SFConst& sf, float4& s
     return pair_math(w.X4[pw & 63], w.X4[(pw >> 6) & 63], pw, sf, s, hyd, hb);
 }
 
-// Energy and lane-distributed gradient of the lane-distributed state x. Returns E (uniform).
-template <int KM, bool DENSE = false>
+// Energy and lane-distributed gradient of the lane-distributed state x. Returns E (uniform). EX: exact pairwise
+// inter energy over the receptor atoms (or a neighbour list) instead of the grids; a compile-time switch, so the grid
+// kernels carry none of its state.
+template <int KM, bool DENSE = false, bool EX = false>
 DEVI float eval_eg(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, int lane, float x, float& gout, float& ea,
                    float& Xx, float& Xy, float& Xz) {
     const float t0 = bcast(x, 0), t1 = bcast(x, 1), t2 = bcast(x, 2);
@@ -296,7 +300,7 @@ DEVI float eval_eg(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, int l
     float el = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
     float v000 = 0.f, v001 = 0.f, v010 = 0.f, v011 = 0.f, v100 = 0.f, v101 = 0.f, v110 = 0.f, v111 = 0.f;
     float wx = 0.f, wy = 0.f, wz = 0.f, in0 = 0.f, in1 = 0.f, in2 = 0.f;
-    const bool grid_atom = c.atom && !c.exact;
+    const bool grid_atom = c.atom && !EX;
     if (grid_atom) {
         const float f0 = (X0 - c.o0) * c.inv_s, f1 = (X1 - c.o1) * c.inv_s, f2 = (X2 - c.o2) * c.inv_s;
         const float fc0 = fminf(fmaxf(f0, 0.f), c.fmx0), fc1 = fminf(fmaxf(f1, 0.f), c.fmx1),
@@ -387,7 +391,7 @@ DEVI float eval_eg(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, int l
         }
     }
     if (c.atom) {
-        if (!c.exact) {
+        if constexpr (!EX) {
             const float c00 = v000 + wz * (v001 - v000), c01 = v010 + wz * (v011 - v010);
             const float c10 = v100 + wz * (v101 - v100), c11 = v110 + wz * (v111 - v110);
             const float c0 = c00 + wy * (c01 - c00), c1 = c10 + wy * (c11 - c10);
@@ -493,7 +497,7 @@ DEVI float plain_move(float x, const ChainCtx& c, int lane, unsigned long long s
 
 // Vina-style BFGS on the lane-distributed state x (in/out); returns the final energy. One evaluation site: the first
 // pass evaluates x itself, every later pass evaluates one line-search trial.
-template <int KM, bool DENSE = false>
+template <int KM, bool DENSE = false, bool EX = false>
 DEVI float bfgs(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, int lane, float& x, int iters, int& nev) {
     constexpr int DV = 7 + KM;
     float h[DV];
@@ -505,7 +509,7 @@ DEVI float bfgs(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, int lane
     while (true) {
         const float xe = trial < 0 ? x : conf_inc(x, p, alpha, lane);
         float ge, ea_, x0_, x1_, x2_;
-        const float fe = eval_eg<KM, DENSE>(c, sf, w, lane, xe, ge, ea_, x0_, x1_, x2_);
+        const float fe = eval_eg<KM, DENSE, EX>(c, sf, w, lane, xe, ge, ea_, x0_, x1_, x2_);
         ++nev;
         if (trial < 0) {
             f0 = fe; f_orig = fe; g = ge;
@@ -575,7 +579,7 @@ DEVI SpecLDS carve_spec(char* base) {
     return SpecLDS{f, f + 512, f + 1024, reinterpret_cast<int*>(f + 1032)};
 }
 
-template <int KM, int NS, bool DENSE>
+template <int KM, int NS, bool DENSE, bool EX>
 DEVI float bfgs_spec(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, const SpecLDS& xs, int lane, int wid,
                      float& x, int iters, int& nev, int& buf) {
     constexpr int DV = 7 + KM;
@@ -584,7 +588,7 @@ DEVI float bfgs_spec(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, con
     for (int j = 0; j < DV; ++j) h[j] = (j == lane) ? 1.f : 0.f;
     const float xo = x;
     float g, ea_, x0_, x1_, x2_;
-    float f0 = eval_eg<KM, DENSE>(c, sf, w, lane, x, g, ea_, x0_, x1_, x2_);
+    float f0 = eval_eg<KM, DENSE, EX>(c, sf, w, lane, x, g, ea_, x0_, x1_, x2_);
     ++nev;
     const float f_orig = f0;
     int step = 0;
@@ -602,7 +606,7 @@ DEVI float bfgs_spec(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, con
             if (t < 10) {
                 const float at = ldexpf(1.f, -t);
                 xe = conf_inc(x, p, at, lane);
-                fe = eval_eg<KM, DENSE>(c, sf, w, lane, xe, ge, ea_, x0_, x1_, x2_);
+                fe = eval_eg<KM, DENSE, EX>(c, sf, w, lane, xe, ge, ea_, x0_, x1_, x2_);
                 ok = fe - f0 < 1e-4f * at * pg;
             }
             const int slot = buf * 4 + wid;
@@ -664,14 +668,14 @@ DEVI float bfgs_spec(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, con
 
 // One MC chain on the calling wave (all 64 lanes active), or on the NS waves of a workgroup with the speculative line
 // search (wave wid; wave 0 writes the results). w: this wave's LDS region.
-template <int KM, int NS = 1, bool DENSE = false>
+template <int KM, int NS = 1, bool DENSE = false, bool EX = false>
 DEVI int run_chain(int ch, int lane, const WaveLDS& w, const LigArgs& L, const GridArgs& G, const SFArgs& S,
-                   const RecArgs& R, int exact, const int* __restrict__ chain_lig, const McArgs& M, int wid = 0,
+                   const RecArgs& R, const int* __restrict__ chain_lig, const McArgs& M, int wid = 0,
                    SpecLDS xs = SpecLDS{}) {
     constexpr int SD = 7 + KM;
     const SFConst sf = make_sf(S.prm);
     const int l = __builtin_amdgcn_readfirstlane(chain_lig[ch]);
-    ChainCtx c = setup_chain<KM, DENSE>(l, lane, L, G, S, R, exact, w);
+    ChainCtx c = setup_chain<KM, DENSE, EX>(l, lane, L, G, S, R, w);
     if (M.nbr) { c.nb = M.nbr + (long long)ch * M.NB_MAX; c.nbn = M.nbr_n[ch]; }
     const int cid = __builtin_amdgcn_readfirstlane(M.chain_id[ch]);
     const int S_ = 7 + c.nt;
@@ -695,8 +699,8 @@ DEVI int run_chain(int ch, int lane, const WaveLDS& w, const LigArgs& L, const G
             x = plain_move(x, c, lane, seed, cid, sg, M.amp_pos, M.amp_rot);
         }
         float en;
-        if constexpr (NS == 1) en = bfgs<KM, DENSE>(c, sf, w, lane, x, iters, nev);
-        else en = bfgs_spec<KM, NS, DENSE>(c, sf, w, xs, lane, wid, x, iters, nev, xbuf);
+        if constexpr (NS == 1) en = bfgs<KM, DENSE, EX>(c, sf, w, lane, x, iters, nev);
+        else en = bfgs_spec<KM, NS, DENSE, EX>(c, sf, w, xs, lane, wid, x, iters, nev, xbuf);
         if (st < 0) {
             ec = en; cur = x;
             if (ec < eb) { eb = ec; best = cur; }
@@ -718,8 +722,8 @@ DEVI int run_chain(int ch, int lane, const WaveLDS& w, const LigArgs& L, const G
     return l;
 }
 
-template <int KM>
-__global__ __launch_bounds__(256) void mc_kernel(LigArgs L, GridArgs G, SFArgs S, RecArgs R, int exact,
+template <int KM, bool EX>
+__global__ __launch_bounds__(256) void mc_kernel(LigArgs L, GridArgs G, SFArgs S, RecArgs R,
                                                   const int* __restrict__ chain_lig, int n_chains, McArgs M) {
     extern __shared__ __attribute__((aligned(16))) char lds[];
     const int lane = lane_id();
@@ -727,12 +731,12 @@ __global__ __launch_bounds__(256) void mc_kernel(LigArgs L, GridArgs G, SFArgs S
     const int ch = blockIdx.x * (blockDim.x / WAVE) + wid;
     if (ch >= n_chains) return;
     const WaveLDS w = carve_lds<KM>(lds + wid * wave_lds_bytes(KM, L.P_MAX), L.P_MAX);
-    run_chain<KM>(ch, lane, w, L, G, S, R, exact, chain_lig, M);
+    run_chain<KM, 1, false, EX>(ch, lane, w, L, G, S, R, chain_lig, M);
 }
 
 // One chain per workgroup of NS waves (speculative line search); grid = n_chains.
-template <int KM, int NS>
-__global__ __launch_bounds__(256) void mc_spec_kernel(LigArgs L, GridArgs G, SFArgs S, RecArgs R, int exact,
+template <int KM, int NS, bool EX>
+__global__ __launch_bounds__(256) void mc_spec_kernel(LigArgs L, GridArgs G, SFArgs S, RecArgs R,
                                                        const int* __restrict__ chain_lig, int n_chains, McArgs M) {
     extern __shared__ __attribute__((aligned(16))) char lds[];
     const int lane = lane_id();
@@ -741,14 +745,14 @@ __global__ __launch_bounds__(256) void mc_spec_kernel(LigArgs L, GridArgs G, SFA
     if (ch >= n_chains) return;
     const int wb = wave_lds_bytes_dense(KM, L.P_MAX, L.DEG_MAX);
     const WaveLDS w = carve_lds<KM, true>(lds + wid * wb, L.P_MAX, L.DEG_MAX);
-    run_chain<KM, NS, true>(ch, lane, w, L, G, S, R, exact, chain_lig, M, wid, carve_spec(lds + NS * wb));
+    run_chain<KM, NS, true, EX>(ch, lane, w, L, G, S, R, chain_lig, M, wid, carve_spec(lds + NS * wb));
 }
 
 // Persistent variant: a grid sized to the chip's resident capacity; every wave pulls chain indices from a global
 // counter (in `order`, e.g. longest chains first) until the queue is empty. Per complex it counts finished chains and
 // stamps the 100 MHz real-time clock when the last one ends (times[0] = earliest wave start).
-template <int KM>
-__global__ __launch_bounds__(256) void mc_persistent_kernel(LigArgs L, GridArgs G, SFArgs S, RecArgs R, int exact,
+template <int KM, bool EX>
+__global__ __launch_bounds__(256) void mc_persistent_kernel(LigArgs L, GridArgs G, SFArgs S, RecArgs R,
                                                              const int* __restrict__ chain_lig, int n_chains, McArgs M,
                                                              QueueArgs Q) {
     extern __shared__ __attribute__((aligned(16))) char lds[];
@@ -762,7 +766,7 @@ __global__ __launch_bounds__(256) void mc_persistent_kernel(LigArgs L, GridArgs 
         idx = bcast_i(idx, 0);
         if (idx >= n_chains) break;
         const int ch = Q.order ? __builtin_amdgcn_readfirstlane(Q.order[idx]) : idx;
-        const int l = run_chain<KM>(ch, lane, w, L, G, S, R, exact, chain_lig, M);
+        const int l = run_chain<KM, 1, false, EX>(ch, lane, w, L, G, S, R, chain_lig, M);
         if (Q.done && lane == 0) {
             const int old = atomicAdd(Q.done + l, 1);
             if (old + 1 == Q.lig_chains[l] && Q.times) Q.times[1 + l] = __builtin_amdgcn_s_memrealtime();
@@ -770,8 +774,8 @@ __global__ __launch_bounds__(256) void mc_persistent_kernel(LigArgs L, GridArgs 
     }
 }
 
-template <int KM>
-__global__ __launch_bounds__(256) void eg_kernel(LigArgs L, GridArgs G, SFArgs S, RecArgs R, int exact,
+template <int KM, bool EX>
+__global__ __launch_bounds__(256) void eg_kernel(LigArgs L, GridArgs G, SFArgs S, RecArgs R,
                                                   const int* __restrict__ chain_lig, int n_chains,
                                                   const float* __restrict__ state, float* __restrict__ e_out,
                                                   float* __restrict__ g_out) {
@@ -783,10 +787,10 @@ __global__ __launch_bounds__(256) void eg_kernel(LigArgs L, GridArgs G, SFArgs S
     if (ch >= n_chains) return;
     const WaveLDS w = carve_lds<KM>(lds + wid * wave_lds_bytes(KM, L.P_MAX), L.P_MAX);
     const SFConst sf = make_sf(S.prm);
-    const ChainCtx c = setup_chain<KM>(__builtin_amdgcn_readfirstlane(chain_lig[ch]), lane, L, G, S, R, exact, w);
+    const ChainCtx c = setup_chain<KM, false, EX>(__builtin_amdgcn_readfirstlane(chain_lig[ch]), lane, L, G, S, R, w);
     const float x = lane < SD ? state[(long long)ch * SD + lane] : 0.f;
     float g, ea_, x0_, x1_, x2_;
-    const float e = eval_eg<KM>(c, sf, w, lane, x, g, ea_, x0_, x1_, x2_);
+    const float e = eval_eg<KM, false, EX>(c, sf, w, lane, x, g, ea_, x0_, x1_, x2_);
     if (lane < 6) g_out[(long long)ch * DM + lane] = g;
     else if (lane >= 7 && lane < 7 + c.nt) g_out[(long long)ch * DM + lane - 1] = g;
     if (lane == 0) e_out[ch] = e;
@@ -846,19 +850,19 @@ inline void allow_lds(const void* fn, size_t bytes) {
     }
 }
 
-template <int KM>
-int launch_mc_t(const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecArgs& R, int exact,
-                const int* chain_lig, int n_chains, const McArgs& M, hipStream_t stream) {
+template <int KM, bool EX>
+int launch_mc_tx(const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecArgs& R, const int* chain_lig,
+                 int n_chains, const McArgs& M, hipStream_t stream) {
     const int ns = line_search_width(n_chains, KM, L.P_MAX, L.DEG_MAX);
     if (ns > 1) {
         const size_t shm = (size_t)ns * wave_lds_bytes_dense(KM, L.P_MAX, L.DEG_MAX) + SPEC_LDS_BYTES;
         if (ns == 4) {
-            allow_lds(reinterpret_cast<const void*>(mc_spec_kernel<KM, 4>), shm);
-            hipLaunchKernelGGL((mc_spec_kernel<KM, 4>), dim3(n_chains), dim3(4 * WAVE), shm, stream, L, G, S, R, exact,
+            allow_lds(reinterpret_cast<const void*>(mc_spec_kernel<KM, 4, EX>), shm);
+            hipLaunchKernelGGL((mc_spec_kernel<KM, 4, EX>), dim3(n_chains), dim3(4 * WAVE), shm, stream, L, G, S, R,
                                chain_lig, n_chains, M);
         } else {
-            allow_lds(reinterpret_cast<const void*>(mc_spec_kernel<KM, 2>), shm);
-            hipLaunchKernelGGL((mc_spec_kernel<KM, 2>), dim3(n_chains), dim3(2 * WAVE), shm, stream, L, G, S, R, exact,
+            allow_lds(reinterpret_cast<const void*>(mc_spec_kernel<KM, 2, EX>), shm);
+            hipLaunchKernelGGL((mc_spec_kernel<KM, 2, EX>), dim3(n_chains), dim3(2 * WAVE), shm, stream, L, G, S, R,
                                chain_lig, n_chains, M);
         }
         return (int)hipGetLastError();
@@ -866,20 +870,28 @@ int launch_mc_t(const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecA
     const int W = fit_waves(waves_for(n_chains), KM, L.P_MAX);
     const size_t shm = (size_t)W * wave_lds_bytes(KM, L.P_MAX);
     const unsigned nblk = (unsigned)((n_chains + W - 1) / W);
-    allow_lds(reinterpret_cast<const void*>(mc_kernel<KM>), shm);
-    hipLaunchKernelGGL(mc_kernel<KM>, dim3(nblk), dim3(W * WAVE), shm, stream, L, G, S, R, exact, chain_lig,
-                       n_chains, M);
+    allow_lds(reinterpret_cast<const void*>(mc_kernel<KM, EX>), shm);
+    hipLaunchKernelGGL((mc_kernel<KM, EX>), dim3(nblk), dim3(W * WAVE), shm, stream, L, G, S, R, chain_lig, n_chains,
+                       M);
     return (int)hipGetLastError();
 }
 
 template <int KM>
-int launch_mc_persistent_t(const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecArgs& R, int exact,
-                           const int* chain_lig, int n_chains, const McArgs& M, const QueueArgs& Q, int blocks,
-                           hipStream_t stream) {
+int launch_mc_t(const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecArgs& R, int exact,
+                const int* chain_lig, int n_chains, const McArgs& M, hipStream_t stream) {
+    return exact ? launch_mc_tx<KM, true>(L, G, S, R, chain_lig, n_chains, M, stream)
+                 : launch_mc_tx<KM, false>(L, G, S, R, chain_lig, n_chains, M, stream);
+}
+
+template <int KM, bool EX>
+int launch_mc_persistent_tx(const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecArgs& R,
+                            const int* chain_lig, int n_chains, const McArgs& M, const QueueArgs& Q, int blocks,
+                            hipStream_t stream) {
     const int W = fit_waves(g_waves_per_block > 0 ? g_waves_per_block : 4, KM, L.P_MAX);
     const size_t shm = (size_t)W * wave_lds_bytes(KM, L.P_MAX);
     if (blocks <= 0) {
-        // resident capacity, queried once per (device, W, LDS size) so later launches (and graph captures) only launch
+        // resident capacity, queried once per (device, kernel, W, LDS size) so later launches (and graph captures)
+        // only launch
         static std::mutex mu;
         static std::map<long long, int> cap;
         int dev = 0;
@@ -890,16 +902,36 @@ int launch_mc_persistent_t(const LigArgs& L, const GridArgs& G, const SFArgs& S,
         if (it == cap.end()) {
             int cus = 0, per_cu = 0;
             (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, mc_persistent_kernel<KM>, W * WAVE, shm);
+            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, mc_persistent_kernel<KM, EX>, W * WAVE, shm);
             it = cap.emplace(key, cus * (per_cu > 0 ? per_cu : 1)).first;
         }
         blocks = it->second;
     }
     const int need = (n_chains + W - 1) / W;
     if (blocks > need) blocks = need;
-    allow_lds(reinterpret_cast<const void*>(mc_persistent_kernel<KM>), shm);
-    hipLaunchKernelGGL(mc_persistent_kernel<KM>, dim3(blocks), dim3(W * WAVE), shm, stream, L, G, S, R, exact,
+    allow_lds(reinterpret_cast<const void*>(mc_persistent_kernel<KM, EX>), shm);
+    hipLaunchKernelGGL((mc_persistent_kernel<KM, EX>), dim3(blocks), dim3(W * WAVE), shm, stream, L, G, S, R,
                        chain_lig, n_chains, M, Q);
+    return (int)hipGetLastError();
+}
+
+template <int KM>
+int launch_mc_persistent_t(const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecArgs& R, int exact,
+                           const int* chain_lig, int n_chains, const McArgs& M, const QueueArgs& Q, int blocks,
+                           hipStream_t stream) {
+    return exact ? launch_mc_persistent_tx<KM, true>(L, G, S, R, chain_lig, n_chains, M, Q, blocks, stream)
+                 : launch_mc_persistent_tx<KM, false>(L, G, S, R, chain_lig, n_chains, M, Q, blocks, stream);
+}
+
+template <int KM, bool EX>
+int launch_eg_tx(const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecArgs& R, const int* chain_lig,
+                 int n_chains, const float* state, float* e_out, float* g_out, hipStream_t stream) {
+    const int W = fit_waves(waves_for(n_chains), KM, L.P_MAX);
+    const size_t shm = (size_t)W * wave_lds_bytes(KM, L.P_MAX);
+    const unsigned nblk = (unsigned)((n_chains + W - 1) / W);
+    allow_lds(reinterpret_cast<const void*>(eg_kernel<KM, EX>), shm);
+    hipLaunchKernelGGL((eg_kernel<KM, EX>), dim3(nblk), dim3(W * WAVE), shm, stream, L, G, S, R, chain_lig, n_chains,
+                       state, e_out, g_out);
     return (int)hipGetLastError();
 }
 
@@ -907,13 +939,8 @@ template <int KM>
 int launch_eg_t(const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecArgs& R, int exact,
                 const int* chain_lig, int n_chains, const float* state, float* e_out, float* g_out,
                 hipStream_t stream) {
-    const int W = fit_waves(waves_for(n_chains), KM, L.P_MAX);
-    const size_t shm = (size_t)W * wave_lds_bytes(KM, L.P_MAX);
-    const unsigned nblk = (unsigned)((n_chains + W - 1) / W);
-    allow_lds(reinterpret_cast<const void*>(eg_kernel<KM>), shm);
-    hipLaunchKernelGGL(eg_kernel<KM>, dim3(nblk), dim3(W * WAVE), shm, stream, L, G, S, R, exact, chain_lig,
-                       n_chains, state, e_out, g_out);
-    return (int)hipGetLastError();
+    return exact ? launch_eg_tx<KM, true>(L, G, S, R, chain_lig, n_chains, state, e_out, g_out, stream)
+                 : launch_eg_tx<KM, false>(L, G, S, R, chain_lig, n_chains, state, e_out, g_out, stream);
 }
 
 
@@ -1056,7 +1083,7 @@ __global__ __launch_bounds__(256) void mc_step_kernel(LigArgs L, GridArgs G, SFA
     const WaveLDS w = carve_lds<KM>(lds + wid * wave_lds_bytes(KM, L.P_MAX), L.P_MAX);
     const SFConst sf = make_sf(S.prm);
     const int l = __builtin_amdgcn_readfirstlane(chain_lig[ch]);
-    const ChainCtx c = setup_chain<KM>(l, lane, L, G, S, R, 0, w);
+    const ChainCtx c = setup_chain<KM>(l, lane, L, G, S, R, w);
     const int cid = __builtin_amdgcn_readfirstlane(M.chain_id[ch]);
     const int S_ = 7 + c.nt;
     const int iters = M.bfgs_iters > 0 ? M.bfgs_iters : (25 + c.na) / 3;
