@@ -48,6 +48,20 @@ int adock_build_grid(const float* rxyz, const int* rtype, int n_rec, const float
                                         spacing, nx, ny, nz, prm, per_term, out, (hipStream_t)stream));
 }
 
+int adock_build_grid_fit(const float* rxyz, const int* rext, int n_rec, const float* rad_rec, const float* rad_lig,
+                         int n_lt, const float* W, int n_rt, float ox, float oy, float oz, float spacing, int nx,
+                         int ny, int nz, const float* prm, float* out, adock_stream_t stream) {
+    if (n_rec < 0 || n_lt < 0 || n_rt < 0 || nx < 0 || ny < 0 || nz < 0) return fail(ADOCK_E_ARG, "negative size");
+    if ((long long)nx * ny * nz == 0 || n_lt == 0) return 0;
+    if (!rad_lig || !prm || !out || (n_rec > 0 && (!rxyz || !rext || !rad_rec || !W)))
+        return fail(ADOCK_E_ARG, "null pointer");
+    if (n_rt > build_grid_fit_rt_max() || (n_rec > 0 && n_rt < 1))
+        return fail(ADOCK_E_TYPES, "n_rt must be in 1..216 receptor types");
+    if ((long long)nx * ny * nz > (1ll << 31) - 1) return fail(ADOCK_E_ARG, "grid too large");
+    return hip_status(launch_build_grid_fit(rxyz, rext, n_rec, rad_rec, rad_lig, n_lt, W, n_rt, ox, oy, oz, spacing,
+                                            nx, ny, nz, prm, out, (hipStream_t)stream));
+}
+
 #define LIG_DECL                                                                                                     \
     const float *ref, const int *typ, const unsigned char *tm, const int *ax, const float *axp, const float *axu,    \
         const int *pr, const int *na, const int *nt, const int *np, const float *rg, int P_MAX, const int *gslot, \

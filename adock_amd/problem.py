@@ -106,6 +106,53 @@ class Problem:
     def subset(self, idx):
         return Problem(self.gs, self.lb.subset(idx), self.rec_idx[idx], self.pot, self.rec)
 
+    def inter(self, X):
+        """Grid inter energy + out-of-box penalty [L, M] of heavy atoms X [L, M, A, 3] (src/adock/search.py:91-97)."""
+        lb = self.lb
+        L, M, A, _ = X.shape
+        t = lb.types[:, None, :].expand(L, M, A)
+        ri = self.rec_idx[:, None, None].expand(L, M, A)
+        return self.gs.lookup(X, t, ri, mask=lb.heavy[:, None, :].expand(L, M, A)).sum(-1)
+
+    def inter_nopen(self, X):
+        """Grid inter energy without the out-of-box penalty (src/adock/search.py:99-105)."""
+        pen, self.gs.penalty = self.gs.penalty, 0.0
+        try:
+            return self.inter(X)
+        finally:
+            self.gs.penalty = pen
+
+    def inter_exact(self, X, chunk=2_000_000):
+        """Direct pairwise ligand-receptor energy [L, M] through pot.energy, no grid (src/adock/search.py:73-89)."""
+        if self.rec is None:
+            raise ValueError("inter_exact needs receptor atoms (Problem.rec)")
+        rx, rt, rm = self.rec
+        lb = self.lb
+        L, M = X.shape[:2]
+        out = torch.zeros(L, M, device=X.device, dtype=X.dtype)
+        for l in range(L):
+            ri = int(self.rec_idx[l])
+            hv = lb.heavy[l]
+            xl = X[l][:, hv]
+            tl = lb.types[l][hv]
+            xr, tr = rx[ri][rm[ri]].to(X.dtype), rt[ri][rm[ri]]
+            if len(xr) == 0 or xl.shape[1] == 0:
+                continue
+            step = max(1, chunk // (xl.shape[1] * len(xr)))
+            for m0 in range(0, M, step):
+                r = torch.cdist(xl[m0:m0 + step], xr[None].expand(len(xl[m0:m0 + step]), -1, -1))
+                e = self.pot.energy(r, tl[None, :, None].expand_as(r), tr[None, None, :].expand_as(r))
+                out[l, m0:m0 + step] = e.sum((-1, -2))
+        return out
+
+    def energy(self, t, q0, r, theta):
+        """(total, inter, intra, X) of poses [L, M] from the grid and the pair potential (the uncompiled branch of
+        src/adock/search.py:123-133)."""
+        from .pose import pose_coords
+        X = pose_coords(self.lb, t, q0, r, theta)
+        ei, ea = self.inter(X), self.intra(X)
+        return ei + ea, ei, ea, X
+
     def intra(self, X):
         """Intra-ligand energy [L, M] of coordinates X [L, M, A, 3] (src/adock/search.py:108-119)."""
         lb = self.lb

@@ -79,25 +79,52 @@ def finalize_kernel(prob, kp, states, chain_lig, L, P, cfg):
                     bth.view(L, P, lb.K))
     etot = kp.energy_grad(chain_lig, states, exact=prob.rec is not None)[0].view(L, P)
     eintra = prob.intra(X)
-    hv = lb.heavy[:, None, None, :].float()
-    D = ((((X[:, :, None] - X[:, None, :]) ** 2).sum(-1) * hv).sum(-1) / hv.sum(-1).clamp(min=1)).sqrt()
+    D = _rmsd(X, lb.heavy)
     Xc, Ec, Ic, Dc = X.cpu().numpy(), etot.cpu().numpy(), eintra.cpu().numpy(), D.cpu().numpy()
     Nt = lb.n_tors_vina.cpu().numpy()
     rw = prob.pot.p["rot_weight"]
-    cut, n_out = cfg["cluster_rmsd"], cfg["n_out"]
-    results = []
-    for l in range(L):
-        reps = []
-        for i in np.argsort(Ec[l], kind="stable"):
-            if all(Dc[l, i, j] >= cut for j in reps):
-                reps.append(i)
-                if len(reps) == n_out:
-                    break
-        ri = np.array(reps)
-        inter = Ec[l, ri] - Ic[l, ri]
-        aff = (inter + Ic[l, ri] - Ic[l, ri[0]]) / (1 + rw * Nt[l])
-        results.append({"coords": Xc[l, ri], "affinity": aff, "inter": inter, "intra": Ic[l, ri], "total": Ec[l, ri]})
-    return results
+    return [_cluster(Xc[l], Ec[l], Ec[l] - Ic[l], Ic[l], Dc[l], Nt[l], rw, cfg["cluster_rmsd"], cfg["n_out"])
+            for l in range(L)]
+
+
+def _rmsd(X, heavy):
+    """Heavy-atom RMSD between every two poses of each complex: X [L, P, A, 3] -> [L, P, P]."""
+    hv = heavy[:, None, None, :].to(X.dtype)
+    return ((((X[:, :, None] - X[:, None, :]) ** 2).sum(-1) * hv).sum(-1) / hv.sum(-1).clamp(min=1)).sqrt()
+
+
+def _cluster(X, E, inter, intra, D, n_tors_vina, rot_weight, cut, n_out):
+    """Greedy clustering of one complex's final poses (lowest energy first; a pose becomes a representative when its
+    RMSD D to every earlier one is at least `cut`; at most n_out) and its result dict with Vina-style affinities
+    (src/adock/search.py:389-396, 520-537)."""
+    reps = []
+    for i in np.argsort(E, kind="stable"):
+        if all(D[i, j] >= cut for j in reps):
+            reps.append(i)
+            if len(reps) == n_out:
+                break
+    ri = np.array(reps)
+    aff = (inter[ri] + intra[ri] - intra[ri[0]]) / (1 + rot_weight * n_tors_vina)
+    return {"coords": X[ri], "affinity": aff, "inter": inter[ri], "intra": intra[ri], "total": E[ri]}
+
+
+@torch.no_grad()
+def finalize(prob, bt, bq, bth, cfg):
+    """Torch finalize of poses [L, M] (the finalize_torch path, src/adock/search.py:476-503): grid energy, then,
+    when receptor atoms are present, the inter part replaced by the exact pairwise energy through prob.pot (any
+    potential, e.g. a fitted one) plus the grid's out-of-box penalty; clustering and affinities as finalize_kernel."""
+    lb = prob.lb
+    etot, einter, eintra, X = prob.energy(bt, bq, torch.zeros_like(bt), bth)
+    if prob.rec is not None:
+        pen = einter - prob.inter_nopen(X)
+        einter = prob.inter_exact(X) + pen
+        etot = einter + eintra
+    D = _rmsd(X, lb.heavy)
+    Xc, Ec, Ei, Ia, Dc = (a.cpu().numpy() for a in (X, etot, einter, eintra, D))
+    Nt = lb.n_tors_vina.cpu().numpy()
+    rw = prob.pot.p["rot_weight"]
+    return [_cluster(Xc[l], Ec[l], Ei[l], Ia[l], Dc[l], Nt[l], rw, cfg["cluster_rmsd"], cfg["n_out"])
+            for l in range(lb.L)]
 
 
 def dock_problem_kernel(prob, boxes_c, boxes_s, cfg, seed=0, log=None, stats=None, kernel_problem=None):
@@ -202,7 +229,11 @@ def dock_problem_kernel(prob, boxes_c, boxes_s, cfg, seed=0, log=None, stats=Non
                   exact=True, nbr=nb[0], nbr_n=nb[1])
         best, M, chain_lig = pb, P, chain_lig[sel]
     mark("polish")
-    res = finalize_kernel(prob, kp, best, chain_lig, L, M, cfg)
+    if cfg.get("finalize_torch", False):
+        bt, bq, bth = kp.unpack(best, lb.K)
+        res = finalize(prob, bt.view(L, M, 3), bq.view(L, M, 4), bth.view(L, M, lb.K), cfg)
+    else:
+        res = finalize_kernel(prob, kp, best, chain_lig, L, M, cfg)
     mark("final")
     if stats is not None:
         ks = list(tick)

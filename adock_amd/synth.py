@@ -252,3 +252,30 @@ def make_multi(seed=0, n_receptors=64, ligands_per=32, chains=32, steps=500):
             rec_idx.append(r)
     return {"receptors": recs, "boxes": boxes, "ligs": ligs, "rec_idx": np.asarray(rec_idx), "chains": chains,
             "steps": steps, "bfgs_iters": 0}
+
+
+def fitted_complex(rec, lig, center, size, spec, seed=0, p_charge=0.3, p_side_chain=0.5):
+    """A complex for the fitted scoring function from a synthetic receptor and ligand: receptor buriedness from its own
+    heavy atoms (count within spec.buried_radius), charge classes drawn on N (cation) and O (anion) atoms with
+    probability p_charge (the annotation the reference derives from residue names and formal charges,
+    src/adock/fit/terms.py:165-230), side-chain flags with probability p_side_chain. Returns the dict that
+    adock_amd.fit.fdock takes (rec_xyz, rec_ext, center, size, lig, lig_ext)."""
+    from .fit.terms import buried_count, ligand_ext_types, receptor_ext_types
+    rng = np.random.default_rng(seed + 77)
+    names = _CFG["_xs_names"]
+    elem = lambda t: names[t].split("_")[0]
+
+    def classes(types):
+        c = np.zeros(len(types), np.int64)
+        for i, t in enumerate(types):
+            if t >= 0 and rng.random() < p_charge:
+                c[i] = {"N": 1, "O": 2}.get(elem(int(t)), 0)
+        return c
+
+    xyz = np.asarray(rec["rec_xyz"], np.float32)
+    ann = {"rec_types": np.asarray(rec["rec_types"], np.int64),
+           "buried_count": buried_count(xyz, xyz, spec.buried_radius),
+           "rec_chg": classes(rec["rec_types"]), "rec_sc": rng.random(len(xyz)) < p_side_chain}
+    return {"rec_xyz": xyz, "rec_ext": receptor_ext_types(ann, spec), "center": np.asarray(center, np.float64),
+            "size": np.broadcast_to(np.asarray(size, np.float64), (3,)).copy(), "lig": lig,
+            "lig_ext": ligand_ext_types(lig.types, classes(lig.types), spec.nxs)}

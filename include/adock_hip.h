@@ -45,6 +45,16 @@ int adock_build_grid(const float* rxyz, const int* rtype, int n_rec, const float
                      float spacing, int nx, int ny, int nz, const float* prm, int per_term, float* out,
                      adock_stream_t stream);
 
+/* Scoring grids of the fitted (linear-in-weights) function. Replaces the reference's build_grid_fit kernel launch
+ * (src/adock/fit/gridfit.py:13-58, launched by build_grid_fit, gridfit.py:70-93); same arguments in the same order,
+ * plus n_rt (rows of W) and the stream. rext [n_rec] receptor ext types in [0, n_rt) (atoms outside are skipped),
+ * rad_rec [n_rt], rad_lig [n_lt] (compact ligand types), W [n_rt, n_lt, 6] pair weights of the basis functions
+ * (gauss1, gauss2, repulsion, hydrophobic, hbond, ionic), prm: 12 floats (cutoff, g1_off, g1_w, g2_off, g2_w,
+ * rep_off, hyd_good, hyd_bad, hb_good, hb_bad, ion_rmin, rep_cap). out [n_lt, nx, ny, nz], z fastest. n_rt <= 216. */
+int adock_build_grid_fit(const float* rxyz, const int* rext, int n_rec, const float* rad_rec, const float* rad_lig,
+                         int n_lt, const float* W, int n_rt, float ox, float oy, float oz, float spacing, int nx,
+                         int ny, int nz, const float* prm, float* out, adock_stream_t stream);
+
 /* Energy + analytic gradient for n_chains poses. Replaces launch_eg_kernel (eg_kernel, src/adock/mckernel.py:411-447;
  * host binding src/adock/cpukernel.py:266-273). state [n, 7+KM], e_out [n], g_out [n, 6+KM] (entries 6+nt.. untouched).
  * prm: 15 floats in search order (g1_off, g1_w, g2_off, g2_w, rep_off, hyd_good, hyd_bad, hb_good, hb_bad, cutoff,

@@ -51,6 +51,9 @@ class _Lib:
         bg = [P, P, I, P, P, P, P, P, I, F, F, F, F, I, I, I, P, I, P]
         L.oracle_build_grid_f64.argtypes = bg
         L.oracle_build_grid_f32.argtypes = bg
+        bf = [P, P, I, P, P, I, P, I, F, F, F, F, I, I, I, P, P]
+        L.oracle_build_grid_fit_f64.argtypes = bf
+        L.oracle_build_grid_fit_f32.argtypes = bf
         L.oracle_eg_f64.argtypes = _LIG + _GRID + _SF + _REC + [I, P, I, P, P, P, I, P, P, P, P]
         L.oracle_eg_f32.argtypes = _LIG + _GRID + _SF + _REC + [I, P, I, P, P, P]
         L.oracle_mc.argtypes = ([I] + _LIG + _GRID + _SF + _REC + [I, P, P, I, P, P, I, P, P, P, P, I, I, I, F, F, F,
@@ -94,6 +97,25 @@ def build_grid(rec_xyz, rec_types, sfp_arrays, prm_grid, origin, spacing, n, lig
        float(origin[1]), float(origin[2]), float(spacing), nx, ny, nz, _ptr(prm), int(per_term), _ptr(out))
     shape = ((5, len(lt)) if per_term else (len(lt),)) + (nx, ny, nz)
     return out.reshape(shape)
+
+
+def build_grid_fit(rec_xyz, rec_ext, rad_rec, rad_lig, W, prm12, origin, spacing, n, f64=True):
+    """Oracle fitted-function grid [n_lt, nx, ny, nz] (src/adock/fit/gridfit.py:13-58). W [n_rt, n_lt, 6],
+    prm12 = cutoff, g1_off, g1_w, g2_off, g2_w, rep_off, hyd_good, hyd_bad, hb_good, hb_bad, ion_rmin, rep_cap."""
+    nx, ny, nz = (int(n),) * 3 if np.isscalar(n) else map(int, n)
+    rx = np.ascontiguousarray(rec_xyz, np.float32).reshape(-1, 3)
+    rt = np.ascontiguousarray(rec_ext, np.int32)
+    rr = np.ascontiguousarray(rad_rec, np.float32)
+    rl = np.ascontiguousarray(rad_lig, np.float32)
+    Wc = np.ascontiguousarray(W, np.float32)
+    assert Wc.ndim == 3 and Wc.shape[1] == len(rl) and Wc.shape[2] == 6
+    pr = np.ascontiguousarray(prm12, np.float32)
+    assert pr.shape == (12,)
+    out = np.zeros((len(rl) * nx * ny * nz,), np.float64 if f64 else np.float32)
+    fn = lib().lib.oracle_build_grid_fit_f64 if f64 else lib().lib.oracle_build_grid_fit_f32
+    fn(_ptr(rx), _ptr(rt), len(rx), _ptr(rr), _ptr(rl), len(rl), _ptr(Wc), Wc.shape[0], float(origin[0]),
+       float(origin[1]), float(origin[2]), float(spacing), nx, ny, nz, _ptr(pr), _ptr(out))
+    return out.reshape(len(rl), nx, ny, nz)
 
 
 class OracleProblem:

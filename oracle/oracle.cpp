@@ -11,6 +11,7 @@
 //   pair_e ........................ src/adock/mckernel.py:44-64 (also scoring.py:18-31, grid.py:40-59)
 //   build_grid .................... src/adock/grid.py:14-71 (tiling of cpukernel.py:102-137 is not needed on a CPU
 //                                    oracle: every point sees every atom, in receptor order)
+//   build_grid_fit ................ src/adock/fit/gridfit.py:13-58 (basis functions as fit/terms.py:233-250)
 //   smix / urand / rand_sphere .... src/adock/mckernel.py:71-90
 //   quat_inc ...................... src/adock/mckernel.py:93-105
 //   eval_eg ....................... src/adock/mckernel.py:108-244
@@ -24,6 +25,7 @@
 #include <cstdint>
 #include <cstring>
 #include <omp.h>
+#include <vector>
 
 namespace {
 
@@ -134,6 +136,54 @@ void build_grid(const float* rxyz, const int* rtype, int n_rec, const float* rad
                 out[(long long)q * npts + p] = e;
             }
         }
+    }
+}
+
+// ---------------------------------------------------------------- fitted grid (src/adock/fit/gridfit.py:13-58)
+// out[q, i, j, k] = sum over receptor atoms a within the cutoff of sum_k W[rext_a, q, k] * basis_k, with the basis
+// (gauss1, gauss2, repulsion capped at rep_cap, hydrophobic ramp, h-bond ramp) of d = r - rad_rec[rext_a] - rad_lig[q]
+// and the ionic basis 1 / max(r, ion_rmin)^2 - 1 / cutoff^2. prm: cutoff, g1_off, g1_w, g2_off, g2_w, rep_off,
+// hyd_good, hyd_bad, hb_good, hb_bad, ion_rmin, rep_cap.
+template <class R>
+void build_grid_fit(const float* rxyz, const int* rext, int n_rec, const float* rad_rec, const float* rad_lig,
+                    int n_lt, const float* W, int n_rt, float ox, float oy, float oz, float spacing, int nx, int ny,
+                    int nz, const float* prm, R* out) {
+    const long long npts = (long long)nx * ny * nz;
+    const float cut2f = prm[0] * prm[0];
+    const R rc = prm[0], g1o = prm[1], g1w = prm[2], g2o = prm[3], g2w = prm[4], repo = prm[5];
+    const R hg = prm[6], hbad = prm[7], bg = prm[8], bbad = prm[9], rmin = prm[10], cap = prm[11];
+#pragma omp parallel for schedule(dynamic, 64)
+    for (long long p = 0; p < npts; ++p) {
+        const int k = (int)(p % nz), j = (int)((p / nz) % ny), i = (int)(p / ((long long)ny * nz));
+        const float xf = std::fma((float)i, spacing, ox), yf = std::fma((float)j, spacing, oy),
+                    zf = std::fma((float)k, spacing, oz);
+        std::vector<R> e(n_lt, R(0));
+        for (int a = 0; a < n_rec; ++a) {
+            const int ta = rext[a];
+            if (ta < 0 || ta >= n_rt) continue;
+            const float dxf = rxyz[3 * a] - xf, dyf = rxyz[3 * a + 1] - yf, dzf = rxyz[3 * a + 2] - zf;
+            const float r2f = std::fma(dxf, dxf, std::fma(dyf, dyf, dzf * dzf));
+            if (!(r2f < cut2f)) continue;
+            const R dx = dxf, dy = dyf, dz = dzf;
+            const R r = std::sqrt(dx * dx + dy * dy + dz * dz);
+            const R rr = std::max(r, rmin);
+            const R ion = R(1) / (rr * rr) - R(1) / (rc * rc);
+            for (int q = 0; q < n_lt; ++q) {
+                const float* w = W + ((long long)ta * n_lt + q) * 6;
+                const R d = r - R(rad_rec[ta]) - R(rad_lig[q]);
+                R u = (d - g1o) / g1w;
+                R sv = R(w[0]) * r_exp(-u * u);
+                u = (d - g2o) / g2w;
+                sv += R(w[1]) * r_exp(-u * u);
+                const R dr = d - repo;
+                if (dr < 0) sv += R(w[2]) * std::min(dr * dr, cap);
+                sv += R(w[3]) * std::min(std::max((hbad - d) / (hbad - hg), R(0)), R(1));
+                sv += R(w[4]) * std::min(std::max((bbad - d) / (bbad - bg), R(0)), R(1));
+                sv += R(w[5]) * ion;
+                e[q] += sv;
+            }
+        }
+        for (int q = 0; q < n_lt; ++q) out[(long long)q * npts + p] = e[q];
     }
 }
 
@@ -544,6 +594,19 @@ void oracle_build_grid_f32(const float* rxyz, const int* rtype, int n_rec, const
                            float* out) {
     build_grid<float>(rxyz, rtype, n_rec, radius, hyd, don, acc, lig_types, n_lt, ox, oy, oz, spacing, nx, ny, nz,
                       prm, per_term, out);
+}
+
+void oracle_build_grid_fit_f64(const float* rxyz, const int* rext, int n_rec, const float* rad_rec,
+                               const float* rad_lig, int n_lt, const float* W, int n_rt, float ox, float oy, float oz,
+                               float spacing, int nx, int ny, int nz, const float* prm, double* out) {
+    build_grid_fit<double>(rxyz, rext, n_rec, rad_rec, rad_lig, n_lt, W, n_rt, ox, oy, oz, spacing, nx, ny, nz, prm,
+                           out);
+}
+void oracle_build_grid_fit_f32(const float* rxyz, const int* rext, int n_rec, const float* rad_rec,
+                               const float* rad_lig, int n_lt, const float* W, int n_rt, float ox, float oy, float oz,
+                               float spacing, int nx, int ny, int nz, const float* prm, float* out) {
+    build_grid_fit<float>(rxyz, rext, n_rec, rad_rec, rad_lig, n_lt, W, n_rt, ox, oy, oz, spacing, nx, ny, nz, prm,
+                          out);
 }
 
 // energy + gradient for N states; nbr/nbr_n optional (exact mode). Outputs FP64 e [N], g [N, 6+KM], X [N, AM, 3],

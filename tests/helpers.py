@@ -169,3 +169,36 @@ def cpu_search_problem(rec, preps, center, size, sf="vina", rec_idx=None):
     ri = torch.zeros(len(preps), dtype=torch.long) if rec_idx is None else torch.as_tensor(rec_idx)
     return Problem(gs, LigandBatch(preps, "cpu"), ri, PairPotential(sf_params(CFG, sf), "cpu"),
                    rec=pad_receptors([rec], "cpu"))
+
+
+def fit_unit_weights(m):
+    """Weights with 1 on term m (name or index) and 0 elsewhere."""
+    from adock_amd.fit.terms import TERM_NAMES
+    w = np.zeros(len(TERM_NAMES), np.float32)
+    w[TERM_NAMES.index(m) if isinstance(m, str) else m] = 1.0
+    return w
+
+
+def cpu_fitted_problem(cxs, spec, w):
+    """A fitted-function search Problem on host tensors: compact ligand types, FittedPotential, grids from the FP64
+    oracle (one common cube per complex, the make_gridset_fit rule) and the padded receptors (pair types TL + ext)."""
+    import torch
+    from adock_amd.fit.fdock import compact_ligands
+    from adock_amd.fit.terms import FittedPotential
+    from adock_amd.grid import GridSet
+    from adock_amd.problem import LigandBatch, Problem, pad_receptors
+    lig_list, ligs = compact_ligands(cxs)
+    pot = FittedPotential(spec, w, lig_list, "cpu")
+    n = int(math.ceil(max(float(np.max(c["size"])) for c in cxs) / SPACING)) + 1
+    grids, org, lo, hi = [], [], [], []
+    for c in cxs:
+        o = np.asarray(c["center"], float) - np.asarray(c["size"], float) / 2
+        grids.append(oracle.build_grid_fit(c["rec_xyz"], c["rec_ext"], spec.rec_radius, pot.radius.numpy(),
+                                           pot.W.numpy(), spec.grid_prm(), o, SPACING, n).astype(np.float32))
+        org.append(o)
+        lo.append(o)
+        hi.append(np.asarray(c["center"], float) + np.asarray(c["size"], float) / 2)
+    T = lambda a: torch.as_tensor(np.array(a), dtype=torch.float32)
+    gs = GridSet(torch.as_tensor(np.stack(grids)), T(org), T(lo), T(hi), SPACING, PENALTY)
+    recs = [{"rec_xyz": c["rec_xyz"], "rec_types": np.asarray(c["rec_ext"], np.int64) + pot.TL} for c in cxs]
+    return Problem(gs, LigandBatch(ligs, "cpu"), torch.arange(len(cxs)), pot, rec=pad_receptors(recs, "cpu"))

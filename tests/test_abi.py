@@ -49,3 +49,12 @@ def test_bucket_validation_without_gpu():
     assert L.adock_set_line_search_width(3) == -1
     assert L.adock_set_line_search_width(0) == 0
     assert L.adock_version().startswith(b"adock_hip")
+    # fitted grid builder: null pointers and more receptor ext types than the LDS table holds are rejected,
+    # empty grids are a no-op
+    fit = [None, None, 1, None, None, 1, None, 16, 0.0, 0.0, 0.0, 0.375, 4, 4, 4, None, None, None]
+    assert L.adock_build_grid_fit(*fit) == -1          # null pointers first
+    assert L.adock_build_grid_fit(*(fit[:12] + [0, 4, 4] + fit[15:])) == 0
+    buf = ctypes.create_string_buffer(64)
+    host = ctypes.cast(buf, ctypes.c_void_p).value       # never dereferenced: the call fails its checks first
+    bad_rt = [None, None, 0, None, host, 1, None, 300] + fit[8:15] + [host, host, None]
+    assert L.adock_build_grid_fit(*bad_rt) == -4
