@@ -10,8 +10,9 @@
 // capped repulsion, hydrophobic and h-bond ramps) and of r (ionic 1/max(r, r_min)^2 - 1/cutoff^2), weighted by a
 // per-(receptor ext type, ligand type) table W[n_rt, n_lt, 6] that is staged in LDS for the pass's ligand types.
 //
-// MI355X design: one workgroup owns a 4x8x8 tile of grid points (one point per thread, z fastest inside a wave so
-// output stores are contiguous). The receptor is streamed through LDS in 256-atom chunks; each chunk is culled against
+// MI355X design: one workgroup owns an 8x8x4 tile of grid points, one point per thread; each wave holds a 4x4x4 cube of
+// it, so the atoms in cutoff of at least one lane of a wave (the lanes that run the type loop) come from a sphere only
+// 1 A wider than the cutoff. The receptor is streamed through LDS in 256-atom chunks; each chunk is culled against
 // the tile's bounding box grown by the cutoff and compacted in receptor order with wave ballots (deterministic
 // summation order). Every thread then walks the compacted list with LDS broadcast reads. The in-cutoff test uses the
 // same float32 operations and fused-multiply-add nesting as the CPU oracle so both see the same atom set.
@@ -21,7 +22,15 @@
 namespace adock {
 namespace {
 
-constexpr int TI = 4, TJ = 8, TK = 8;          // tile: 256 points
+constexpr int TI = 8, TJ = 8, TK = 4;          // tile: 256 points, four 4x4x4 wave cubes
+
+// thread -> point of the tile: wave w takes the cube at i = 4 (w >> 1), j = 4 (w & 1); lanes run z fastest
+__device__ __forceinline__ void tile_point(int tid, int& ti, int& tj, int& tk) {
+    const int w = tid >> 6, l = tid & 63;
+    ti = ((w >> 1) << 2) + (l >> 4);
+    tj = ((w & 1) << 2) + ((l >> 2) & 3);
+    tk = l & 3;
+}
 constexpr int GB = TI * TJ * TK;
 constexpr int TQ = 8;                          // ligand types per pass
 
@@ -40,8 +49,8 @@ __global__ __launch_bounds__(GB) void build_grid_kernel(
     const int bk = tile % tiles_k; tile /= tiles_k;
     const int bj = tile % tiles_j;
     const int bi = tile / tiles_j;
-    // thread -> point: k fastest
-    const int tk = tid % TK, tj = (tid / TK) % TJ, ti = tid / (TK * TJ);
+    int ti, tj, tk;
+    tile_point(tid, ti, tj, tk);
     const int i = bi * TI + ti, j = bj * TJ + tj, k = bk * TK + tk;
     const bool valid = (i < nx) && (j < ny) && (k < nz);
     const float x = fmaf((float)i, spacing, ox), y = fmaf((float)j, spacing, oy), z = fmaf((float)k, spacing, oz);
@@ -167,7 +176,8 @@ __global__ __launch_bounds__(GB) void build_grid_fit_kernel(
     const int bk = tile % tiles_k; tile /= tiles_k;
     const int bj = tile % tiles_j;
     const int bi = tile / tiles_j;
-    const int tk = tid % TK, tj = (tid / TK) % TJ, ti = tid / (TK * TJ);
+    int ti, tj, tk;
+    tile_point(tid, ti, tj, tk);
     const int i = bi * TI + ti, j = bj * TJ + tj, k = bk * TK + tk;
     const bool valid = (i < nx) && (j < ny) && (k < nz);
     const float x = fmaf((float)i, spacing, ox), y = fmaf((float)j, spacing, oy), z = fmaf((float)k, spacing, oz);

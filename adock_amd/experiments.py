@@ -3,6 +3,8 @@
 screen():  one receptor grid set, many uneven ligands, all chains in one persistent HIP-graph-replayed launch.
 multi():   many receptors (grid sets resident, admitted in groups under an HBM budget), per-complex latency from the
            kernel's completion stamps.
+grids():   both grid builders (Vina XS and the fitted function) on the single-complex receptor, with the in-cutoff
+           work counted exactly, for the builders' FP32 VALU roofline.
 learned(): trains the proposal network on accepted moves logged by the plain method on training complexes, then
            compares plain and learned search on disjoint evaluation complexes at equal MC-step budgets.
 """
@@ -98,6 +100,94 @@ def complexes(seeds, config=2):
         ligs.append(ws["lig"])
     prob = workload.gpu_problem(recs, ligs, np.arange(len(seeds)), with_rec=False)
     return recs, ligs, prob
+
+
+# algorithmic FP32 operations of the grid builders (transcendentals counted as one): per in-cutoff (point, atom) pair
+# and per (pair, ligand type) triple
+GRID_OPS = {"vina": (10, 26), "fitted": (15, 38)}
+
+
+def _in_cutoff_pairs(xyz, origin, dims, spacing, cutoff, dev):
+    """Exact count of (grid point, receptor atom) pairs closer than the cutoff."""
+    R = torch.as_tensor(np.asarray(xyz, np.float32), device=dev)
+    ax = [torch.arange(n, device=dev, dtype=torch.float32) * spacing + float(o) for n, o in zip(dims, origin)]
+    total = 0
+    for i in range(dims[0]):
+        P = torch.stack(torch.meshgrid(ax[0][i:i + 1], ax[1], ax[2], indexing="ij"), -1).reshape(-1, 3)
+        total += int((torch.cdist(P, R) < cutoff).sum())
+    return total
+
+
+def grids(seed=0, reps=10):
+    """Both grid builders on the single-complex receptor (71^3 points). kernel_ms: the C entry point on device-resident
+    inputs between HIP events on its stream (one launch per 8 ligand types); call_ms: the Python facade, which also
+    uploads the receptor and tables on every call."""
+    from . import _lib
+    from .config import grid_prm, sf_params
+    from .fit.fdock import DATA, compact_ligands, load_fitted
+    from .fit.gridfit import build_grid_fit
+    from .fit.terms import FittedPotential
+    from .grid import build_grid
+    dev = torch.device("cuda:0")
+    w = synth.make_single(2, seed)
+    rec = w["receptor"]
+    size = float(np.max(w["size"]))
+    nx, ny, nz = dims = synth.grid_dims(size, workload.SPACING)
+    origin = np.asarray(w["center"], float) - size / 2
+    sfp = sf_params(workload.CFG, "vina")
+    lt = np.unique(np.concatenate([[0], w["lig"].types[w["lig"].types >= 0]])).astype(np.int32)
+    spec, wts = load_fitted(DATA / "adock_fitted.json", workload.CFG)
+    cx = synth.fitted_complex(rec, w["lig"], w["center"], size, spec, seed=seed)
+    lig_list, _ = compact_ligands([cx])
+    pot = FittedPotential(spec, wts, lig_list, dev)
+    D = lambda a, t=np.float32: torch.as_tensor(np.ascontiguousarray(a, t), device=dev)
+    rx = D(rec["rec_xyz"])
+    vina_args = [D(rec["rec_types"], np.int32), D(sfp["radius"]), D(sfp["hydrophobic"], np.int32),
+                 D(sfp["donor"], np.int32), D(sfp["acceptor"], np.int32), D(lt, np.int32), D(grid_prm(sfp))]
+    fit_args = [D(cx["rec_ext"], np.int32), D(spec.rec_radius), pot.radius.float().contiguous(),
+                pot.W.float().contiguous(), D(spec.grid_prm())]
+    out_v = torch.empty(len(lt) * nx * ny * nz, device=dev)
+    out_f = torch.empty(pot.TL * nx * ny * nz, device=dev)
+    L, p, o = _lib.lib(), _lib.ptr, [float(x) for x in origin]
+    stream = _lib.stream_of(dev)
+    a = vina_args
+    raw = {"vina": lambda: L.adock_build_grid(p(rx), p(a[0]), len(rx), p(a[1]), p(a[2]), p(a[3]), p(a[4]), p(a[5]),
+                                              len(lt), *o, workload.SPACING, nx, ny, nz, p(a[6]), 0, p(out_v), stream),
+           "fitted": lambda: L.adock_build_grid_fit(p(rx), p(fit_args[0]), len(rx), p(fit_args[1]), p(fit_args[2]),
+                                                    pot.TL, p(fit_args[3]), spec.n_rt, *o, workload.SPACING, nx, ny,
+                                                    nz, p(fit_args[4]), p(out_f), stream)}
+    facade = {"vina": lambda: build_grid(rec["rec_xyz"], rec["rec_types"], sfp, origin, workload.SPACING, dims,
+                                         lig_types=lt, device=dev),
+              "fitted": lambda: build_grid_fit(cx["rec_xyz"], cx["rec_ext"], pot, origin, workload.SPACING, dims,
+                                               device=dev)}
+    pairs = _in_cutoff_pairs(rec["rec_xyz"], origin, dims, workload.SPACING, float(sfp["cutoff"]), dev)
+    out = {"workload": f"grid sets of the single-complex receptor: {len(rec['rec_xyz'])} atoms, "
+                       f"{nx}x{ny}x{nz} points", "in_cutoff_point_atom_pairs": pairs}
+
+    def event_ms(fn):
+        fn()
+        torch.cuda.synchronize()
+        ms = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            r = fn()
+            e1.record()
+            torch.cuda.synchronize()
+            if isinstance(r, int):
+                _lib.check(r)
+            ms.append(e0.elapsed_time(e1))
+        return float(np.mean(ms))
+
+    for name, T in (("vina", len(lt)), ("fitted", pot.TL)):
+        kms, cms = event_ms(raw[name]), event_ms(facade[name])
+        t = kms * 1e-3
+        per_pair, per_triple = GRID_OPS[name]
+        flops = pairs * per_pair + pairs * T * per_triple
+        out[name] = {"types": int(T), "kernel_ms": kms, "call_ms": cms, "grid_sets_per_s": 1.0 / t,
+                     "point_atom_pairs_per_s": pairs / t, "triples_per_s": pairs * T / t,
+                     "achieved_tflops": flops / t / 1e12, "ops_per_pair": per_pair, "ops_per_triple": per_triple}
+    return out
 
 
 def _timed(fn):

@@ -68,7 +68,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--chains-per-block", type=int, default=0)
-    ap.add_argument("--workload", choices=["single", "screen", "multi", "learned"], default="single",
+    ap.add_argument("--workload", choices=["single", "screen", "multi", "learned", "grids"], default="single",
                     help="single = the headline line (default); the others print their own JSON line")
     ap.add_argument("--small", action="store_true", help="scaled-down sizes for the opt-in workloads")
     ap.add_argument("--target", choices=["move", "jump"], default="move",
@@ -84,6 +84,11 @@ def main():
         from adock_amd import experiments as ex
         if args.workload == "screen":
             out = ex.screen(n_ligands=64 if args.small else 512, chains=64, steps=50 if args.small else 500)
+        elif args.workload == "grids":
+            out = ex.grids()
+            for k in ("vina", "fitted"):
+                out[k]["roofline"] = {"bound": "valu", "achieved": out[k]["achieved_tflops"], "peak": PEAK_FP32_TFLOPS,
+                                      "unit": "TFLOP/s", "frac": out[k]["achieved_tflops"] / PEAK_FP32_TFLOPS}
         elif args.workload == "multi":
             out = ex.multi(n_receptors=8 if args.small else 64, ligands_per=8 if args.small else 32, chains=32,
                            steps=50 if args.small else 500)
