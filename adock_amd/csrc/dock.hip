@@ -30,16 +30,20 @@ namespace adock {
 namespace {
 
 constexpr int AMK = 64;   // heavy-atom bucket handled by these kernels (one atom per lane)
+constexpr unsigned PAIR_HYD = 1u << 14, PAIR_HB = 1u << 15;       // pair-word class bits
+constexpr unsigned short PAIR_PAD = (unsigned short)AMK;        // (dummy atom, atom 0)
 
 // Per-wave LDS layout, in bytes (P128 = P_MAX rounded up to a multiple of 128, KM + 1 torsion records):
-//   X4  [64]        float4  world xyz + radius of each atom
+//   X4  [64 + 1]    float4  world xyz + radius of each atom; entry 64 is a dummy atom far beyond the cutoff (padding
+//                           pairs point at it, so they contribute exactly zero without a validity flag)
 //   TR0, TR1, TR2 [KM + 1] float4  torsion k's rotation about its reference axis as an affine map (row i: R_i0,
 //                                  R_i1, R_i2, b_i; p -> R p + b), rebuilt every evaluation (identity past nt)
 //   TU  [KM + 1]    float4  current (world) axis unit vector u_k
 //   TM  [KM + 1]    float4  current axis moment m_k = p_k x u_k (torque about the axis: u.(X x F) + m.F)
 //   FS  [P128 + 1]  float4  gradient slot of pair q (dE/dX of its first atom); slot P128 stays zero
 //   flg [64]        int     atom type flags
-//   pl  [P128]      u16     pair words, strided over lanes (pair q on lane q % 64 in round q / 64)
+//   pl  [P128]      u16     pair words i | j << 7 | hydrophobic << 14 | h-bond << 15, strided over lanes (pair q on
+//                           lane q % 64 in round q / 64); padding pairs are (64, 0)
 //   inc [2 P128 + 512] u16  per-atom pair-slot lists, copied from the host-built packed ligand (each atom's list
 //                           padded to 8 entries; start and padded count live in the atom's lane); bit 15 = negate
 // Dense variant (the latency-bound speculative kernel): FS is replaced by FD [64 DEG_MAX + 1] float4, row k column
@@ -51,12 +55,12 @@ __host__ __device__ inline int pad128(int P_MAX) { return (P_MAX + 127) & ~127; 
 __host__ __device__ inline int inc_stride(int P_MAX) { return 2 * pad128(P_MAX) + 512; }
 __host__ __device__ inline int wave_lds_bytes(int KM, int P_MAX) {
     const int P = pad128(P_MAX);
-    return 16 * AMK + 5 * 16 * (KM + 1) + 16 * (P + 1) + 4 * AMK + 2 * P + 2 * inc_stride(P_MAX);
+    return 16 * (AMK + 1) + 5 * 16 * (KM + 1) + 16 * (P + 1) + 4 * AMK + 2 * P + 2 * inc_stride(P_MAX);
 }
 __host__ __device__ inline int pad512(int P_MAX) { return (P_MAX + 511) & ~511; }
 __host__ __device__ inline int wave_lds_bytes_dense(int KM, int P_MAX, int DEG_MAX) {
     const int P = pad512(P_MAX);   // pair words and destinations padded to whole 8-round chunks
-    return 16 * AMK + 5 * 16 * (KM + 1) + 16 * (AMK * DEG_MAX + 1) + 4 * AMK + 2 * P + 4 * P;
+    return 16 * (AMK + 1) + 5 * 16 * (KM + 1) + 16 * (AMK * DEG_MAX + 1) + 4 * AMK + 2 * P + 4 * P;
 }
 template <bool DENSE>
 __host__ __device__ inline int lds_bytes(int KM, int P_MAX, int DEG_MAX) {
@@ -102,7 +106,7 @@ DEVI WaveLDS carve_lds(char* base, int P_MAX, int DEG_MAX = 0) {
     WaveLDS w;
     w.P128 = DENSE ? pad512(P_MAX) : pad128(P_MAX);
     float4* f = reinterpret_cast<float4*>(base);
-    w.X4 = f; f += AMK;
+    w.X4 = f; f += AMK + 1;
     w.TR0 = f; f += KM + 1;
     w.TR1 = f; f += KM + 1;
     w.TR2 = f; f += KM + 1;
@@ -177,6 +181,7 @@ DEVI ChainCtx setup_chain(int l, int lane, const LigArgs& L, const GridArgs& G, 
         w.TM[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
     w.flg[lane] = c.flg;
+    if (lane == 0) w.X4[AMK] = make_float4(1e18f, 1e18f, 1e18f, 0.f);
     if constexpr (DENSE) {
         for (int i = lane; i <= w.trash; i += WAVE) w.FD[i] = make_float4(0.f, 0.f, 0.f, 0.f);
         for (int q = lane; q < w.P128; q += WAVE) w.pdst[q] = (unsigned)w.trash | ((unsigned)w.trash << 16);
@@ -184,13 +189,13 @@ DEVI ChainCtx setup_chain(int l, int lane, const LigArgs& L, const GridArgs& G, 
         w.FS[w.P128] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
     wave_lds_sync();
-    // pair word: i | j << 6 | (hydrophobic pair) << 12 | (h-bond pair) << 13; 0x8000 marks padding (and self-pairs)
+    // pair word: i | j << 7 | (hydrophobic pair) << 14 | (h-bond pair) << 15; padding (and self-pairs): the dummy atom
     const int* pr = L.pr + (long long)l * L.P_MAX * 2;
     for (int q = lane; q < w.P128; q += WAVE) {
-        unsigned short word = 0x8000;
+        unsigned short word = PAIR_PAD;
         if (q < c.np) {
             const int i = pr[2 * q], j = pr[2 * q + 1];
-            if (i != j) word = (unsigned short)(i | (j << 6) | (pair_flags(w.flg[i], w.flg[j]) << 12));
+            if (i != j) word = (unsigned short)(i | (j << 7) | (pair_flags(w.flg[i], w.flg[j]) << 14));
         }
         w.pl[q] = word;
     }
@@ -231,19 +236,20 @@ DEVI ChainCtx setup_chain(int l, int lane, const LigArgs& L, const GridArgs& G, 
 // energy of pair word pw (0 for padding) at atom coordinates + radii xi, xj, and its gradient slot (dE/dX of the
 // first atom; the second gets minus it)
 DEVI float pair_math(float4 xi, float4 xj, unsigned pw, const SFConst& sf, float4& s, bool hyd, bool hb) {
-    const float valid = (pw & 0x8000u) ? 0.f : 1.f;
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-    const float rr = fast_sqrt(rx * rx + ry * ry + rz * rz);
+    const float r2 = rx * rx + ry * ry + rz * rz;
+    const float rinv = fast_rsq(fmaxf(r2, 1e-12f));
+    const float rr = r2 * rinv;
     float de;
-    const float e = pair_e_nb(rr - xi.w - xj.w, (int)((pw >> 12) & 3u), sf, de, hyd, hb);
-    const float in = rr < sf.cut ? valid : 0.f;
-    const float sc = in * de * fast_rcp(fmaxf(rr, 1e-6f));
+    const float e = pair_e_nb(rr - xi.w - xj.w, (int)(pw >> 14), sf, de, hyd, hb);
+    const float in = rr < sf.cut ? 1.f : 0.f;
+    const float sc = in * de * rinv;
     s = make_float4(sc * rx, sc * ry, sc * rz, 0.f);
     return in * e;
 }
 
 DEVI float pair_slot(const WaveLDS& w, unsigned pw, const SFConst& sf, float4& s, bool hyd, bool hb) {
-    return pair_math(w.X4[pw & 63], w.X4[(pw >> 6) & 63], pw, sf, s, hyd, hb);
+    return pair_math(w.X4[pw & 127], w.X4[(pw >> 7) & 127], pw, sf, s, hyd, hb);
 }
 
 // Energy and lane-distributed gradient of the lane-distributed state x. Returns E (uniform). EX: exact pairwise
@@ -339,11 +345,14 @@ DEVI float eval_eg(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, int l
 #pragma unroll
             for (int r = 0; r < 8; ++r) { pw[r] = w.pl[cb + r * WAVE + lane]; dd[r] = w.pdst[cb + r * WAVE + lane]; }
 #pragma unroll
-            for (int r = 0; r < 8; ++r) { xi[r] = w.X4[pw[r] & 63]; xj[r] = w.X4[(pw[r] >> 6) & 63]; }
+            for (int r = 0; r < 8; ++r) { xi[r] = w.X4[pw[r] & 127]; xj[r] = w.X4[(pw[r] >> 7) & 127]; }
 #pragma unroll
             for (int r = 0; r < 8; ++r) {
+                // pairs are sorted h-bond, hydrophobic, other: most rounds skip both slope terms (uniform test)
+                const bool hyd = __ballot((pw[r] & PAIR_HYD) != 0u) != 0ull;
+                const bool hb = __ballot((pw[r] & PAIR_HB) != 0u) != 0ull;
                 float4 s0;
-                ep_l += pair_math(xi[r], xj[r], pw[r], sf, s0, true, true);
+                ep_l += pair_math(xi[r], xj[r], pw[r], sf, s0, hyd, hb);
                 w.FD[dd[r] & 0xffffu] = s0;
                 w.FD[dd[r] >> 16] = make_float4(-s0.x, -s0.y, -s0.z, 0.f);
             }
@@ -351,8 +360,8 @@ DEVI float eval_eg(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, int l
     } else {
         for (int q0 = lane; q0 < c.np2; q0 += 2 * WAVE) {
             const unsigned pw0 = w.pl[q0], pw1 = w.pl[q0 + WAVE];
-            const bool hyd = __ballot(((pw0 | pw1) & 0x1000u) != 0u) != 0ull;
-            const bool hb = __ballot(((pw0 | pw1) & 0x2000u) != 0u) != 0ull;
+            const bool hyd = __ballot(((pw0 | pw1) & PAIR_HYD) != 0u) != 0ull;
+            const bool hb = __ballot(((pw0 | pw1) & PAIR_HB) != 0u) != 0ull;
             float4 s0, s1;
             ep_l += pair_slot(w, pw0, sf, s0, hyd, hb);
             ep_l += pair_slot(w, pw1, sf, s1, hyd, hb);
