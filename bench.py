@@ -68,6 +68,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--chains-per-block", type=int, default=0)
+    ap.add_argument("--line-search-width", type=int, default=0, help="trials per speculative batch (0 = adaptive)")
     ap.add_argument("--workload", choices=["single", "screen", "multi", "learned", "grids"], default="single",
                     help="single = the headline line (default); the others print their own JSON line")
     ap.add_argument("--small", action="store_true", help="scaled-down sizes for the opt-in workloads")
@@ -101,9 +102,10 @@ def main():
     dev = torch.device("cuda:0")
     w, prob = workload.single(2, seed=0, device=dev)
     kp = KernelProblem(prob)
+    from adock_amd import _lib
     if args.chains_per_block:
-        from adock_amd import _lib
         _lib.check(_lib.lib().adock_set_chains_per_block(args.chains_per_block))
+    _lib.check(_lib.lib().adock_set_line_search_width(args.line_search_width))
     sfp = sf_params(workload.CFG, "vina")
     rec = workload.receptor_dict(w["receptor"], w["center"], w["size"])
     lt = [np.unique(w["lig"].types)]
