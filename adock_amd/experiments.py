@@ -91,11 +91,12 @@ def multi(n_receptors=64, ligands_per=32, chains=32, steps=500, seed=0, hbm_budg
             "latency_p99_s": float(np.percentile(lat, 99)), "best_score_mean": float(best.mean())}
 
 
-def complexes(seeds, config=2):
-    """Config-2-sized complexes, one receptor + one ligand per seed, batched into one Problem."""
+def complexes(seeds, name="single"):
+    """Single-workload-sized complexes (4,000-atom receptor, 35-atom ligand), one receptor + one ligand per seed,
+    batched into one Problem."""
     recs, ligs = [], []
     for s in seeds:
-        ws = synth.make_single(config, s)
+        ws = synth.make_single(name, s)
         recs.append(workload.receptor_dict(ws["receptor"], ws["center"], ws["size"]))
         ligs.append(ws["lig"])
     prob = workload.gpu_problem(recs, ligs, np.arange(len(seeds)), with_rec=False)
@@ -129,7 +130,7 @@ def grids(seed=0, reps=10):
     from .fit.terms import FittedPotential
     from .grid import build_grid
     dev = torch.device("cuda:0")
-    w = synth.make_single(2, seed)
+    w = synth.make_single("single", seed)
     rec = w["receptor"]
     size = float(np.max(w["size"]))
     nx, ny, nz = dims = synth.grid_dims(size, workload.SPACING)
@@ -209,7 +210,7 @@ def _budget_to_gap(rows, key, target):
 def learned(n_train=512, train_chains=64, train_steps=400, n_eval=512, eval_chains=64,
             budgets=(10, 15, 20, 30, 40, 60, 80, 120, 160, 240, 320, 480), full_steps=1000, epochs=5, p_plain=0.25,
             gap_target=0.1, seed=0, target="move", every=2, log=print):
-    """Config 5: the proposal network trained on the plain method's runs over n_train complexes, then plain (fused
+    """Learned proposal: the proposal network trained on the plain method's runs over n_train complexes, then plain (fused
     kernel, one launch) and learned (step kernel + MFMA inference, the whole search captured in one HIP graph)
     searches on n_eval disjoint complexes at a range of MC-step budgets. Gap = mean over complexes of
     best(budget) - best(plain, full_steps). Reports, per method, the smallest budget reaching gap_target and the
@@ -277,7 +278,7 @@ def learned(n_train=512, train_chains=64, train_steps=400, n_eval=512, eval_chai
             f"({tl:.2f} s)")
     sp, tp = _budget_to_gap(rows, "plain", gap_target)
     sl, tl = _budget_to_gap(rows, "learned", gap_target)
-    out = {"workload": f"learned proposal ({target} target): trained on {n_moves} samples from {L} config-2 complexes, "
+    out = {"workload": f"learned proposal ({target} target): trained on {n_moves} samples from {L} single complexes, "
                        f"evaluated on {E} disjoint complexes x {eval_chains} chains ({E * eval_chains} chains per step)",
            "gap_target_kcal": gap_target, "full_plain_steps": full_steps, "full_plain_s": t_full, "log_s": t_log,
            "train_s": t_train, "train_loss": losses, "rows": rows, "total_s": time.perf_counter() - t_all,

@@ -207,17 +207,19 @@ def start_states(seed, n, center, size, n_tors, K):
 
 
 # Workload definitions. Receptor radius / cavity / atom counts, ligand sizes, box edges, chains and steps.
-CONFIGS = {
-    1: dict(rec=dict(n_atoms=1500, radius=20.0, cavity=6.0, min_sep=3.0), lig=(20, 4), box=20.0, chains=8, steps=200,
+WORKLOADS = {
+    "small": dict(rec=dict(n_atoms=1500, radius=20.0, cavity=6.0, min_sep=3.0), lig=(20, 4), box=20.0, chains=8, steps=200,
             bfgs_iters=0),
-    2: dict(rec=dict(n_atoms=4000, radius=26.0, cavity=8.0, min_sep=2.8), lig=(35, 8), box=26.0, chains=256,
+    "single": dict(rec=dict(n_atoms=4000, radius=26.0, cavity=8.0, min_sep=2.8), lig=(35, 8), box=26.0, chains=256,
             steps=1000, bfgs_iters=0),
 }
 
 
-def make_single(config, seed=0):
-    """A single-complex workload (configs 1 and 2): receptor, ligand prep, box centre and size."""
-    c = CONFIGS[config]
+def make_single(name, seed=0):
+    """A single-complex workload ("small": 1,500-atom receptor, 20-atom / 4-torsion ligand, 20 A box, 8 chains x 200 steps;
+    "single": 4,000-atom receptor, 35-atom / 8-torsion ligand, 26 A box, 256 chains x 1,000 steps): receptor, ligand
+    prep, box centre and size."""
+    c = WORKLOADS[name]
     rec = make_receptor(seed, **c["rec"])
     lig = make_ligand(seed + 1, *c["lig"])
     return {"receptor": rec, "lig": lig, "center": np.zeros(3), "size": np.full(3, c["box"]), "chains": c["chains"],
@@ -225,18 +227,18 @@ def make_single(config, seed=0):
 
 
 def make_screen(seed=0, n_ligands=512, chains=64, steps=500):
-    """Config 3: one config-1 receptor and box, n_ligands ligands with heavy atoms ~ U[10, 60] and torsions
+    """Screening batch: one small-workload receptor and box, n_ligands ligands with heavy atoms ~ U[10, 60] and torsions
     ~ U[0, 15] (capped by the ligand's non-terminal bonds)."""
     rng = np.random.default_rng(seed + 991)
-    rec = make_receptor(seed, **CONFIGS[1]["rec"])
+    rec = make_receptor(seed, **WORKLOADS["small"]["rec"])
     ligs = [make_ligand(seed * 100003 + 17 + i, int(rng.integers(10, 61)), int(rng.integers(0, 16)))
             for i in range(n_ligands)]
-    return {"receptor": rec, "ligs": ligs, "center": np.zeros(3), "size": np.full(3, CONFIGS[1]["box"]),
+    return {"receptor": rec, "ligs": ligs, "center": np.zeros(3), "size": np.full(3, WORKLOADS["small"]["box"]),
             "chains": chains, "steps": steps, "bfgs_iters": 0}
 
 
 def make_multi(seed=0, n_receptors=64, ligands_per=32, chains=32, steps=500):
-    """Config 4: n_receptors receptors of 2,000-6,000 atoms (sphere radius for ~0.05 atoms/A^3 around a 6-8 A
+    """Multi-receptor batch: n_receptors receptors of 2,000-6,000 atoms (sphere radius for ~0.05 atoms/A^3 around a 6-8 A
     cavity), box edges ~ U[16, 30] A, ligands_per ligands each from the U[10, 60] atoms / U[0, 15] torsions mix."""
     rng = np.random.default_rng(seed + 4242)
     recs, boxes, ligs, rec_idx = [], [], [], []

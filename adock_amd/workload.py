@@ -61,16 +61,17 @@ def host_arrays(prob):
     return AM, KM, lig, grid, sf, rec
 
 
-def single(config, seed=0, sf="vina", device="cuda"):
-    """Configs 1 / 2: one receptor, one ligand. Returns (workload dict, Problem)."""
-    w = synth.make_single(config, seed)
+def single(name, seed=0, sf="vina", device="cuda"):
+    """A single-complex workload ("small" or "single", synth.make_single): one receptor, one ligand. Returns (workload
+    dict, Problem)."""
+    w = synth.make_single(name, seed)
     rec = receptor_dict(w["receptor"], w["center"], w["size"])
     prob = gpu_problem([rec], [w["lig"]], [0], sf=sf, device=device)
     return w, prob
 
 
 def screen(seed=0, n_ligands=512, chains=64, steps=500, sf="vina", device="cuda"):
-    """Config 3: one grid set (union of ligand types + C_H) shared by n_ligands ligands."""
+    """Screening batch: one grid set (union of ligand types + C_H) shared by n_ligands ligands."""
     w = synth.make_screen(seed, n_ligands, chains, steps)
     rec = receptor_dict(w["receptor"], w["center"], w["size"])
     prob = gpu_problem([rec], w["ligs"], np.zeros(len(w["ligs"]), np.int64), sf=sf, device=device, with_rec=False)
@@ -78,7 +79,7 @@ def screen(seed=0, n_ligands=512, chains=64, steps=500, sf="vina", device="cuda"
 
 
 def multi(seed=0, n_receptors=64, ligands_per=32, chains=32, steps=500, sf="vina", device="cuda"):
-    """Config 4: one grid set per receptor (its ligands' types + C_H), all resident."""
+    """Multi-receptor batch: one grid set per receptor (its ligands' types + C_H), all resident."""
     w = synth.make_multi(seed, n_receptors, ligands_per, chains, steps)
     recs = [receptor_dict(r, np.zeros(3), b) for r, b in zip(w["receptors"], w["boxes"])]
     prob = gpu_problem(recs, w["ligs"], w["rec_idx"], sf=sf, device=device, with_rec=False)

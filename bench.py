@@ -100,7 +100,7 @@ def main():
         print(json.dumps(out), flush=True)
         return
     dev = torch.device("cuda:0")
-    w, prob = workload.single(2, seed=0, device=dev)
+    w, prob = workload.single("single", seed=0, device=dev)
     kp = KernelProblem(prob)
     from adock_amd import _lib
     if args.chains_per_block:

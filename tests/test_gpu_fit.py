@@ -95,7 +95,7 @@ def test_fit_grid_rejects_bad_types():
 
 
 def _complexes(spec):
-    w1 = synth.make_single(1, 0)
+    w1 = synth.make_single("small", 0)
     rec = workload.receptor_dict(w1["receptor"], w1["center"], w1["size"])
     ligs = [w1["lig"], synth.make_ligand(31, 16, 3)]
     s = float(np.max(w1["size"]))

@@ -14,8 +14,8 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.fixture(scope="module")
-def cfg1():
-    w, prob = workload.single(1, seed=0)
+def small_cx():
+    w, prob = workload.single("small", seed=0)
     kp = KernelProblem(prob)
     N = 32
     t, q, th = synth.start_states(4, N, w["center"], w["size"], w["lig"].n_tors, kp.KM)
@@ -28,9 +28,9 @@ def _bufs(st):
     return st.clone(), torch.zeros(N).cuda(), st.clone(), torch.full((N,), float("inf")).cuda()
 
 
-def test_step_pipeline_equals_fused_plain(cfg1):
+def test_step_pipeline_equals_fused_plain(small_cx):
     print("[gpu] step pipeline vs fused", flush=True)
-    w, kp, st = cfg1
+    w, kp, st = small_cx
     N, S = st.shape[0], 12
     cl = torch.zeros(N, dtype=torch.int32).cuda()
     ref = _bufs(st)
@@ -51,9 +51,9 @@ def test_step_pipeline_equals_fused_plain(cfg1):
     assert (np.abs(X) <= 64).all()
 
 
-def test_network_search_invariants_and_graph_replay(cfg1):
+def test_network_search_invariants_and_graph_replay(small_cx):
     print("[gpu] network search + graph", flush=True)
-    w, kp, st = cfg1
+    w, kp, st = small_cx
     N, S = st.shape[0], 8
     cl = torch.zeros(N, dtype=torch.int32).cuda()
     net = ProposalNet("cuda", seed=1)
@@ -77,9 +77,9 @@ def test_network_search_invariants_and_graph_replay(cfg1):
             np.testing.assert_array_equal(a.cpu().numpy(), b.cpu().numpy())
 
 
-def test_training_on_logged_moves_reduces_loss(cfg1):
+def test_training_on_logged_moves_reduces_loss(small_cx):
     print("[gpu] train on logged moves", flush=True)
-    w, kp, st = cfg1
+    w, kp, st = small_cx
     N = st.shape[0]
     cl = torch.zeros(N, dtype=torch.int32).cuda()
     ss = StepSearch(kp, cl, log_cap=50000)

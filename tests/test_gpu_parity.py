@@ -31,14 +31,14 @@ def _oracle_of(prob):
 
 
 @pytest.fixture(scope="module")
-def cfg1():
-    w, prob = workload.single(1, seed=0)
+def small_cx():
+    w, prob = workload.single("small", seed=0)
     return w, prob, KernelProblem(prob), _oracle_of(prob)
 
 
 @pytest.fixture(scope="module")
-def cfg2():
-    w, prob = workload.single(2, seed=0)
+def single_cx():
+    w, prob = workload.single("single", seed=0)
     return w, prob, KernelProblem(prob), _oracle_of(prob)
 
 
@@ -59,10 +59,10 @@ def test_grid_random_receptor(sf):
     assert rel_err(gp, op, GRID_FLOOR, GRID_REL) < GRID_REL
 
 
-def test_grid_config1_full(cfg1):
-    """Whole config-1 grid set (55^3 x types): 1,500 receptor atoms."""
-    print("[gpu] grid config 1", flush=True)
-    w, prob, kp, op = cfg1
+def test_grid_small_full(small_cx):
+    """Whole grid set of the small workload (55^3 x types): 1,500 receptor atoms."""
+    print("[gpu] grid small complex", flush=True)
+    w, prob, kp, op = small_cx
     sfp = sf_params(CFG, "vina")
     rec = w["receptor"]
     types = np.nonzero(prob.gs.slot[0].cpu().numpy() >= 0)[0]
@@ -71,11 +71,11 @@ def test_grid_config1_full(cfg1):
     assert rel_err(g, o, GRID_FLOOR, GRID_REL) < GRID_REL
 
 
-def test_grid_config2_subbox(cfg2):
-    """Config 2 (71^3, 4,000 atoms): a 14^3 sub-box of the GPU grid vs the oracle built on that sub-box. The origin
+def test_grid_single_subbox(single_cx):
+    """Single-complex workload (71^3, 4,000 atoms): a 14^3 sub-box of the GPU grid vs the oracle built on that sub-box. The origin
     -13 A and h = 0.375 make every point an exact multiple of 1/8, so both sides place the points identically."""
-    print("[gpu] grid config 2 sub-box", flush=True)
-    w, prob, kp, op = cfg2
+    print("[gpu] grid single complex sub-box", flush=True)
+    w, prob, kp, op = single_cx
     sfp = sf_params(CFG, "vina")
     types = np.nonzero(prob.gs.slot[0].cpu().numpy() >= 0)[0]
     g = prob.gs.flat.cpu().numpy().reshape(len(types), 71, 71, 71)
@@ -104,31 +104,31 @@ def _eg_parity(kp, op, w, st, exact=False):
 
 
 @pytest.mark.parametrize("which", ["near", "box"])
-def test_eg_config1(cfg1, which):
-    print(f"[gpu] eg config 1 {which}", flush=True)
-    w, prob, kp, op = cfg1
+def test_eg_small(small_cx, which):
+    print(f"[gpu] eg small complex {which}", flush=True)
+    w, prob, kp, op = small_cx
     st = states(5, 256, w["center"], w["size"], w["lig"], kp.KM, sigma=1.5 if which == "near" else None)
     _eg_parity(kp, op, w, st)
 
 
-def test_eg_config2(cfg2):
-    print("[gpu] eg config 2", flush=True)
-    w, prob, kp, op = cfg2
+def test_eg_single(single_cx):
+    print("[gpu] eg single complex", flush=True)
+    w, prob, kp, op = single_cx
     st = states(6, 256, w["center"], w["size"], w["lig"], kp.KM, sigma=2.0)
     _eg_parity(kp, op, w, st)
 
 
 def test_eg_vinardo():
     print("[gpu] eg vinardo", flush=True)
-    w, prob = workload.single(1, seed=3, sf="vinardo")
+    w, prob = workload.single("small", seed=3, sf="vinardo")
     kp, op = KernelProblem(prob), _oracle_of(prob)
     st = states(7, 256, w["center"], w["size"], w["lig"], kp.KM, sigma=1.5)
     _eg_parity(kp, op, w, st)
 
 
-def test_eg_exact_mode(cfg1):
+def test_eg_exact_mode(small_cx):
     print("[gpu] eg exact", flush=True)
-    w, prob, kp, op = cfg1
+    w, prob, kp, op = small_cx
     st = states(8, 128, w["center"], w["size"], w["lig"], kp.KM, sigma=1.5)
     _eg_parity(kp, op, w, st, exact=True)
 
@@ -202,9 +202,9 @@ def _run_both(kp, op, st, init_opt, mc_steps, iters, seed=0, step_offset=0, f64=
     return out
 
 
-def test_bfgs_one_iteration(cfg1):
+def test_bfgs_one_iteration(small_cx):
     print("[gpu] bfgs 1 iteration", flush=True)
-    w, prob, kp, op = cfg1
+    w, prob, kp, op = small_cx
     st = states(9, 512, w["center"], w["size"], w["lig"], kp.KM, sigma=1.0)
     st[:, 7:] = 0
     out = _run_both(kp, op, st, True, 0, 1)
@@ -213,9 +213,9 @@ def test_bfgs_one_iteration(cfg1):
     assert ok.mean() >= 0.99, ok.mean()
 
 
-def test_bfgs_100_iterations(cfg1):
+def test_bfgs_100_iterations(small_cx):
     print("[gpu] bfgs 100 iterations", flush=True)
-    w, prob, kp, op = cfg1
+    w, prob, kp, op = small_cx
     st = states(10, 512, w["center"], w["size"], w["lig"], kp.KM, sigma=1.0)
     out = _run_both(kp, op, st, True, 0, 100)
     eg, ec = out["gpu"][3], out["cpu"][3]
@@ -224,11 +224,11 @@ def test_bfgs_100_iterations(cfg1):
     assert (np.abs(eg - ec) <= 1e-3 * np.maximum(np.abs(ec), 1)).mean() > 0.2
 
 
-def test_mc_rng_and_mutation_match(cfg1):
+def test_mc_rng_and_mutation_match(small_cx):
     """init_opt off, one MC step, one BFGS iteration: the move (entity, ball vector, torsion draw) comes from the
     counter-based RNG, so matching energies show the GPU draws the same numbers as the oracle."""
     print("[gpu] mc one step", flush=True)
-    w, prob, kp, op = cfg1
+    w, prob, kp, op = small_cx
     st = states(11, 512, w["center"], w["size"], w["lig"], kp.KM, sigma=1.0)
     out = _run_both(kp, op, st, False, 1, 1, seed=777, step_offset=5)
     xg, xc = out["gpu"][0], out["cpu"][0]
@@ -237,10 +237,10 @@ def test_mc_rng_and_mutation_match(cfg1):
     assert ok.mean() >= 0.97, ok.mean()
 
 
-def test_mc_config1_properties(cfg1):
-    """The config-1 workload itself: 8 chains x 200 MC steps, Vina-rule BFGS, seed 0."""
-    print("[gpu] mc config 1 (8 x 200)", flush=True)
-    w, prob, kp, op = cfg1
+def test_mc_small_properties(small_cx):
+    """The small workload itself: 8 chains x 200 MC steps, Vina-rule BFGS, seed 0."""
+    print("[gpu] mc small complex (8 x 200)", flush=True)
+    w, prob, kp, op = small_cx
     N = w["chains"]
     st = states(0, N, w["center"], w["size"], w["lig"], kp.KM)
     cl, ci = torch.zeros(N, dtype=torch.int32).cuda(), torch.arange(N, dtype=torch.int32).cuda()
@@ -267,10 +267,10 @@ def test_mc_config1_properties(cfg1):
     assert be.min() < 0.0
 
 
-def test_mc_best_score_distribution(cfg1):
+def test_mc_best_score_distribution(small_cx):
     """Same seeded starts, same seed: the per-chain best-score distributions of GPU and FP64 oracle agree."""
     print("[gpu] mc distribution (64 x 30)", flush=True)
-    w, prob, kp, op = cfg1
+    w, prob, kp, op = small_cx
     st = states(12, 64, w["center"], w["size"], w["lig"], kp.KM)
     out = _run_both(kp, op, st, True, 30, 0, seed=12345)
     eg, ec = out["gpu"][3], out["cpu"][3]

@@ -29,7 +29,7 @@ def _cfg(**kw):
 
 
 def _complexes():
-    w1 = synth.make_single(1, 0)
+    w1 = synth.make_single("small", 0)
     rec = workload.receptor_dict(w1["receptor"], w1["center"], w1["size"])
     preps = [w1["lig"], synth.make_ligand(31, 16, 3)]
     return rec, preps, np.asarray(w1["center"], float), float(np.max(w1["size"]))

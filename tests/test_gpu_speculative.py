@@ -29,8 +29,8 @@ def _run(kp, st, cl, steps, width, seed=77):
 
 @pytest.mark.parametrize("width", [2, 4])
 def test_single_complex_bit_identical(width):
-    print(f"[gpu] speculative line search width {width}, config-2 complex", flush=True)
-    w, prob = workload.single(2, seed=0, device="cuda")
+    print(f"[gpu] speculative line search width {width}, single complex", flush=True)
+    w, prob = workload.single("single", seed=0, device="cuda")
     kp = KernelProblem(prob)
     N = 48
     t, q, th = synth.start_states(3, N, w["center"], w["size"], w["lig"].n_tors, kp.KM)

@@ -302,15 +302,15 @@ def test_bfgs_converges_on_quadratic():
 
 
 @pytest.fixture(scope="module")
-def config1():
-    w = synth.make_single(1, 0)
+def small_cx():
+    w = synth.make_single("small", 0)
     rec = w["receptor"]
     prob, meta = cpu_problem(rec, [w["lig"]], w["center"], w["size"])
     return w, prob
 
 
-def test_mc_monotone_and_best_re_evaluates(config1):
-    w, prob = config1
+def test_mc_monotone_and_best_re_evaluates(small_cx):
+    w, prob = small_cx
     lig = w["lig"]
     N = 8
     st = states(0, N, w["center"], w["size"], lig, prob.KM)
@@ -325,10 +325,10 @@ def test_mc_monotone_and_best_re_evaluates(config1):
     assert (ev > 5).all()
 
 
-def test_mc_segments_equal_one_launch(config1):
+def test_mc_segments_equal_one_launch(small_cx):
     """Counter-based RNG: 2 segments of 3 steps (step_offset 0, 3) == one launch of 6 steps, bit for bit, in the
     float32 restatement (its state never leaves float32, so the segment boundary is lossless)."""
-    w, prob = config1
+    w, prob = small_cx
     N = 4
     st = states(1, N, w["center"], w["size"], w["lig"], prob.KM)
     outs = []
@@ -345,9 +345,9 @@ def test_mc_segments_equal_one_launch(config1):
         np.testing.assert_array_equal(a, b)
 
 
-def test_exact_mode_matches_direct_pairwise(config1):
+def test_exact_mode_matches_direct_pairwise(small_cx):
     """exact=1: E = sum over (ligand atom, receptor atom) pairs of the two-atom energy + penalty + intra."""
-    w, prob = config1
+    w, prob = small_cx
     N = 6
     st = states(2, N, w["center"], w["size"], w["lig"], prob.KM, sigma=1.0)
     e, g, X, _ = prob.energy_grad(np.zeros(N, np.int32), st, exact=True)
