@@ -5,6 +5,7 @@ multi():   many receptors (grid sets resident, admitted in groups under an HBM b
            kernel's completion stamps.
 grids():   both grid builders (Vina XS and the fitted function) on the single-complex receptor, with the in-cutoff
            work counted exactly, for the builders' FP32 VALU roofline.
+eg():      the energy + gradient kernel alone on many independent poses of the single complex (throughput bound).
 learned(): trains the proposal network on accepted moves logged by the plain method on training complexes, then
            compares plain and learned search on disjoint evaluation complexes at equal MC-step budgets.
 """
@@ -121,8 +122,8 @@ def _in_cutoff_pairs(xyz, origin, dims, spacing, cutoff, dev):
 
 def grids(seed=0, reps=10):
     """Both grid builders on the single-complex receptor (71^3 points). kernel_ms: the C entry point on device-resident
-    inputs between HIP events on its stream (one launch per 8 ligand types); call_ms: the Python facade, which also
-    uploads the receptor and tables on every call."""
+    inputs, `reps` calls back to back between HIP events on its stream (one launch per 8 ligand types); call_ms: the
+    Python facade, which also uploads the receptor and tables on every call."""
     from . import _lib
     from .config import grid_prm, sf_params
     from .fit.fdock import DATA, compact_ligands, load_fitted
@@ -166,19 +167,18 @@ def grids(seed=0, reps=10):
                        f"{nx}x{ny}x{nz} points", "in_cutoff_point_atom_pairs": pairs}
 
     def event_ms(fn):
+        """Mean time per call of `reps` back-to-back calls between one pair of HIP events (launch gaps amortised)."""
         fn()
         torch.cuda.synchronize()
-        ms = []
-        for _ in range(reps):
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record()
-            r = fn()
-            e1.record()
-            torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        st = [fn() for _ in range(reps)]
+        e1.record()
+        torch.cuda.synchronize()
+        for r in st:
             if isinstance(r, int):
                 _lib.check(r)
-            ms.append(e0.elapsed_time(e1))
-        return float(np.mean(ms))
+        return e0.elapsed_time(e1) / reps
 
     for name, T in (("vina", len(lt)), ("fitted", pot.TL)):
         kms, cms = event_ms(raw[name]), event_ms(facade[name])
@@ -189,6 +189,28 @@ def grids(seed=0, reps=10):
                      "point_atom_pairs_per_s": pairs / t, "triples_per_s": pairs * T / t,
                      "achieved_tflops": flops / t / 1e12, "ops_per_pair": per_pair, "ops_per_triple": per_triple}
     return out
+
+
+def eg(n_poses=65536, reps=10, seed=0):
+    """adock_eg on n_poses random poses of the single-complex workload (one launch per call, `reps` calls back to back
+    between HIP events on the launch stream)."""
+    dev = torch.device("cuda:0")
+    w, prob = workload.single("single", seed=seed, device=dev)
+    kp = KernelProblem(prob)
+    t, q, th = synth.start_states(seed + 5, n_poses, w["center"], w["size"], w["lig"].n_tors, kp.KM)
+    st = kp.pack(torch.as_tensor(t, device=dev), torch.as_tensor(q, device=dev), torch.as_tensor(th, device=dev))
+    cl = torch.zeros(n_poses, dtype=torch.int32, device=dev)
+    kp.energy_grad(cl, st)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        kp.energy_grad(cl, st)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    return {"workload": f"energy + gradient of {n_poses} independent poses, single complex (35-atom ligand, 454 pairs)",
+            "kernel_ms": ms, "evals_per_s": n_poses / (ms * 1e-3), "lig": w["lig"]}
 
 
 def _timed(fn):

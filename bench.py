@@ -69,7 +69,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--chains-per-block", type=int, default=0)
     ap.add_argument("--line-search-width", type=int, default=0, help="trials per speculative batch (0 = adaptive)")
-    ap.add_argument("--workload", choices=["single", "screen", "multi", "learned", "grids"], default="single",
+    ap.add_argument("--workload", choices=["single", "screen", "multi", "learned", "grids", "eg"], default="single",
                     help="single = the headline line (default); the others print their own JSON line")
     ap.add_argument("--small", action="store_true", help="scaled-down sizes for the opt-in workloads")
     ap.add_argument("--target", choices=["move", "jump"], default="move",
@@ -85,6 +85,12 @@ def main():
         from adock_amd import experiments as ex
         if args.workload == "screen":
             out = ex.screen(n_ligands=64 if args.small else 512, chains=64, steps=50 if args.small else 500)
+        elif args.workload == "eg":
+            out = ex.eg()
+            fl = flops_per_eval(out.pop("lig"))
+            tf = out["evals_per_s"] * fl / 1e12
+            out["roofline"] = {"bound": "valu", "achieved": tf, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                               "frac": tf / PEAK_FP32_TFLOPS, "flops_per_eval": fl}
         elif args.workload == "grids":
             out = ex.grids()
             for k in ("vina", "fitted"):
@@ -165,7 +171,7 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": measured_traffic(mc_kernel_name(kp.KM, N)),
                      "kernel": mc_kernel_name(kp.KM, N), "flops_per_eval": fl,
-                     "traffic_source": "profiles/mc_spec_kernel_config2_traffic.json (rocprofv3 FETCH_SIZE / WRITE_SIZE, "
+                     "traffic_source": "profiles/mc_spec_kernel_single_complex_traffic.json (rocprofv3 FETCH_SIZE / WRITE_SIZE, "
                                        "calibrated on a 1 GiB device copy)"},
     }
     if args.copy_calib_mib > 0:
@@ -183,7 +189,8 @@ def main():
 def measured_traffic(kernel):
     """HBM bytes per launch of this workload's search kernel from the committed counter summary, or None when the
     summary is absent or was measured on another kernel."""
-    f = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "mc_spec_kernel_config2_traffic.json")
+    f = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                     "mc_spec_kernel_single_complex_traffic.json")
     try:
         with open(f) as fh:
             d = json.load(fh)
@@ -200,7 +207,7 @@ def mc_kernel_name(KM, n_chains):
     if ns == 0:
         cus = torch.cuda.get_device_properties(0).multi_processor_count
         ns = 4 if n_chains <= cus else 2 if n_chains <= 2 * cus else 1
-    return f"mc_spec_kernel<{KM}, {ns}>" if ns > 1 else f"mc_kernel<{KM}>"
+    return f"mc_spec_kernel<{KM}, {ns}, false>" if ns > 1 else f"mc_kernel<{KM}, false>"
 
 
 def kp_chains_per_block():
