@@ -55,9 +55,12 @@ class BatchRunner:
         kp = self.kp
         L = _lib.lib()
         st = _lib.stream_of(kp.device)
+        kp.check_chain_buffers(self.N, cur, cur_e, best, best_e)
         _lib.check(L.adock_batch_reset(_lib.ptr(self.counter), _lib.ptr(self.done), self.L, _lib.ptr(self.times), st))
         ci = chain_id if chain_id.dtype == torch.int32 else chain_id.to(torch.int32)
         self._ci = ci.contiguous()
+        if self._ci.shape != (self.N,):
+            raise _lib.AdockError("chain_id must hold one entry per chain")
         args = (kp._args(False) + [_lib.ptr(self.chain_lig), _lib.ptr(self._ci), self.N, None, None, 0,
                                    _lib.ptr(cur), _lib.ptr(cur_e), _lib.ptr(best), _lib.ptr(best_e), int(bool(init_opt)),
                                    int(mc_steps), int(bfgs_iters), float(temp), float(amp_pos), float(amp_rot),

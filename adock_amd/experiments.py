@@ -5,6 +5,7 @@ multi():   many receptors (grid sets resident, admitted in groups under an HBM b
            kernel's completion stamps.
 grids():   both grid builders (Vina XS and the fitted function) on the single-complex receptor, with the in-cutoff
            work counted exactly, for the builders' FP32 VALU roofline.
+wide():    the single complex widened to 4,096 chains at constant chain-steps (the reference's min_total_chains rule).
 eg():      the energy + gradient kernel alone on many independent poses of the single complex (throughput bound).
 learned(): trains the proposal network on accepted moves logged by the plain method on training complexes, then
            compares plain and learned search on disjoint evaluation complexes at equal MC-step budgets.
@@ -189,6 +190,37 @@ def grids(seed=0, reps=10):
                      "point_atom_pairs_per_s": pairs / t, "triples_per_s": pairs * T / t,
                      "achieved_tflops": flops / t / 1e12, "ops_per_pair": per_pair, "ops_per_triple": per_triple}
     return out
+
+
+def wide(chains=4096, base_chains=256, base_steps=1000, reps=3, seed=0):
+    """The single complex with the chains widened by powers of two at constant chain-steps, as the search driver does
+    for small batches (src/adock/search.py:563-569): 256 x 1,000 -> 4,096 chains x 62 steps, Vina-rule BFGS. One
+    docked ligand = one launch over all chains; also the best score against the 256 x 1,000 run's."""
+    dev = torch.device("cuda:0")
+    w, prob = workload.single("single", seed=seed, device=dev)
+    kp = KernelProblem(prob)
+    f = chains // base_chains
+    steps = base_steps // f
+    t, q, th = synth.start_states(1, chains, w["center"], w["size"], w["lig"].n_tors, kp.KM)
+    st = kp.pack(torch.as_tensor(t, device=dev), torch.as_tensor(q, device=dev), torch.as_tensor(th, device=dev))
+    cl = torch.zeros(chains, dtype=torch.int32, device=dev)
+    ci = torch.arange(chains, dtype=torch.int32, device=dev)
+    seed64 = seed * 1_000_003 + 12345
+    times, best = [], None
+    for r in range(reps + 1):
+        cur, ce, b, be = _bufs(st)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        kp.run_mc(cl, ci, cur, ce, b, be, True, steps, 0, 1.2, 2.0, 2.0, seed64, 0)
+        e1.record()
+        torch.cuda.synchronize()
+        if r:
+            times.append(e0.elapsed_time(e1) * 1e-3)
+        best = float(be.min())
+    dt = float(np.mean(times))
+    return {"workload": f"single complex widened: {chains} chains x {steps} MC steps (= {base_chains} x {base_steps} "
+                        "chain-steps), Vina-rule BFGS", "docked_ligands_per_s": 1.0 / dt, "seconds": dt,
+            "best_score": best}
 
 
 def eg(n_poses=65536, reps=10, seed=0):

@@ -97,6 +97,8 @@ class KernelProblem:
         if exact and self.RN_MAX <= 0:
             raise _lib.AdockError("exact mode needs receptor atoms (Problem.rec)")
         N = state.shape[0]
+        if state.dim() != 2 or state.shape[1] != self.SD or chain_lig.shape != (N,):
+            raise _lib.AdockError(f"state must be [N, {self.SD}] and chain_lig [N]")
         e = torch.zeros(N, device=self.device)
         g = torch.zeros(N, self.DM, device=self.device)
         cl = chain_lig.to(device=self.device, dtype=torch.int32).contiguous()
@@ -159,6 +161,17 @@ class KernelProblem:
             nbr_n[idx] = near.sum(1).to(torch.int32)
         return nbr, nbr_n
 
+    def check_chain_buffers(self, N, cur, cur_e, best, best_e):
+        """cur / best [N, SD] and cur_e / best_e [N]: contiguous float32 tensors on this problem's device (the kernels
+        read and write exactly these extents)."""
+        for x in (cur, cur_e, best, best_e):
+            if not (x.is_contiguous() and x.dtype == torch.float32 and x.device == self.device):
+                raise _lib.AdockError("cur/cur_e/best/best_e must be contiguous float32 tensors on the problem's device")
+        if cur.shape != (N, self.SD) or best.shape != (N, self.SD):
+            raise _lib.AdockError(f"cur and best must be [{N}, {self.SD}] (chains x 7 + KM)")
+        if cur_e.shape != (N,) or best_e.shape != (N,):
+            raise _lib.AdockError(f"cur_e and best_e must be [{N}]")
+
     def run_mc(self, chain_lig, chain_id, cur, cur_e, best, best_e, init_opt, mc_steps, bfgs_iters, temp,
                amp_pos, amp_rot, seed, step_offset, exact=False, nbr=None, nbr_n=None, count_evals=False):
         """In place on cur/cur_e/best/best_e (float32 contiguous device tensors). exact=True with nbr [N, NB] /
@@ -168,14 +181,16 @@ class KernelProblem:
             return
         if exact and self.RN_MAX <= 0:
             raise _lib.AdockError("exact mode needs receptor atoms (Problem.rec)")
-        for x in (cur, cur_e, best, best_e):
-            if not (x.is_contiguous() and x.dtype == torch.float32 and x.is_cuda):
-                raise _lib.AdockError("cur/cur_e/best/best_e must be contiguous float32 device tensors")
+        self.check_chain_buffers(N, cur, cur_e, best, best_e)
         cl = chain_lig.to(device=self.device, dtype=torch.int32).contiguous()
         ci = chain_id.to(device=self.device, dtype=torch.int32).contiguous()
+        if cl.shape != (N,) or ci.shape != (N,):
+            raise _lib.AdockError("chain_lig and chain_id must hold one entry per chain")
         if nbr is not None:
             nbr = nbr.to(device=self.device, dtype=torch.int32).contiguous()
             nbr_n = nbr_n.to(device=self.device, dtype=torch.int32).contiguous()
+            if nbr.dim() != 2 or nbr.shape[0] != N or nbr_n.shape != (N,):
+                raise _lib.AdockError("nbr must be [N, NB] and nbr_n [N]")
             nb_args = [_lib.ptr(nbr), _lib.ptr(nbr_n), int(nbr.shape[1])]
         else:
             nb_args = [None, None, 0]

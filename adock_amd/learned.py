@@ -40,6 +40,7 @@ class StepSearch:
     def _step(self, cur, cur_e, best, best_e, init, step, bfgs_iters, temp, amp_pos, amp_rot, seed, use_net, p_plain,
               sig, log, feat_on, count_evals):
         kp = self.kp
+        kp.check_chain_buffers(self.N, cur, cur_e, best, best_e)
         if use_net and not init:
             self.net.infer(self.feat, self.Y)
         args = (kp._args(False)[:-1] +

@@ -58,3 +58,21 @@ def test_bucket_validation_without_gpu():
     host = ctypes.cast(buf, ctypes.c_void_p).value       # never dereferenced: the call fails its checks first
     bad_rt = [None, None, 0, None, host, 1, None, 300] + fit[8:15] + [host, host, None]
     assert L.adock_build_grid_fit(*bad_rt) == -4
+
+
+def test_facade_rejects_mismatched_chain_buffers():
+    """KernelProblem.run_mc / BatchRunner check every buffer's extent before anything reaches the kernels: a state
+    buffer passed as an energy vector (or the reverse) is an AdockError, not an out-of-bounds write."""
+    import numpy as np
+    import torch
+    from adock_amd import _lib
+    from adock_amd.kernel import KernelProblem
+    kp = KernelProblem.__new__(KernelProblem)
+    kp.SD, kp.device = 15, torch.device("cpu")
+    N = 4
+    st, e = torch.zeros(N, 15), torch.zeros(N)
+    kp.check_chain_buffers(N, st, e, st.clone(), e.clone())
+    for bad in ((st, st.clone(), e, e.clone()), (st, e, e.clone(), e.clone()), (st[:, :8].contiguous(), e, st, e),
+                (st, e.double(), st, e), (st, e, st, torch.zeros(N + 1))):
+        with pytest.raises(_lib.AdockError):
+            kp.check_chain_buffers(N, *bad)
