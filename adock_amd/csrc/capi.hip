@@ -79,7 +79,7 @@ static int common_checks(int AM, int KM, int P_MAX, const void* ref, const void*
     if (DEG_MAX < 0 || DEG_MAX > 64 || (DEG_MAX & 7))
         return fail(ADOCK_E_ARG, "DEG_MAX must be a multiple of 8 in 0..64");
     if (n_chains < 0) return fail(ADOCK_E_ARG, "negative n_chains");
-    if (!ref || !inc || !ist || !grid || !prm || !chain_lig) return fail(ADOCK_E_ARG, "null pointer");
+    if (!ref || !inc || !ist || !grid || !prm || (n_chains > 0 && !chain_lig)) return fail(ADOCK_E_ARG, "null pointer");
     if (exact && (!rxyz || !rn || RN_MAX <= 0)) return fail(ADOCK_E_ARG, "exact mode needs receptor atoms");
     return 0;
 }

@@ -1,0 +1,102 @@
+"""Edge cases on the GPU: grids of an empty receptor (both builders), the larger torsion buckets (KM = 24, 32) against
+the FP64 oracle, MC invariants on a 31-torsion ligand, the speculative search bit-identical where the LDS budget
+lowers its width (64 atoms, 1,761 pairs), and an empty chain list."""
+import numpy as np
+import pytest
+import torch
+
+from adock_amd import _lib, synth, workload
+from adock_amd.config import sf_params
+from adock_amd.fit.gridfit import build_grid_fit
+from adock_amd.fit.terms import FittedPotential, Spec, default_weights
+from adock_amd.grid import build_grid
+from adock_amd.kernel import KernelProblem
+from helpers import CFG, EG_FLOOR, EG_REL, SPACING, rel_err, states, vec_rel
+from test_gpu_parity import MARGIN, _oracle_of
+
+pytestmark = pytest.mark.gpu
+
+
+def test_empty_receptor_grids_are_zero():
+    print("[gpu] empty receptor grids", flush=True)
+    sfp = sf_params(CFG, "vina")
+    g = build_grid(np.zeros((0, 3), np.float32), np.zeros(0, np.int32), sfp, np.zeros(3), SPACING, (9, 7, 5),
+                   lig_types=np.arange(10))
+    assert g.shape == (10, 9, 7, 5) and float(g.abs().max()) == 0.0
+    spec = Spec(CFG, "vina")
+    pot = FittedPotential(spec, default_weights(spec), np.arange(11), "cuda")
+    gf = build_grid_fit(np.zeros((0, 3), np.float32), np.zeros(0, np.int32), pot, np.zeros(3), SPACING, (9, 7, 5))
+    assert gf.shape == (11, 9, 7, 5) and float(gf.abs().max()) == 0.0
+
+
+@pytest.mark.parametrize("n_heavy,n_tors,KM", [(50, 20, 24), (64, 31, 32)])
+def test_eg_large_torsion_buckets(n_heavy, n_tors, KM):
+    print(f"[gpu] eg {n_heavy} atoms / {n_tors} torsions (KM {KM})", flush=True)
+    rec = workload.receptor_dict(synth.make_receptor(3, 1500, 20.0, 6.0), np.zeros(3), 24.0)
+    lig = synth.make_ligand(101, n_heavy, n_tors)
+    assert lig.n_tors == n_tors
+    prob = workload.gpu_problem([rec], [lig], [0])
+    kp, op = KernelProblem(prob), _oracle_of(prob)
+    assert kp.KM == KM
+    N = 64
+    st = states(77, N, np.zeros(3), 20.0, lig, kp.KM, sigma=2.0)
+    cl = np.zeros(N, np.int32)
+    e, g = kp.energy_grad(torch.as_tensor(cl).cuda(), torch.as_tensor(st).cuda())
+    e, g = e.cpu().numpy(), g.cpu().numpy()
+    eo, go, _, m = op.energy_grad(cl, st)
+    keep = m >= MARGIN
+    assert keep.sum() > N // 2
+    D = 6 + n_tors
+    assert rel_err(e[keep], eo[keep], EG_FLOOR, EG_REL) < EG_REL
+    assert vec_rel(g[keep, :D], go[keep, :D], EG_FLOOR) < EG_REL
+
+
+def _mc(kp, st, cl, steps, width):
+    L = _lib.lib()
+    _lib.check(L.adock_set_line_search_width(width))
+    try:
+        N = len(cl)
+        cur, best = st.clone(), st.clone()
+        ce, be = torch.zeros(N).cuda(), torch.full((N,), float("inf")).cuda()
+        kp.eval_count.zero_()
+        kp.run_mc(cl, torch.arange(N, dtype=torch.int32).cuda(), cur, ce, best, be, True, steps, 0, 1.2, 2.0, 2.0, 5,
+                  0, count_evals=True)
+        torch.cuda.synchronize()
+        return [x.cpu().numpy() for x in (cur, ce, best, be)] + [int(kp.eval_count.item())]
+    finally:
+        _lib.check(L.adock_set_line_search_width(0))
+
+
+def test_mc_31_torsions_widths_and_invariants():
+    """64 heavy atoms, 31 torsions (KM 32), 1,761 pairs: the dense layout's LDS lowers the speculative width; every
+    width gives the same chains bit for bit, best <= every start energy, and best_e re-evaluates to itself."""
+    print("[gpu] mc 64 atoms / 31 torsions", flush=True)
+    rec = workload.receptor_dict(synth.make_receptor(3, 1500, 20.0, 6.0), np.zeros(3), 24.0)
+    lig = synth.make_ligand(101, 64, 31)
+    prob = workload.gpu_problem([rec], [lig], [0])
+    kp = KernelProblem(prob)
+    N = 16
+    st = torch.as_tensor(states(5, N, np.zeros(3), 20.0, lig, kp.KM, sigma=2.0)).cuda()
+    cl = torch.zeros(N, dtype=torch.int32).cuda()
+    e0 = kp.energy_grad(cl, st)[0].cpu().numpy()
+    ref = _mc(kp, st, cl, 4, 1)
+    for width in (2, 4):
+        got = _mc(kp, st, cl, 4, width)
+        assert got[4] == ref[4]
+        for a, b in zip(ref[:4], got[:4]):
+            np.testing.assert_array_equal(a, b)
+    best, best_e = ref[2], ref[3]
+    assert np.all(best_e <= e0 + 1e-4)
+    eb = kp.energy_grad(cl, torch.as_tensor(best).cuda())[0].cpu().numpy()
+    np.testing.assert_allclose(eb, best_e, rtol=1e-5, atol=1e-4)
+
+
+def test_empty_chain_list_is_a_no_op():
+    w, prob = workload.single("small", seed=0)
+    kp = KernelProblem(prob)
+    z = torch.zeros(0, kp.SD, device="cuda")
+    ze = torch.zeros(0, device="cuda")
+    zi = torch.zeros(0, dtype=torch.int32, device="cuda")
+    kp.run_mc(zi, zi, z, ze, z.clone(), ze.clone(), True, 5, 0, 1.2, 2.0, 2.0, 0, 0)
+    e, g = kp.energy_grad(zi, z)
+    assert e.shape == (0,) and g.shape == (0, kp.DM)
