@@ -5,6 +5,8 @@ multi():   many receptors (grid sets resident, admitted in groups under an HBM b
            kernel's completion stamps.
 grids():   both grid builders (Vina XS and the fitted function) on the single-complex receptor, with the in-cutoff
            work counted exactly, for the builders' FP32 VALU roofline.
+driver():  the full search driver (Stage A, MC segments, refine, exact polish, finalize) with a search preset on a
+           one-pocket screen, grid build included, per docked ligand (Vina XS, or the fitted function's path).
 wide():    the single complex widened to 4,096 chains at constant chain-steps (the reference's min_total_chains rule).
 eg():      the energy + gradient kernel alone on many independent poses of the single complex (throughput bound).
 learned(): trains the proposal network on accepted moves logged by the plain method on training complexes, then
@@ -190,6 +192,45 @@ def grids(seed=0, reps=10):
                      "point_atom_pairs_per_s": pairs / t, "triples_per_s": pairs * T / t,
                      "achieved_tflops": flops / t / 1e12, "ops_per_pair": per_pair, "ops_per_triple": per_triple}
     return out
+
+
+def driver(n_ligands=256, mode="v2_default", fitted=False, seed=0, warmup=True):
+    """Seconds per docked ligand through search.dock_problem_kernel (or fit.fdock.dock_fitted) on n_ligands synthetic
+    ligands (10-60 heavy atoms, 0-15 torsions) in the one receptor of the screening batch, grid build included;
+    ligand preparation is not part of it (the ligands are generated already prepared)."""
+    from .search import dock_problem_kernel
+    dev = torch.device("cuda:0")
+    w = synth.make_screen(seed=seed, n_ligands=n_ligands)
+    L = len(w["ligs"])
+    rec = workload.receptor_dict(w["receptor"], w["center"], w["size"])
+    c, s = np.asarray(w["center"], float), np.broadcast_to(np.asarray(w["size"], float), (3,))
+    cfg = workload.CFG
+    if fitted:
+        from .fit.fdock import DATA, dock_fitted, load_fitted
+        spec, wts = load_fitted(DATA / "adock_fitted.json", cfg)
+        cxs = [synth.fitted_complex(rec, lg, c, s, spec, seed=i) for i, lg in enumerate(w["ligs"])]
+
+        def run(n):
+            return dock_fitted(cxs[:n], spec, wts, mode=mode, seed=seed, cfg=cfg, stats=stats)
+    else:
+        def run(n):
+            prob = workload.gpu_problem([rec], w["ligs"][:n], [0] * n, device=dev)
+            return dock_problem_kernel(prob, [c] * n, [s] * n, cfg["search"][mode], seed=seed, stats=stats)
+    stats = {}
+    if warmup:
+        run(min(4, L))
+    stats = {}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = run(L)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    best = np.array([r["total"][0] for r in res])
+    return {"workload": f"search driver, preset {mode}{' (fitted function)' if fitted else ''}: {L} ligands (10-60 heavy "
+                        "atoms, 0-15 torsions) in one pocket, grid build + Stage A + MC + refine + polish + finalize",
+            "seconds": dt, "seconds_per_docked_ligand": dt / L, "docked_ligands_per_s": L / dt,
+            "phase_seconds": {k: v for k, v in stats.items() if k.startswith("t_")}, "widen": stats.get("widen"),
+            "best_total_mean": float(best.mean())}
 
 
 def wide(chains=4096, base_chains=256, base_steps=1000, reps=3, seed=0):

@@ -87,10 +87,17 @@ def finalize_kernel(prob, kp, states, chain_lig, L, P, cfg):
             for l in range(L)]
 
 
-def _rmsd(X, heavy):
-    """Heavy-atom RMSD between every two poses of each complex: X [L, P, A, 3] -> [L, P, P]."""
-    hv = heavy[:, None, None, :].to(X.dtype)
-    return ((((X[:, :, None] - X[:, None, :]) ** 2).sum(-1) * hv).sum(-1) / hv.sum(-1).clamp(min=1)).sqrt()
+def _rmsd(X, heavy, chunk=50_000_000):
+    """Heavy-atom RMSD between every two poses of each complex: X [L, P, A, 3] -> [L, P, P], over groups of complexes
+    so the [l, P, P, A] difference stays near `chunk` elements."""
+    L, P, A = X.shape[:3]
+    out = torch.empty(L, P, P, device=X.device, dtype=X.dtype)
+    step = max(1, chunk // max(1, P * P * A))
+    for l0 in range(0, L, step):
+        Xs, hv = X[l0:l0 + step], heavy[l0:l0 + step, None, None, :].to(X.dtype)
+        d2 = ((Xs[:, :, None] - Xs[:, None, :]) ** 2).sum(-1)
+        out[l0:l0 + step] = ((d2 * hv).sum(-1) / hv.sum(-1).clamp(min=1)).sqrt()
+    return out
 
 
 def _cluster(X, E, inter, intra, D, n_tors_vina, rot_weight, cut, n_out):

@@ -69,7 +69,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--chains-per-block", type=int, default=0)
     ap.add_argument("--line-search-width", type=int, default=0, help="trials per speculative batch (0 = adaptive)")
-    ap.add_argument("--workload", choices=["single", "screen", "multi", "learned", "grids", "eg", "wide"], default="single",
+    ap.add_argument("--workload", choices=["single", "screen", "multi", "learned", "grids", "eg", "wide", "driver", "driver_fitted"], default="single",
                     help="single = the headline line (default); the others print their own JSON line")
     ap.add_argument("--small", action="store_true", help="scaled-down sizes for the opt-in workloads")
     ap.add_argument("--target", choices=["move", "jump"], default="move",
@@ -85,6 +85,8 @@ def main():
         from adock_amd import experiments as ex
         if args.workload == "screen":
             out = ex.screen(n_ligands=64 if args.small else 512, chains=64, steps=50 if args.small else 500)
+        elif args.workload in ("driver", "driver_fitted"):
+            out = ex.driver(n_ligands=32 if args.small else 256, fitted=args.workload == "driver_fitted")
         elif args.workload == "wide":
             out = ex.wide()
         elif args.workload == "eg":
