@@ -58,8 +58,10 @@ def lib():
         L.adock_stage_a.argtypes = [I, P, P, P, I, I, P, I, P, P, I] + [P] * 7 + [F, F, F, P, P, P]
         L.adock_set_line_search_width.argtypes = [I]
         L.adock_get_line_search_width.restype = I
+        L.adock_set_exact_staging.argtypes = [I]
+        L.adock_get_exact_staging.restype = I
         for name in ("adock_build_grid", "adock_build_grid_fit", "adock_eg", "adock_mc", "adock_mc_counted", "adock_mc_batch",
-                     "adock_batch_reset", "adock_set_chains_per_block", "adock_set_line_search_width", "adock_mc_step",
+                     "adock_batch_reset", "adock_set_chains_per_block", "adock_set_line_search_width", "adock_set_exact_staging", "adock_mc_step",
                      "adock_mlp_infer", "adock_gemm_f32", "adock_stage_a",
                      "adock_bias_act", "adock_relu_bwd", "adock_colsum", "adock_mse", "adock_adam", "adock_to_bf16"):
             getattr(L, name).restype = I

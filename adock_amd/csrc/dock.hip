@@ -78,6 +78,9 @@ struct WaveLDS {
     float4* FD;           // dense variant
     unsigned* pdst;
     int P128, trash;
+    float4* RN4;          // exact mode: staged receptor atoms (xyz, radius) of the chain's list, RNF their type flags
+    int* RNF;
+    int nrs;              // staging capacity (0: none)
 };
 
 struct ChainCtx {
@@ -98,6 +101,7 @@ struct ChainCtx {
     float o0, o1, o2, lo0, lo1, lo2, hi0, hi1, hi2, inv_s, pen, fmx0, fmx1, fmx2;
     int exact;
     const float* rrx; const int* rrt; int rn; const int* nb; int nbn;
+    bool staged;          // exact mode: the receptor atoms of the loop sit in LDS (w.RN4 / w.RNF)
     const float* sfrad; const int* sfhyd; const int* sfdon; const int* sfacc;
 };
 
@@ -113,6 +117,9 @@ DEVI WaveLDS carve_lds(char* base, int P_MAX, int DEG_MAX = 0) {
     w.TU = f; f += KM + 1;
     w.TM = f; f += KM + 1;
     w.FS = w.FD = nullptr;
+    w.RN4 = nullptr;
+    w.RNF = nullptr;
+    w.nrs = 0;
     w.inc = nullptr;
     w.pdst = nullptr;
     w.trash = AMK * DEG_MAX;
@@ -123,6 +130,15 @@ DEVI WaveLDS carve_lds(char* base, int P_MAX, int DEG_MAX = 0) {
     if constexpr (DENSE) w.pdst = reinterpret_cast<unsigned*>(w.pl + w.P128);
     else w.inc = w.pl + w.P128;
     return w;
+}
+
+// exact-mode staging area of a wave: nrs float4 atoms + nrs int flags, after the wave's regular region
+__host__ __device__ inline int stage_bytes(int nrs) { return 20 * nrs; }
+DEVI void attach_stage(WaveLDS& w, char* base, int nrs) {
+    if (nrs <= 0) return;
+    w.RN4 = reinterpret_cast<float4*>(base);
+    w.RNF = reinterpret_cast<int*>(w.RN4 + nrs);
+    w.nrs = nrs;
 }
 
 template <int KM, bool DENSE = false, bool EX = false>
@@ -222,7 +238,7 @@ DEVI ChainCtx setup_chain(int l, int lane, const LigArgs& L, const GridArgs& G, 
         c.D8 = (int)wave_max_f((float)c.ic8);
     }
     c.exact = EX ? 1 : 0;
-    c.nb = nullptr; c.nbn = 0;
+    c.nb = nullptr; c.nbn = 0; c.staged = false;
     if constexpr (EX) {
         c.rrx = R.rxyz + (long long)ri * R.RN_MAX * 3;
         c.rrt = R.rtyp + (long long)ri * R.RN_MAX;
@@ -413,6 +429,20 @@ DEVI float eval_eg(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, int l
         } else {
             const float cut2 = sf.cut * sf.cut;
             const int nloop = c.nb ? c.nbn : c.rn;
+            if (c.staged) {
+                // same terms, in the same order, from the atoms staged in LDS (broadcast reads)
+                for (int bb = 0; bb < nloop; ++bb) {
+                    const float4 a = w.RN4[bb];
+                    const float rx = X0 - a.x, ry = X1 - a.y, rz = X2 - a.z;
+                    const float r2 = rx * rx + ry * ry + rz * rz;
+                    if (r2 >= cut2) continue;
+                    const float r = fast_sqrt(r2);
+                    float de;
+                    el += pair_e(r - c.rad - a.w, pair_flags(c.flg, w.RNF[bb]), sf, de);
+                    const float sc = de * fast_rcp(fmaxf(r, 1e-6f));
+                    gx += sc * rx; gy += sc * ry; gz += sc * rz;
+                }
+            } else
             for (int bb = 0; bb < nloop; ++bb) {
                 const int b = c.nb ? c.nb[bb] : bb;
                 const float rx = X0 - c.rrx[3 * b], ry = X1 - c.rrx[3 * b + 1], rz = X2 - c.rrx[3 * b + 2];
@@ -685,7 +715,24 @@ DEVI int run_chain(int ch, int lane, const WaveLDS& w, const LigArgs& L, const G
     const SFConst sf = make_sf(S.prm);
     const int l = __builtin_amdgcn_readfirstlane(chain_lig[ch]);
     ChainCtx c = setup_chain<KM, DENSE, EX>(l, lane, L, G, S, R, w);
-    if (M.nbr) { c.nb = M.nbr + (long long)ch * M.NB_MAX; c.nbn = M.nbr_n[ch]; }
+    if (M.nbr) {
+        c.nb = M.nbr + (long long)ch * M.NB_MAX;
+        c.nbn = min(max(__builtin_amdgcn_readfirstlane(M.nbr_n[ch]), 0), M.NB_MAX);
+    }
+    if constexpr (EX) {
+        // the chain's receptor atoms (neighbour list or whole receptor) into LDS once, when they fit the staging area
+        const int nloop = c.nb ? c.nbn : c.rn;
+        c.staged = w.nrs > 0 && nloop <= w.nrs;
+        if (c.staged) {
+            for (int bb = lane; bb < nloop; bb += WAVE) {
+                const int b = c.nb ? c.nb[bb] : bb;
+                const int t = c.rrt[b];
+                w.RN4[bb] = make_float4(c.rrx[3 * b], c.rrx[3 * b + 1], c.rrx[3 * b + 2], c.sfrad[t]);
+                w.RNF[bb] = c.sfhyd[t] | (c.sfdon[t] << 1) | (c.sfacc[t] << 2);
+            }
+            wave_lds_sync();
+        }
+    }
     const int cid = __builtin_amdgcn_readfirstlane(M.chain_id[ch]);
     const int S_ = 7 + c.nt;
     const int iters = M.bfgs_iters > 0 ? M.bfgs_iters : (25 + c.na) / 3;
@@ -739,7 +786,9 @@ __global__ __launch_bounds__(256) void mc_kernel(LigArgs L, GridArgs G, SFArgs S
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
     const int ch = blockIdx.x * (blockDim.x / WAVE) + wid;
     if (ch >= n_chains) return;
-    const WaveLDS w = carve_lds<KM>(lds + wid * wave_lds_bytes(KM, L.P_MAX), L.P_MAX);
+    const int wl = wave_lds_bytes(KM, L.P_MAX), wb = wl + stage_bytes(M.NBS);
+    WaveLDS w = carve_lds<KM>(lds + wid * wb, L.P_MAX);
+    attach_stage(w, lds + wid * wb + wl, M.NBS);
     run_chain<KM, 1, false, EX>(ch, lane, w, L, G, S, R, chain_lig, M);
 }
 
@@ -752,8 +801,9 @@ __global__ __launch_bounds__(256) void mc_spec_kernel(LigArgs L, GridArgs G, SFA
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
     const int ch = blockIdx.x;
     if (ch >= n_chains) return;
-    const int wb = wave_lds_bytes_dense(KM, L.P_MAX, L.DEG_MAX);
-    const WaveLDS w = carve_lds<KM, true>(lds + wid * wb, L.P_MAX, L.DEG_MAX);
+    const int wl = wave_lds_bytes_dense(KM, L.P_MAX, L.DEG_MAX), wb = wl + stage_bytes(M.NBS);
+    WaveLDS w = carve_lds<KM, true>(lds + wid * wb, L.P_MAX, L.DEG_MAX);
+    attach_stage(w, lds + wid * wb + wl, M.NBS);
     run_chain<KM, NS, true, EX>(ch, lane, w, L, G, S, R, chain_lig, M, wid, carve_spec(lds + NS * wb));
 }
 
@@ -828,19 +878,28 @@ inline int auto_waves(int n_chains) {
 inline int waves_for(int n_chains) { return g_waves_per_block > 0 ? g_waves_per_block : auto_waves(n_chains); }
 
 // chains per workgroup actually launched: the requested count, lowered until the workgroup's LDS fits
-inline int fit_waves(int W, int KM, int P_MAX) {
-    while (W > 1 && (size_t)W * wave_lds_bytes(KM, P_MAX) > (size_t)LDS_PER_WORKGROUP) --W;
+inline int fit_waves(int W, int KM, int P_MAX, int extra = 0) {
+    while (W > 1 && (size_t)W * (wave_lds_bytes(KM, P_MAX) + extra) > (size_t)LDS_PER_WORKGROUP) --W;
     return W;
+}
+
+// exact mode: receptor atoms staged per wave: the longest neighbour list (or the whole receptor), padded to 64, at
+// most STAGE_MAX; a chain whose list is longer than the staging area reads its atoms from memory
+constexpr int STAGE_MAX = 1024;
+int g_exact_staging = 1;
+inline int exact_stage(const McArgs& M, const RecArgs& R) {
+    const int n = ((M.nbr ? M.NB_MAX : R.RN_MAX) + WAVE - 1) & ~(WAVE - 1);
+    return g_exact_staging && n > 0 ? min(n, STAGE_MAX) : 0;
 }
 
 int g_line_search_width = 0;   // 0: adaptive (line_search_width)
 
 // waves per chain of the speculative line search: 4 while the chains fit one per CU, 2 up to two per CU, else 1;
 // lowered until the workgroup's LDS fits
-inline int line_search_width(int n_chains, int KM, int P_MAX, int DEG_MAX) {
+inline int line_search_width(int n_chains, int KM, int P_MAX, int DEG_MAX, int extra = 0) {
     int ns = g_line_search_width;
     if (ns <= 0) ns = n_chains <= cu_count() ? 4 : n_chains <= 2 * cu_count() ? 2 : 1;
-    while (ns > 1 && (size_t)ns * wave_lds_bytes_dense(KM, P_MAX, DEG_MAX) + SPEC_LDS_BYTES >
+    while (ns > 1 && (size_t)ns * (wave_lds_bytes_dense(KM, P_MAX, DEG_MAX) + extra) + SPEC_LDS_BYTES >
                          (size_t)LDS_PER_WORKGROUP)
         ns >>= 1;
     return ns;
@@ -861,10 +920,13 @@ inline void allow_lds(const void* fn, size_t bytes) {
 
 template <int KM, bool EX>
 int launch_mc_tx(const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecArgs& R, const int* chain_lig,
-                 int n_chains, const McArgs& M, hipStream_t stream) {
-    const int ns = line_search_width(n_chains, KM, L.P_MAX, L.DEG_MAX);
+                 int n_chains, const McArgs& M0, hipStream_t stream) {
+    McArgs M = M0;
+    M.NBS = EX ? exact_stage(M, R) : 0;
+    const int extra = stage_bytes(M.NBS);
+    const int ns = line_search_width(n_chains, KM, L.P_MAX, L.DEG_MAX, extra);
     if (ns > 1) {
-        const size_t shm = (size_t)ns * wave_lds_bytes_dense(KM, L.P_MAX, L.DEG_MAX) + SPEC_LDS_BYTES;
+        const size_t shm = (size_t)ns * (wave_lds_bytes_dense(KM, L.P_MAX, L.DEG_MAX) + extra) + SPEC_LDS_BYTES;
         if (ns == 4) {
             allow_lds(reinterpret_cast<const void*>(mc_spec_kernel<KM, 4, EX>), shm);
             hipLaunchKernelGGL((mc_spec_kernel<KM, 4, EX>), dim3(n_chains), dim3(4 * WAVE), shm, stream, L, G, S, R,
@@ -876,8 +938,8 @@ int launch_mc_tx(const LigArgs& L, const GridArgs& G, const SFArgs& S, const Rec
         }
         return (int)hipGetLastError();
     }
-    const int W = fit_waves(waves_for(n_chains), KM, L.P_MAX);
-    const size_t shm = (size_t)W * wave_lds_bytes(KM, L.P_MAX);
+    const int W = fit_waves(waves_for(n_chains), KM, L.P_MAX, extra);
+    const size_t shm = (size_t)W * (wave_lds_bytes(KM, L.P_MAX) + extra);
     const unsigned nblk = (unsigned)((n_chains + W - 1) / W);
     allow_lds(reinterpret_cast<const void*>(mc_kernel<KM, EX>), shm);
     hipLaunchKernelGGL((mc_kernel<KM, EX>), dim3(nblk), dim3(W * WAVE), shm, stream, L, G, S, R, chain_lig, n_chains,
@@ -1179,6 +1241,12 @@ int set_line_search_width(int ns) {
     return 0;
 }
 int get_line_search_width() { return g_line_search_width; }
+int set_exact_staging(int on) {
+    if (on != 0 && on != 1) return -1;
+    g_exact_staging = on;
+    return 0;
+}
+int get_exact_staging() { return g_exact_staging; }
 int lds_budget_ok(int KM, int P_MAX) { return wave_lds_bytes(KM, P_MAX) <= LDS_PER_WORKGROUP; }
 
 int launch_mc(int KM, const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecArgs& R, int exact,

@@ -19,6 +19,7 @@ struct McArgs {
     float* cur; float* cur_e; float* best; float* best_e;
     int init_opt, mc_steps, bfgs_iters; float temp, amp_pos, amp_rot; unsigned long long seed; int step_offset;
     unsigned long long* eval_count;
+    int NBS = 0;   // exact mode: receptor atoms staged in each wave's LDS (set by the launcher; 0 = read from memory)
 };
 // work queue of the persistent chain kernel: order [n] (chain indices, may be NULL = identity), counter (1 int, 0 at
 // launch), done [L] (finished chains per complex, 0 at launch), lig_chains [L] (chains per complex), times [1 + L]
@@ -65,5 +66,7 @@ int set_waves_per_block(int w);
 int get_waves_per_block();
 int set_line_search_width(int ns);
 int get_line_search_width();
+int set_exact_staging(int on);
+int get_exact_staging();
 int lds_budget_ok(int KM, int P_MAX);
 }  // namespace adock

@@ -133,33 +133,34 @@ class KernelProblem:
             G["spacing"], G["penalty"], float(cap), _lib.ptr(be), _lib.ptr(bi), _lib.stream_of(self.device)))
         return be.view(L, C * Q), bi.view(L, C * Q).long()
 
-    def receptor_neighbours(self, prob, chain_lig, states, K, margin):
-        """Per chain: the receptor atoms within cutoff + margin of any heavy atom of the chain's pose, nearest list
-        first in receptor order -> (nbr [N, NB] int32, nbr_n [N] int32) (src/adock/mckernel.py:662-688)."""
+    def receptor_neighbours(self, prob, chain_lig, states, K, margin, chunk=100_000_000):
+        """Per chain: the receptor atoms within cutoff + margin of any heavy atom of the chain's pose, in receptor order
+        -> (nbr [N, NB] int32, nbr_n [N] int32), NB the longest list (src/adock/mckernel.py:662-688). All chains at
+        once, in groups sized so the [chains, atoms, receptor atoms] distance block stays near `chunk` elements."""
         from .pose import pose_coords
         rx, rt, rm = prob.rec
         lb = prob.lb
+        dev = states.device
         cut = float(prob.pot.p["cutoff"]) + margin
         N = states.shape[0]
-        lists = []
-        cl = chain_lig.long()
-        for l in torch.unique(cl).tolist():
-            idx = (cl == l).nonzero().squeeze(1)
-            t, q, th = self.unpack(states[idx], K)
-            sub = lb.subset(torch.tensor([l], device=states.device))
-            X = pose_coords(sub, t[None], q[None], torch.zeros_like(t)[None], th[None])[0][:, lb.heavy[l]]
-            ri = int(prob.rec_idx[l])
-            xr = rx[ri][rm[ri]].to(X.device).float()
-            near = (torch.cdist(X.reshape(-1, 3), xr).view(len(idx), -1, len(xr)) < cut).any(1)
-            lists.append((idx, near))
-        NB = max(1, max(int(n.sum(1).max()) for _, n in lists))
-        nbr = torch.zeros(N, NB, dtype=torch.int32, device=states.device)
-        nbr_n = torch.zeros(N, dtype=torch.int32, device=states.device)
-        for idx, near in lists:
-            order = torch.argsort((~near).to(torch.int8), dim=1, stable=True)[:, :NB]
-            nbr[idx, :order.shape[1]] = order.to(torch.int32)
-            nbr_n[idx] = near.sum(1).to(torch.int32)
-        return nbr, nbr_n
+        cl = chain_lig.to(device=dev).long()
+        t, q, th = self.unpack(states, K)
+        X = pose_coords(lb.subset(cl), t[:, None], q[:, None], torch.zeros_like(t)[:, None], th[:, None])[:, 0]
+        heavy = lb.heavy[cl]
+        ri = prob.rec_idx.to(dev)[cl]
+        rxd, rmd = rx.to(dev).float(), rm.to(dev)
+        near = torch.zeros(N, rxd.shape[1], dtype=torch.bool, device=dev)
+        step = max(1, chunk // max(1, X.shape[1] * rxd.shape[1]))
+        for c0 in range(0, N, step):
+            d = torch.cdist(X[c0:c0 + step].float(), rxd[ri[c0:c0 + step]])
+            d = d.masked_fill(~heavy[c0:c0 + step, :, None], float("inf"))
+            near[c0:c0 + step] = (d < cut).any(1) & rmd[ri[c0:c0 + step]]
+        cnt = near.sum(1)
+        NB = max(1, int(cnt.max())) if N else 1
+        order = torch.argsort((~near).to(torch.int8), dim=1, stable=True)[:, :NB]
+        nbr = torch.zeros(N, NB, dtype=torch.int32, device=dev)
+        nbr[:, :order.shape[1]] = order.to(torch.int32)
+        return nbr, cnt.to(torch.int32)
 
     def check_chain_buffers(self, N, cur, cur_e, best, best_e):
         """cur / best [N, SD] and cur_e / best_e [N]: contiguous float32 tensors on this problem's device (the kernels

@@ -232,6 +232,7 @@ def dock_problem_kernel(prob, boxes_c, boxes_s, cfg, seed=0, log=None, stats=Non
         pe, pbe = torch.zeros(L * P, device=dev), torch.full((L * P,), float("inf"), device=dev)
         margin = cfg.get("polish_nbr_margin", 2.0)
         nb = kp.receptor_neighbours(prob, chain_lig[sel], pc, lb.K, margin) if margin > 0 else (None, None)
+        mark("neighbours")
         kp.run_mc(chain_lig[sel], chain_id[sel], pc, pe, pb, pbe, True, 0, cfg["polish_iters"], *mc_args[1:], 0,
                   exact=True, nbr=nb[0], nbr_n=nb[1])
         best, M, chain_lig = pb, P, chain_lig[sel]

@@ -108,3 +108,27 @@ def test_widening_rule():
     cfg = _search_cfg(n_chains=4, mc_steps=16, mc_segment=16, min_total_chains=16, min_chain_steps=4, polish_top=0)
     dock_problem_kernel(prob, [np.zeros(3)], [np.full(3, 14.0)], cfg, seed=0, stats=stats, kernel_problem=OracleKernel)
     assert stats["widen"] == 4 and stats["chain_steps"] == 16 * 4
+
+
+def test_receptor_neighbour_lists():
+    """Neighbour lists (receptor atoms within cutoff + margin of any heavy atom of each chain's pose, receptor order)
+    against a per-chain brute force, for chains of two ligands."""
+    from adock_amd import workload
+    rec = workload.receptor_dict(synth.make_receptor(4, 700, 16.0, 5.0), np.zeros(3), 16.0)
+    preps = [synth.make_ligand(21, 14, 3), synth.make_ligand(22, 18, 2)]
+    prob = cpu_search_problem(rec, preps, np.zeros(3), 16.0)
+    tw = OracleKernel(prob)
+    cl = torch.tensor([0, 1, 1, 0, 1], dtype=torch.int32)
+    st = torch.as_tensor(np.concatenate([states(40 + i, 1, np.zeros(3), 10.0, preps[int(c)], tw.KM)
+                                         for i, c in enumerate(cl)]))
+    nbr, nbr_n = OracleKernel.receptor_neighbours(tw, prob, cl, st, prob.lb.K, 2.0)
+    rx = np.asarray(rec["rec_xyz"], np.float64)
+    cut = float(prob.pot.p["cutoff"]) + 2.0
+    for i, c in enumerate(cl.tolist()):
+        t, q, th = tw.unpack(st[i:i + 1], prob.lb.K)
+        sub = prob.lb.subset(torch.tensor([c]))
+        X = pose_coords(sub, t[None], q[None], torch.zeros_like(t)[None], th[None])[0, 0][prob.lb.heavy[c]].numpy()
+        d = np.sqrt(((X[:, None, :] - rx[None]) ** 2).sum(-1)).min(0)
+        want = np.nonzero(d < cut)[0]
+        assert int(nbr_n[i]) == len(want)
+        np.testing.assert_array_equal(nbr[i, :len(want)].numpy(), want)
