@@ -281,3 +281,44 @@ def fitted_complex(rec, lig, center, size, spec, seed=0, p_charge=0.3, p_side_ch
     return {"rec_xyz": xyz, "rec_ext": receptor_ext_types(ann, spec), "center": np.asarray(center, np.float64),
             "size": np.broadcast_to(np.asarray(size, np.float64), (3,)).copy(), "lig": lig,
             "lig_ext": ligand_ext_types(lig.types, classes(lig.types), spec.nxs)}
+
+
+def make_native_complex(seed, n_heavy=20, n_tors=4, shell=(3.6, 4.6), outer=(4.6, 7.5), min_sep=3.0):
+    """A complex with a known native pose: the ligand in its reference conformation at the origin (translation 0,
+    identity rotation, torsions 0) and receptor atoms grown around it. Inner shell: points shell[0]..shell[1] A from
+    the nearest ligand heavy atom, typed to complement it (O_A facing a ligand donor, N_D facing an acceptor, C_H
+    otherwise); outer layer: C_H / C_P; every receptor pair at least min_sep apart. Returns dict(receptor, lig,
+    center, native [n, 3] world coordinates)."""
+    rng = np.random.default_rng(seed)
+    lig = make_ligand(seed + 3, n_heavy, n_tors)
+    X = lig.ref[lig.heavy].astype(np.float64)
+    names = _CFG["_xs_names"]
+    don = np.asarray(_CFG["_xs_table"]["donor"], bool)
+    acc = np.asarray(_CFG["_xs_table"]["acceptor"], bool)
+    lo, hi = X.min(0) - outer[1] - 1.0, X.max(0) + outer[1] + 1.0
+    pts = rng.uniform(lo, hi, (40000, 3))
+    d = np.sqrt(((pts[:, None, :] - X[None]) ** 2).sum(-1))
+    dmin, near = d.min(1), d.argmin(1)
+    keep = (dmin >= shell[0]) & (dmin <= outer[1])
+    pts, dmin, near = pts[keep], dmin[keep], near[keep]
+    order = np.argsort(dmin, kind="stable")            # inner points first
+    chosen = []
+    for i in order:
+        p = pts[i]
+        if chosen and np.min(np.sum((np.asarray(chosen) - p) ** 2, 1)) < min_sep ** 2:
+            continue
+        chosen.append(p)
+    chosen = np.asarray(chosen)
+    dc = np.sqrt(((chosen[:, None, :] - X[None]) ** 2).sum(-1))
+    types = np.empty(len(chosen), np.int32)
+    lt = lig.types[lig.heavy]
+    for k in range(len(chosen)):
+        j = int(dc[k].argmin())
+        if dc[k, j] <= shell[1]:
+            t = int(lt[j])
+            types[k] = IDX["O_A"] if don[t] else IDX["N_D"] if acc[t] else IDX["C_H"]
+        else:
+            types[k] = IDX["C_H"] if rng.random() < 0.5 else IDX["C_P"]
+    native = lig.ref.astype(np.float64)
+    return {"receptor": {"rec_xyz": chosen.astype(np.float32), "rec_types": types}, "lig": lig,
+            "center": np.zeros(3), "native": native, "xs_names": names}
