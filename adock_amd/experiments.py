@@ -8,6 +8,7 @@ grids():   both grid builders (Vina XS and the fitted function) on the single-co
 driver():  the full search driver (Stage A, MC segments, refine, exact polish, finalize) with a search preset on a
            one-pocket screen, grid build included, per docked ligand (Vina XS, or the fitted function's path).
 wide():    the single complex widened to 4,096 chains at constant chain-steps (the reference's min_total_chains rule).
+mlp():     the proposal network's kernels alone: MFMA inference on one step's batch, one FP32 training step.
 eg():      the energy + gradient kernel alone on many independent poses of the single complex (throughput bound).
 learned(): trains the proposal network on accepted moves logged by the plain method on training complexes, then
            compares plain and learned search on disjoint evaluation complexes at equal MC-step budgets.
@@ -262,6 +263,45 @@ def wide(chains=4096, base_chains=256, base_steps=1000, reps=3, seed=0):
     return {"workload": f"single complex widened: {chains} chains x {steps} MC steps (= {base_chains} x {base_steps} "
                         "chain-steps), Vina-rule BFGS", "docked_ligands_per_s": 1.0 / dt, "seconds": dt,
             "best_score": best}
+
+
+MLP_INFER_FLOP = 2 * (128 * 256 + 256 * 256 + 256 * 23)   # per row: the three matrix products (208,384)
+
+
+def mlp(rows=32768, train_batch=4096, reps=20, seed=0):
+    """Proposal network kernels: inference of `rows` feature rows (one MC step of the learned search's batch) on MFMA
+    (BF16 in, FP32 accumulate), and one training step (forward + backward + Adam, FP32 GEMM kernels) on train_batch
+    rows, each timed over `reps` back-to-back calls between HIP events on the launch stream."""
+    from .mlp import IN, OUT, ProposalNet
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev).manual_seed(seed)
+    net = ProposalNet(device=dev, seed=seed)
+    X = torch.randn(rows, IN, generator=g, device=dev)
+    Y = torch.empty(rows, OUT, device=dev)
+    Xt = torch.randn(train_batch, IN, generator=g, device=dev)
+    Tt = torch.randn(train_batch, OUT, generator=g, device=dev)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps * 1e-3
+
+    def train_step():
+        net.backward(net.forward(Xt), Tt)
+        net.step()
+
+    ti = timed(lambda: net.infer(X, Y))
+    tt = timed(train_step)
+    return {"workload": f"proposal network 128-256-256-23: inference of {rows} rows (MFMA, BF16 / FP32 accumulate); "
+                        f"training step of {train_batch} rows (FP32)",
+            "infer_ms": ti * 1e3, "infer_rows_per_s": rows / ti, "infer_tflops": rows * MLP_INFER_FLOP / ti / 1e12,
+            "train_step_ms": tt * 1e3, "train_tflops": 3 * train_batch * MLP_INFER_FLOP / tt / 1e12}
 
 
 def eg(n_poses=65536, reps=10, seed=0):

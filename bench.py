@@ -22,6 +22,7 @@ from adock_amd.grid import make_gridset
 from adock_amd.kernel import KernelProblem
 
 PEAK_FP32_TFLOPS = 157.3
+PEAK_BF16_TFLOPS = 2500.0   # dense MFMA
 HBM_PEAK_GBS = 8000.0
 
 
@@ -69,7 +70,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--chains-per-block", type=int, default=0)
     ap.add_argument("--line-search-width", type=int, default=0, help="trials per speculative batch (0 = adaptive)")
-    ap.add_argument("--workload", choices=["single", "screen", "multi", "learned", "grids", "eg", "wide", "driver", "driver_fitted"], default="single",
+    ap.add_argument("--workload", choices=["single", "screen", "multi", "learned", "grids", "eg", "wide", "driver", "driver_fitted", "mlp"], default="single",
                     help="single = the headline line (default); the others print their own JSON line")
     ap.add_argument("--small", action="store_true", help="scaled-down sizes for the opt-in workloads")
     ap.add_argument("--target", choices=["move", "jump"], default="move",
@@ -87,6 +88,12 @@ def main():
             out = ex.screen(n_ligands=64 if args.small else 512, chains=64, steps=50 if args.small else 500)
         elif args.workload in ("driver", "driver_fitted"):
             out = ex.driver(n_ligands=32 if args.small else 256, fitted=args.workload == "driver_fitted")
+        elif args.workload == "mlp":
+            out = ex.mlp()
+            out["roofline"] = {"infer": {"bound": "mfma", "achieved": out["infer_tflops"], "peak": PEAK_BF16_TFLOPS,
+                                         "unit": "TFLOP/s", "frac": out["infer_tflops"] / PEAK_BF16_TFLOPS},
+                               "train": {"bound": "valu", "achieved": out["train_tflops"], "peak": PEAK_FP32_TFLOPS,
+                                         "unit": "TFLOP/s", "frac": out["train_tflops"] / PEAK_FP32_TFLOPS}}
         elif args.workload == "wide":
             out = ex.wide()
         elif args.workload == "eg":
