@@ -78,3 +78,29 @@ def test_facade_rejects_mismatched_chain_buffers():
                 (st, e.double(), st, e), (st, e, st, torch.zeros(N + 1))):
         with pytest.raises(_lib.AdockError):
             kp.check_chain_buffers(N, *bad)
+
+
+def test_product_path_fails_loudly_without_gpu_or_library(monkeypatch, tmp_path):
+    """No CPU fallback: without a GPU the facades raise, and a missing library is an error naming the build step."""
+    import numpy as np
+    import torch
+    from adock_amd import _lib
+    from adock_amd.config import sf_params
+    from adock_amd.grid import build_grid
+    from adock_amd.kernel import KernelProblem
+    from helpers import CFG
+    if not torch.cuda.is_available():
+        with pytest.raises(_lib.AdockError):
+            _lib.require_gpu()
+        with pytest.raises(_lib.AdockError):
+            KernelProblem(object())
+        with pytest.raises(_lib.AdockError):
+            build_grid(np.zeros((1, 3), np.float32), np.zeros(1, np.int32), sf_params(CFG, "vina"), np.zeros(3),
+                       0.375, 4, device="cuda")
+    with pytest.raises(_lib.AdockError):
+        build_grid(np.zeros((1, 3), np.float32), np.zeros(1, np.int32), sf_params(CFG, "vina"), np.zeros(3), 0.375,
+                   4, device="cpu")
+    monkeypatch.setattr(_lib, "_LIB", None)
+    monkeypatch.setattr(_lib, "LIB_PATH", tmp_path / "missing.so")
+    with pytest.raises(_lib.AdockError, match="not built"):
+        _lib.lib()
