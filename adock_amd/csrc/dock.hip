@@ -40,12 +40,14 @@ constexpr unsigned short PAIR_PAD = (unsigned short)AMK;        // (dummy atom, 
 //                                  R_i1, R_i2, b_i; p -> R p + b), rebuilt every evaluation (identity past nt)
 //   TU  [KM + 1]    float4  current (world) axis unit vector u_k
 //   TM  [KM + 1]    float4  current axis moment m_k = p_k x u_k (torque about the axis: u.(X x F) + m.F)
-//   FS  [P128 + 1]  float4  gradient slot of pair q (dE/dX of its first atom); slot P128 stays zero
+//   FS  [P128 + 4]  float   slot of pair q: (dE/dr) / r, so dE/dX of either atom is FS[q] (X_self - X_partner);
+//                           slot P128 stays zero
 //   flg [64]        int     atom type flags
-//   pl  [P128]      u16     pair words i | j << 7 | hydrophobic << 14 | h-bond << 15, strided over lanes (pair q on
-//                           lane q % 64 in round q / 64); padding pairs are (64, 0)
+//   pl  [P128 + 8]  u16     pair words i | j << 7 | hydrophobic << 14 | h-bond << 15, strided over lanes (pair q on
+//                           lane q % 64 in round q / 64); padding pairs (and entry P128) are (64, 0)
 //   inc [2 P128 + 512] u16  per-atom pair-slot lists, copied from the host-built packed ligand (each atom's list
-//                           padded to 8 entries; start and padded count live in the atom's lane); bit 15 = negate
+//                           padded to 8 entries; start and padded count live in the atom's lane); bit 15 = the atom is
+//                           the pair's second one (its partner is then the first)
 // Dense variant (the latency-bound speculative kernel): FS is replaced by FD [64 DEG_MAX + 1] float4, row k column
 // a ^ k = the gradient of atom a's k-th pair (sign applied by the writer; unused slots stay zero, the last one takes
 // the padding pairs' writes), and inc by pdst [P128] u32 (destination slot of pair q's first atom | second atom << 16).
@@ -55,12 +57,12 @@ __host__ __device__ inline int pad128(int P_MAX) { return (P_MAX + 127) & ~127; 
 __host__ __device__ inline int inc_stride(int P_MAX) { return 2 * pad128(P_MAX) + 512; }
 __host__ __device__ inline int wave_lds_bytes(int KM, int P_MAX) {
     const int P = pad128(P_MAX);
-    return 16 * (AMK + 1) + 5 * 16 * (KM + 1) + 16 * (P + 1) + 4 * AMK + 2 * P + 2 * inc_stride(P_MAX);
+    return 16 * (AMK + 1) + 5 * 16 * (KM + 1) + 4 * (P + 4) + 4 * AMK + 2 * (P + 8) + 2 * inc_stride(P_MAX);
 }
 __host__ __device__ inline int pad512(int P_MAX) { return (P_MAX + 511) & ~511; }
 __host__ __device__ inline int wave_lds_bytes_dense(int KM, int P_MAX, int DEG_MAX) {
     const int P = pad512(P_MAX);   // pair words and destinations padded to whole 8-round chunks
-    return 16 * (AMK + 1) + 5 * 16 * (KM + 1) + 16 * (AMK * DEG_MAX + 1) + 4 * AMK + 2 * P + 4 * P;
+    return 16 * (AMK + 1) + 5 * 16 * (KM + 1) + 16 * (AMK * DEG_MAX + 1) + 4 * AMK + 2 * (P + 8) + 4 * P;
 }
 template <bool DENSE>
 __host__ __device__ inline int lds_bytes(int KM, int P_MAX, int DEG_MAX) {
@@ -71,7 +73,7 @@ constexpr int LDS_PER_WORKGROUP = 160 * 1024;   // gfx950: 160 KB of LDS per CU,
 struct WaveLDS {
     float4* X4;
     float4* TR0; float4* TR1; float4* TR2; float4* TU; float4* TM;
-    float4* FS;
+    float* FS;
     int* flg;
     unsigned short* pl;
     unsigned short* inc;
@@ -116,7 +118,8 @@ DEVI WaveLDS carve_lds(char* base, int P_MAX, int DEG_MAX = 0) {
     w.TR2 = f; f += KM + 1;
     w.TU = f; f += KM + 1;
     w.TM = f; f += KM + 1;
-    w.FS = w.FD = nullptr;
+    w.FS = nullptr;
+    w.FD = nullptr;
     w.RN4 = nullptr;
     w.RNF = nullptr;
     w.nrs = 0;
@@ -124,11 +127,11 @@ DEVI WaveLDS carve_lds(char* base, int P_MAX, int DEG_MAX = 0) {
     w.pdst = nullptr;
     w.trash = AMK * DEG_MAX;
     if constexpr (DENSE) { w.FD = f; f += AMK * DEG_MAX + 1; }
-    else { w.FS = f; f += w.P128 + 1; }
+    else { w.FS = reinterpret_cast<float*>(f); f += (w.P128 + 4) / 4; }
     w.flg = reinterpret_cast<int*>(f);
     w.pl = reinterpret_cast<unsigned short*>(w.flg + AMK);
-    if constexpr (DENSE) w.pdst = reinterpret_cast<unsigned*>(w.pl + w.P128);
-    else w.inc = w.pl + w.P128;
+    if constexpr (DENSE) w.pdst = reinterpret_cast<unsigned*>(w.pl + w.P128 + 8);
+    else w.inc = w.pl + w.P128 + 8;
     return w;
 }
 
@@ -202,8 +205,9 @@ DEVI ChainCtx setup_chain(int l, int lane, const LigArgs& L, const GridArgs& G, 
         for (int i = lane; i <= w.trash; i += WAVE) w.FD[i] = make_float4(0.f, 0.f, 0.f, 0.f);
         for (int q = lane; q < w.P128; q += WAVE) w.pdst[q] = (unsigned)w.trash | ((unsigned)w.trash << 16);
     } else if (lane == 0) {
-        w.FS[w.P128] = make_float4(0.f, 0.f, 0.f, 0.f);
+        w.FS[w.P128] = 0.f;
     }
+    if (lane < 8) w.pl[w.P128 + lane] = PAIR_PAD;
     wave_lds_sync();
     // pair word: i | j << 7 | (hydrophobic pair) << 14 | (h-bond pair) << 15; padding (and self-pairs): the dummy atom
     const int* pr = L.pr + (long long)l * L.P_MAX * 2;
@@ -260,7 +264,7 @@ DEVI float pair_math(float4 xi, float4 xj, unsigned pw, const SFConst& sf, float
     const float e = pair_e_nb(rr - xi.w - xj.w, (int)(pw >> 14), sf, de, hyd, hb);
     const float in = rr < sf.cut ? 1.f : 0.f;
     const float sc = in * de * rinv;
-    s = make_float4(sc * rx, sc * ry, sc * rz, 0.f);
+    s = make_float4(sc * rx, sc * ry, sc * rz, sc);
     return in * e;
 }
 
@@ -381,8 +385,8 @@ DEVI float eval_eg(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, int l
             float4 s0, s1;
             ep_l += pair_slot(w, pw0, sf, s0, hyd, hb);
             ep_l += pair_slot(w, pw1, sf, s1, hyd, hb);
-            w.FS[q0] = s0;
-            w.FS[q0 + WAVE] = s1;
+            w.FS[q0] = s0.w;
+            w.FS[q0 + WAVE] = s1.w;
         }
     }
     wave_lds_sync();
@@ -399,6 +403,8 @@ DEVI float eval_eg(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, int l
             }
         }
     } else {
+        // each entry: the pair's slot (dE/dr)/r and the partner atom, from the pair word; the product is rounded
+        // before the sum (as the dense rows' (dE/dX) slots are), so both layouts give the same bits
         const uint4* inc4 = reinterpret_cast<const uint4*>(w.inc);
         for (int k = 0; k < c.D8; k += 8) {
             const bool ok = k < c.ic8;
@@ -407,11 +413,18 @@ DEVI float eval_eg(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, int l
             const unsigned ws[8] = {ok ? v.x & 0xffffu : zs, ok ? v.x >> 16 : zs, ok ? v.y & 0xffffu : zs,
                                     ok ? v.y >> 16 : zs,     ok ? v.z & 0xffffu : zs, ok ? v.z >> 16 : zs,
                                     ok ? v.w & 0xffffu : zs, ok ? v.w >> 16 : zs};
+            float sc[8];
+            unsigned pw[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) { sc[u] = w.FS[ws[u] & 0x7fffu]; pw[u] = w.pl[ws[u] & 0x7fffu]; }
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
-                const float4 f = w.FS[ws[u] & 0x7fffu];
-                const float sg = (ws[u] & 0x8000u) ? -1.f : 1.f;
-                pfx += sg * f.x; pfy += sg * f.y; pfz += sg * f.z;
+#pragma clang fp contract(off)
+                const unsigned pt = (ws[u] & 0x8000u) ? (pw[u] & 127u) : ((pw[u] >> 7) & 127u);
+                const float4 xp = w.X4[pt];
+                pfx += sc[u] * (X0 - xp.x);
+                pfy += sc[u] * (X1 - xp.y);
+                pfz += sc[u] * (X2 - xp.z);
             }
         }
     }
