@@ -122,27 +122,39 @@ class Problem:
         finally:
             self.gs.penalty = pen
 
-    def inter_exact(self, X, chunk=2_000_000):
-        """Direct pairwise ligand-receptor energy [L, M] through pot.energy, no grid (src/adock/search.py:73-89)."""
+    def inter_exact(self, X, chunk=20_000_000):
+        """Direct pairwise ligand-receptor energy [L, M] through pot.energy, no grid (src/adock/search.py:73-89); groups
+        of complexes (or of poses of one complex) at a time, so the [complexes, poses, atoms, receptor atoms] distance
+        block stays near `chunk` elements."""
         if self.rec is None:
             raise ValueError("inter_exact needs receptor atoms (Problem.rec)")
         rx, rt, rm = self.rec
         lb = self.lb
-        L, M = X.shape[:2]
-        out = torch.zeros(L, M, device=X.device, dtype=X.dtype)
-        for l in range(L):
-            ri = int(self.rec_idx[l])
-            hv = lb.heavy[l]
-            xl = X[l][:, hv]
-            tl = lb.types[l][hv]
-            xr, tr = rx[ri][rm[ri]].to(X.dtype), rt[ri][rm[ri]]
-            if len(xr) == 0 or xl.shape[1] == 0:
-                continue
-            step = max(1, chunk // (xl.shape[1] * len(xr)))
-            for m0 in range(0, M, step):
-                r = torch.cdist(xl[m0:m0 + step], xr[None].expand(len(xl[m0:m0 + step]), -1, -1))
-                e = self.pot.energy(r, tl[None, :, None].expand_as(r), tr[None, None, :].expand_as(r))
-                out[l, m0:m0 + step] = e.sum((-1, -2))
+        L, M, A = X.shape[:3]
+        dev = X.device
+        rx, rt, rm = rx.to(dev), rt.to(dev), rm.to(dev)
+        # padding slots get a real receptor type (they are masked out; the potential only sees valid pair types)
+        rt = torch.where(rm, rt, rt[rm].max() if bool(rm.any()) else rt.new_zeros(()))
+        NR = rx.shape[1]
+        out = torch.zeros(L, M, device=dev, dtype=X.dtype)
+        per_l = M * A * NR
+        lstep = max(1, chunk // max(1, per_l))
+        mstep = M if per_l <= chunk else max(1, chunk // max(1, A * NR))
+        for l0 in range(0, L, lstep):
+            ls = slice(l0, min(L, l0 + lstep))
+            ri = self.rec_idx[ls].to(dev)
+            xr = rx[ri].to(X.dtype)
+            n = xr.shape[0]
+            mask = lb.heavy[ls][:, None, :, None] & rm[ri][:, None, None, :]
+            tl = lb.types[ls].clamp(min=0)[:, None, :, None]
+            tr = rt[ri][:, None, None, :]
+            for m0 in range(0, M, mstep):
+                ms = slice(m0, min(M, m0 + mstep))
+                Xs = X[ls, ms]
+                m = Xs.shape[1]
+                r = torch.cdist(Xs.reshape(n, m * A, 3), xr).view(n, m, A, NR)
+                e = self.pot.energy(r, tl.expand(n, m, A, NR), tr.expand(n, m, A, NR))
+                out[ls, ms] = torch.where(mask, e, torch.zeros_like(e)).sum((-1, -2))
         return out
 
     def energy(self, t, q0, r, theta):
