@@ -62,6 +62,23 @@ int adock_build_grid_fit(const float* rxyz, const int* rext, int n_rec, const fl
                                             nx, ny, nz, prm, out, (hipStream_t)stream));
 }
 
+int adock_inter_exact_fit(const float* xyz, const int* lig_type, int n_cx, int n_pose, int n_atom,
+                          const int* rec_idx, const float* rxyz, const int* rext, int n_sets, int n_rec,
+                          const float* rad_rec, const float* rad_lig, int n_lt, const float* W, int n_rt,
+                          const float* prm, float* out, adock_stream_t stream) {
+    if (n_cx < 0 || n_pose < 0 || n_atom < 0 || n_sets < 0 || n_rec < 0 || n_lt < 0 || n_rt < 0)
+        return fail(ADOCK_E_ARG, "negative size");
+    if ((long long)n_cx * n_pose == 0) return 0;
+    if (!out || !rec_idx || !prm || (n_atom > 0 && (!xyz || !lig_type)))
+        return fail(ADOCK_E_ARG, "null pointer");
+    if (n_sets < 1) return fail(ADOCK_E_ARG, "at least one receptor set");
+    if (n_rec > 0 && (!rxyz || !rext || !rad_rec || !W || !rad_lig || n_rt < 1 || n_lt < 1))
+        return fail(ADOCK_E_ARG, "receptor atoms need rxyz, rext, rad_rec, rad_lig, W and n_rt, n_lt >= 1");
+    if ((long long)n_cx * ((n_pose + 3) / 4) > (1ll << 31) - 1) return fail(ADOCK_E_ARG, "too many poses");
+    return hip_status(launch_inter_exact_fit(xyz, lig_type, n_cx, n_pose, n_atom, rec_idx, rxyz, rext, n_sets, n_rec,
+                                             rad_rec, rad_lig, n_lt, W, n_rt, prm, out, (hipStream_t)stream));
+}
+
 #define LIG_DECL                                                                                                     \
     const float *ref, const int *typ, const unsigned char *tm, const int *ax, const float *axp, const float *axu,    \
         const int *pr, const int *na, const int *nt, const int *np, const float *rg, int P_MAX, const int *gslot, \

@@ -161,6 +161,39 @@ __global__ __launch_bounds__(GB) void build_grid_kernel(
 constexpr int FIT_RT_MAX = 216;
 constexpr int FIT_NB = 6;
 
+// Fitted-function constants from prm (cutoff, g1_off, g1_w, g2_off, g2_w, rep_off, hyd_good, hyd_bad, hb_good, hb_bad,
+// ion_rmin, rep_cap) in evaluation form: Gaussian widths folded with sqrt(log2 e) so exp(-u^2) = exp2(-(u')^2).
+struct FitConst {
+    float cut, cut2, g1o, g1s, g2o, g2s, repo, hbad, hinv, bbad, binv, rmin, cap, icut2;
+};
+DEVI FitConst make_fit(const float* __restrict__ prm) {
+    FitConst f;
+    const float rl = 1.2011224087864498f;   // sqrt(log2 e)
+    f.cut = prm[0];
+    f.cut2 = f.cut * f.cut;
+    f.g1o = prm[1]; f.g1s = rl / prm[2]; f.g2o = prm[3]; f.g2s = rl / prm[4]; f.repo = prm[5];
+    f.hbad = prm[7]; f.hinv = 1.f / (prm[7] - prm[6]);
+    f.bbad = prm[9]; f.binv = 1.f / (prm[9] - prm[8]);
+    f.rmin = prm[10]; f.cap = prm[11]; f.icut2 = 1.f / f.cut2;
+    return f;
+}
+// ionic basis 1 / max(r, r_min)^2 - 1 / cutoff^2 (depends on r only)
+DEVI float fit_ion(const FitConst& f, float rr) {
+    const float rc = fmaxf(rr, f.rmin);
+    return 1.f / (rc * rc) - f.icut2;
+}
+// sum_k w[k] basis_k at surface distance d (gauss1, gauss2, capped repulsion, hydrophobic and h-bond ramps) + w[5] ion
+DEVI float fit_pair(const FitConst& f, const float* wq, float d, float ion) {
+    const float u1 = (d - f.g1o) * f.g1s, u2 = (d - f.g2o) * f.g2s;
+    float sv = wq[0] * fast_exp2(-u1 * u1);
+    sv = fmaf(wq[1], fast_exp2(-u2 * u2), sv);
+    const float dr = fminf(d - f.repo, 0.f);
+    sv = fmaf(wq[2], fminf(dr * dr, f.cap), sv);
+    sv = fmaf(wq[3], fminf(fmaxf((f.hbad - d) * f.hinv, 0.f), 1.f), sv);
+    sv = fmaf(wq[4], fminf(fmaxf((f.bbad - d) * f.binv, 0.f), 1.f), sv);
+    return fmaf(wq[5], ion, sv);
+}
+
 __global__ __launch_bounds__(GB) void build_grid_fit_kernel(
     const float* __restrict__ rxyz, const int* __restrict__ rext, int n_rec, const float* __restrict__ rad_rec,
     const float* __restrict__ rad_lig, int n_lt, const float* __restrict__ W, int n_rt, int q0, float ox, float oy,
@@ -181,18 +214,11 @@ __global__ __launch_bounds__(GB) void build_grid_fit_kernel(
     const int i = bi * TI + ti, j = bj * TJ + tj, k = bk * TK + tk;
     const bool valid = (i < nx) && (j < ny) && (k < nz);
     const float x = fmaf((float)i, spacing, ox), y = fmaf((float)j, spacing, oy), z = fmaf((float)k, spacing, oz);
-    // prm: cutoff, g1_off, g1_w, g2_off, g2_w, rep_off, hyd_good, hyd_bad, hb_good, hb_bad, ion_rmin, rep_cap
-    const float cut = prm[0];
-    const float cut2 = cut * cut;
-    const float gr = cut + 1e-3f;
+    const FitConst f = make_fit(prm);
+    const float gr = f.cut + 1e-3f;
     const float blo0 = fmaf((float)(bi * TI), spacing, ox) - gr, bhi0 = fmaf((float)(bi * TI + TI - 1), spacing, ox) + gr;
     const float blo1 = fmaf((float)(bj * TJ), spacing, oy) - gr, bhi1 = fmaf((float)(bj * TJ + TJ - 1), spacing, oy) + gr;
     const float blo2 = fmaf((float)(bk * TK), spacing, oz) - gr, bhi2 = fmaf((float)(bk * TK + TK - 1), spacing, oz) + gr;
-    const float rl = 1.2011224087864498f;   // sqrt(log2 e): exp(-u^2) = exp2(-(u rl)^2)
-    const float g1o = prm[1], g1s = rl / prm[2], g2o = prm[3], g2s = rl / prm[4], repo = prm[5];
-    const float hbad = prm[7], hinv = 1.f / (prm[7] - prm[6]);
-    const float bbad = prm[9], binv = 1.f / (prm[9] - prm[8]);
-    const float rmin = prm[10], cap = prm[11], icut2 = 1.f / cut2;
 
     const int nq = min(TQ, n_lt - q0);
     const int n_w = n_rec > 0 ? min(n_rt, FIT_RT_MAX) * TQ * FIT_NB : 0;
@@ -240,26 +266,15 @@ __global__ __launch_bounds__(GB) void build_grid_fit_kernel(
             const float4 r = s_atom[b];
             const float dx = r.x - x, dy = r.y - y, dz = r.z - z;
             const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-            if (r2 < cut2) {
+            if (r2 < f.cut2) {
                 const float rr = fast_sqrt(r2);
-                const float rc = fmaxf(rr, rmin);
-                const float ion = 1.f / (rc * rc) - icut2;
+                const float ion = fit_ion(f, rr);
                 const float rmr = rr - r.w;
                 const float* wb = s_w + s_rt[b] * (TQ * FIT_NB);
 #pragma unroll
                 for (int q = 0; q < TQ; ++q) {
                     if (q >= nq) break;
-                    const float* wq = wb + q * FIT_NB;
-                    const float d = rmr - rq[q];
-                    const float u1 = (d - g1o) * g1s, u2 = (d - g2o) * g2s;
-                    float sv = wq[0] * fast_exp2(-u1 * u1);
-                    sv = fmaf(wq[1], fast_exp2(-u2 * u2), sv);
-                    const float dr = fminf(d - repo, 0.f);
-                    sv = fmaf(wq[2], fminf(dr * dr, cap), sv);
-                    sv = fmaf(wq[3], fminf(fmaxf((hbad - d) * hinv, 0.f), 1.f), sv);
-                    sv = fmaf(wq[4], fminf(fmaxf((bbad - d) * binv, 0.f), 1.f), sv);
-                    sv = fmaf(wq[5], ion, sv);
-                    acc_e[q] += sv;
+                    acc_e[q] += fit_pair(f, wb + q * FIT_NB, rmr - rq[q], ion);
                 }
             }
         }
@@ -275,6 +290,98 @@ __global__ __launch_bounds__(GB) void build_grid_fit_kernel(
     }
 }
 
+// Exact pairwise fitted inter energy of docked poses (src/adock/search.py:73-89 with FittedPotential.energy,
+// src/adock/fit/terms.py:339-344): out[l, m] = sum over ligand atoms a of pose (l, m) with a type in [0, n_lt) and
+// receptor atoms b of receptor rec_idx[l] with an ext type in [0, n_rt) and |x_a - x_b|^2 < cutoff^2 of
+// sum_k W[rt_b, lt_a, k] basis_k, with the same basis as the fitted grid (a grid point is an atom here).
+//
+// One wave per pose, one lane per ligand atom (poses with more than 64 atom slots take several 64-atom slices), four
+// independent poses per workgroup. Each wave streams the receptor in 64-atom chunks, keeps the atoms inside its own
+// pose's box grown by the cutoff (ballot-compacted in receptor order into the wave's LDS slice) and walks them with LDS
+// broadcast reads; the six pair weights come from W in memory (all lanes of a wave read the same receptor type row,
+// so the reads hit a few cache lines). Per-lane sums, then a fixed-order wave sum: the result is deterministic. A
+// complex whose rec_idx lies outside [0, n_sets) gets NaN.
+constexpr int XP = 4;                          // poses (waves) per workgroup
+
+__global__ __launch_bounds__(XP * WAVE) void inter_exact_fit_kernel(
+    const float* __restrict__ X, const int* __restrict__ lig_type, int M, int A, const int* __restrict__ rec_idx,
+    const float* __restrict__ rxyz, const int* __restrict__ rext, int n_sets, int NR,
+    const float* __restrict__ rad_rec, const float* __restrict__ rad_lig, int n_lt, const float* __restrict__ W,
+    int n_rt, const float* __restrict__ prm, float* __restrict__ out, int blocks_m) {
+    __shared__ float4 s_atom[XP][WAVE];   // x, y, z, radius
+    __shared__ int s_rt[XP][WAVE];
+    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+    const int l = blockIdx.x / blocks_m;
+    const int m = (blockIdx.x % blocks_m) * XP + wid;
+    if (m >= M) return;                   // waves are independent: no workgroup barrier below
+    const int ri = rec_idx[l];
+    if (ri < 0 || ri >= n_sets) {
+        if (lane == 0) out[(long long)l * M + m] = __builtin_nanf("");
+        return;
+    }
+    const FitConst f = make_fit(prm);
+    const float gr = f.cut + 1e-3f;
+    const float* rx = rxyz + (long long)ri * NR * 3;
+    const int* re = rext + (long long)ri * NR;
+    float4* sa = s_atom[wid];
+    int* st = s_rt[wid];
+    float acc = 0.f;
+    for (int a0 = 0; a0 < A; a0 += WAVE) {
+        const int a = a0 + lane;
+        const int lt = a < A ? lig_type[(long long)l * A + a] : -1;
+        const bool act = lt >= 0 && lt < n_lt;
+        float x = 0.f, y = 0.f, z = 0.f;
+        if (act) {
+            const float* p = X + (((long long)l * M + m) * A + a) * 3;
+            x = p[0]; y = p[1]; z = p[2];
+        }
+        // the pose slice's box grown by the cutoff (+1e-3 A: the cull is conservative under rounding); empty when no
+        // lane is active, then no receptor atom passes
+        const float blo0 = -wave_max_f(act ? -x : -INFINITY) - gr, bhi0 = wave_max_f(act ? x : -INFINITY) + gr;
+        const float blo1 = -wave_max_f(act ? -y : -INFINITY) - gr, bhi1 = wave_max_f(act ? y : -INFINITY) + gr;
+        const float blo2 = -wave_max_f(act ? -z : -INFINITY) - gr, bhi2 = wave_max_f(act ? z : -INFINITY) + gr;
+        const float rq = act ? rad_lig[lt] : 0.f;
+        const float* wl = W + (long long)(act ? lt : 0) * FIT_NB;
+        const long long wrow = (long long)n_lt * FIT_NB;
+        for (int base = 0; base < NR; base += WAVE) {
+            const int b = base + lane;
+            float4 at = make_float4(0.f, 0.f, 0.f, 0.f);
+            int rt = 0;
+            bool keep = false;
+            if (b < NR) {
+                at.x = rx[3 * b]; at.y = rx[3 * b + 1]; at.z = rx[3 * b + 2];
+                rt = re[b];
+                const bool typed = rt >= 0 && rt < n_rt;
+                at.w = typed ? rad_rec[rt] : 0.f;
+                keep = typed && at.x >= blo0 && at.x <= bhi0 && at.y >= blo1 && at.y <= bhi1 && at.z >= blo2 &&
+                       at.z <= bhi2;
+            }
+            const uint64_t mk = __ballot(keep);
+            if (keep) {
+                const int pos = __popcll(mk & ((1ull << lane) - 1ull));
+                sa[pos] = at;
+                st[pos] = rt;
+            }
+            wave_lds_sync();
+            const int cnt = __popcll(mk);
+            if (act) {
+                for (int c = 0; c < cnt; ++c) {
+                    const float4 r = sa[c];
+                    const float dx = r.x - x, dy = r.y - y, dz = r.z - z;
+                    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                    if (r2 < f.cut2) {
+                        const float rr = fast_sqrt(r2);
+                        acc += fit_pair(f, wl + st[c] * wrow, rr - r.w - rq, fit_ion(f, rr));
+                    }
+                }
+            }
+            wave_lds_sync();
+        }
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) out[(long long)l * M + m] = acc;
+}
+
 }  // namespace
 
 int build_grid_fit_rt_max() { return FIT_RT_MAX; }
@@ -288,6 +395,18 @@ int launch_build_grid_fit(const float* rxyz, const int* rext, int n_rec, const f
         hipLaunchKernelGGL(build_grid_fit_kernel, dim3((unsigned)nblk), dim3(GB), 0, stream, rxyz, rext, n_rec,
                            rad_rec, rad_lig, n_lt, W, n_rt, q0, ox, oy, oz, spacing, nx, ny, nz, prm, out, tiles_j,
                            tiles_k);
+    return (int)hipGetLastError();
+}
+
+int launch_inter_exact_fit(const float* X, const int* lig_type, int L, int M, int A, const int* rec_idx,
+                           const float* rxyz, const int* rext, int n_sets, int NR, const float* rad_rec,
+                           const float* rad_lig, int n_lt, const float* W, int n_rt, const float* prm, float* out,
+                           hipStream_t stream) {
+    if (L == 0 || M == 0) return 0;
+    const int blocks_m = (M + XP - 1) / XP;
+    const long long nblk = (long long)L * blocks_m;
+    hipLaunchKernelGGL(inter_exact_fit_kernel, dim3((unsigned)nblk), dim3(XP * WAVE), 0, stream, X, lig_type, M, A,
+                       rec_idx, rxyz, rext, n_sets, NR, rad_rec, rad_lig, n_lt, W, n_rt, prm, out, blocks_m);
     return (int)hipGetLastError();
 }
 

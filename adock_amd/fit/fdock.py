@@ -5,8 +5,9 @@
     res = dock_fitted(preps, spec, w, mode="v2_default", seed=0)
 
 MC chains and BFGS run in the HIP kernels on the fitted grids (adock_build_grid_fit) with the intra-ligand Vina form
-at the fitted intra weights. The kernels' exact pairwise mode implements only the stock function, so the exact polish
-is off and finalize rescores the final poses with the exact pairwise fitted energy in torch (kernel_cfg).
+at the fitted intra weights. The chain kernels' exact pairwise mode implements only the stock function, so the exact
+polish is off and finalize rescores the final poses with the exact pairwise fitted energy from the HIP kernel
+adock_inter_exact_fit (kernel_cfg; Problem.inter_exact dispatches to FittedPotential.inter_exact).
 
 A complex here is a dict with rec_xyz [N,3], rec_ext [N] (receptor ext types, terms.receptor_ext_types), center,
 size and lig (a LigandPrep with XS types) plus lig_ext (its ext types, terms.ligand_ext_types). The reference builds
@@ -47,8 +48,8 @@ def save_fitted(path, spec, w, extra=None):
 
 
 def kernel_cfg(scfg):
-    """A kernel search preset adapted to a fitted function: no exact polish, torch finalize with the exact pairwise
-    fitted energy (src/adock/fit/fdock.py:109-116)."""
+    """A kernel search preset adapted to a fitted function: no exact polish, the torch-orchestrated finalize with the
+    exact pairwise fitted energy (src/adock/fit/fdock.py:109-116)."""
     return dict(scfg, polish_top=0, polish_iters=0, finalize_torch=True)
 
 

@@ -64,3 +64,43 @@ def make_gridset_fit(preps, pot, spacing, penalty, device="cuda"):
         hi.append(c + s / 2)
     T = lambda a: torch.as_tensor(np.array(a), dtype=torch.float32, device=device)
     return GridSet(torch.stack(grids), T(origins), T(lo), T(hi), spacing, penalty)
+
+
+def inter_exact_fit(X, lig_types, rec_idx, rec_xyz, rec_ext, pot):
+    """Exact pairwise fitted inter energy [L, M] of poses X [L, M, A, 3] by the HIP kernel adock_inter_exact_fit:
+    the reference's Problem.inter_exact over FittedPotential.energy (src/adock/search.py:73-89,
+    src/adock/fit/terms.py:339-344). lig_types [L, A] compact ligand types (-1: hydrogen / padding, skipped), rec_idx
+    [L] receptor set of each complex, rec_xyz [R, N, 3], rec_ext [R, N] receptor ext types (-1: padding, skipped)."""
+    _lib.require_gpu()
+    if not X.is_cuda:
+        raise _lib.AdockError("inter_exact_fit runs on the GPU only (X must be a cuda tensor)")
+    dev = X.device
+    L, M, A = X.shape[:3]
+    spec = pot.spec
+    xs = X.detach().to(torch.float32).contiguous()
+    lt = torch.as_tensor(lig_types, device=dev).to(torch.int32).contiguous()
+    ri = torch.as_tensor(rec_idx, device=dev).to(torch.int32).contiguous()
+    rx = torch.as_tensor(rec_xyz, device=dev).to(torch.float32).contiguous()
+    re = torch.as_tensor(rec_ext, device=dev).to(torch.int32).contiguous()
+    if tuple(lt.shape) != (L, A) or tuple(ri.shape) != (L,):
+        raise _lib.AdockError("lig_types must be [L, A] and rec_idx [L] for X [L, M, A, 3]")
+    if rx.dim() != 3 or rx.shape[2] != 3 or tuple(re.shape) != tuple(rx.shape[:2]):
+        raise _lib.AdockError("rec_xyz must be [R, N, 3] and rec_ext [R, N]")
+    R, N = re.shape
+    if L and R == 0:
+        raise _lib.AdockError("no receptor sets")
+    if L and (int(ri.min()) < 0 or int(ri.max()) >= R):
+        raise _lib.AdockError(f"rec_idx must lie in [0, {R})")
+    if tuple(pot.W.shape) != (spec.n_rt, pot.TL, 6):
+        raise _lib.AdockError("pot.W must be [n_rt, TL, 6]")
+    keep = [pot.W.to(dev, torch.float32).contiguous(), torch.as_tensor(spec.rec_radius, dtype=torch.float32,
+            device=dev), pot.radius.to(dev, torch.float32).contiguous(),
+            torch.as_tensor(np.asarray(spec.grid_prm(), np.float32), device=dev)]
+    out = torch.empty((L, M), dtype=torch.float32, device=dev)
+    with torch.cuda.device(dev):
+        st = _lib.lib().adock_inter_exact_fit(
+            _lib.ptr(xs), _lib.ptr(lt), L, M, A, _lib.ptr(ri), _lib.ptr(rx), _lib.ptr(re), R, N, _lib.ptr(keep[1]),
+            _lib.ptr(keep[2]), pot.TL, _lib.ptr(keep[0]), spec.n_rt, _lib.ptr(keep[3]), _lib.ptr(out),
+            _lib.stream_of(dev))
+    _lib.check(st)
+    return out

@@ -272,7 +272,8 @@ class FittedPotential:
     Ligand types are compact indices 0..TL-1 into lig_ext_list; receptor types in pair calls are TL + receptor ext
     type. energy(r, ti, tj) is the fitted inter energy when tj >= TL and the Vina-form intra energy with the intra
     weights otherwise. `p` carries what the search kernels read: the base parameters, the intra weights as w_g1..w_hb
-    and the per-compact-type radius / flag arrays. The kernels' exact pairwise mode does not implement this function."""
+    and the per-compact-type radius / flag arrays. The search kernels' exact pairwise mode does not implement this
+    function; inter_exact (exact rescoring of final poses) runs the fitted-function kernel adock_inter_exact_fit."""
 
     def __init__(self, spec, weights, lig_ext_list, device="cpu"):
         w = np.asarray(weights, np.float32)
@@ -305,6 +306,14 @@ class FittedPotential:
 
     def weights(self, device=None):
         return self.w_intra if device is None else self.w_intra.to(device)
+
+    def inter_exact(self, X, lig_types, rec_idx, rec):
+        """Exact pairwise inter energy [L, M] of cuda poses X [L, M, A, 3] by the HIP kernel (gridfit.inter_exact_fit);
+        rec = (xyz [R, N, 3], pair types TL + ext type [R, N], mask [R, N]) as Problem.rec."""
+        from .gridfit import inter_exact_fit
+        rx, rt, rm = (t.to(X.device) for t in rec)
+        rext = torch.where(rm, rt - self.TL, torch.full_like(rt, -1))
+        return inter_exact_fit(X, lig_types, rec_idx, rx, rext, self)
 
     def energy(self, r, ti, tj):
         from ..scoring import pair_terms

@@ -123,11 +123,16 @@ class Problem:
             self.gs.penalty = pen
 
     def inter_exact(self, X, chunk=20_000_000):
-        """Direct pairwise ligand-receptor energy [L, M] through pot.energy, no grid (src/adock/search.py:73-89); groups
-        of complexes (or of poses of one complex) at a time, so the [complexes, poses, atoms, receptor atoms] distance
-        block stays near `chunk` elements."""
+        """Direct pairwise ligand-receptor energy [L, M], no grid (src/adock/search.py:73-89). For cuda poses and a
+        potential with its own exact kernel (pot.inter_exact: the fitted function) that kernel; otherwise through
+        pot.energy in torch, over groups of complexes (or of poses of one complex) at a time, so the [complexes, poses,
+        atoms, receptor atoms] distance block stays near `chunk` elements."""
         if self.rec is None:
             raise ValueError("inter_exact needs receptor atoms (Problem.rec)")
+        kernel = getattr(self.pot, "inter_exact", None)
+        if kernel is not None and X.is_cuda:
+            # a potential with its own exact kernel (FittedPotential: adock_inter_exact_fit)
+            return kernel(X, self.lb.types, self.rec_idx, self.rec).to(X.dtype)
         rx, rt, rm = self.rec
         lb = self.lb
         L, M, A = X.shape[:3]

@@ -55,6 +55,18 @@ int adock_build_grid_fit(const float* rxyz, const int* rext, int n_rec, const fl
                          int n_lt, const float* W, int n_rt, float ox, float oy, float oz, float spacing, int nx,
                          int ny, int nz, const float* prm, float* out, adock_stream_t stream);
 
+/* Exact pairwise fitted inter energy of docked poses (no grid). Replaces the torch loop Problem.inter_exact
+ * (src/adock/search.py:73-89) over FittedPotential.energy (src/adock/fit/terms.py:339-344) that finalize runs for the
+ * fitted function (search.py:483-489). xyz [n_cx, n_pose, n_atom, 3] pose coordinates, lig_type [n_cx, n_atom]
+ * compact ligand types (atoms outside [0, n_lt), e.g. -1 hydrogens and padding, are skipped), rec_idx [n_cx] receptor
+ * set of each complex in [0, n_sets) (else that complex's energies are NaN), rxyz [n_sets, n_rec, 3], rext
+ * [n_sets, n_rec] receptor ext types (outside [0, n_rt): padding, skipped), rad_rec [n_rt], rad_lig [n_lt],
+ * W [n_rt, n_lt, 6] and prm (12 floats) as adock_build_grid_fit. out [n_cx, n_pose]. Deterministic. */
+int adock_inter_exact_fit(const float* xyz, const int* lig_type, int n_cx, int n_pose, int n_atom,
+                          const int* rec_idx, const float* rxyz, const int* rext, int n_sets, int n_rec,
+                          const float* rad_rec, const float* rad_lig, int n_lt, const float* W, int n_rt,
+                          const float* prm, float* out, adock_stream_t stream);
+
 /* Energy + analytic gradient for n_chains poses. Replaces launch_eg_kernel (eg_kernel, src/adock/mckernel.py:411-447;
  * host binding src/adock/cpukernel.py:266-273). state [n, 7+KM], e_out [n], g_out [n, 6+KM] (entries 6+nt.. untouched).
  * prm: 15 floats in search order (g1_off, g1_w, g2_off, g2_w, rep_off, hyd_good, hyd_bad, hb_good, hb_bad, cutoff,

@@ -54,6 +54,7 @@ class _Lib:
         bf = [P, P, I, P, P, I, P, I, F, F, F, F, I, I, I, P, P]
         L.oracle_build_grid_fit_f64.argtypes = bf
         L.oracle_build_grid_fit_f32.argtypes = bf
+        L.oracle_inter_exact_fit_f64.argtypes = [P, P, I, I, I, P, P, P, I, I, P, P, I, P, I, P, P]
         L.oracle_eg_f64.argtypes = _LIG + _GRID + _SF + _REC + [I, P, I, P, P, P, I, P, P, P, P]
         L.oracle_eg_f32.argtypes = _LIG + _GRID + _SF + _REC + [I, P, I, P, P, P]
         L.oracle_mc.argtypes = ([I] + _LIG + _GRID + _SF + _REC + [I, P, P, I, P, P, I, P, P, P, P, I, I, I, F, F, F,
@@ -116,6 +117,31 @@ def build_grid_fit(rec_xyz, rec_ext, rad_rec, rad_lig, W, prm12, origin, spacing
     fn(_ptr(rx), _ptr(rt), len(rx), _ptr(rr), _ptr(rl), len(rl), _ptr(Wc), Wc.shape[0], float(origin[0]),
        float(origin[1]), float(origin[2]), float(spacing), nx, ny, nz, _ptr(pr), _ptr(out))
     return out.reshape(len(rl), nx, ny, nz)
+
+
+def inter_exact_fit(X, lig_type, rec_idx, rec_xyz, rec_ext, rad_rec, rad_lig, W, prm12):
+    """Oracle exact pairwise fitted inter energy [L, M] in FP64 (src/adock/search.py:73-89 with FittedPotential.energy,
+    src/adock/fit/terms.py:339-344). X [L, M, A, 3], lig_type [L, A] (outside [0, n_lt): skipped), rec_idx [L],
+    rec_xyz [R, N, 3], rec_ext [R, N] (outside [0, n_rt): skipped), W [n_rt, n_lt, 6], prm12 as build_grid_fit."""
+    c = np.ascontiguousarray
+    X = c(X, np.float32)
+    L, M, A = X.shape[:3]
+    lt = c(lig_type, np.int32).reshape(L, A)
+    ri = c(rec_idx, np.int32).reshape(L)
+    rx = c(rec_xyz, np.float32)
+    re = c(rec_ext, np.int32)
+    R, N = re.shape
+    assert rx.shape == (R, N, 3)
+    Wc = c(W, np.float32)
+    rl = c(rad_lig, np.float32)
+    assert Wc.ndim == 3 and Wc.shape[1] == len(rl) and Wc.shape[2] == 6
+    pr = c(prm12, np.float32)
+    assert pr.shape == (12,)
+    out = np.zeros((L, M), np.float64)
+    lib().lib.oracle_inter_exact_fit_f64(_ptr(X), _ptr(lt), L, M, A, _ptr(ri), _ptr(rx), _ptr(re), R, N,
+                                         _ptr(c(rad_rec, np.float32)), _ptr(rl), len(rl), _ptr(Wc), Wc.shape[0],
+                                         _ptr(pr), _ptr(out))
+    return out
 
 
 class OracleProblem:

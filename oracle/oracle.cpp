@@ -11,7 +11,8 @@
 //   pair_e ........................ src/adock/mckernel.py:44-64 (also scoring.py:18-31, grid.py:40-59)
 //   build_grid .................... src/adock/grid.py:14-71 (tiling of cpukernel.py:102-137 is not needed on a CPU
 //                                    oracle: every point sees every atom, in receptor order)
-//   build_grid_fit ................ src/adock/fit/gridfit.py:13-58 (basis functions as fit/terms.py:233-250)
+//   build_grid_fit ................ src/adock/fit/gridfit.py:13-58 (basis functions as fit/terms.py:239-254)
+//   inter_exact_fit ............... src/adock/search.py:73-89 with FittedPotential.energy, fit/terms.py:339-344
 //   smix / urand / rand_sphere .... src/adock/mckernel.py:71-90
 //   quat_inc ...................... src/adock/mckernel.py:93-105
 //   eval_eg ....................... src/adock/mckernel.py:108-244
@@ -139,19 +140,45 @@ void build_grid(const float* rxyz, const int* rtype, int n_rec, const float* rad
     }
 }
 
+// ---------------------------------------------------------------- fitted basis (src/adock/fit/terms.py:239-254)
+// sum_k W[rt, q, k] * basis_k of one receptor-ligand pair at distance r: gauss1, gauss2, repulsion capped at rep_cap,
+// hydrophobic ramp and h-bond ramp of d = r - rad_rec[rt] - rad_lig[q], and the ionic basis
+// 1 / max(r, ion_rmin)^2 - 1 / cutoff^2. prm: cutoff, g1_off, g1_w, g2_off, g2_w, rep_off, hyd_good, hyd_bad,
+// hb_good, hb_bad, ion_rmin, rep_cap.
+template <class R>
+struct FitBasis {
+    R rc, g1o, g1w, g2o, g2w, repo, hg, hbad, bg, bbad, rmin, cap;
+    explicit FitBasis(const float* prm)
+        : rc(prm[0]), g1o(prm[1]), g1w(prm[2]), g2o(prm[3]), g2w(prm[4]), repo(prm[5]), hg(prm[6]), hbad(prm[7]),
+          bg(prm[8]), bbad(prm[9]), rmin(prm[10]), cap(prm[11]) {}
+    R ion(R r) const {
+        const R rr = std::max(r, rmin);
+        return R(1) / (rr * rr) - R(1) / (rc * rc);
+    }
+    R pair(const float* w, R d, R ion_r) const {
+        R u = (d - g1o) / g1w;
+        R sv = R(w[0]) * r_exp(-u * u);
+        u = (d - g2o) / g2w;
+        sv += R(w[1]) * r_exp(-u * u);
+        const R dr = d - repo;
+        if (dr < 0) sv += R(w[2]) * std::min(dr * dr, cap);
+        sv += R(w[3]) * std::min(std::max((hbad - d) / (hbad - hg), R(0)), R(1));
+        sv += R(w[4]) * std::min(std::max((bbad - d) / (bbad - bg), R(0)), R(1));
+        sv += R(w[5]) * ion_r;
+        return sv;
+    }
+};
+
 // ---------------------------------------------------------------- fitted grid (src/adock/fit/gridfit.py:13-58)
-// out[q, i, j, k] = sum over receptor atoms a within the cutoff of sum_k W[rext_a, q, k] * basis_k, with the basis
-// (gauss1, gauss2, repulsion capped at rep_cap, hydrophobic ramp, h-bond ramp) of d = r - rad_rec[rext_a] - rad_lig[q]
-// and the ionic basis 1 / max(r, ion_rmin)^2 - 1 / cutoff^2. prm: cutoff, g1_off, g1_w, g2_off, g2_w, rep_off,
-// hyd_good, hyd_bad, hb_good, hb_bad, ion_rmin, rep_cap.
+// out[q, i, j, k] = sum over receptor atoms a (ext type in [0, n_rt)) within the cutoff of the FitBasis pair sum with
+// W[rext_a, q].
 template <class R>
 void build_grid_fit(const float* rxyz, const int* rext, int n_rec, const float* rad_rec, const float* rad_lig,
                     int n_lt, const float* W, int n_rt, float ox, float oy, float oz, float spacing, int nx, int ny,
                     int nz, const float* prm, R* out) {
     const long long npts = (long long)nx * ny * nz;
     const float cut2f = prm[0] * prm[0];
-    const R rc = prm[0], g1o = prm[1], g1w = prm[2], g2o = prm[3], g2w = prm[4], repo = prm[5];
-    const R hg = prm[6], hbad = prm[7], bg = prm[8], bbad = prm[9], rmin = prm[10], cap = prm[11];
+    const FitBasis<R> fb(prm);
 #pragma omp parallel for schedule(dynamic, 64)
     for (long long p = 0; p < npts; ++p) {
         const int k = (int)(p % nz), j = (int)((p / nz) % ny), i = (int)(p / ((long long)ny * nz));
@@ -166,24 +193,52 @@ void build_grid_fit(const float* rxyz, const int* rext, int n_rec, const float* 
             if (!(r2f < cut2f)) continue;
             const R dx = dxf, dy = dyf, dz = dzf;
             const R r = std::sqrt(dx * dx + dy * dy + dz * dz);
-            const R rr = std::max(r, rmin);
-            const R ion = R(1) / (rr * rr) - R(1) / (rc * rc);
-            for (int q = 0; q < n_lt; ++q) {
-                const float* w = W + ((long long)ta * n_lt + q) * 6;
-                const R d = r - R(rad_rec[ta]) - R(rad_lig[q]);
-                R u = (d - g1o) / g1w;
-                R sv = R(w[0]) * r_exp(-u * u);
-                u = (d - g2o) / g2w;
-                sv += R(w[1]) * r_exp(-u * u);
-                const R dr = d - repo;
-                if (dr < 0) sv += R(w[2]) * std::min(dr * dr, cap);
-                sv += R(w[3]) * std::min(std::max((hbad - d) / (hbad - hg), R(0)), R(1));
-                sv += R(w[4]) * std::min(std::max((bbad - d) / (bbad - bg), R(0)), R(1));
-                sv += R(w[5]) * ion;
-                e[q] += sv;
-            }
+            const R ion = fb.ion(r);
+            for (int q = 0; q < n_lt; ++q)
+                e[q] += fb.pair(W + ((long long)ta * n_lt + q) * 6, r - R(rad_rec[ta]) - R(rad_lig[q]), ion);
         }
         for (int q = 0; q < n_lt; ++q) out[(long long)q * npts + p] = e[q];
+    }
+}
+
+// ---------------------------------------------------------------- exact fitted inter energy (src/adock/search.py:73-89)
+// out[l, m] = sum over ligand atoms a of pose (l, m) with lig_type in [0, n_lt) and receptor atoms b of set
+// rec_idx[l] with rext in [0, n_rt) and |x_b - x_a|^2 < cutoff^2 (decided in float32 with the fused-multiply-add
+// nesting of the HIP kernel) of the FitBasis pair sum with W[rext_b, lig_type_a] (FittedPotential.energy,
+// src/adock/fit/terms.py:339-344). A complex whose rec_idx lies outside [0, n_sets) gets NaN.
+template <class R>
+void inter_exact_fit(const float* X, const int* lig_type, int L, int M, int A, const int* rec_idx, const float* rxyz,
+                     const int* rext, int n_sets, int NR, const float* rad_rec, const float* rad_lig, int n_lt,
+                     const float* W, int n_rt, const float* prm, R* out) {
+    const float cut2f = prm[0] * prm[0];
+    const FitBasis<R> fb(prm);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (long long lm = 0; lm < (long long)L * M; ++lm) {
+        const int l = (int)(lm / M);
+        const int ri = rec_idx[l];
+        if (ri < 0 || ri >= n_sets) {
+            out[lm] = std::nan("");
+            continue;
+        }
+        const float* rx = rxyz + (long long)ri * NR * 3;
+        const int* re = rext + (long long)ri * NR;
+        R e = 0;
+        for (int a = 0; a < A; ++a) {
+            const int q = lig_type[(long long)l * A + a];
+            if (q < 0 || q >= n_lt) continue;
+            const float* xa = X + (lm * A + a) * 3;
+            for (int b = 0; b < NR; ++b) {
+                const int tb = re[b];
+                if (tb < 0 || tb >= n_rt) continue;
+                const float dxf = rx[3 * b] - xa[0], dyf = rx[3 * b + 1] - xa[1], dzf = rx[3 * b + 2] - xa[2];
+                const float r2f = std::fma(dxf, dxf, std::fma(dyf, dyf, dzf * dzf));
+                if (!(r2f < cut2f)) continue;
+                const R dx = dxf, dy = dyf, dz = dzf;
+                const R r = std::sqrt(dx * dx + dy * dy + dz * dz);
+                e += fb.pair(W + ((long long)tb * n_lt + q) * 6, r - R(rad_rec[tb]) - R(rad_lig[q]), fb.ion(r));
+            }
+        }
+        out[lm] = e;
     }
 }
 
@@ -607,6 +662,14 @@ void oracle_build_grid_fit_f32(const float* rxyz, const int* rext, int n_rec, co
                                float spacing, int nx, int ny, int nz, const float* prm, float* out) {
     build_grid_fit<float>(rxyz, rext, n_rec, rad_rec, rad_lig, n_lt, W, n_rt, ox, oy, oz, spacing, nx, ny, nz, prm,
                           out);
+}
+
+void oracle_inter_exact_fit_f64(const float* X, const int* lig_type, int L, int M, int A, const int* rec_idx,
+                                const float* rxyz, const int* rext, int n_sets, int NR, const float* rad_rec,
+                                const float* rad_lig, int n_lt, const float* W, int n_rt, const float* prm,
+                                double* out) {
+    inter_exact_fit<double>(X, lig_type, L, M, A, rec_idx, rxyz, rext, n_sets, NR, rad_rec, rad_lig, n_lt, W, n_rt,
+                            prm, out);
 }
 
 // energy + gradient for N states; nbr/nbr_n optional (exact mode). Outputs FP64 e [N], g [N, 6+KM], X [N, AM, 3],

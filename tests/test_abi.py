@@ -60,6 +60,14 @@ def test_bucket_validation_without_gpu():
     host = ctypes.cast(buf, ctypes.c_void_p).value       # never dereferenced: the call fails its checks first
     bad_rt = [None, None, 0, None, host, 1, None, 300] + fit[8:15] + [host, host, None]
     assert L.adock_build_grid_fit(*bad_rt) == -4
+    # exact fitted inter energy: empty pose sets are a no-op; null pointers, negative sizes, no receptor set and
+    # receptor atoms without their tables are rejected before any launch
+    ex = [host, host, 2, 3, 5, host, None, None, 1, 0, None, None, 0, None, 0, host, host, None]
+    assert L.adock_inter_exact_fit(*(ex[:2] + [0] + ex[3:])) == 0
+    assert L.adock_inter_exact_fit(*(ex[:16] + [None, None])) == -1
+    assert L.adock_inter_exact_fit(*(ex[:2] + [-1] + ex[3:])) == -1
+    assert L.adock_inter_exact_fit(*(ex[:8] + [0] + ex[9:])) == -1
+    assert L.adock_inter_exact_fit(*(ex[:9] + [7] + ex[10:])) == -1
 
 
 def test_facade_rejects_mismatched_chain_buffers():

@@ -16,8 +16,8 @@ from adock_amd.fit.terms import (N_INTER, TERM_NAMES, FittedPotential, Spec, def
                                  intra_features)
 from adock_amd.grid import build_grid
 from adock_amd.search import dock_problem_kernel
-from helpers import (CFG, GRID_FLOOR, GRID_REL, IDX, SPACING, OracleKernel, cpu_fitted_problem, random_receptor,
-                     rel_err)
+from helpers import (CFG, EG_FLOOR, EG_REL, GRID_FLOOR, GRID_REL, IDX, SPACING, OracleKernel, cpu_fitted_problem,
+                     random_receptor, rel_err)
 
 pytestmark = pytest.mark.gpu
 NXS = len(CFG["_xs_names"])
@@ -92,6 +92,89 @@ def test_fit_grid_rejects_bad_types():
     pot = FittedPotential(spec, default_weights(spec), np.array([0]), "cuda")
     with pytest.raises(Exception):
         build_grid_fit(np.zeros((2, 3), np.float32), np.array([0, spec.n_rt]), pot, np.zeros(3), 0.5, 4)
+
+
+def _exact_case(seed, L, M, A, n_rec, spec, lig_list):
+    """Random complexes around two receptor sets of different sizes (padded slots -1), hydrogens (-1) in every ligand,
+    poses spread over the receptor: (X, lig_types, rec_idx, rec_xyz, rec_ext)."""
+    rng = np.random.default_rng(seed)
+    N = max(n_rec)
+    rx = np.zeros((len(n_rec), N, 3), np.float32)
+    re = -np.ones((len(n_rec), N), np.int32)
+    for s, n in enumerate(n_rec):
+        xyz, _ = random_receptor(n, seed=seed + s, extent=16.0)
+        rx[s, :n] = xyz
+        re[s, :n] = _types(rng, spec, n)
+    rec_idx = rng.integers(0, len(n_rec), L).astype(np.int32)
+    lt = rng.integers(0, len(lig_list), (L, A)).astype(np.int32)
+    lt[rng.random((L, A)) < 0.2] = -1
+    X = rng.uniform(-2.0, 18.0, (L, M, A, 3)).astype(np.float32)
+    return X, lt, rec_idx, rx, re
+
+
+@pytest.mark.parametrize("L,M,A", [(3, 37, 70), (5, 8, 23)])
+def test_inter_exact_fit_matches_oracle(L, M, A):
+    """adock_inter_exact_fit against the FP64 oracle: pose counts not a multiple of the four poses per workgroup,
+    more than 64 atom slots (two lane slices) and a small ligand, two receptor sets, random weights."""
+    print(f"[gpu] exact fitted inter energy vs oracle L={L} M={M} A={A}", flush=True)
+    from adock_amd.fit.gridfit import inter_exact_fit
+    spec = Spec(CFG, "vinardo", buried_threshold=1, rep_cap=1.5, sc_radius_scale=0.9)
+    rng = np.random.default_rng(11)
+    w = rng.normal(size=len(TERM_NAMES)).astype(np.float32)
+    lig_list = np.unique(rng.integers(0, spec.n_lt, 9))
+    pot = FittedPotential(spec, w, lig_list, "cuda")
+    X, lt, ri, rx, re = _exact_case(L + M, L, M, A, (600, 420), spec, lig_list)
+    g = inter_exact_fit(torch.as_tensor(X).cuda(), lt, ri, rx, re, pot).cpu().numpy()
+    o = oracle.inter_exact_fit(X, lt, ri, rx, re, spec.rec_radius, pot.radius.cpu().numpy(), pot.W.cpu().numpy(),
+                               spec.grid_prm())
+    assert g.shape == o.shape == (L, M) and np.isfinite(o).all()
+    err = rel_err(g, o, EG_FLOOR, EG_REL)
+    print(f"  max rel err {err:.2e}, |E| up to {np.abs(o).max():.1f}", flush=True)
+    assert err < EG_REL
+    again = inter_exact_fit(torch.as_tensor(X).cuda(), lt, ri, rx, re, pot).cpu().numpy()
+    np.testing.assert_array_equal(g, again)
+
+
+def test_inter_exact_fit_equals_grid_at_grid_points():
+    """Atoms placed on grid points: the exact kernel's pose energy equals the sum of the fitted grid kernel's values at
+    those points (the two kernels share the basis; the reference checks grid against exact the same way,
+    tests/test_fit_features.py:103-123), and Problem.inter_exact on cuda poses takes the kernel."""
+    print("[gpu] exact fitted inter energy vs fitted grid at grid points", flush=True)
+    from adock_amd.fit.gridfit import inter_exact_fit
+    spec = Spec(CFG, "vina", buried_threshold=1)
+    rng = np.random.default_rng(2)
+    w = rng.normal(size=len(TERM_NAMES)).astype(np.float32)
+    lig_list = np.array([IDX["C_H"], IDX["N_D"] + NXS, IDX["O_A"] + 2 * NXS, IDX["O_A"]])
+    pot = FittedPotential(spec, w, lig_list, "cuda")
+    xyz, _ = random_receptor(500, seed=8, extent=14.0)
+    rext = _types(rng, spec, len(xyz))
+    n = 29
+    G = build_grid_fit(xyz, rext, pot, np.zeros(3), SPACING, n).cpu().numpy()
+    M, A = 16, 12
+    ijk = rng.integers(0, n, (M, A, 3))
+    X = (ijk * np.float32(SPACING)).astype(np.float32)[None]
+    lt = rng.integers(0, len(lig_list), (1, A)).astype(np.int32)
+    lt[0, 5] = -1
+    e = inter_exact_fit(torch.as_tensor(X).cuda(), lt, np.array([0], np.int32), xyz[None],
+                        rext[None].astype(np.int32), pot).cpu().numpy()
+    want = np.array([sum(float(G[lt[0, a], i, j, k]) for a, (i, j, k) in enumerate(ijk[m]) if lt[0, a] >= 0)
+                     for m in range(M)])
+    np.testing.assert_allclose(e[0], want, rtol=1e-5, atol=1e-5)
+
+
+def test_problem_inter_exact_uses_the_kernel():
+    """The fitted Problem's inter_exact on cuda poses (the kernel) equals the torch pairwise sum on the CPU."""
+    print("[gpu] Problem.inter_exact kernel vs torch", flush=True)
+    spec, w = load_fitted(DATA / "adock_fitted.json", CFG)
+    cxs, _ = _complexes(spec)
+    prob = build_problem(cxs, spec, w, CFG, "cuda")
+    rng = np.random.default_rng(3)
+    X = torch.as_tensor(rng.normal(0, 4, (2, 6, prob.lb.A, 3)).astype(np.float32)) + torch.as_tensor(
+        np.asarray(cxs[0]["center"], np.float32))
+    got = prob.inter_exact(X.cuda()).cpu().numpy()
+    cpu = cpu_fitted_problem(cxs, spec, w)
+    want = cpu.inter_exact(X.double()).numpy()
+    assert rel_err(got, want, EG_FLOOR, EG_REL) < EG_REL
 
 
 def _complexes(spec):
