@@ -274,7 +274,7 @@ int adock_set_line_search_width(int ns) {
 }
 int adock_get_line_search_width(void) { return get_line_search_width(); }
 int adock_set_exact_staging(int on) {
-    if (set_exact_staging(on)) return fail(ADOCK_E_ARG, "exact_staging must be 0 or 1");
+    if (set_exact_staging(on)) return fail(ADOCK_E_ARG, "exact_staging must be 0, 1 or 2");
     return 0;
 }
 int adock_get_exact_staging(void) { return get_exact_staging(); }

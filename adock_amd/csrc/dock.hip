@@ -82,6 +82,7 @@ struct WaveLDS {
     int P128, trash;
     float4* RN4;          // exact mode: staged receptor atoms (xyz, radius) of the chain's list, RNF their type flags
     int* RNF;
+    unsigned* MK;         // exact mode: per 32 staged atoms, one bit mask per lane (column per lane); NULL: no masks
     int nrs;              // staging capacity (0: none)
 };
 
@@ -104,6 +105,10 @@ struct ChainCtx {
     int exact;
     const float* rrx; const int* rrt; int rn; const int* nb; int nbn;
     bool staged;          // exact mode: the receptor atoms of the loop sit in LDS (w.RN4 / w.RNF)
+    bool masked;          // ... and each atom walks only the staged atoms its masks (w.MK) mark
+    // masks: built from this atom's position (mx0..2) when mvalid; rebuilt once any atom has moved EXACT_MARGIN
+    mutable bool mvalid;
+    mutable float mx0, mx1, mx2;
     const float* sfrad; const int* sfhyd; const int* sfdon; const int* sfacc;
 };
 
@@ -122,6 +127,7 @@ DEVI WaveLDS carve_lds(char* base, int P_MAX, int DEG_MAX = 0) {
     w.FD = nullptr;
     w.RN4 = nullptr;
     w.RNF = nullptr;
+    w.MK = nullptr;
     w.nrs = 0;
     w.inc = nullptr;
     w.pdst = nullptr;
@@ -135,13 +141,30 @@ DEVI WaveLDS carve_lds(char* base, int P_MAX, int DEG_MAX = 0) {
     return w;
 }
 
-// exact-mode staging area of a wave: nrs float4 atoms + nrs int flags, after the wave's regular region
-__host__ __device__ inline int stage_bytes(int nrs) { return 20 * nrs; }
-DEVI void attach_stage(WaveLDS& w, char* base, int nrs) {
+// exact-mode staging area of a wave, after the wave's regular region: nrs float4 atoms + nrs int flags (+ with masks
+// nrs / 32 rows of 64 lane masks); nrs is a multiple of 64
+__host__ __device__ inline int stage_bytes(int nrs, int masks) { return nrs * (masks ? 28 : 20); }
+DEVI void attach_stage(WaveLDS& w, char* base, int nrs, int masks) {
     if (nrs <= 0) return;
     w.RN4 = reinterpret_cast<float4*>(base);
     w.RNF = reinterpret_cast<int*>(w.RN4 + nrs);
+    w.MK = masks ? reinterpret_cast<unsigned*>(w.RNF + nrs) : nullptr;
     w.nrs = nrs;
+}
+
+// Masks of the exact loop cover the receptor atoms within cutoff + EXACT_MARGIN (+ 0.01 A against rounding) of where
+// the atom was when they were built; while no atom has moved more than EXACT_MARGIN since, every receptor atom within
+// the cutoff of an atom is marked, and the masked atoms are visited in list order: the sums equal the full loop's.
+constexpr float EXACT_MARGIN = 1.0f;
+
+// one receptor atom's exact term for an atom with r2 < cutoff^2 (offset rx, ry, rz); every exact loop uses it
+DEVI void exact_add(const ChainCtx& c, const SFConst& sf, float rx, float ry, float rz, float r2, float rad_b,
+                    int flg_b, float& el, float& gx, float& gy, float& gz) {
+    const float r = fast_sqrt(r2);
+    float de;
+    el += pair_e(r - c.rad - rad_b, pair_flags(c.flg, flg_b), sf, de);
+    const float sc = de * fast_rcp(fmaxf(r, 1e-6f));
+    gx += sc * rx; gy += sc * ry; gz += sc * rz;
 }
 
 template <int KM, bool DENSE = false, bool EX = false>
@@ -242,7 +265,8 @@ DEVI ChainCtx setup_chain(int l, int lane, const LigArgs& L, const GridArgs& G, 
         c.D8 = (int)wave_max_f((float)c.ic8);
     }
     c.exact = EX ? 1 : 0;
-    c.nb = nullptr; c.nbn = 0; c.staged = false;
+    c.nb = nullptr; c.nbn = 0; c.staged = false; c.masked = false;
+    c.mvalid = false; c.mx0 = c.mx1 = c.mx2 = 0.f;
     if constexpr (EX) {
         c.rrx = R.rxyz + (long long)ri * R.RN_MAX * 3;
         c.rrt = R.rtyp + (long long)ri * R.RN_MAX;
@@ -442,18 +466,46 @@ DEVI float eval_eg(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, int l
         } else {
             const float cut2 = sf.cut * sf.cut;
             const int nloop = c.nb ? c.nbn : c.rn;
-            if (c.staged) {
+            if (c.masked) {
+                // masks stale (first evaluation, or an atom moved more than the margin): rebuild from here
+                const float ddx = X0 - c.mx0, ddy = X1 - c.mx1, ddz = X2 - c.mx2;
+                const bool moved = ddx * ddx + ddy * ddy + ddz * ddz > EXACT_MARGIN * EXACT_MARGIN;
+                if (!c.mvalid || __ballot(moved) != 0ull) {
+                    const float lim = sf.cut + EXACT_MARGIN + 0.01f, lim2 = lim * lim;
+                    for (int cb = 0; cb < nloop; cb += 32) {
+                        const int ne = min(32, nloop - cb);
+                        unsigned m = 0u;
+                        for (int b = 0; b < ne; ++b) {
+                            const float4 a = w.RN4[cb + b];
+                            const float rx = X0 - a.x, ry = X1 - a.y, rz = X2 - a.z;
+                            if (rx * rx + ry * ry + rz * rz < lim2) m |= 1u << b;
+                        }
+                        w.MK[(cb >> 5) * WAVE + lane] = m;
+                    }
+                    c.mx0 = X0; c.mx1 = X1; c.mx2 = X2;
+                    c.mvalid = true;
+                }
+                // the marked atoms of each 32, in list order
+                for (int cb = 0; cb < nloop; cb += 32) {
+                    unsigned m = w.MK[(cb >> 5) * WAVE + lane];
+                    while (m != 0u) {
+                        const int bb = cb + __builtin_ctz(m);
+                        m &= m - 1u;
+                        const float4 a = w.RN4[bb];
+                        const float rx = X0 - a.x, ry = X1 - a.y, rz = X2 - a.z;
+                        const float r2 = rx * rx + ry * ry + rz * rz;
+                        if (r2 >= cut2) continue;
+                        exact_add(c, sf, rx, ry, rz, r2, a.w, w.RNF[bb], el, gx, gy, gz);
+                    }
+                }
+            } else if (c.staged) {
                 // same terms, in the same order, from the atoms staged in LDS (broadcast reads)
                 for (int bb = 0; bb < nloop; ++bb) {
                     const float4 a = w.RN4[bb];
                     const float rx = X0 - a.x, ry = X1 - a.y, rz = X2 - a.z;
                     const float r2 = rx * rx + ry * ry + rz * rz;
                     if (r2 >= cut2) continue;
-                    const float r = fast_sqrt(r2);
-                    float de;
-                    el += pair_e(r - c.rad - a.w, pair_flags(c.flg, w.RNF[bb]), sf, de);
-                    const float sc = de * fast_rcp(fmaxf(r, 1e-6f));
-                    gx += sc * rx; gy += sc * ry; gz += sc * rz;
+                    exact_add(c, sf, rx, ry, rz, r2, a.w, w.RNF[bb], el, gx, gy, gz);
                 }
             } else
             for (int bb = 0; bb < nloop; ++bb) {
@@ -462,12 +514,8 @@ DEVI float eval_eg(const ChainCtx& c, const SFConst& sf, const WaveLDS& w, int l
                 const float r2 = rx * rx + ry * ry + rz * rz;
                 if (r2 >= cut2) continue;
                 const int tb = c.rrt[b];
-                const float r = fast_sqrt(r2);
-                const int fb = c.sfhyd[tb] | (c.sfdon[tb] << 1) | (c.sfacc[tb] << 2);
-                float de;
-                el += pair_e(r - c.rad - c.sfrad[tb], pair_flags(c.flg, fb), sf, de);
-                const float sc = de * fast_rcp(fmaxf(r, 1e-6f));
-                gx += sc * rx; gy += sc * ry; gz += sc * rz;
+                exact_add(c, sf, rx, ry, rz, r2, c.sfrad[tb], c.sfhyd[tb] | (c.sfdon[tb] << 1) | (c.sfacc[tb] << 2),
+                          el, gx, gy, gz);
             }
         }
         const float l0 = fmaxf(c.lo0 - X0, 0.f), h0 = fmaxf(X0 - c.hi0, 0.f);
@@ -736,6 +784,8 @@ DEVI int run_chain(int ch, int lane, const WaveLDS& w, const LigArgs& L, const G
         // the chain's receptor atoms (neighbour list or whole receptor) into LDS once, when they fit the staging area
         const int nloop = c.nb ? c.nbn : c.rn;
         c.staged = w.nrs > 0 && nloop <= w.nrs;
+        c.masked = c.staged && w.MK != nullptr;
+        c.mvalid = false;
         if (c.staged) {
             for (int bb = lane; bb < nloop; bb += WAVE) {
                 const int b = c.nb ? c.nb[bb] : bb;
@@ -799,9 +849,9 @@ __global__ __launch_bounds__(256) void mc_kernel(LigArgs L, GridArgs G, SFArgs S
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
     const int ch = blockIdx.x * (blockDim.x / WAVE) + wid;
     if (ch >= n_chains) return;
-    const int wl = wave_lds_bytes(KM, L.P_MAX), wb = wl + stage_bytes(M.NBS);
+    const int wl = wave_lds_bytes(KM, L.P_MAX), wb = wl + stage_bytes(M.NBS, M.NBM);
     WaveLDS w = carve_lds<KM>(lds + wid * wb, L.P_MAX);
-    attach_stage(w, lds + wid * wb + wl, M.NBS);
+    attach_stage(w, lds + wid * wb + wl, M.NBS, M.NBM);
     run_chain<KM, 1, false, EX>(ch, lane, w, L, G, S, R, chain_lig, M);
 }
 
@@ -814,9 +864,9 @@ __global__ __launch_bounds__(256) void mc_spec_kernel(LigArgs L, GridArgs G, SFA
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
     const int ch = blockIdx.x;
     if (ch >= n_chains) return;
-    const int wl = wave_lds_bytes_dense(KM, L.P_MAX, L.DEG_MAX), wb = wl + stage_bytes(M.NBS);
+    const int wl = wave_lds_bytes_dense(KM, L.P_MAX, L.DEG_MAX), wb = wl + stage_bytes(M.NBS, M.NBM);
     WaveLDS w = carve_lds<KM, true>(lds + wid * wb, L.P_MAX, L.DEG_MAX);
-    attach_stage(w, lds + wid * wb + wl, M.NBS);
+    attach_stage(w, lds + wid * wb + wl, M.NBS, M.NBM);
     run_chain<KM, NS, true, EX>(ch, lane, w, L, G, S, R, chain_lig, M, wid, carve_spec(lds + NS * wb));
 }
 
@@ -899,7 +949,7 @@ inline int fit_waves(int W, int KM, int P_MAX, int extra = 0) {
 // exact mode: receptor atoms staged per wave: the longest neighbour list (or the whole receptor), padded to 64, at
 // most STAGE_MAX; a chain whose list is longer than the staging area reads its atoms from memory
 constexpr int STAGE_MAX = 1024;
-int g_exact_staging = 1;
+int g_exact_staging = 2;   // 0: read from memory, 1: staged, 2: staged with per-atom masks
 inline int exact_stage(const McArgs& M, const RecArgs& R) {
     const int n = ((M.nbr ? M.NB_MAX : R.RN_MAX) + WAVE - 1) & ~(WAVE - 1);
     return g_exact_staging && n > 0 ? min(n, STAGE_MAX) : 0;
@@ -936,7 +986,8 @@ int launch_mc_tx(const LigArgs& L, const GridArgs& G, const SFArgs& S, const Rec
                  int n_chains, const McArgs& M0, hipStream_t stream) {
     McArgs M = M0;
     M.NBS = EX ? exact_stage(M, R) : 0;
-    const int extra = stage_bytes(M.NBS);
+    M.NBM = M.NBS > 0 && g_exact_staging == 2 ? 1 : 0;
+    const int extra = stage_bytes(M.NBS, M.NBM);
     const int ns = line_search_width(n_chains, KM, L.P_MAX, L.DEG_MAX, extra);
     if (ns > 1) {
         const size_t shm = (size_t)ns * (wave_lds_bytes_dense(KM, L.P_MAX, L.DEG_MAX) + extra) + SPEC_LDS_BYTES;
@@ -1255,7 +1306,7 @@ int set_line_search_width(int ns) {
 }
 int get_line_search_width() { return g_line_search_width; }
 int set_exact_staging(int on) {
-    if (on != 0 && on != 1) return -1;
+    if (on < 0 || on > 2) return -1;
     g_exact_staging = on;
     return 0;
 }

@@ -20,6 +20,7 @@ struct McArgs {
     int init_opt, mc_steps, bfgs_iters; float temp, amp_pos, amp_rot; unsigned long long seed; int step_offset;
     unsigned long long* eval_count;
     int NBS = 0;   // exact mode: receptor atoms staged in each wave's LDS (set by the launcher; 0 = read from memory)
+    int NBM = 0;   // exact mode: per-atom masks over the staged atoms (set by the launcher)
 };
 // work queue of the persistent chain kernel: order [n] (chain indices, may be NULL = identity), counter (1 int, 0 at
 // launch), done [L] (finished chains per complex, 0 at launch), lig_chains [L] (chains per complex), times [1 + L]

@@ -212,10 +212,12 @@ int adock_get_chains_per_block(void);
 int adock_set_line_search_width(int waves);
 int adock_get_line_search_width(void);
 
-/* Exact (pairwise) mode of adock_mc: 1 (default) stages each chain's receptor atoms (its neighbour list, or the whole
- * receptor) in LDS once per launch when they number at most 1,024 (longer lists are read from memory), so the
- * per-atom loop reads LDS broadcasts instead of dependent global loads; 0 always reads them from memory. Results are the same bit for bit. No counterpart in the
- * reference. Returns 0, or ADOCK_E_ARG. */
+/* Exact (pairwise) mode of adock_mc: 1 stages each chain's receptor atoms (its neighbour list, or the whole receptor)
+ * in LDS once per launch when they number at most 1,024 (longer lists are read from memory), so the per-atom loop
+ * reads LDS broadcasts instead of dependent global loads; 2 (default) also keeps, per atom, bit masks of the staged
+ * atoms within cutoff + 1 A of where the atom was when they were built (rebuilt once any atom has moved 1 A), and each
+ * atom visits only those; 0 always reads the atoms from memory. Results are the same bit for bit in all three modes.
+ * No counterpart in the reference. Returns 0, or ADOCK_E_ARG. */
 int adock_set_exact_staging(int on);
 int adock_get_exact_staging(void);
 

@@ -48,8 +48,10 @@ def test_bucket_validation_without_gpu():
     assert L.adock_set_chains_per_block(5) == -1
     assert L.adock_set_line_search_width(3) == -1
     assert L.adock_set_line_search_width(0) == 0
-    assert L.adock_set_exact_staging(2) == -1
+    assert L.adock_get_exact_staging() == 2
+    assert L.adock_set_exact_staging(3) == -1 and L.adock_set_exact_staging(-1) == -1
     assert L.adock_set_exact_staging(1) == 0 and L.adock_get_exact_staging() == 1
+    assert L.adock_set_exact_staging(2) == 0
     assert L.adock_version().startswith(b"adock_hip")
     # fitted grid builder: null pointers and more receptor ext types than the LDS table holds are rejected,
     # empty grids are a no-op

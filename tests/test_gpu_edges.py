@@ -1,7 +1,7 @@
 """Edge cases on the GPU: grids of an empty receptor (both builders), the larger torsion buckets (KM = 24, 32) against
 the FP64 oracle, MC invariants on a 31-torsion ligand, the speculative search bit-identical where the LDS budget
 lowers its width (64 atoms, 1,761 pairs), an empty chain list, and exact mode with its receptor neighbour lists staged
-in LDS against read from memory."""
+in LDS and walked through per-atom masks, staged and walked whole, and read from memory (all bit-identical)."""
 import numpy as np
 import pytest
 import torch
@@ -105,9 +105,10 @@ def test_empty_chain_list_is_a_no_op():
 
 @pytest.mark.parametrize("width", [1, 4])
 def test_exact_staging_bit_identical(width):
-    """Exact mode over receptor neighbour lists: with the lists staged in LDS (default) and read from memory, the
-    chains end bit for bit the same, in the one-wave kernel and in the speculative one."""
-    print(f"[gpu] exact mode, staged vs memory neighbour lists, width {width}", flush=True)
+    """Exact mode over receptor neighbour lists: with the lists staged in LDS and walked through per-atom masks
+    (default), staged and walked whole, and read from memory, the chains end bit for bit the same, in the one-wave
+    kernel and in the speculative one; enough MC steps that atoms move past the masks' margin and they are rebuilt."""
+    print(f"[gpu] exact mode, masked / staged / memory neighbour lists, width {width}", flush=True)
     w, prob = workload.single("small", seed=0)
     kp = KernelProblem(prob)
     N = 24
@@ -117,21 +118,24 @@ def test_exact_staging_bit_identical(width):
     assert 0 < nbr.shape[1] <= 1024
     L = _lib.lib()
 
-    def run(staging):
+    def run(staging, steps, nb):
         _lib.check(L.adock_set_exact_staging(staging))
         _lib.check(L.adock_set_line_search_width(width))
         try:
             cur, best = st.clone(), st.clone()
             ce, be = torch.zeros(N).cuda(), torch.full((N,), float("inf")).cuda()
-            kp.run_mc(cl, torch.arange(N, dtype=torch.int32).cuda(), cur, ce, best, be, True, 2, 15, 1.2, 2.0, 2.0, 3,
-                      0, exact=True, nbr=nbr, nbr_n=nbr_n)
+            kw = dict(nbr=nbr, nbr_n=nbr_n) if nb else {}
+            kp.run_mc(cl, torch.arange(N, dtype=torch.int32).cuda(), cur, ce, best, be, True, steps, 15, 1.2, 2.0,
+                      2.0, 3, 0, exact=True, **kw)
             torch.cuda.synchronize()
             return [x.cpu().numpy() for x in (cur, ce, best, be)]
         finally:
-            _lib.check(L.adock_set_exact_staging(1))
+            _lib.check(L.adock_set_exact_staging(2))
             _lib.check(L.adock_set_line_search_width(0))
 
-    a, b = run(1), run(0)
-    for x, y in zip(a, b):
-        np.testing.assert_array_equal(x, y)
-    assert np.isfinite(a[3]).all()
+    for steps, nb in ((2, True), (6, True), (2, False)):
+        a, b, c = run(2, steps, nb), run(1, steps, nb), run(0, steps, nb)
+        for x, y, z in zip(a, b, c):
+            np.testing.assert_array_equal(x, y)
+            np.testing.assert_array_equal(x, z)
+        assert np.isfinite(a[3]).all()
