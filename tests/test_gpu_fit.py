@@ -162,6 +162,37 @@ def test_inter_exact_fit_equals_grid_at_grid_points():
     np.testing.assert_allclose(e[0], want, rtol=1e-5, atol=1e-5)
 
 
+def test_inter_exact_fit_raw_abi_edges():
+    """Through the C ABI: a receptor set with no atoms gives zero energies, and a complex whose receptor-set index lies
+    outside [0, n_sets) gets NaN from the kernel's own check (the Python wrapper rejects such an index before)."""
+    print("[gpu] exact fitted inter energy: empty receptor, receptor index out of range", flush=True)
+    from adock_amd import _lib
+    spec = Spec(CFG, "vina")
+    pot = FittedPotential(spec, default_weights(spec), np.array([0, IDX["N_D"]]), "cuda")
+    L, M, A = 2, 5, 3
+    X = torch.zeros(L, M, A, 3, device="cuda")
+    lt = torch.zeros(L, A, dtype=torch.int32, device="cuda")
+    rx = torch.zeros(1, 4, 3, device="cuda")
+    re = torch.zeros(1, 4, dtype=torch.int32, device="cuda")
+    prm = torch.as_tensor(np.asarray(spec.grid_prm(), np.float32), device="cuda")
+    rr = torch.as_tensor(spec.rec_radius, dtype=torch.float32, device="cuda")
+    W, rl = pot.W.float().contiguous(), pot.radius.float().contiguous()
+    lib = _lib.lib()
+
+    def call(ri, n_rec):
+        out = torch.full((L, M), 7.0, device="cuda")
+        ri_t = torch.as_tensor(ri, dtype=torch.int32, device="cuda")
+        _lib.check(lib.adock_inter_exact_fit(_lib.ptr(X), _lib.ptr(lt), L, M, A, _lib.ptr(ri_t), _lib.ptr(rx),
+                                             _lib.ptr(re), 1, n_rec, _lib.ptr(rr), _lib.ptr(rl), pot.TL, _lib.ptr(W),
+                                             spec.n_rt, _lib.ptr(prm), _lib.ptr(out), None))
+        torch.cuda.synchronize()
+        return out.cpu().numpy()
+
+    assert np.array_equal(call([0, 0], 0), np.zeros((L, M), np.float32))
+    bad = call([0, 3], 4)
+    assert np.isfinite(bad[0]).all() and np.isnan(bad[1]).all()
+
+
 def test_problem_inter_exact_uses_the_kernel():
     """The fitted Problem's inter_exact on cuda poses (the kernel) equals the torch pairwise sum on the CPU."""
     print("[gpu] Problem.inter_exact kernel vs torch", flush=True)
