@@ -40,9 +40,9 @@ def lib():
         L.adock_mc_counted.argtypes = (LIG + GRID + SF + REC +
                                        [I, P, P, I, P, P, I, P, P, P, P, I, I, I, F, F, F, U64, I, P, P])
         L.adock_mc_batch.argtypes = (LIG + GRID + SF + REC +
-                                     [I, P, P, I, P, P, I, P, P, P, P, I, I, I, F, F, F, U64, I, P, P, P, P, P, I, P,
-                                      P])
-        L.adock_batch_reset.argtypes = [P, P, I, P, P]
+                                     [I, P, P, I, P, P, I, P, P, P, P, I, I, I, F, F, F, U64, I, P, P, I, P, P, P, P, I,
+                                      P, P])
+        L.adock_batch_reset.argtypes = [P, I, P, I, P, P]
         LL = ctypes.c_longlong
         L.adock_mc_step.argtypes = (LIG + GRID + SF + REC +
                                     [P, P, I, P, P, P, P, I, I, I, F, F, F, U64, P, P, P, P, I, I, F, F, F, F, P, P])

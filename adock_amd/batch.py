@@ -1,8 +1,9 @@
 """Batched screening: many ligands (and receptor grid sets) in one persistent launch.
 
-BatchRunner drives adock_mc_batch: all chains of all complexes in a single launch whose wavefronts pull chains from a
-device work queue, longest chains first (cost model below), so uneven ligands (10-60 atoms, 0-15 torsions) balance
-without host round trips. The queue reset and the launch are both kernels on the current stream, so the pair can be
+BatchRunner drives adock_mc_batch: all chains of all complexes in a single launch whose wavefronts pull chains from
+device work queues, longest chains first (cost model below), so uneven ligands (10-60 atoms, 0-15 torsions) balance
+without host round trips. There is one queue per XCD holding the complexes of a few receptors (queue_partition), so
+the grids a die's chains read stay in that die's L2; a wave whose queue is drained serves the others. The queue reset and the launch are both kernels on the current stream, so the pair can be
 captured once in a HIP graph (torch.cuda.CUDAGraph) and replayed. Per-complex completion stamps (100 MHz real-time
 clock) give per-complex latency from inside the kernel.
 
@@ -15,6 +16,7 @@ import torch
 from . import _lib
 
 RT_CLOCK_HZ = 100e6   # s_memrealtime frequency
+XCDS = 8              # MI355X: 8 accelerator dies (XCDs), each with its own L2; workgroups are dealt to them in turn
 
 
 def chain_cost(preps, iters_rule=True):
@@ -29,10 +31,51 @@ def chain_cost(preps, iters_rule=True):
     return np.asarray(out, np.float64)
 
 
+def queue_partition(cost, group, n_queues):
+    """(order [N] int32, queue_start [n_queues + 1] int32) for adock_mc_batch: the N chains (per-chain cost, group =
+    receptor set of each chain's complex) in n_queues queues of about equal cost. Groups are kept whole where they fit
+    one queue's share (so a queue, i.e. one XCD, touches the grids of a few receptors only) and otherwise dealt over
+    ceil(cost / share) queues; groups go to the least-loaded queue, largest first; each queue runs longest chains
+    first. n_queues <= 1: one queue, longest first."""
+    cost = np.asarray(cost, np.float64)
+    group = np.asarray(group)
+    N = len(cost)
+    if n_queues <= 1 or N == 0:
+        return np.argsort(-cost, kind="stable").astype(np.int32), np.array([0, N], np.int32)
+    share = cost.sum() / n_queues
+    units = []
+    for g in np.unique(group):
+        idx = np.nonzero(group == g)[0]
+        gc = float(cost[idx].sum())
+        k = min(len(idx), max(1, int(np.ceil(gc / share - 1e-9)))) if share > 0 else 1
+        if k == 1:
+            units.append((gc, idx))
+            continue
+        srt = idx[np.argsort(-cost[idx], kind="stable")]
+        for j in range(k):
+            part = srt[j::k]
+            units.append((float(cost[part].sum()), part))
+    units.sort(key=lambda u: -u[0])
+    load = np.zeros(n_queues)
+    members = [[] for _ in range(n_queues)]
+    for c, idx in units:
+        q = int(np.argmin(load))
+        load[q] += c
+        members[q].append(idx)
+    order, start = [], [0]
+    for m in members:
+        idx = np.concatenate(m) if m else np.zeros(0, np.int64)
+        idx = idx[np.argsort(-cost[idx], kind="stable")]
+        order.append(idx)
+        start.append(start[-1] + len(idx))
+    return np.concatenate(order).astype(np.int32), np.array(start, np.int32)
+
+
 class BatchRunner:
-    def __init__(self, kp, chain_lig, n_complexes=None, cost=None):
+    def __init__(self, kp, chain_lig, n_complexes=None, cost=None, groups=None, queues=XCDS):
         """kp: KernelProblem; chain_lig [N] (complex of each chain, host or device); cost: per-complex cost (default
-        chain_cost of kp's ligands)."""
+        chain_cost of kp's ligands); groups: per-complex receptor set (default kp's rec_idx); queues: work queues, one
+        per XCD (queue_partition)."""
         self.kp = kp
         dev = kp.device
         cl = torch.as_tensor(chain_lig).to(dtype=torch.int32)
@@ -40,11 +83,14 @@ class BatchRunner:
         self.L = int(n_complexes if n_complexes is not None else int(cl.max()) + 1)
         self.chain_lig = cl.to(dev).contiguous()
         cost = np.asarray(cost if cost is not None else kp.cost, np.float64)
-        c = cost[cl.cpu().numpy()]
-        order = np.argsort(-c, kind="stable").astype(np.int32)
+        cl_h = cl.cpu().numpy()
+        grp = np.asarray(groups if groups is not None else kp.grid["rec_idx"].cpu().numpy())
+        order, qstart = queue_partition(cost[cl_h], grp[cl_h], int(queues))
         self.order = torch.as_tensor(order, device=dev)
-        self.lig_chains = torch.as_tensor(np.bincount(cl.cpu().numpy(), minlength=self.L).astype(np.int32), device=dev)
-        self.counter = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.qstart = torch.as_tensor(qstart, device=dev)
+        self.nq = len(qstart) - 1
+        self.lig_chains = torch.as_tensor(np.bincount(cl_h, minlength=self.L).astype(np.int32), device=dev)
+        self.counter = torch.zeros(max(self.nq, 1), dtype=torch.int32, device=dev)
         self.done = torch.zeros(self.L, dtype=torch.int32, device=dev)
         self.times = torch.zeros(1 + self.L, dtype=torch.int64, device=dev)
         self.blocks = 0
@@ -56,7 +102,8 @@ class BatchRunner:
         L = _lib.lib()
         st = _lib.stream_of(kp.device)
         kp.check_chain_buffers(self.N, cur, cur_e, best, best_e)
-        _lib.check(L.adock_batch_reset(_lib.ptr(self.counter), _lib.ptr(self.done), self.L, _lib.ptr(self.times), st))
+        _lib.check(L.adock_batch_reset(_lib.ptr(self.counter), int(self.counter.numel()), _lib.ptr(self.done), self.L,
+                                       _lib.ptr(self.times), st))
         ci = chain_id if chain_id.dtype == torch.int32 else chain_id.to(torch.int32)
         self._ci = ci.contiguous()
         if self._ci.shape != (self.N,):
@@ -65,7 +112,7 @@ class BatchRunner:
                                    _lib.ptr(cur), _lib.ptr(cur_e), _lib.ptr(best), _lib.ptr(best_e), int(bool(init_opt)),
                                    int(mc_steps), int(bfgs_iters), float(temp), float(amp_pos), float(amp_rot),
                                    int(seed) & (2 ** 64 - 1), int(step_offset), _lib.ptr(self.order),
-                                   _lib.ptr(self.counter), _lib.ptr(self.done), _lib.ptr(self.lig_chains),
+                                   _lib.ptr(self.qstart), self.nq, _lib.ptr(self.counter), _lib.ptr(self.done), _lib.ptr(self.lig_chains),
                                    _lib.ptr(self.times), int(self.blocks),
                                    _lib.ptr(kp.eval_count) if count_evals else None, st])
         _lib.check(L.adock_mc_batch(*args))

@@ -152,14 +152,16 @@ int adock_mc_batch(int AM, int KM, LIG_DECL, GRID_DECL, SF_DECL, REC_DECL, int e
                    const int* chain_id, int n_chains, const int* nbr, const int* nbr_n, int NB_MAX, float* cur,
                    float* cur_e, float* best, float* best_e, int init_opt, int mc_steps, int bfgs_iters, float temp,
                    float amp_pos, float amp_rot, unsigned long long seed, int step_offset, const int* order,
-                   int* counter, int* done, const int* lig_chains, unsigned long long* times, int blocks,
-                   unsigned long long* eval_count, adock_stream_t stream) {
+                   const int* queue_start, int n_queues, int* counter, int* done, const int* lig_chains,
+                   unsigned long long* times, int blocks, unsigned long long* eval_count, adock_stream_t stream) {
     if (int st = common_checks(AM, KM, P_MAX, ref, inc, ist, DEG_MAX, grid, prm, exact, rxyz, rn, RN_MAX, chain_lig,
                                n_chains))
         return st;
     if (n_chains == 0) return 0;
     if (!chain_id || !cur || !cur_e || !best || !best_e || !counter) return fail(ADOCK_E_ARG, "null pointer");
     if (done && !lig_chains) return fail(ADOCK_E_ARG, "done needs lig_chains");
+    if (n_queues < 0 || n_queues > 1024 || (n_queues > 1 && !queue_start))
+        return fail(ADOCK_E_ARG, "n_queues must be in 0..1024, with queue_start when above 1");
     if (mc_steps < 0) return fail(ADOCK_E_ARG, "negative mc_steps");
     if (nbr && (!nbr_n || NB_MAX <= 0)) return fail(ADOCK_E_ARG, "nbr needs nbr_n and NB_MAX > 0");
     LigArgs L{ref, typ, tm, ax, axp, axu, pr, na, nt, np, rg, P_MAX, gslot, inc, ist, DEG_MAX};
@@ -168,14 +170,17 @@ int adock_mc_batch(int AM, int KM, LIG_DECL, GRID_DECL, SF_DECL, REC_DECL, int e
     RecArgs R{rxyz, rtyp, rn, RN_MAX};
     McArgs M{chain_id, nbr, nbr_n, NB_MAX, cur, cur_e, best, best_e, init_opt, mc_steps, bfgs_iters, temp, amp_pos,
              amp_rot, seed, step_offset, eval_count};
-    QueueArgs Q{order, counter, done, lig_chains, times};
+    QueueArgs Q{order, queue_start, n_queues, counter, done, lig_chains, times};
     return hip_status(launch_mc_persistent(KM, L, G, S, R, exact, chain_lig, n_chains, M, Q, blocks,
                                            (hipStream_t)stream));
 }
 
-int adock_batch_reset(int* counter, int* done, int n_complexes, unsigned long long* times, adock_stream_t stream) {
-    if (!counter || n_complexes < 0) return fail(ADOCK_E_ARG, "null counter or negative size");
-    return hip_status(launch_batch_reset(counter, done, done ? n_complexes : 0, times, times ? 1 + n_complexes : 0,
+int adock_batch_reset(int* counter, int n_counters, int* done, int n_complexes, unsigned long long* times,
+                      adock_stream_t stream) {
+    if (!counter || n_complexes < 0 || n_counters < 1 || n_counters > 1024)
+        return fail(ADOCK_E_ARG, "null counter, n_counters outside 1..1024 or negative size");
+    return hip_status(launch_batch_reset(counter, n_counters, done, done ? n_complexes : 0, times,
+                                         times ? 1 + n_complexes : 0,
                                          (hipStream_t)stream));
 }
 

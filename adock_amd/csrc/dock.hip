@@ -870,9 +870,13 @@ __global__ __launch_bounds__(256) void mc_spec_kernel(LigArgs L, GridArgs G, SFA
     run_chain<KM, NS, true, EX>(ch, lane, w, L, G, S, R, chain_lig, M, wid, carve_spec(lds + NS * wb));
 }
 
-// Persistent variant: a grid sized to the chip's resident capacity; every wave pulls chain indices from a global
-// counter (in `order`, e.g. longest chains first) until the queue is empty. Per complex it counts finished chains and
-// stamps the 100 MHz real-time clock when the last one ends (times[0] = earliest wave start).
+// Persistent variant: a grid sized to the chip's resident capacity; every wave pulls chain indices (from `order`, e.g.
+// longest chains first) with an atomic counter until its queue is empty, then serves the other queues in turn.
+// Workgroup b starts on queue b % nq: the dispatcher deals workgroups to the 8 XCDs in turn, so with nq = 8 each queue's
+// chains (the host puts the complexes of a few receptors in each) run on one XCD and their grids share its L2. Every
+// queue is clamped to [0, n) and a wave leaves a queue for good once it is drained, so the loop ends. Per complex it
+// counts finished chains and stamps the 100 MHz real-time clock when the last one ends (times[0] = earliest wave
+// start).
 template <int KM, bool EX>
 __global__ __launch_bounds__(256) void mc_persistent_kernel(LigArgs L, GridArgs G, SFArgs S, RecArgs R,
                                                              const int* __restrict__ chain_lig, int n_chains, McArgs M,
@@ -882,12 +886,21 @@ __global__ __launch_bounds__(256) void mc_persistent_kernel(LigArgs L, GridArgs 
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
     const WaveLDS w = carve_lds<KM>(lds + wid * wave_lds_bytes(KM, L.P_MAX), L.P_MAX);
     if (Q.times && lane == 0) atomicMin(Q.times, (unsigned long long)__builtin_amdgcn_s_memrealtime());
-    while (true) {
+    const int nq = Q.qstart && Q.nq > 0 ? Q.nq : 1;
+    int q = (int)(blockIdx.x % (unsigned)nq);
+    for (int left = nq; left > 0;) {
+        const int qs = nq > 1 ? min(max(Q.qstart[q], 0), n_chains) : 0;
+        const int qe = nq > 1 ? min(max(Q.qstart[q + 1], qs), n_chains) : n_chains;
         int idx = 0;
-        if (lane == 0) idx = atomicAdd(Q.counter, 1);
+        if (lane == 0) idx = atomicAdd(Q.counter + q, 1);
         idx = bcast_i(idx, 0);
-        if (idx >= n_chains) break;
-        const int ch = Q.order ? __builtin_amdgcn_readfirstlane(Q.order[idx]) : idx;
+        if (idx >= qe - qs) {
+            q = q + 1 == nq ? 0 : q + 1;
+            --left;
+            continue;
+        }
+        const int pos = qs + idx;
+        const int ch = Q.order ? __builtin_amdgcn_readfirstlane(Q.order[pos]) : pos;
         const int l = run_chain<KM, 1, false, EX>(ch, lane, w, L, G, S, R, chain_lig, M);
         if (Q.done && lane == 0) {
             const int old = atomicAdd(Q.done + l, 1);
@@ -1336,19 +1349,21 @@ int launch_mc_persistent(int KM, const LigArgs& L, const GridArgs& G, const SFAr
     }
 }
 
-__global__ void batch_reset_kernel(int* counter, int* done, int n_done, unsigned long long* times, int n_times) {
+__global__ void batch_reset_kernel(int* counter, int n_counters, int* done, int n_done, unsigned long long* times,
+                                   int n_times) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0 && counter) counter[0] = 0;
+    if (counter && i < n_counters) counter[i] = 0;
     if (done && i < n_done) done[i] = 0;
     if (times && i < n_times) times[i] = i == 0 ? ~0ull : 0ull;
 }
 
-int launch_batch_reset(int* counter, int* done, int n_done, unsigned long long* times, int n_times,
+int launch_batch_reset(int* counter, int n_counters, int* done, int n_done, unsigned long long* times, int n_times,
                        hipStream_t stream) {
     int n = n_done > n_times ? n_done : n_times;
+    if (n < n_counters) n = n_counters;
     if (n < 1) n = 1;
-    hipLaunchKernelGGL(batch_reset_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, counter, done, n_done, times,
-                       n_times);
+    hipLaunchKernelGGL(batch_reset_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, counter, n_counters, done,
+                       n_done, times, n_times);
     return (int)hipGetLastError();
 }
 

@@ -22,11 +22,13 @@ struct McArgs {
     int NBS = 0;   // exact mode: receptor atoms staged in each wave's LDS (set by the launcher; 0 = read from memory)
     int NBM = 0;   // exact mode: per-atom masks over the staged atoms (set by the launcher)
 };
-// work queue of the persistent chain kernel: order [n] (chain indices, may be NULL = identity), counter (1 int, 0 at
-// launch), done [L] (finished chains per complex, 0 at launch), lig_chains [L] (chains per complex), times [1 + L]
-// (times[0] = ~0 at launch; 100 MHz real-time stamps). done / lig_chains / times may be NULL.
+// work queues of the persistent chain kernel: order [n] (chain indices, may be NULL = identity) holds nq consecutive
+// queues, queue q = order[qstart[q] .. qstart[q + 1]) (qstart NULL or nq = 0: one queue over all n); counter [max(nq,
+// 1)] (0 at launch), done [L] (finished chains per complex, 0 at launch), lig_chains [L] (chains per complex), times
+// [1 + L] (times[0] = ~0 at launch; 100 MHz real-time stamps). done / lig_chains / times may be NULL.
 struct QueueArgs {
-    const int* order; int* counter; int* done; const int* lig_chains; unsigned long long* times;
+    const int* order; const int* qstart; int nq; int* counter; int* done; const int* lig_chains;
+    unsigned long long* times;
 };
 // learned-proposal MC step: Y [N,23] network output (NULL: plain moves), feat [N,128] features (in: at the current
 // pose; out: at the new current pose; NULL: not computed), log_buf [cap, 128+23+23+1] accepted-move records.
@@ -47,7 +49,8 @@ int launch_eg(int KM, const LigArgs&, const GridArgs&, const SFArgs&, const RecA
               const float*, float*, float*, hipStream_t);
 int launch_mc_persistent(int KM, const LigArgs&, const GridArgs&, const SFArgs&, const RecArgs&, int, const int*, int,
                          const McArgs&, const QueueArgs&, int blocks, hipStream_t);
-int launch_batch_reset(int* counter, int* done, int n_done, unsigned long long* times, int n_times, hipStream_t);
+int launch_batch_reset(int* counter, int n_counters, int* done, int n_done, unsigned long long* times,
+                       int n_times, hipStream_t);
 int launch_mlp_infer(int B, const float* X, const unsigned short* W1, const float* b1, const unsigned short* W2,
                      const float* b2, const unsigned short* W3, const float* b3, float* Y, hipStream_t s);
 int launch_gemm_f32(int ta, int tb, int M, int N, int K, float alpha, const float* A, int lda, const float* Bm,

@@ -116,9 +116,14 @@ int adock_mc_counted(int AM, int KM,
                      int bfgs_iters, float temp, float amp_pos, float amp_rot, unsigned long long seed,
                      int step_offset, unsigned long long* eval_count, adock_stream_t stream);
 
-/* Batched screening in one persistent launch: same chain arguments as adock_mc, plus a work queue. The grid is sized to
- * the chip's resident capacity (blocks <= 0: CUs x occupancy) and every wavefront pulls chain indices from *counter
- * through `order` (NULL = 0..n-1; pass longest-first for load balance) until the queue is empty. Optional per-complex
+/* Batched screening in one persistent launch: same chain arguments as adock_mc, plus work queues. The grid is sized to
+ * the chip's resident capacity (blocks <= 0: CUs x occupancy). `order` (NULL = 0..n-1) holds n_queues consecutive
+ * queues of chain indices, queue q = order[queue_start[q] .. queue_start[q+1]) (queue_start [n_queues + 1], from 0 to
+ * n; n_queues 0 or 1: one queue over all n), each with its counter in counter[q] (counter [max(n_queues, 1)]).
+ * Workgroup b pulls chains from queue b % n_queues until it is empty, then from the others in turn; the dispatcher
+ * deals workgroups to the 8 XCDs in turn, so with 8 queues each queue runs on one XCD (put the complexes of a few
+ * receptors in each, longest chains first, so their grids share that XCD's L2). Results do not depend on the queues
+ * (every chain's computation is its own). Optional per-complex
  * bookkeeping: done [L] counts finished chains, lig_chains [L] gives chains per complex, times [1+L] receives the
  * 100 MHz real-time stamp of the first wave start (times[0]) and of each complex's last chain (times[1+l]).
  * counter, done and times must be reset before every launch (adock_batch_reset: a kernel, so the pair can be captured
@@ -136,12 +141,14 @@ int adock_mc_batch(int AM, int KM,
                    const int* chain_lig, const int* chain_id, int n_chains, const int* nbr, const int* nbr_n,
                    int NB_MAX, float* cur, float* cur_e, float* best, float* best_e, int init_opt, int mc_steps,
                    int bfgs_iters, float temp, float amp_pos, float amp_rot, unsigned long long seed,
-                   int step_offset, const int* order, int* counter, int* done, const int* lig_chains,
-                   unsigned long long* times, int blocks, unsigned long long* eval_count, adock_stream_t stream);
+                   int step_offset, const int* order, const int* queue_start, int n_queues, int* counter,
+                   int* done, const int* lig_chains, unsigned long long* times, int blocks,
+                   unsigned long long* eval_count, adock_stream_t stream);
 
-/* Resets the work queue of adock_mc_batch with a kernel launch: *counter = 0, done[0..L) = 0, times[0] = ~0,
- * times[1..L] = 0 (done / times may be NULL). */
-int adock_batch_reset(int* counter, int* done, int n_complexes, unsigned long long* times, adock_stream_t stream);
+/* Resets the work queues of adock_mc_batch with a kernel launch: counter[0..n_counters) = 0, done[0..L) = 0,
+ * times[0] = ~0, times[1..L] = 0 (done / times may be NULL). n_counters in 1..1024. */
+int adock_batch_reset(int* counter, int n_counters, int* done, int n_complexes, unsigned long long* times,
+                      adock_stream_t stream);
 
 /* ---- learned MC proposal network (128 -> 256 -> 256 -> 23, ReLU). No reference counterpart: the reference proposes
  * moves at random only (src/adock/mckernel.py:381-398). ---- */

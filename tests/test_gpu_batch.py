@@ -30,11 +30,15 @@ def _run_plain(kp, st, cl, steps):
     return [x.cpu().numpy() for x in (cur, ce, best, be)]
 
 
-def test_batch_equals_per_chain_launch(small_screen):
-    print("[gpu] batch vs plain", flush=True)
+@pytest.mark.parametrize("queues", [8, 3, 1])
+def test_batch_equals_per_chain_launch(small_screen, queues):
+    """Bit-identical to the per-chain launch whatever the queues (one per XCD, an odd count, one global queue); with
+    3 queues the complexes are put in 5 groups, as several receptor sets would be."""
+    print(f"[gpu] batch vs plain, {queues} queues", flush=True)
     w, kp, st, cl = small_screen
     ref = _run_plain(kp, st, cl, w["steps"])
-    r = BatchRunner(kp, cl, n_complexes=len(w["ligs"]))
+    groups = np.arange(len(w["ligs"])) % 5 if queues == 3 else None
+    r = BatchRunner(kp, cl, n_complexes=len(w["ligs"]), groups=groups, queues=queues)
     N = len(cl)
     cur, best = st.clone(), st.clone()
     ce, be = torch.zeros(N).cuda(), torch.full((N,), float("inf")).cuda()
