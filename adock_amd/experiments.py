@@ -20,7 +20,7 @@ import torch
 
 from . import synth, workload
 from .autotune import Autotuner
-from .batch import BatchRunner, GraphedBatch, admit
+from .batch import XCDS, BatchRunner, GraphedBatch, admit
 from .kernel import KernelProblem
 from .learned import StepSearch, jump_dataset, train_proposal
 
@@ -30,14 +30,14 @@ def _bufs(st):
     return st.clone(), torch.zeros(N, device=st.device), st.clone(), torch.full((N,), float("inf"), device=st.device)
 
 
-def screen(n_ligands=512, chains=64, steps=500, seed=0, reps=2, tune=True, log=print):
+def screen(n_ligands=512, chains=64, steps=500, seed=0, reps=2, tune=True, log=print, queues=XCDS):
     w, prob = workload.screen(seed=seed, n_ligands=n_ligands, chains=chains, steps=steps)
     kp = KernelProblem(prob)
     L = len(w["ligs"])
     st, cl = workload.chain_starts(seed, w["ligs"], [w["center"]] * L, [w["size"]] * L, chains, kp.KM)
     st = torch.as_tensor(st, device=kp.device)
     cl = torch.as_tensor(cl, device=kp.device)
-    runner = BatchRunner(kp, cl, n_complexes=L)
+    runner = BatchRunner(kp, cl, n_complexes=L, queues=queues)
     shape = Autotuner().tune(runner, st, probe_steps=2, reps=1) if tune else None
     bufs = _bufs(st)
     ci = torch.arange(len(cl), dtype=torch.int32, device=kp.device)
@@ -61,7 +61,7 @@ def screen(n_ligands=512, chains=64, steps=500, seed=0, reps=2, tune=True, log=p
             "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99))}
 
 
-def multi(n_receptors=64, ligands_per=32, chains=32, steps=500, seed=0, hbm_budget=64e9, log=print):
+def multi(n_receptors=64, ligands_per=32, chains=32, steps=500, seed=0, hbm_budget=64e9, log=print, queues=XCDS):
     w, prob = workload.multi(seed=seed, n_receptors=n_receptors, ligands_per=ligands_per, chains=chains, steps=steps)
     grid_bytes = [int(prob.gs.dims[r].prod()) * int((prob.gs.slot[r] >= 0).sum()) * 4 for r in range(n_receptors)]
     groups = admit(grid_bytes, hbm_budget)
@@ -81,7 +81,7 @@ def multi(n_receptors=64, ligands_per=32, chains=32, steps=500, seed=0, hbm_budg
         sel = torch.as_tensor(np.concatenate([np.arange(l * chains, (l + 1) * chains) for l in ligs]),
                               device=kp.device)
         c_, ce_, b_, be_ = [x[sel].contiguous() for x in bufs]
-        runner = BatchRunner(kp, cl_all[sel], n_complexes=L)
+        runner = BatchRunner(kp, cl_all[sel], n_complexes=L, queues=queues)
         t1 = time.perf_counter()
         runner.launch(ci[sel].contiguous(), c_, ce_, b_, be_, True, steps, 0, 1.2, 2.0, 2.0, seed * 1_000_003 + 12345, 0)
         torch.cuda.synchronize()

@@ -76,6 +76,8 @@ def main():
     ap.add_argument("--workload", choices=["single", "screen", "multi", "learned", "grids", "eg", "wide", "driver", "driver_fitted", "mlp"], default="single",
                     help="single = the headline line (default); the others print their own JSON line")
     ap.add_argument("--small", action="store_true", help="scaled-down sizes for the opt-in workloads")
+    ap.add_argument("--queues", type=int, default=8,
+                    help="screen / multi: work queues of the persistent launch (8: one per XCD; 1: one global queue)")
     ap.add_argument("--target", choices=["move", "jump"], default="move",
                     help="training target of the learned proposal (--workload learned)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
@@ -88,7 +90,8 @@ def main():
     if args.workload != "single":
         from adock_amd import experiments as ex
         if args.workload == "screen":
-            out = ex.screen(n_ligands=64 if args.small else 512, chains=64, steps=50 if args.small else 500)
+            out = ex.screen(n_ligands=64 if args.small else 512, chains=64, steps=50 if args.small else 500,
+                            queues=args.queues)
         elif args.workload in ("driver", "driver_fitted"):
             out = ex.driver(n_ligands=32 if args.small else 256, fitted=args.workload == "driver_fitted")
         elif args.workload == "mlp":
@@ -112,7 +115,7 @@ def main():
                                       "unit": "TFLOP/s", "frac": out[k]["achieved_tflops"] / PEAK_FP32_TFLOPS}
         elif args.workload == "multi":
             out = ex.multi(n_receptors=8 if args.small else 64, ligands_per=8 if args.small else 32, chains=32,
-                           steps=50 if args.small else 500)
+                           steps=50 if args.small else 500, queues=args.queues)
         else:
             out = (ex.learned(n_train=8, train_steps=40, n_eval=8, budgets=(10, 20, 40), full_steps=100, epochs=2,
                               target=args.target)
