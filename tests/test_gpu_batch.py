@@ -88,3 +88,35 @@ def test_autotuner_picks_a_shape(small_screen):
     np.testing.assert_array_equal(ref[3], be.cpu().numpy())
     from adock_amd import _lib
     _lib.check(_lib.lib().adock_set_chains_per_block(0))
+
+
+def test_run_mc_routes_many_chains_through_queues():
+    """KernelProblem.run_mc with QUEUE_MIN_CHAINS chains or more (grid mode) takes the persistent queued launch; the
+    result and the evaluation count equal adock_mc's bit for bit."""
+    print("[gpu] run_mc: queued launch vs adock_mc at 4096 chains", flush=True)
+    from adock_amd import kernel as kmod
+    w, prob = workload.screen(seed=4, n_ligands=64, chains=64, steps=3)
+    kp = KernelProblem(prob)
+    L = len(w["ligs"])
+    st, cl = workload.chain_starts(4, w["ligs"], [w["center"]] * L, [w["size"]] * L, w["chains"], kp.KM)
+    st, cl = torch.as_tensor(st).cuda(), torch.as_tensor(cl).cuda()
+    N = len(cl)
+    assert N >= kmod.QUEUE_MIN_CHAINS
+
+    def run(queues):
+        kp.use_queues = queues
+        try:
+            cur, best = st.clone(), st.clone()
+            ce, be = torch.zeros(N).cuda(), torch.full((N,), float("inf")).cuda()
+            kp.eval_count.zero_()
+            kp.run_mc(cl, torch.arange(N, dtype=torch.int32).cuda(), cur, ce, best, be, True, w["steps"], 0, 1.2, 2.0,
+                      2.0, 9, 0, count_evals=True)
+            torch.cuda.synchronize()
+            return [x.cpu().numpy() for x in (cur, ce, best, be)] + [int(kp.eval_count.item())]
+        finally:
+            kp.use_queues = True
+
+    a, b = run(True), run(False)
+    assert a[4] == b[4] > 0
+    for x, y in zip(a[:4], b[:4]):
+        np.testing.assert_array_equal(x, y)
