@@ -18,6 +18,39 @@ import torch
 from .pose import internal_coords, pose_coords, random_quat
 
 
+def _translation_lattices(prob, boxes_c, boxes_s, cfg):
+    """(trans [L, maxT, 3] f32, nT [L]): each complex's translation lattice (step stageA_translation_step inside its
+    box shrunk by 1 A), points whose nearest-grid C_H probe energy exceeds stageA_probe_max dropped (the box centre
+    alone when none is left). All complexes' lattices go through one probe lookup and one host round trip."""
+    gs = prob.gs
+    dev = prob.lb.ref.device
+    L = len(boxes_c)
+    step = cfg["stageA_translation_step"]
+    centres = torch.as_tensor(np.stack([np.asarray(c, np.float64) for c in boxes_c]), device=dev)
+    Tall, sizes = [], []
+    for l in range(L):
+        s = boxes_s[l]
+        axes = [torch.arange(-float(s[d]) / 2 + 1.0, float(s[d]) / 2 - 1.0 + 1e-6, step, device=dev) for d in range(3)]
+        T = torch.stack(torch.meshgrid(*axes, indexing="ij"), -1).reshape(-1, 3) + centres[l]
+        Tall.append(T)
+        sizes.append(T.shape[0])
+    owner = torch.repeat_interleave(torch.arange(L, device=dev), torch.as_tensor(sizes, device=dev))
+    T = torch.cat(Tall)
+    ep = gs.lookup_nearest(T, torch.zeros(len(T), dtype=torch.long, device=dev), prob.rec_idx.to(dev)[owner])
+    keep = ep <= cfg.get("stageA_probe_max", 0.0)
+    T, owner = T[keep], owner[keep]
+    counts = torch.bincount(owner, minlength=L)
+    first = torch.cumsum(counts, 0) - counts
+    nT_h = counts.cpu().numpy()
+    empty = np.nonzero(nT_h == 0)[0]
+    nT_h = np.maximum(nT_h, 1)
+    trans = torch.zeros(L, int(nT_h.max()), 3, device=dev)
+    trans[owner, torch.arange(len(owner), device=dev) - first[owner]] = T.to(trans.dtype)
+    for l in empty:
+        trans[l, 0] = centres[l].to(trans.dtype)
+    return trans, torch.as_tensor(nT_h, device=dev)
+
+
 @torch.no_grad()
 def stage_a_kernel(prob, kp, boxes_c, boxes_s, cfg, gen):
     """Rigid enumeration: NT random torsion draws per ligand, the C lowest-intra-energy conformers x Q random rotations
@@ -38,25 +71,14 @@ def stage_a_kernel(prob, kp, boxes_c, boxes_s, cfg, gen):
     e_c = torch.gather(e_intra, 1, ci)
     Xc = internal_coords(lb, th_c)
     C = Xc.shape[1]
-    hidx = torch.zeros(L, kp.AM, dtype=torch.long, device=dev)
+    hidx_h = np.zeros((L, kp.AM), np.int64)
     for l, p in enumerate(lb.preps):
-        hv = torch.as_tensor(np.where(p.heavy)[0], device=dev)
-        hidx[l, :len(hv)] = hv
+        hv = np.where(p.heavy)[0]
+        hidx_h[l, :len(hv)] = hv
+    hidx = torch.as_tensor(hidx_h, device=dev)
     conf_h = torch.gather(Xc, 2, hidx[:, None, :, None].expand(L, C, kp.AM, 3))
     quats = random_quat((Q,), gen, dev)
-    step = cfg["stageA_translation_step"]
-    Ts = []
-    for l in range(L):
-        c, s = boxes_c[l], boxes_s[l]
-        axes = [torch.arange(-float(s[d]) / 2 + 1.0, float(s[d]) / 2 - 1.0 + 1e-6, step, device=dev) for d in range(3)]
-        T = torch.stack(torch.meshgrid(*axes, indexing="ij"), -1).reshape(-1, 3) + torch.as_tensor(c, device=dev)
-        ep = gs.lookup_nearest(T, torch.zeros(len(T), dtype=torch.long, device=dev), prob.rec_idx[l].expand(len(T)))
-        T = T[ep <= cfg.get("stageA_probe_max", 0.0)]
-        Ts.append(T if len(T) else torch.as_tensor(c, device=dev, dtype=torch.float32)[None])
-    nT = torch.tensor([len(T) for T in Ts], device=dev)
-    trans = torch.zeros(L, int(nT.max()), 3, device=dev)
-    for l, T in enumerate(Ts):
-        trans[l, :len(T)] = T
+    trans, nT = _translation_lattices(prob, boxes_c, boxes_s, cfg)
     best_e, best_i = kp.stage_a(conf_h, quats, trans, nT, cfg.get("stageA_atom_cap", 1e9))
     best_e = best_e + e_c.repeat_interleave(Q, dim=1)
     k = min(M, C * Q)
