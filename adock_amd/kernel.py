@@ -21,9 +21,11 @@ def ligand_arrays(prob):
     return pack_ligands(prob.lb.preps, slot, prob.rec_idx.cpu().numpy())
 
 
-# from this many chains in grid mode run_mc uses the persistent queued launch: its longest-first queues keep every CU
-# busy to the end of a launch of uneven chains (8 % off the search driver's MC phase at 65,536 chains)
+# from this many chains in grid mode, over complexes whose chain costs differ by more than UNEVEN_COST, run_mc uses the
+# persistent queued launch: its longest-first queues keep every CU busy to the end of a launch of uneven chains (8 % off
+# the search driver's MC phase at 65,536 chains); equal chains (one complex, widened) are faster in adock_mc
 QUEUE_MIN_CHAINS = 4096
+UNEVEN_COST = 1.1
 
 
 class KernelProblem:
@@ -184,8 +186,9 @@ class KernelProblem:
                amp_pos, amp_rot, seed, step_offset, exact=False, nbr=None, nbr_n=None, count_evals=False):
         """In place on cur/cur_e/best/best_e (float32 contiguous device tensors). exact=True with nbr [N, NB] /
         nbr_n [N]: each chain's exact inter energy loops only over its receptor neighbour list. From QUEUE_MIN_CHAINS
-        chains in grid mode (and use_queues) the chains run in the persistent queued launch (adock_mc_batch),
-        otherwise in adock_mc (speculative line search for few chains); the results are the same bit for bit."""
+        chains in grid mode over complexes of uneven cost (and use_queues) the chains run in the persistent queued
+        launch (adock_mc_batch), otherwise in adock_mc (speculative line search for few chains); the results are the
+        same bit for bit."""
         N = cur.shape[0]
         if N == 0:
             return
@@ -204,15 +207,18 @@ class KernelProblem:
             nb_args = [_lib.ptr(nbr), _lib.ptr(nbr_n), int(nbr.shape[1])]
         else:
             nb_args = [None, None, 0]
-        if self.use_queues and not exact and nbr is None and N >= QUEUE_MIN_CHAINS:
-            # many chains in grid mode: the persistent launch with longest-first queues, one per XCD (same results
-            # bit for bit; the tail of uneven chains no longer idles CUs)
-            from .batch import BatchRunner
-            runner = BatchRunner(self, cl, n_complexes=self.L)
-            runner.launch(ci, cur, cur_e, best, best_e, init_opt, mc_steps, bfgs_iters, temp, amp_pos, amp_rot, seed,
-                          step_offset, count_evals=count_evals)
-            self._keep = (cl, ci, runner)
-            return
+        if self.use_queues and not exact and nbr is None and N >= QUEUE_MIN_CHAINS and self.L > 1:
+            # many uneven chains in grid mode: the persistent launch with longest-first queues, one per XCD (same
+            # results bit for bit; the tail of long chains no longer idles CUs). Chains of equal cost stay on adock_mc.
+            cl_h = cl.cpu().numpy()
+            c = np.asarray(self.cost, np.float64)[np.unique(cl_h)]
+            if c.max() > UNEVEN_COST * c.min():
+                from .batch import BatchRunner
+                runner = BatchRunner(self, cl_h, n_complexes=self.L)
+                runner.launch(ci, cur, cur_e, best, best_e, init_opt, mc_steps, bfgs_iters, temp, amp_pos, amp_rot,
+                              seed, step_offset, count_evals=count_evals)
+                self._keep = (cl, ci, runner)
+                return
         self._keep = (cl, ci, nbr, nbr_n)
         args = (self._args(exact) + [_lib.ptr(cl), _lib.ptr(ci), N] + nb_args +
                 [_lib.ptr(cur), _lib.ptr(cur_e), _lib.ptr(best), _lib.ptr(best_e), int(bool(init_opt)), int(mc_steps),
