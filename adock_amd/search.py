@@ -102,11 +102,8 @@ def finalize_kernel(prob, kp, states, chain_lig, L, P, cfg):
     etot = kp.energy_grad(chain_lig, states, exact=prob.rec is not None)[0].view(L, P)
     eintra = prob.intra(X)
     D = _rmsd(X, lb.heavy)
-    Xc, Ec, Ic, Dc = X.cpu().numpy(), etot.cpu().numpy(), eintra.cpu().numpy(), D.cpu().numpy()
-    Nt = lb.n_tors_vina.cpu().numpy()
-    rw = prob.pot.p["rot_weight"]
-    return [_cluster(Xc[l], Ec[l], Ec[l] - Ic[l], Ic[l], Dc[l], Nt[l], rw, cfg["cluster_rmsd"], cfg["n_out"])
-            for l in range(L)]
+    return _cluster_all(X, etot, eintra, D, lb.n_tors_vina, prob.pot.p["rot_weight"], cfg["cluster_rmsd"],
+                        cfg["n_out"])
 
 
 def _rmsd(X, heavy, chunk=50_000_000):
@@ -122,19 +119,42 @@ def _rmsd(X, heavy, chunk=50_000_000):
     return out
 
 
-def _cluster(X, E, inter, intra, D, n_tors_vina, rot_weight, cut, n_out):
-    """Greedy clustering of one complex's final poses (lowest energy first; a pose becomes a representative when its
-    RMSD D to every earlier one is at least `cut`; at most n_out) and its result dict with Vina-style affinities
-    (src/adock/search.py:389-396, 520-537)."""
-    reps = []
-    for i in np.argsort(E, kind="stable"):
-        if all(D[i, j] >= cut for j in reps):
-            reps.append(i)
-            if len(reps) == n_out:
-                break
-    ri = np.array(reps)
-    aff = (inter[ri] + intra[ri] - intra[ri[0]]) / (1 + rot_weight * n_tors_vina)
-    return {"coords": X[ri], "affinity": aff, "inter": inter[ri], "intra": intra[ri], "total": E[ri]}
+def _cluster_all(X, E, intra, D, n_tors_vina, rot_weight, cut, n_out, inter=None):
+    """Greedy clustering of every complex's final poses at once (lowest energy first; a pose becomes a representative
+    when its RMSD D to every earlier one is at least `cut`; at most n_out) and the result dicts with Vina-style
+    affinities (src/adock/search.py:389-396, 520-537). X [L, M, A, 3], E / intra / inter [L, M] (inter None: E -
+    intra), D [L, M, M], n_tors_vina [L], on any device. Round k takes, in every complex, the first pose in energy order
+    (stable) that is at least `cut` from all k earlier representatives: the sequential rule's picks, since a pose it
+    rejected was within `cut` of an earlier representative and stays so. Only the representatives leave the device."""
+    L, M = E.shape
+    dev = E.device
+    order = torch.sort(E, dim=1, stable=True).indices
+    rank = torch.empty_like(order)
+    rank.scatter_(1, order, torch.arange(M, device=dev).expand(L, M).contiguous())
+    eligible = torch.ones(L, M, dtype=torch.bool, device=dev)
+    reps = torch.full((L, n_out), -1, dtype=torch.long, device=dev)
+    li = torch.arange(L, device=dev)
+    for k in range(n_out):
+        first = torch.where(eligible, rank, torch.full_like(rank, M)).min(1)
+        has = first.values < M
+        pick = first.indices
+        reps[:, k] = torch.where(has, pick, torch.full_like(pick, -1))
+        ok = D[li, pick] >= cut
+        ok[li, pick] = False
+        eligible &= ok | ~has[:, None]
+    take = reps.clamp(min=0)
+    Xr = X[li[:, None], take].cpu().numpy()
+    Er, Ir = E.gather(1, take).cpu().numpy(), intra.gather(1, take).cpu().numpy()
+    Fr = inter.gather(1, take).cpu().numpy() if inter is not None else None
+    Nt, rh = n_tors_vina.cpu().numpy(), reps.cpu().numpy()
+    out = []
+    for l in range(L):
+        k = int((rh[l] >= 0).sum())
+        e, ia = Er[l, :k], Ir[l, :k]
+        fi = Fr[l, :k] if Fr is not None else e - ia
+        aff = (fi + ia - ia[0]) / (1 + rot_weight * Nt[l])
+        out.append({"coords": Xr[l, :k], "affinity": aff, "inter": fi, "intra": ia, "total": e})
+    return out
 
 
 @torch.no_grad()
@@ -149,11 +169,8 @@ def finalize(prob, bt, bq, bth, cfg):
         einter = prob.inter_exact(X) + pen
         etot = einter + eintra
     D = _rmsd(X, lb.heavy)
-    Xc, Ec, Ei, Ia, Dc = (a.cpu().numpy() for a in (X, etot, einter, eintra, D))
-    Nt = lb.n_tors_vina.cpu().numpy()
-    rw = prob.pot.p["rot_weight"]
-    return [_cluster(Xc[l], Ec[l], Ei[l], Ia[l], Dc[l], Nt[l], rw, cfg["cluster_rmsd"], cfg["n_out"])
-            for l in range(lb.L)]
+    return _cluster_all(X, etot, eintra, D, lb.n_tors_vina, prob.pot.p["rot_weight"], cfg["cluster_rmsd"],
+                        cfg["n_out"], inter=einter)
 
 
 def dock_problem_kernel(prob, boxes_c, boxes_s, cfg, seed=0, log=None, stats=None, kernel_problem=None):

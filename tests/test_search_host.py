@@ -132,3 +132,42 @@ def test_receptor_neighbour_lists():
         want = np.nonzero(d < cut)[0]
         assert int(nbr_n[i]) == len(want)
         np.testing.assert_array_equal(nbr[i, :len(want)].numpy(), want)
+
+
+def _greedy_reference(X, E, inter, intra, D, nt, rw, cut, n_out):
+    """The sequential greedy rule of one complex, as the reference states it (src/adock/search.py:389-396, 520-537)."""
+    reps = []
+    for i in np.argsort(E, kind="stable"):
+        if all(D[i, j] >= cut for j in reps):
+            reps.append(i)
+            if len(reps) == n_out:
+                break
+    ri = np.array(reps)
+    aff = (inter[ri] + intra[ri] - intra[ri[0]]) / (1 + rw * nt)
+    return {"coords": X[ri], "affinity": aff, "inter": inter[ri], "intra": intra[ri], "total": E[ri]}
+
+
+@pytest.mark.parametrize("cut", [2.0, 0.0, 5.5])
+def test_cluster_all_equals_sequential_greedy(cut):
+    """The all-complex clustering picks the sequential rule's representatives, in its order, with the same values
+    bit for bit: ties in energy (stable order), complexes with fewer than n_out clusters, a zero cutoff."""
+    from adock_amd.search import _cluster_all
+    rng = np.random.default_rng(int(cut * 10))
+    L, M, A, n_out = 7, 40, 5, 9
+    X = rng.normal(0, 3, (L, M, A, 3)).astype(np.float32)
+    E = np.round(rng.normal(-5, 2, (L, M)), 1).astype(np.float32)         # many ties
+    intra = rng.normal(0, 1, (L, M)).astype(np.float32)
+    inter = (E - intra).astype(np.float32)
+    D = rng.uniform(0, 8, (L, M, M)).astype(np.float32)
+    D = (D + D.transpose(0, 2, 1)) / 2
+    D[:, np.arange(M), np.arange(M)] = 0
+    D[2] *= 0.05                                                          # one complex with a single cluster
+    nt = rng.integers(0, 8, L).astype(np.float32)
+    T = torch.as_tensor
+    for inter_arg in (None, T(inter)):
+        got = _cluster_all(T(X), T(E), T(intra), T(D), T(nt), 0.05846, cut, n_out, inter=inter_arg)
+        for l in range(L):
+            want = _greedy_reference(X[l], E[l], E[l] - intra[l] if inter_arg is None else inter[l], intra[l], D[l],
+                                     nt[l], 0.05846, cut, n_out)
+            for key in want:
+                np.testing.assert_array_equal(got[l][key], want[key])
