@@ -120,3 +120,21 @@ def test_run_mc_routes_many_chains_through_queues():
     assert a[4] == b[4] > 0
     for x, y in zip(a[:4], b[:4]):
         np.testing.assert_array_equal(x, y)
+
+
+def test_skewed_queues_are_stolen_and_complete(small_screen):
+    """All chains in the last of 8 queues (seven empty): every workgroup finds its own queue drained and serves the
+    others, every chain runs exactly once (per-complex done counts), results bit-identical to the per-chain launch."""
+    print("[gpu] batch with one loaded queue out of 8", flush=True)
+    w, kp, st, cl = small_screen
+    ref = _run_plain(kp, st, cl, w["steps"])
+    r = BatchRunner(kp, cl, n_complexes=len(w["ligs"]), queues=8)
+    N = len(cl)
+    r.qstart = torch.as_tensor(np.array([0] * 8 + [N], np.int32)).cuda()
+    cur, best = st.clone(), st.clone()
+    ce, be = torch.zeros(N).cuda(), torch.full((N,), float("inf")).cuda()
+    r.launch(torch.arange(N, dtype=torch.int32).cuda(), cur, ce, best, be, True, w["steps"], 0, 1.2, 2.0, 2.0, 55, 0)
+    torch.cuda.synchronize()
+    for a, b in zip(ref, (cur, ce, best, be)):
+        np.testing.assert_array_equal(a, b.cpu().numpy())
+    assert (r.done.cpu().numpy() == w["chains"]).all()
