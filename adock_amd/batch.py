@@ -34,9 +34,10 @@ def chain_cost(preps, iters_rule=True):
 def queue_partition(cost, group, n_queues):
     """(order [N] int32, queue_start [n_queues + 1] int32) for adock_mc_batch: the N chains (per-chain cost, group =
     receptor set of each chain's complex) in n_queues queues of about equal cost. Groups are kept whole where they fit
-    one queue's share (so a queue, i.e. one XCD, touches the grids of a few receptors only) and otherwise dealt over
-    ceil(cost / share) queues; groups go to the least-loaded queue, largest first; each queue runs longest chains
-    first. n_queues <= 1: one queue, longest first."""
+    one queue's share and otherwise dealt over ceil(cost / share) units; units go to the least-loaded queue, largest
+    first. Inside a queue the units run one after another, largest first, each with its longest chains first: a
+    queue (one XCD) works on one receptor's grids at a time and ends on short work. n_queues <= 1: one queue, longest
+    chains first."""
     cost = np.asarray(cost, np.float64)
     group = np.asarray(group)
     N = len(cost)
@@ -64,8 +65,9 @@ def queue_partition(cost, group, n_queues):
         members[q].append(idx)
     order, start = [], [0]
     for m in members:
-        idx = np.concatenate(m) if m else np.zeros(0, np.int64)
-        idx = idx[np.argsort(-cost[idx], kind="stable")]
+        # members arrive largest unit first (LPT order); each unit longest chains first
+        idx = (np.concatenate([u[np.argsort(-cost[u], kind="stable")] for u in m]) if m
+               else np.zeros(0, np.int64))
         order.append(idx)
         start.append(start[-1] + len(idx))
     return np.concatenate(order).astype(np.int32), np.array(start, np.int32)

@@ -284,4 +284,10 @@ int adock_set_exact_staging(int on) {
 }
 int adock_get_exact_staging(void) { return get_exact_staging(); }
 
+int adock_mc_resident_waves(int KM, int P_MAX, int persistent) {
+    if (int st = check_buckets(64, KM, P_MAX)) return st;
+    const int w = mc_resident_waves(KM, P_MAX, persistent ? 1 : 0);
+    return w < 0 ? fail(ADOCK_E_ARG, "occupancy query failed") : w;
+}
+
 }  // extern "C"

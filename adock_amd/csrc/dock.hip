@@ -20,6 +20,7 @@
 //   Gradient lane layout: 0-2 translation, 3-5 torque, 6 always zero, 7+k torsion k (state layout t(3) q(4) theta:
 //   torsion k sits on the same lane in the state and in the gradient; the zero lane is inert in BFGS).
 #include <map>
+#include <tuple>
 #include <mutex>
 
 #include "common.hpp"
@@ -994,6 +995,36 @@ inline void allow_lds(const void* fn, size_t bytes) {
     }
 }
 
+// chains per workgroup, at most W0, that keep the most chains resident per CU at wb LDS bytes per wave, from the
+// runtime's occupancy calculator: once the per-wave LDS is large, four waves per workgroup leave LDS unused that one
+// or two would fill. Ties go to the larger workgroup (equal chains share its instruction stream: the widened single
+// complex runs 4 % faster), or with prefer_small to the smaller one, whose slot frees as soon as its own chains end
+// (uneven chains: the exact polish of many ligands runs 8 % faster). Cached per (device, kernel, W0, wb).
+inline int packed_waves(const void* fn, int W0, size_t wb, bool prefer_small = false) {
+    if (W0 <= 1) return W0;
+    static std::mutex mu;
+    static std::map<std::tuple<int, const void*, int, size_t>, int> cache;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    const auto key = std::make_tuple(dev, fn, W0, wb);
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        const auto it = cache.find(key);
+        if (it != cache.end()) return it->second;
+    }
+    int best_w = W0, best_res = -1;
+    for (int w = W0; w >= 1; --w) {
+        if ((size_t)w * wb > (size_t)LDS_PER_WORKGROUP) continue;
+        allow_lds(fn, (size_t)w * wb);
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, w * WAVE, (size_t)w * wb) != hipSuccess) continue;
+        if (per_cu * w > best_res || (prefer_small && per_cu * w == best_res)) { best_res = per_cu * w; best_w = w; }
+    }
+    std::lock_guard<std::mutex> lock(mu);
+    cache[key] = best_w;
+    return best_w;
+}
+
 template <int KM, bool EX>
 int launch_mc_tx(const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecArgs& R, const int* chain_lig,
                  int n_chains, const McArgs& M0, hipStream_t stream) {
@@ -1015,7 +1046,9 @@ int launch_mc_tx(const LigArgs& L, const GridArgs& G, const SFArgs& S, const Rec
         }
         return (int)hipGetLastError();
     }
-    const int W = fit_waves(waves_for(n_chains), KM, L.P_MAX, extra);
+    int W = fit_waves(waves_for(n_chains), KM, L.P_MAX, extra);
+    if (g_waves_per_block <= 0)
+        W = packed_waves(reinterpret_cast<const void*>(mc_kernel<KM, EX>), W, wave_lds_bytes(KM, L.P_MAX) + extra, EX);
     const size_t shm = (size_t)W * (wave_lds_bytes(KM, L.P_MAX) + extra);
     const unsigned nblk = (unsigned)((n_chains + W - 1) / W);
     allow_lds(reinterpret_cast<const void*>(mc_kernel<KM, EX>), shm);
@@ -1035,7 +1068,9 @@ template <int KM, bool EX>
 int launch_mc_persistent_tx(const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecArgs& R,
                             const int* chain_lig, int n_chains, const McArgs& M, const QueueArgs& Q, int blocks,
                             hipStream_t stream) {
-    const int W = fit_waves(g_waves_per_block > 0 ? g_waves_per_block : 4, KM, L.P_MAX);
+    int W = fit_waves(g_waves_per_block > 0 ? g_waves_per_block : 4, KM, L.P_MAX);
+    if (g_waves_per_block <= 0)
+        W = packed_waves(reinterpret_cast<const void*>(mc_persistent_kernel<KM, EX>), W, wave_lds_bytes(KM, L.P_MAX));
     const size_t shm = (size_t)W * wave_lds_bytes(KM, L.P_MAX);
     if (blocks <= 0) {
         // resident capacity, queried once per (device, kernel, W, LDS size) so later launches (and graph captures)
@@ -1333,6 +1368,31 @@ int launch_mc(int KM, const LigArgs& L, const GridArgs& G, const SFArgs& S, cons
         case 16: return launch_mc_t<16>(L, G, S, R, exact, chain_lig, n_chains, M, stream);
         case 24: return launch_mc_t<24>(L, G, S, R, exact, chain_lig, n_chains, M, stream);
         case 32: return launch_mc_t<32>(L, G, S, R, exact, chain_lig, n_chains, M, stream);
+        default: return -1;
+    }
+}
+
+// resident waves per CU of the grid-mode chain kernels at a launch's bucket and per-wave LDS: the per-chain kernel
+// (mc_kernel, as launched for many chains) or the persistent one, from the runtime's occupancy calculator
+template <int KM>
+int mc_resident_waves_t(int P_MAX, int persistent) {
+    int W = fit_waves(persistent ? (g_waves_per_block > 0 ? g_waves_per_block : 4) : waves_for(1 << 20), KM, P_MAX);
+    const void* fn = persistent ? reinterpret_cast<const void*>(mc_persistent_kernel<KM, false>)
+                                : reinterpret_cast<const void*>(mc_kernel<KM, false>);
+    if (g_waves_per_block <= 0) W = packed_waves(fn, W, wave_lds_bytes(KM, P_MAX));
+    const size_t shm = (size_t)W * wave_lds_bytes(KM, P_MAX);
+    allow_lds(fn, shm);
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, W * WAVE, shm) != hipSuccess) return -1;
+    return per_cu * W;
+}
+
+int mc_resident_waves(int KM, int P_MAX, int persistent) {
+    switch (KM) {
+        case 8: return mc_resident_waves_t<8>(P_MAX, persistent);
+        case 16: return mc_resident_waves_t<16>(P_MAX, persistent);
+        case 24: return mc_resident_waves_t<24>(P_MAX, persistent);
+        case 32: return mc_resident_waves_t<32>(P_MAX, persistent);
         default: return -1;
     }
 }

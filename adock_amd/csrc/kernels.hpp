@@ -75,4 +75,5 @@ int get_line_search_width();
 int set_exact_staging(int on);
 int get_exact_staging();
 int lds_budget_ok(int KM, int P_MAX);
+int mc_resident_waves(int KM, int P_MAX, int persistent);
 }  // namespace adock

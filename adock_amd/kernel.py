@@ -21,9 +21,10 @@ def ligand_arrays(prob):
     return pack_ligands(prob.lb.preps, slot, prob.rec_idx.cpu().numpy())
 
 
-# from this many chains in grid mode, over complexes whose chain costs differ by more than UNEVEN_COST, run_mc uses the
-# persistent queued launch: its longest-first queues keep every CU busy to the end of a launch of uneven chains (8 % off
-# the search driver's MC phase at 65,536 chains); equal chains (one complex, widened) are faster in adock_mc
+# from this many chains in grid mode, over complexes whose chain costs differ by more than UNEVEN_COST, and where the
+# persistent kernel keeps as many waves resident as the per-chain one, run_mc uses the persistent queued launch: its
+# longest-first queues keep every CU busy to the end of a launch of uneven chains (8 % off the search driver's MC phase
+# at 65,536 chains); equal chains (one complex, widened) are faster in adock_mc
 QUEUE_MIN_CHAINS = 4096
 UNEVEN_COST = 1.1
 
@@ -32,6 +33,7 @@ class KernelProblem:
     """State layout per chain: [t(3), q(4) (w,x,y,z), theta(KM)]; gradient [d/dt(3), torque(3), d/dtheta(KM)]."""
 
     use_queues = True
+    _queue_occupancy = None
 
     def __init__(self, prob):
         _lib.require_gpu()
@@ -182,6 +184,18 @@ class KernelProblem:
         if cur_e.shape != (N,) or best_e.shape != (N,):
             raise _lib.AdockError(f"cur_e and best_e must be [{N}]")
 
+    def queues_keep_occupancy(self):
+        """True when the persistent kernel keeps at least as many chains resident per CU as the per-chain kernel at
+        this problem's bucket and pair capacity (the runtime's occupancy calculator, asked once): the queued launch
+        only pays where it does not lose a wave per SIMD to its extra registers."""
+        if self._queue_occupancy is None:
+            L = _lib.lib()
+            a = L.adock_mc_resident_waves(self.KM, self.P_MAX, 1)
+            b = L.adock_mc_resident_waves(self.KM, self.P_MAX, 0)
+            self._queue_occupancy = (a, b)
+        a, b = self._queue_occupancy
+        return a > 0 and b > 0 and a >= b
+
     def run_mc(self, chain_lig, chain_id, cur, cur_e, best, best_e, init_opt, mc_steps, bfgs_iters, temp,
                amp_pos, amp_rot, seed, step_offset, exact=False, nbr=None, nbr_n=None, count_evals=False):
         """In place on cur/cur_e/best/best_e (float32 contiguous device tensors). exact=True with nbr [N, NB] /
@@ -212,7 +226,7 @@ class KernelProblem:
             # results bit for bit; the tail of long chains no longer idles CUs). Chains of equal cost stay on adock_mc.
             cl_h = cl.cpu().numpy()
             c = np.asarray(self.cost, np.float64)[np.unique(cl_h)]
-            if c.max() > UNEVEN_COST * c.min():
+            if c.max() > UNEVEN_COST * c.min() and self.queues_keep_occupancy():
                 from .batch import BatchRunner
                 runner = BatchRunner(self, cl_h, n_complexes=self.L)
                 runner.launch(ci, cur, cur_e, best, best_e, init_opt, mc_steps, bfgs_iters, temp, amp_pos, amp_rot,

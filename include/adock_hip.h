@@ -228,6 +228,12 @@ int adock_get_line_search_width(void);
 int adock_set_exact_staging(int on);
 int adock_get_exact_staging(void);
 
+/* Resident wavefronts per CU of the grid-mode chain kernels for bucket KM and pair capacity P_MAX, from the runtime's
+ * occupancy calculator: persistent = 0 the per-chain kernel of adock_mc (as launched for many chains), 1 the persistent
+ * kernel of adock_mc_batch. Lets a caller pick the launch that keeps more chains resident. No counterpart in the
+ * reference. Returns the count (> 0), or a negative ADOCK_E_* code. */
+int adock_mc_resident_waves(int KM, int P_MAX, int persistent);
+
 #ifdef __cplusplus
 }
 #endif

@@ -53,6 +53,7 @@ def test_bucket_validation_without_gpu():
     assert L.adock_set_exact_staging(1) == 0 and L.adock_get_exact_staging() == 1
     assert L.adock_set_exact_staging(2) == 0
     assert L.adock_version().startswith(b"adock_hip")
+    assert L.adock_mc_resident_waves(12, 64, 1) == -2 and L.adock_mc_resident_waves(16, -1, 0) < 0
     # fitted grid builder: null pointers and more receptor ext types than the LDS table holds are rejected,
     # empty grids are a no-op
     fit = [None, None, 1, None, None, 1, None, 16, 0.0, 0.0, 0.0, 0.375, 4, 4, 4, None, None, None]

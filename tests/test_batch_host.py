@@ -29,9 +29,17 @@ def _check_partition(cost, group, nq):
     order, start = queue_partition(cost, group, nq)
     assert sorted(order.tolist()) == list(range(len(cost)))
     assert start[0] == 0 and start[-1] == len(cost) and np.all(np.diff(start) >= 0) and len(start) == max(nq, 1) + 1
-    for q in range(len(start) - 1):
-        c = cost[order[start[q]:start[q + 1]]]
-        assert np.all(np.diff(c) <= 0)                       # longest chains first within a queue
+    for q in range(len(start) - 1 if nq > 1 else 0):
+        o = order[start[q]:start[q + 1]]
+        g = group[o]
+        # each group's chains form one run inside a queue (one receptor at a time), runs in descending total cost,
+        # longest chains first inside a run
+        runs = np.split(np.arange(len(o)), np.nonzero(np.diff(g))[0] + 1) if len(o) else []
+        assert len({int(g[r[0]]) for r in runs}) == len(runs)
+        tot = [cost[o[r]].sum() for r in runs]
+        assert all(a >= b for a, b in zip(tot, tot[1:]))
+        for r in runs:
+            assert np.all(np.diff(cost[o[r]]) <= 0)
     return order, start
 
 

@@ -112,10 +112,14 @@ def test_run_mc_routes_many_chains_through_queues():
             kp.run_mc(cl, torch.arange(N, dtype=torch.int32).cuda(), cur, ce, best, be, True, w["steps"], 0, 1.2, 2.0,
                       2.0, 9, 0, count_evals=True)
             torch.cuda.synchronize()
+            assert isinstance(kp._keep[-1], BatchRunner) == queues      # the launch that ran
             return [x.cpu().numpy() for x in (cur, ce, best, be)] + [int(kp.eval_count.item())]
         finally:
             kp.use_queues = True
 
+    ok = kp.queues_keep_occupancy()
+    print(f"  resident waves per CU (persistent, per-chain): {kp._queue_occupancy}", flush=True)
+    assert kp.KM == 16 and ok
     a, b = run(True), run(False)
     assert a[4] == b[4] > 0
     for x, y in zip(a[:4], b[:4]):
