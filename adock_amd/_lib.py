@@ -34,6 +34,7 @@ def lib():
         L.adock_last_error.restype = ctypes.c_char_p
         L.adock_build_grid.argtypes = [P, P, I, P, P, P, P, P, I, F, F, F, F, I, I, I, P, I, P, P]
         L.adock_build_grid_fit.argtypes = [P, P, I, P, P, I, P, I, F, F, F, F, I, I, I, P, P, P]
+        L.adock_build_grid_fit_sets.argtypes = [P, P, I, I, P, P, I, P, I, P, F, I, I, I, P, P, P]
         L.adock_inter_exact_fit.argtypes = [P, P, I, I, I, P, P, P, I, I, P, P, I, P, I, P, P, P]
         L.adock_eg.argtypes = LIG + GRID + SF + REC + [I, P, I, P, P, P, P]
         L.adock_mc.argtypes = LIG + GRID + SF + REC + [I, P, P, I, P, P, I, P, P, P, P, I, I, I, F, F, F, U64, I, P]
@@ -63,7 +64,7 @@ def lib():
         L.adock_get_exact_staging.restype = I
         L.adock_mc_resident_waves.argtypes = [I, I, I]
         L.adock_mc_resident_waves.restype = I
-        for name in ("adock_build_grid", "adock_build_grid_fit", "adock_inter_exact_fit", "adock_eg", "adock_mc", "adock_mc_counted", "adock_mc_batch",
+        for name in ("adock_build_grid", "adock_build_grid_fit", "adock_build_grid_fit_sets", "adock_inter_exact_fit", "adock_eg", "adock_mc", "adock_mc_counted", "adock_mc_batch",
                      "adock_batch_reset", "adock_set_chains_per_block", "adock_set_line_search_width", "adock_set_exact_staging", "adock_mc_step",
                      "adock_mlp_infer", "adock_gemm_f32", "adock_stage_a",
                      "adock_bias_act", "adock_relu_bwd", "adock_colsum", "adock_mse", "adock_adam", "adock_to_bf16"):

@@ -62,6 +62,23 @@ int adock_build_grid_fit(const float* rxyz, const int* rext, int n_rec, const fl
                                             nx, ny, nz, prm, out, (hipStream_t)stream));
 }
 
+int adock_build_grid_fit_sets(const float* rxyz, const int* rext, int n_sets, int n_rec, const float* rad_rec,
+                              const float* rad_lig, int n_lt, const float* W, int n_rt, const float* origins,
+                              float spacing, int nx, int ny, int nz, const float* prm, float* out,
+                              adock_stream_t stream) {
+    if (n_sets < 0 || n_rec < 0 || n_lt < 0 || n_rt < 0 || nx < 0 || ny < 0 || nz < 0)
+        return fail(ADOCK_E_ARG, "negative size");
+    if ((long long)nx * ny * nz == 0 || n_lt == 0 || n_sets == 0) return 0;
+    if (n_sets > 65535) return fail(ADOCK_E_ARG, "at most 65,535 receptor sets per call");
+    if (!rad_lig || !prm || !out || !origins || (n_rec > 0 && (!rxyz || !rext || !rad_rec || !W)))
+        return fail(ADOCK_E_ARG, "null pointer");
+    if (n_rt > build_grid_fit_rt_max() || (n_rec > 0 && n_rt < 1))
+        return fail(ADOCK_E_TYPES, "n_rt must be in 1..216 receptor types");
+    if ((long long)nx * ny * nz > (1ll << 31) - 1) return fail(ADOCK_E_ARG, "grid too large");
+    return hip_status(launch_build_grid_fit_sets(rxyz, rext, n_sets, n_rec, rad_rec, rad_lig, n_lt, W, n_rt, origins,
+                                                 spacing, nx, ny, nz, prm, out, (hipStream_t)stream));
+}
+
 int adock_inter_exact_fit(const float* xyz, const int* lig_type, int n_cx, int n_pose, int n_atom,
                           const int* rec_idx, const float* rxyz, const int* rext, int n_sets, int n_rec,
                           const float* rad_rec, const float* rad_lig, int n_lt, const float* W, int n_rt,

@@ -194,15 +194,22 @@ DEVI float fit_pair(const FitConst& f, const float* wq, float d, float ion) {
     return fmaf(wq[5], ion, sv);
 }
 
+// Several receptor sets in one launch: blockIdx.y = set, whose atoms start set_atoms atoms further on, whose origin is
+// origins[3 set ..] (NULL: ox, oy, oz) and whose grids start set_out floats further on (one set: 0, NULL, 0).
 __global__ __launch_bounds__(GB) void build_grid_fit_kernel(
     const float* __restrict__ rxyz, const int* __restrict__ rext, int n_rec, const float* __restrict__ rad_rec,
     const float* __restrict__ rad_lig, int n_lt, const float* __restrict__ W, int n_rt, int q0, float ox, float oy,
     float oz, float spacing, int nx, int ny, int nz, const float* __restrict__ prm, float* __restrict__ out,
-    int tiles_j, int tiles_k) {
+    int tiles_j, int tiles_k, long long set_atoms, const float* __restrict__ origins, long long set_out) {
     __shared__ float4 s_atom[GB];      // x, y, z, radius
     __shared__ int s_rt[GB];           // receptor ext type
     __shared__ int s_wcount[GB / WAVE];
     __shared__ float s_w[FIT_RT_MAX * TQ * FIT_NB];
+    const int set = blockIdx.y;
+    rxyz += (long long)set * set_atoms * 3;
+    rext += (long long)set * set_atoms;
+    out += (long long)set * set_out;
+    if (origins) { ox = origins[3 * set]; oy = origins[3 * set + 1]; oz = origins[3 * set + 2]; }
     const int tid = threadIdx.x;
     const int lane = tid & (WAVE - 1), wid = tid / WAVE;
     int tile = blockIdx.x;
@@ -394,7 +401,21 @@ int launch_build_grid_fit(const float* rxyz, const int* rext, int n_rec, const f
     for (int q0 = 0; q0 < n_lt; q0 += TQ)
         hipLaunchKernelGGL(build_grid_fit_kernel, dim3((unsigned)nblk), dim3(GB), 0, stream, rxyz, rext, n_rec,
                            rad_rec, rad_lig, n_lt, W, n_rt, q0, ox, oy, oz, spacing, nx, ny, nz, prm, out, tiles_j,
-                           tiles_k);
+                           tiles_k, 0ll, (const float*)nullptr, 0ll);
+    return (int)hipGetLastError();
+}
+
+int launch_build_grid_fit_sets(const float* rxyz, const int* rext, int n_sets, int n_rec, const float* rad_rec,
+                               const float* rad_lig, int n_lt, const float* W, int n_rt, const float* origins,
+                               float spacing, int nx, int ny, int nz, const float* prm, float* out,
+                               hipStream_t stream) {
+    const int tiles_i = (nx + TI - 1) / TI, tiles_j = (ny + TJ - 1) / TJ, tiles_k = (nz + TK - 1) / TK;
+    const long long nblk = (long long)tiles_i * tiles_j * tiles_k;
+    const long long set_out = (long long)n_lt * nx * ny * nz;
+    for (int q0 = 0; q0 < n_lt; q0 += TQ)
+        hipLaunchKernelGGL(build_grid_fit_kernel, dim3((unsigned)nblk, (unsigned)n_sets), dim3(GB), 0, stream, rxyz,
+                           rext, n_rec, rad_rec, rad_lig, n_lt, W, n_rt, q0, 0.f, 0.f, 0.f, spacing, nx, ny, nz, prm,
+                           out, tiles_j, tiles_k, (long long)n_rec, origins, set_out);
     return (int)hipGetLastError();
 }
 

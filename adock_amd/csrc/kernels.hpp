@@ -41,6 +41,8 @@ int launch_build_grid(const float*, const int*, int, const float*, const int*, c
 int launch_build_grid_fit(const float*, const int*, int, const float*, const float*, int, const float*, int, float,
                           float, float, float, int, int, int, const float*, float*, hipStream_t);
 int build_grid_fit_rt_max();
+int launch_build_grid_fit_sets(const float*, const int*, int, int, const float*, const float*, int, const float*, int,
+                               const float*, float, int, int, int, const float*, float*, hipStream_t);
 int launch_inter_exact_fit(const float*, const int*, int, int, int, const int*, const float*, const int*, int, int,
                            const float*, const float*, int, const float*, int, const float*, float*, hipStream_t);
 int launch_mc(int KM, const LigArgs&, const GridArgs&, const SFArgs&, const RecArgs&, int, const int*, int,

@@ -52,18 +52,50 @@ def build_grid_fit(rec_xyz, rec_ext, pot, origin, spacing, n, device="cuda"):
 def make_gridset_fit(preps, pot, spacing, penalty, device="cuda"):
     """GridSet of the complexes `preps` (dicts with rec_xyz, rec_ext, center, size): one [TL, n, n, n] cube each with
     the common edge n = ceil(max box edge / spacing) + 1, origin = centre - size / 2, box = centre -+ size / 2
-    (src/adock/fit/gridfit.py:96-108)."""
+    (src/adock/fit/gridfit.py:96-108). All complexes in one adock_build_grid_fit_sets launch per 8 ligand types
+    (receptors padded with ext type -1, which the kernel skips: each cube equals build_grid_fit's bit for bit)."""
+    if str(device).startswith("cpu"):
+        raise _lib.AdockError("adock_amd builds grids on the GPU only (device='cuda')")
+    _lib.require_gpu()
+    spec = pot.spec
     n = int(math.ceil(max(float(np.max(p["size"])) for p in preps) / spacing)) + 1
-    grids, origins, lo, hi = [], [], [], []
-    for p in preps:
+    R = len(preps)
+    NR = max(max(len(p["rec_ext"]) for p in preps), 1)
+    rx = np.zeros((R, NR, 3), np.float32)
+    re = -np.ones((R, NR), np.int32)
+    origins, lo, hi = [], [], []
+    for i, p in enumerate(preps):
+        e = np.asarray(p["rec_ext"], np.int32)
+        x = np.asarray(p["rec_xyz"], np.float32).reshape(-1, 3)
+        if len(e) != len(x):
+            raise _lib.AdockError("rec_xyz and rec_ext differ in length")
+        if len(e) and (e.min() < 0 or e.max() >= spec.n_rt):
+            raise _lib.AdockError(f"receptor ext types must lie in [0, {spec.n_rt})")
+        rx[i, :len(x)], re[i, :len(e)] = x, e
         c = np.asarray(p["center"], np.float64)
-        s = np.broadcast_to(np.asarray(p["size"], np.float64), (3,))
-        grids.append(build_grid_fit(p["rec_xyz"], p["rec_ext"], pot, c - s / 2, spacing, n, device=device))
-        origins.append(c - s / 2)
-        lo.append(c - s / 2)
-        hi.append(c + s / 2)
+        sz = np.broadcast_to(np.asarray(p["size"], np.float64), (3,))
+        origins.append(c - sz / 2)
+        lo.append(c - sz / 2)
+        hi.append(c + sz / 2)
+    if tuple(pot.W.shape) != (spec.n_rt, pot.TL, 6):
+        raise _lib.AdockError("pot.W must be [n_rt, TL, 6]")
+    if spec.n_rt > MAX_RT:
+        raise _lib.AdockError(f"at most {MAX_RT} receptor ext types")
+    dev = torch.device(device)
+    f32 = lambda a: torch.as_tensor(np.ascontiguousarray(a, np.float32), device=dev)
+    keep = [f32(rx), torch.as_tensor(re, device=dev), f32(spec.rec_radius),
+            pot.radius.to(dev, torch.float32).contiguous(), pot.W.to(dev, torch.float32).contiguous(),
+            f32(np.array(origins)), f32(spec.grid_prm())]
+    TL = pot.TL
+    out = torch.empty((R, TL, n, n, n), dtype=torch.float32, device=dev)
+    with torch.cuda.device(dev):
+        st = _lib.lib().adock_build_grid_fit_sets(
+            _lib.ptr(keep[0]), _lib.ptr(keep[1]), R, NR, _lib.ptr(keep[2]), _lib.ptr(keep[3]), TL, _lib.ptr(keep[4]),
+            spec.n_rt, _lib.ptr(keep[5]), float(spacing), n, n, n, _lib.ptr(keep[6]), _lib.ptr(out),
+            _lib.stream_of(dev))
+    _lib.check(st)
     T = lambda a: torch.as_tensor(np.array(a), dtype=torch.float32, device=device)
-    return GridSet(torch.stack(grids), T(origins), T(lo), T(hi), spacing, penalty)
+    return GridSet(out, T(origins), T(lo), T(hi), spacing, penalty)
 
 
 def inter_exact_fit(X, lig_types, rec_idx, rec_xyz, rec_ext, pot):

@@ -55,6 +55,16 @@ int adock_build_grid_fit(const float* rxyz, const int* rext, int n_rec, const fl
                          int n_lt, const float* W, int n_rt, float ox, float oy, float oz, float spacing, int nx,
                          int ny, int nz, const float* prm, float* out, adock_stream_t stream);
 
+/* adock_build_grid_fit for n_sets receptor sets in one launch per 8 ligand types, as make_gridset_fit needs them (the
+ * reference builds them one by one, src/adock/fit/gridfit.py:96-108). rxyz [n_sets, n_rec, 3], rext [n_sets, n_rec]
+ * (pad shorter sets with ext type -1: such atoms are skipped, so each set's grids equal adock_build_grid_fit's of its
+ * own atoms bit for bit), origins [n_sets, 3] (device), out [n_sets, n_lt, nx, ny, nz]; the rest as
+ * adock_build_grid_fit. n_sets <= 65,535. */
+int adock_build_grid_fit_sets(const float* rxyz, const int* rext, int n_sets, int n_rec, const float* rad_rec,
+                              const float* rad_lig, int n_lt, const float* W, int n_rt, const float* origins,
+                              float spacing, int nx, int ny, int nz, const float* prm, float* out,
+                              adock_stream_t stream);
+
 /* Exact pairwise fitted inter energy of docked poses (no grid). Replaces the torch loop Problem.inter_exact
  * (src/adock/search.py:73-89) over FittedPotential.energy (src/adock/fit/terms.py:339-344) that finalize runs for the
  * fitted function (search.py:483-489). xyz [n_cx, n_pose, n_atom, 3] pose coordinates, lig_type [n_cx, n_atom]

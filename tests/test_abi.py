@@ -63,6 +63,12 @@ def test_bucket_validation_without_gpu():
     host = ctypes.cast(buf, ctypes.c_void_p).value       # never dereferenced: the call fails its checks first
     bad_rt = [None, None, 0, None, host, 1, None, 300] + fit[8:15] + [host, host, None]
     assert L.adock_build_grid_fit(*bad_rt) == -4
+    # fitted grids of several receptor sets: no sets is a no-op, a missing origin array or too many sets is rejected
+    fs = [host, host, 2, 4, host, host, 1, host, 16, host, 0.375, 4, 4, 4, host, host, None]
+    assert L.adock_build_grid_fit_sets(*(fs[:2] + [0] + fs[3:])) == 0
+    assert L.adock_build_grid_fit_sets(*(fs[:9] + [None] + fs[10:])) == -1
+    assert L.adock_build_grid_fit_sets(*(fs[:2] + [70000] + fs[3:])) == -1
+    assert L.adock_build_grid_fit_sets(*(fs[:8] + [300] + fs[9:])) == -4
     # exact fitted inter energy: empty pose sets are a no-op; null pointers, negative sizes, no receptor set and
     # receptor atoms without their tables are rejected before any launch
     ex = [host, host, 2, 3, 5, host, None, None, 1, 0, None, None, 0, None, 0, host, host, None]
