@@ -934,26 +934,52 @@ __global__ __launch_bounds__(256) void mc_persistent_kernel(LigArgs L, GridArgs 
     }
 }
 
-template <int KM, bool EX>
+// MULTI = false: one state per wave. MULTI = true: each wave evaluates per_wave consecutive states and sets the
+// ligand up again only when chain_lig changes (what the setup leaves in LDS - the dummy atom, the zero slot, identity
+// maps past nt, pair words, slot lists - survives evaluations, as in a chain); it holds more registers, so it pays
+// only when there are enough states for several per wave.
+template <int KM, bool EX, bool MULTI>
 __global__ __launch_bounds__(256) void eg_kernel(LigArgs L, GridArgs G, SFArgs S, RecArgs R,
                                                   const int* __restrict__ chain_lig, int n_chains,
                                                   const float* __restrict__ state, float* __restrict__ e_out,
-                                                  float* __restrict__ g_out) {
+                                                  float* __restrict__ g_out, int per_wave) {
     extern __shared__ __attribute__((aligned(16))) char lds[];
     constexpr int SD = 7 + KM, DM = 6 + KM;
     const int lane = lane_id();
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
-    const int ch = blockIdx.x * (blockDim.x / WAVE) + wid;
-    if (ch >= n_chains) return;
+    const long long wave = (long long)blockIdx.x * (blockDim.x / WAVE) + wid;
+    const long long first = MULTI ? wave * per_wave : wave;
+    if (first >= n_chains) return;
     const WaveLDS w = carve_lds<KM>(lds + wid * wave_lds_bytes(KM, L.P_MAX), L.P_MAX);
     const SFConst sf = make_sf(S.prm);
-    const ChainCtx c = setup_chain<KM, false, EX>(__builtin_amdgcn_readfirstlane(chain_lig[ch]), lane, L, G, S, R, w);
-    const float x = lane < SD ? state[(long long)ch * SD + lane] : 0.f;
-    float g, ea_, x0_, x1_, x2_;
-    const float e = eval_eg<KM, false, EX>(c, sf, w, lane, x, g, ea_, x0_, x1_, x2_);
-    if (lane < 6) g_out[(long long)ch * DM + lane] = g;
-    else if (lane >= 7 && lane < 7 + c.nt) g_out[(long long)ch * DM + lane - 1] = g;
-    if (lane == 0) e_out[ch] = e;
+    if constexpr (!MULTI) {
+        const int ch = (int)first;
+        const ChainCtx c = setup_chain<KM, false, EX>(__builtin_amdgcn_readfirstlane(chain_lig[ch]), lane, L, G, S, R,
+                                                      w);
+        const float x = lane < SD ? state[(long long)ch * SD + lane] : 0.f;
+        float g, ea_, x0_, x1_, x2_;
+        const float e = eval_eg<KM, false, EX>(c, sf, w, lane, x, g, ea_, x0_, x1_, x2_);
+        if (lane < 6) g_out[(long long)ch * DM + lane] = g;
+        else if (lane >= 7 && lane < 7 + c.nt) g_out[(long long)ch * DM + lane - 1] = g;
+        if (lane == 0) e_out[ch] = e;
+        return;
+    }
+    const int last = (int)min(first + per_wave, (long long)n_chains);
+    ChainCtx c;
+    int l_set = -1;
+    for (int ch = (int)first; ch < last; ++ch) {
+        const int l = __builtin_amdgcn_readfirstlane(chain_lig[ch]);
+        if (l != l_set) {
+            c = setup_chain<KM, false, EX>(l, lane, L, G, S, R, w);
+            l_set = l;
+        }
+        const float x = lane < SD ? state[(long long)ch * SD + lane] : 0.f;
+        float g, ea_, x0_, x1_, x2_;
+        const float e = eval_eg<KM, false, EX>(c, sf, w, lane, x, g, ea_, x0_, x1_, x2_);
+        if (lane < 6) g_out[(long long)ch * DM + lane] = g;
+        else if (lane >= 7 && lane < 7 + c.nt) g_out[(long long)ch * DM + lane - 1] = g;
+        if (lane == 0) e_out[ch] = e;
+    }
 }
 
 inline int cu_count() {
@@ -1130,15 +1156,27 @@ int launch_mc_persistent_t(const LigArgs& L, const GridArgs& G, const SFArgs& S,
                  : launch_mc_persistent_tx<KM, false>(L, G, S, R, chain_lig, n_chains, M, Q, blocks, stream);
 }
 
+constexpr int EG_PER_WAVE = 8, EG_MULTI_WAVES_PER_CU = 16;
+
 template <int KM, bool EX>
 int launch_eg_tx(const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecArgs& R, const int* chain_lig,
                  int n_chains, const float* state, float* e_out, float* g_out, hipStream_t stream) {
     const int W = fit_waves(waves_for(n_chains), KM, L.P_MAX);
     const size_t shm = (size_t)W * wave_lds_bytes(KM, L.P_MAX);
+    // from EG_MULTI_WAVES_PER_CU x CUs waves' worth of EG_PER_WAVE states, several states per wave (65,536 poses of one
+    // complex: 250 -> 284 M evaluations/s)
+    if ((long long)n_chains >= (long long)EG_MULTI_WAVES_PER_CU * cu_count() * EG_PER_WAVE) {
+        const long long waves = ((long long)n_chains + EG_PER_WAVE - 1) / EG_PER_WAVE;
+        const unsigned nblk = (unsigned)((waves + W - 1) / W);
+        allow_lds(reinterpret_cast<const void*>(eg_kernel<KM, EX, true>), shm);
+        hipLaunchKernelGGL((eg_kernel<KM, EX, true>), dim3(nblk), dim3(W * WAVE), shm, stream, L, G, S, R, chain_lig,
+                           n_chains, state, e_out, g_out, EG_PER_WAVE);
+        return (int)hipGetLastError();
+    }
     const unsigned nblk = (unsigned)((n_chains + W - 1) / W);
-    allow_lds(reinterpret_cast<const void*>(eg_kernel<KM, EX>), shm);
-    hipLaunchKernelGGL((eg_kernel<KM, EX>), dim3(nblk), dim3(W * WAVE), shm, stream, L, G, S, R, chain_lig, n_chains,
-                       state, e_out, g_out);
+    allow_lds(reinterpret_cast<const void*>(eg_kernel<KM, EX, false>), shm);
+    hipLaunchKernelGGL((eg_kernel<KM, EX, false>), dim3(nblk), dim3(W * WAVE), shm, stream, L, G, S, R, chain_lig,
+                       n_chains, state, e_out, g_out, 1);
     return (int)hipGetLastError();
 }
 

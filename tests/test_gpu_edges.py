@@ -139,3 +139,30 @@ def test_exact_staging_bit_identical(width):
             np.testing.assert_array_equal(x, y)
             np.testing.assert_array_equal(x, z)
         assert np.isfinite(a[3]).all()
+
+
+@pytest.mark.parametrize("shuffle", [False, True])
+def test_energy_kernel_many_states_per_wave(shuffle):
+    """From 16 x CUs x 8 states, energy_grad evaluates 8 consecutive states per wave and sets a ligand up again only
+    when it changes: the results equal those of 4,096-state calls (one state per wave) bit for bit, with the states of
+    20 ligands in ligand order and shuffled (a new ligand at almost every state)."""
+    print(f"[gpu] energy kernel, several states per wave (shuffled {shuffle})", flush=True)
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    w, prob = workload.screen(seed=5, n_ligands=20, chains=8, steps=1)
+    kp = KernelProblem(prob)
+    L = len(w["ligs"])
+    per = max(40_000, 16 * cus * 8 + 100) // L + 1
+    st, cl = workload.chain_starts(5, w["ligs"], [w["center"]] * L, [w["size"]] * L, per, kp.KM)
+    if shuffle:
+        perm = np.random.default_rng(0).permutation(len(cl))
+        st, cl = st[perm], cl[perm]
+    st, cl = torch.as_tensor(st).cuda(), torch.as_tensor(cl).cuda()
+    N = len(cl)
+    assert N >= 16 * cus * 8
+    e, g = kp.energy_grad(cl, st)
+    parts = [kp.energy_grad(cl[i:i + 4096], st[i:i + 4096]) for i in range(0, N, 4096)]
+    e1, g1 = torch.cat([p[0] for p in parts]), torch.cat([p[1] for p in parts])
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(e.cpu().numpy(), e1.cpu().numpy())
+    np.testing.assert_array_equal(g.cpu().numpy(), g1.cpu().numpy())
+    assert np.isfinite(e.cpu().numpy()).all()
