@@ -49,6 +49,7 @@ def lib():
                                     [P, P, I, P, P, P, P, I, I, I, F, F, F, U64, P, P, P, P, I, I, F, F, F, F, P, P])
         L.adock_mlp_infer.argtypes = [I, P, P, P, P, P, P, P, P, P]
         L.adock_gemm_f32.argtypes = [I, I, I, I, I, F, P, I, P, I, F, P, I, P]
+        L.adock_gemm_f32_splitk.argtypes = [I, I, I, I, I, F, P, I, P, I, F, P, I, I, P, P]
         L.adock_bias_act.argtypes = [I, I, P, P, I, P]
         L.adock_relu_bwd.argtypes = [LL, P, P, P]
         L.adock_colsum.argtypes = [I, I, P, P, P]
@@ -66,7 +67,7 @@ def lib():
         L.adock_mc_resident_waves.restype = I
         for name in ("adock_build_grid", "adock_build_grid_fit", "adock_build_grid_fit_sets", "adock_inter_exact_fit", "adock_eg", "adock_mc", "adock_mc_counted", "adock_mc_batch",
                      "adock_batch_reset", "adock_set_chains_per_block", "adock_set_line_search_width", "adock_set_exact_staging", "adock_mc_step",
-                     "adock_mlp_infer", "adock_gemm_f32", "adock_stage_a",
+                     "adock_mlp_infer", "adock_gemm_f32", "adock_gemm_f32_splitk", "adock_stage_a",
                      "adock_bias_act", "adock_relu_bwd", "adock_colsum", "adock_mse", "adock_adam", "adock_to_bf16"):
             getattr(L, name).restype = I
         _LIB = L

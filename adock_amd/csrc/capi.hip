@@ -239,6 +239,18 @@ int adock_gemm_f32(int trans_a, int trans_b, int M, int N, int K, float alpha, c
     return hip_status(launch_gemm_f32(trans_a, trans_b, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
                                       (hipStream_t)stream));
 }
+int adock_gemm_f32_splitk(int trans_a, int trans_b, int M, int N, int K, float alpha, const float* A, int lda,
+                          const float* B, int ldb, float beta, float* C, int ldc, int splits, float* work,
+                          adock_stream_t stream) {
+    if (M < 0 || N < 0 || K < 0) return fail(ADOCK_E_ARG, "negative size");
+    if ((long long)M * N == 0) return 0;
+    if (splits < 1 || splits > 1024) return fail(ADOCK_E_ARG, "splits must be in 1..1024");
+    if (!A || !B || !C || !work) return fail(ADOCK_E_ARG, "null pointer");
+    if (K == 0) return hip_status(launch_gemm_f32(trans_a, trans_b, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
+                                                  (hipStream_t)stream));
+    return hip_status(launch_gemm_f32_splitk(trans_a, trans_b, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, splits,
+                                             work, (hipStream_t)stream));
+}
 int adock_bias_act(int M, int N, float* Y, const float* b, int relu, adock_stream_t stream) {
     if ((long long)M * N <= 0) return M < 0 || N < 0 ? fail(ADOCK_E_ARG, "negative size") : 0;
     return hip_status(launch_bias_act(M, N, Y, b, relu, (hipStream_t)stream));

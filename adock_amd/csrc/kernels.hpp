@@ -57,6 +57,8 @@ int launch_mlp_infer(int B, const float* X, const unsigned short* W1, const floa
                      const float* b2, const unsigned short* W3, const float* b3, float* Y, hipStream_t s);
 int launch_gemm_f32(int ta, int tb, int M, int N, int K, float alpha, const float* A, int lda, const float* Bm,
                     int ldb, float beta, float* C, int ldc, hipStream_t s);
+int launch_gemm_f32_splitk(int ta, int tb, int M, int N, int K, float alpha, const float* A, int lda, const float* Bm,
+                           int ldb, float beta, float* C, int ldc, int splits, float* work, hipStream_t s);
 int launch_bias_act(int M, int N, float* Y, const float* b, int relu, hipStream_t s);
 int launch_relu_bwd(long long n, float* dH, const float* H, hipStream_t s);
 int launch_colsum(int M, int N, const float* dY, float* db, hipStream_t s);

@@ -95,30 +95,36 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(int B, const float* __re
 }
 
 // C[M,N] = alpha op(A) op(B) + beta C, row-major; op(A) = A [M,K] (lda) or A^T with A [K,M]; same for B.
-template <bool TA, bool TB>
+// SPLIT: workgroup z sums k in [z kc, (z + 1) kc) only and stores its partial tile to P + z M N (row-major [M, N]);
+// splitk_reduce_kernel adds the partials in z order (deterministic) and applies alpha and beta. For the weight gradients
+// (M, N <= 256, K = the batch): 16 output tiles alone leave most of the chip idle.
+template <bool TA, bool TB, bool SPLIT = false>
 __global__ __launch_bounds__(256) void gemm_f32_kernel(int M, int N, int K, float alpha, const float* __restrict__ A,
                                                         int lda, const float* __restrict__ Bm, int ldb, float beta,
-                                                        float* __restrict__ C, int ldc) {
+                                                        float* __restrict__ C, int ldc, int kc = 0,
+                                                        float* __restrict__ P = nullptr) {
     constexpr int BM = 64, BN = 64, BK = 16;
     __shared__ float sA[BK][BM + 4];
     __shared__ float sB[BK][BN + 4];
     const int tid = threadIdx.x;
     const int tx = tid % 16, ty = tid / 16;
     const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+    const int kb = SPLIT ? blockIdx.z * kc : 0;
+    const int ke = SPLIT ? min(K, kb + kc) : K;
     float acc[4][4] = {};
-    for (int k0 = 0; k0 < K; k0 += BK) {
+    for (int k0 = kb; k0 < ke; k0 += BK) {
         for (int e = tid; e < BK * BM; e += 256) {
             const int kk = e / BM, mm = e % BM;
             const int m = m0 + mm, k = k0 + kk;
             float v = 0.f;
-            if (m < M && k < K) v = TA ? A[(long long)k * lda + m] : A[(long long)m * lda + k];
+            if (m < M && k < ke) v = TA ? A[(long long)k * lda + m] : A[(long long)m * lda + k];
             sA[kk][mm] = v;
         }
         for (int e = tid; e < BK * BN; e += 256) {
             const int kk = e / BN, nn = e % BN;
             const int n = n0 + nn, k = k0 + kk;
             float v = 0.f;
-            if (n < N && k < K) v = TB ? Bm[(long long)n * ldb + k] : Bm[(long long)k * ldb + n];
+            if (n < N && k < ke) v = TB ? Bm[(long long)n * ldb + k] : Bm[(long long)k * ldb + n];
             sB[kk][nn] = v;
         }
         __syncthreads();
@@ -144,10 +150,25 @@ __global__ __launch_bounds__(256) void gemm_f32_kernel(int M, int N, int K, floa
         for (int j = 0; j < 4; ++j) {
             const int n = n0 + tx * 4 + j;
             if (n >= N) continue;
-            float* c = C + (long long)m * ldc + n;
-            *c = alpha * acc[i][j] + (beta != 0.f ? beta * *c : 0.f);
+            if constexpr (SPLIT) {
+                P[((long long)blockIdx.z * M + m) * N + n] = acc[i][j];
+            } else {
+                float* c = C + (long long)m * ldc + n;
+                *c = alpha * acc[i][j] + (beta != 0.f ? beta * *c : 0.f);
+            }
         }
     }
+}
+
+__global__ void splitk_reduce_kernel(int M, int N, int S, const float* __restrict__ P, float alpha, float beta,
+                                     float* __restrict__ C, int ldc) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long long)M * N) return;
+    const long long mn = (long long)M * N;
+    float acc = 0.f;
+    for (int z = 0; z < S; ++z) acc += P[z * mn + e];
+    float* c = C + (e / N) * ldc + e % N;
+    *c = alpha * acc + (beta != 0.f ? beta * *c : 0.f);
 }
 
 // Y[M,N] += b (broadcast over rows), then ReLU if relu
@@ -230,6 +251,20 @@ int launch_gemm_f32(int ta, int tb, int M, int N, int K, float alpha, const floa
     else if (!ta && tb) hipLaunchKernelGGL((gemm_f32_kernel<false, true>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, Bm, ldb, beta, C, ldc);
     else if (ta && !tb) hipLaunchKernelGGL((gemm_f32_kernel<true, false>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, Bm, ldb, beta, C, ldc);
     else hipLaunchKernelGGL((gemm_f32_kernel<true, true>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, Bm, ldb, beta, C, ldc);
+    return (int)hipGetLastError();
+}
+
+int launch_gemm_f32_splitk(int ta, int tb, int M, int N, int K, float alpha, const float* A, int lda, const float* Bm,
+                           int ldb, float beta, float* C, int ldc, int splits, float* work, hipStream_t s) {
+    const int kc = ((K + splits - 1) / splits + 15) & ~15;   // whole 16-deep K steps per split
+    const int S = (K + kc - 1) / kc;
+    const dim3 grid((N + 63) / 64, (M + 63) / 64, S);
+    if (!ta && !tb) hipLaunchKernelGGL((gemm_f32_kernel<false, false, true>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, Bm, ldb, beta, C, ldc, kc, work);
+    else if (!ta && tb) hipLaunchKernelGGL((gemm_f32_kernel<false, true, true>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, Bm, ldb, beta, C, ldc, kc, work);
+    else if (ta && !tb) hipLaunchKernelGGL((gemm_f32_kernel<true, false, true>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, Bm, ldb, beta, C, ldc, kc, work);
+    else hipLaunchKernelGGL((gemm_f32_kernel<true, true, true>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, Bm, ldb, beta, C, ldc, kc, work);
+    hipLaunchKernelGGL(splitk_reduce_kernel, dim3(nb((long long)M * N, 256)), dim3(256), 0, s, M, N, S, work, alpha,
+                       beta, C, ldc);
     return (int)hipGetLastError();
 }
 

@@ -19,12 +19,34 @@ def _s(dev):
     return _lib.stream_of(dev)
 
 
+SPLIT_TILES = 128       # below this many 64x64 output tiles and with K >= SPLIT_MIN_K, K is split over workgroups
+SPLIT_MIN_K = 1024
+_WORK = {}
+
+
+def _work(dev, n):
+    """Split-K workspace of at least n floats on dev, kept for reuse (only the most recent launch reads it)."""
+    w = _WORK.get(dev)
+    if w is None or w.numel() < n:
+        w = _WORK[dev] = torch.empty(n, dtype=torch.float32, device=dev)
+    return w
+
+
 def gemm(A, B, C, trans_a=False, trans_b=False, alpha=1.0, beta=0.0):
-    """C = alpha op(A) op(B) + beta C on the GPU (row-major FP32, contiguous)."""
+    """C = alpha op(A) op(B) + beta C on the GPU (row-major FP32, contiguous). Products with few output tiles and a
+    long K (the weight gradients, K = the batch) split K over workgroups and add the partials in a fixed order."""
     M = A.shape[1] if trans_a else A.shape[0]
     K = A.shape[0] if trans_a else A.shape[1]
     N = B.shape[0] if trans_b else B.shape[1]
     assert C.shape == (M, N) and (B.shape[1] if trans_b else B.shape[0]) == K
+    tiles = ((M + 63) // 64) * ((N + 63) // 64)
+    if tiles < SPLIT_TILES and K >= SPLIT_MIN_K:
+        splits = min(max(1, 256 // tiles), K // 256)
+        w = _work(C.device, splits * M * N)
+        _lib.check(_lib.lib().adock_gemm_f32_splitk(
+            int(trans_a), int(trans_b), M, N, K, float(alpha), _lib.ptr(A), A.shape[1], _lib.ptr(B), B.shape[1],
+            float(beta), _lib.ptr(C), C.shape[1], splits, _lib.ptr(w), _s(C.device)))
+        return C
     _lib.check(_lib.lib().adock_gemm_f32(int(trans_a), int(trans_b), M, N, K, float(alpha), _lib.ptr(A), A.shape[1],
                                          _lib.ptr(B), B.shape[1], float(beta), _lib.ptr(C), C.shape[1], _s(C.device)))
     return C

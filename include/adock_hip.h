@@ -195,6 +195,12 @@ int adock_mlp_infer(int B, const float* X, const unsigned short* W1, const float
  * (dY = 2 mask (Y - T) inv_norm, *loss += sum mask (Y - T)^2 inv_norm); bias-corrected Adam step t; FP32 -> BF16. */
 int adock_gemm_f32(int trans_a, int trans_b, int M, int N, int K, float alpha, const float* A, int lda, const float* B,
                    int ldb, float beta, float* C, int ldc, adock_stream_t stream);
+/* adock_gemm_f32 with K split over up to `splits` workgroup layers (1..1024), for products with few output tiles and a
+ * long K (the weight gradients: K = the batch): work holds splits x M x N floats of partial tiles, which a second
+ * kernel adds in layer order (deterministic) before alpha and beta. */
+int adock_gemm_f32_splitk(int trans_a, int trans_b, int M, int N, int K, float alpha, const float* A, int lda,
+                          const float* B, int ldb, float beta, float* C, int ldc, int splits, float* work,
+                          adock_stream_t stream);
 int adock_bias_act(int M, int N, float* Y, const float* b, int relu, adock_stream_t stream);
 int adock_relu_bwd(long long n, float* dH, const float* H, adock_stream_t stream);
 int adock_colsum(int M, int N, const float* dY, float* db, adock_stream_t stream);

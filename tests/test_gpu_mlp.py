@@ -16,11 +16,14 @@ def _rel(a, b):
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-12))
 
 
+@pytest.mark.parametrize("K", [130, 3000])
 @pytest.mark.parametrize("ta,tb", [(False, False), (False, True), (True, False), (True, True)])
-def test_gemm_all_transposes(ta, tb):
-    print(f"[gpu] gemm ta={ta} tb={tb}", flush=True)
+def test_gemm_all_transposes(ta, tb, K):
+    """Both GEMM paths against FP64: K = 130 (one K range) and K = 3,000 over 2 output tiles (K split over
+    workgroups, partials added in order; K not a multiple of the split)."""
+    print(f"[gpu] gemm ta={ta} tb={tb} K={K}", flush=True)
     g = np.random.default_rng(0)
-    M, N, K = 77, 45, 130
+    M, N = 77, 45
     A = g.standard_normal((K, M) if ta else (M, K)).astype(np.float32)
     B = g.standard_normal((N, K) if tb else (K, N)).astype(np.float32)
     C0 = g.standard_normal((M, N)).astype(np.float32)
@@ -30,12 +33,14 @@ def test_gemm_all_transposes(ta, tb):
     assert _rel(C.cpu().numpy(), want) < 1e-5
 
 
-def test_forward_backward_adam_match_oracle():
-    print("[gpu] mlp fwd/bwd", flush=True)
+@pytest.mark.parametrize("B", [300, 2048])
+def test_forward_backward_adam_match_oracle(B):
+    """Forward, loss, gradients and one Adam step against the numpy FP64 oracle; with 2,048 rows the weight gradients
+    take the split-K GEMM."""
+    print(f"[gpu] mlp fwd/bwd B={B}", flush=True)
     net = ProposalNet("cuda", seed=3)
     p = {k: v.astype(np.float64) for k, v in net.numpy_params().items()}
     g = np.random.default_rng(4)
-    B = 300
     X = g.standard_normal((B, 128)).astype(np.float32)
     T = g.standard_normal((B, 23)).astype(np.float32)
     M = (g.random((B, 23)) < 0.7).astype(np.float32)
