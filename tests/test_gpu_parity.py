@@ -254,6 +254,24 @@ def test_bfgs_one_iteration_large_buckets(n_heavy, n_tors, KM):
     assert ok.mean() >= 0.99, ok.mean()
 
 
+@pytest.mark.parametrize("n_heavy,n_tors,KM", [(50, 20, 24), (64, 31, 32)])
+def test_mc_step_large_buckets(n_heavy, n_tors, KM):
+    """One MC step (the counter-RNG mutation: translation, rotation or one of up to 31 torsions) plus one BFGS
+    iteration at the larger buckets against the FP64 oracle: >= 97 % of 256 chains within 1e-3, as at KM 8."""
+    print(f"[gpu] mc one step, {n_heavy} atoms / {n_tors} torsions (KM {KM})", flush=True)
+    rec = workload.receptor_dict(synth.make_receptor(3, 1500, 20.0, 6.0), np.zeros(3), 24.0)
+    lig = synth.make_ligand(101 + KM, n_heavy, n_tors)
+    prob = workload.gpu_problem([rec], [lig], [0])
+    kp, op = KernelProblem(prob), _oracle_of(prob)
+    assert kp.KM == KM
+    st = states(40 + KM, 256, np.zeros(3), 20.0, lig, kp.KM, sigma=1.5)
+    out = _run_both(kp, op, st, False, 1, 1, seed=4242, step_offset=3)
+    eg, ec = out["gpu"][1], out["cpu"][1]
+    ok = np.abs(eg - ec) <= 1e-3 * np.maximum(np.abs(ec), 1)
+    print(f"  within 1e-3: {ok.mean():.4f}", flush=True)
+    assert ok.mean() >= 0.97, ok.mean()
+
+
 def test_bfgs_100_iterations(small_cx):
     print("[gpu] bfgs 100 iterations", flush=True)
     w, prob, kp, op = small_cx
