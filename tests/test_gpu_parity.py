@@ -254,6 +254,46 @@ def test_bfgs_one_iteration_large_buckets(n_heavy, n_tors, KM):
     assert ok.mean() >= 0.99, ok.mean()
 
 
+def test_bfgs_one_iteration_vinardo():
+    """One BFGS iteration under the Vinardo function (its own radii, Gaussian and repulsion terms, no h-bond
+    distance offset) against the FP64 oracle: >= 99 % of 512 chains within 1e-3."""
+    print("[gpu] bfgs 1 iteration vinardo", flush=True)
+    w, prob = workload.single("small", seed=3, sf="vinardo")
+    kp, op = KernelProblem(prob), _oracle_of(prob)
+    st = states(19, 512, w["center"], w["size"], w["lig"], kp.KM, sigma=1.0)
+    out = _run_both(kp, op, st, True, 0, 1)
+    eg, ec = out["gpu"][3], out["cpu"][3]
+    ok = np.abs(eg - ec) <= 1e-3 * np.maximum(np.abs(ec), 1)
+    print(f"  within 1e-3: {ok.mean():.4f}", flush=True)
+    assert ok.mean() >= 0.99, ok.mean()
+
+
+def test_exact_bfgs_large_bucket():
+    """Exact mode at the KM 32 bucket (64 heavy atoms, 31 torsions): one BFGS iteration over receptor neighbour lists
+    against the FP64 oracle on the same lists; lists longer than the 1,024 staged atoms read their atoms from memory
+    in the same launch. >= 99 % of 256 chains within 1e-3."""
+    print("[gpu] exact bfgs, 64 atoms / 31 torsions (KM 32)", flush=True)
+    # 2,600 receptor atoms at >= 2.3 A: the chains' lists straddle the 1,024-atom staging area
+    rec = workload.receptor_dict(synth.make_receptor(3, 2600, 20.0, 6.0, min_sep=2.3), np.zeros(3), 24.0)
+    lig = synth.make_ligand(133, 64, 31)
+    assert lig.n_tors == 31
+    prob = workload.gpu_problem([rec], [lig], [0])
+    kp, op = KernelProblem(prob), _oracle_of(prob)
+    assert kp.KM == 32
+    N = 256
+    st = states(50, N, np.zeros(3), 20.0, lig, kp.KM, sigma=1.5)
+    cl = np.zeros(N, np.int32)
+    nbr, nbr_n = kp.receptor_neighbours(prob, torch.as_tensor(cl).cuda(), torch.as_tensor(st).cuda(), prob.lb.K, 2.0)
+    nn = nbr_n.cpu().numpy()
+    print(f"  neighbour lists: {nn.min()}-{nn.max()} atoms, {(nn > 1024).sum()} of {N} longer than 1,024", flush=True)
+    assert 0 < (nn > 1024).sum() < N
+    out = _run_both(kp, op, st, True, 0, 1, cl=cl, nbr=(nbr.cpu().numpy(), nn))
+    eg, ec = out["gpu"][3], out["cpu"][3]
+    ok = np.abs(eg - ec) <= 1e-3 * np.maximum(np.abs(ec), 1)
+    print(f"  within 1e-3: {ok.mean():.4f}", flush=True)
+    assert ok.mean() >= 0.99, ok.mean()
+
+
 @pytest.mark.parametrize("n_heavy,n_tors,KM", [(50, 20, 24), (64, 31, 32)])
 def test_mc_step_large_buckets(n_heavy, n_tors, KM):
     """One MC step (the counter-RNG mutation: translation, rotation or one of up to 31 torsions) plus one BFGS
