@@ -43,6 +43,9 @@ def lib():
         L.adock_mc_batch.argtypes = (LIG + GRID + SF + REC +
                                      [I, P, P, I, P, P, I, P, P, P, P, I, I, I, F, F, F, U64, I, P, P, I, P, P, P, P, I,
                                       P, P])
+        L.adock_mc_batch_noalloc.argtypes = (LIG + GRID + SF + REC +
+                                             [I, P, P, I, P, P, I, P, P, P, P, I, I, I, F, F, F, U64, I, P, P, I, I, P,
+                                              P, I, P, P])
         L.adock_batch_reset.argtypes = [P, I, P, I, P, P]
         LL = ctypes.c_longlong
         L.adock_mc_step.argtypes = (LIG + GRID + SF + REC +
@@ -66,7 +69,7 @@ def lib():
         L.adock_mc_resident_waves.argtypes = [I, I, I]
         L.adock_mc_resident_waves.restype = I
         for name in ("adock_build_grid", "adock_build_grid_fit", "adock_build_grid_fit_sets", "adock_inter_exact_fit", "adock_eg", "adock_mc", "adock_mc_counted", "adock_mc_batch",
-                     "adock_batch_reset", "adock_set_chains_per_block", "adock_set_line_search_width", "adock_set_exact_staging", "adock_mc_step",
+                     "adock_mc_batch_noalloc", "adock_batch_reset", "adock_set_chains_per_block", "adock_set_line_search_width", "adock_set_exact_staging", "adock_mc_step",
                      "adock_mlp_infer", "adock_gemm_f32", "adock_gemm_f32_splitk", "adock_stage_a",
                      "adock_bias_act", "adock_relu_bwd", "adock_colsum", "adock_mse", "adock_adam", "adock_to_bf16"):
             getattr(L, name).restype = I

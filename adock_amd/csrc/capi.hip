@@ -1,6 +1,7 @@
 // extern "C" entry points of libadock_hip.so (declared in include/adock_hip.h): argument validation, bucket
 // dispatch, error text. No allocation, no copies, no synchronisation.
 #include <cstdio>
+#include <mutex>
 #include <string>
 
 #include "../../include/adock_hip.h"
@@ -188,6 +189,81 @@ int adock_mc_batch(int AM, int KM, LIG_DECL, GRID_DECL, SF_DECL, REC_DECL, int e
     McArgs M{chain_id, nbr, nbr_n, NB_MAX, cur, cur_e, best, best_e, init_opt, mc_steps, bfgs_iters, temp, amp_pos,
              amp_rot, seed, step_offset, eval_count};
     QueueArgs Q{order, queue_start, n_queues, counter, done, lig_chains, times};
+    return hip_status(launch_mc_persistent(KM, L, G, S, R, exact, chain_lig, n_chains, M, Q, blocks,
+                                           (hipStream_t)stream));
+}
+
+// ---- allocation-free batched launch ----
+// Workspace of adock_mc_batch_noalloc: the work-queue counters [ADOCK_NOALLOC_MAX_QUEUES], then the finished-chain
+// counts [ADOCK_NOALLOC_MAX_COMPLEXES]. One copy per GPU for the life of the process (264 KiB). Host code never
+// passes the array's name to a kernel: it is a host-side stand-in, not a device address. Each device's address comes
+// from hipGetSymbolAddress, taken once per device with that device current, and only that address is used.
+static __device__ __align__(256) int scratch[ADOCK_NOALLOC_MAX_QUEUES + ADOCK_NOALLOC_MAX_COMPLEXES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+int* g_scratch_dev[MAX_DEVICES] = {};
+
+// device address of the current device's scratch array (looked up on the first call on that device)
+int scratch_of_current_device(int** out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_status((int)e);
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(ADOCK_E_ARG, "device ordinal out of range for the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            g_err = std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e);
+            return e != hipSuccess ? (int)e : ADOCK_E_ARG;
+        }
+        g_scratch_dev[dev] = static_cast<int*>(p);
+    }
+    *out = g_scratch_dev[dev];
+    return 0;
+}
+}  // namespace
+
+int adock_mc_batch_noalloc(int AM, int KM, LIG_DECL, GRID_DECL, SF_DECL, REC_DECL, int exact, const int* chain_lig,
+                           const int* chain_id, int n_chains, const int* nbr, const int* nbr_n, int NB_MAX, float* cur,
+                           float* cur_e, float* best, float* best_e, int init_opt, int mc_steps, int bfgs_iters,
+                           float temp, float amp_pos, float amp_rot, unsigned long long seed, int step_offset,
+                           const int* order, const int* queue_start, int n_queues, int n_complexes,
+                           const int* lig_chains, unsigned long long* times, int blocks,
+                           unsigned long long* eval_count, adock_stream_t stream) {
+    if (n_queues < 0 || n_queues > ADOCK_NOALLOC_MAX_QUEUES)
+        return fail(ADOCK_E_ARG, "n_queues must be in 0..1024 for the allocation-free launch");
+    if (n_complexes < 0 || n_complexes > ADOCK_NOALLOC_MAX_COMPLEXES)
+        return fail(ADOCK_E_ARG, "n_complexes must be in 0..65536 for the allocation-free launch");
+    if (times && (!lig_chains || n_complexes < 1)) return fail(ADOCK_E_ARG, "times needs lig_chains and n_complexes");
+    // the checks adock_mc_batch makes, before anything is launched
+    if (int st = common_checks(AM, KM, P_MAX, ref, inc, ist, DEG_MAX, grid, prm, exact, rxyz, rn, RN_MAX, chain_lig,
+                               n_chains))
+        return st;
+    if (n_chains == 0) return 0;
+    if (!chain_id || !cur || !cur_e || !best || !best_e) return fail(ADOCK_E_ARG, "null pointer");
+    if (n_queues > 1 && !queue_start) return fail(ADOCK_E_ARG, "n_queues above 1 needs queue_start");
+    if (mc_steps < 0) return fail(ADOCK_E_ARG, "negative mc_steps");
+    if (nbr && (!nbr_n || NB_MAX <= 0)) return fail(ADOCK_E_ARG, "nbr needs nbr_n and NB_MAX > 0");
+    int* base = nullptr;
+    if (int st = scratch_of_current_device(&base)) return st;
+    int* counter = base;
+    int* done = times ? base + ADOCK_NOALLOC_MAX_QUEUES : nullptr;
+    const int nc = n_queues > 1 ? n_queues : 1;
+    // the array keeps the last call's values: clear this call's counters with a kernel (no memset node in a graph)
+    if (int st = hip_status(launch_batch_reset(counter, nc, done, done ? n_complexes : 0, times,
+                                               times ? 1 + n_complexes : 0, (hipStream_t)stream)))
+        return st;
+    LigArgs L{ref, typ, tm, ax, axp, axu, pr, na, nt, np, rg, P_MAX, gslot, inc, ist, DEG_MAX};
+    GridArgs G{grid, origin, lo, hi, rec_idx, goffs, gdims, T, spacing, pen};
+    SFArgs S{prm, rad, hyd, don, acc};
+    RecArgs R{rxyz, rtyp, rn, RN_MAX};
+    McArgs M{chain_id, nbr, nbr_n, NB_MAX, cur, cur_e, best, best_e, init_opt, mc_steps, bfgs_iters, temp, amp_pos,
+             amp_rot, seed, step_offset, eval_count};
+    QueueArgs Q{order, queue_start, n_queues, counter, done, lig_chains, times};
+    Q.n_done = n_complexes;   // a chain of a complex past n_complexes is run but not counted (its slot is not ours)
     return hip_status(launch_mc_persistent(KM, L, G, S, R, exact, chain_lig, n_chains, M, Q, blocks,
                                            (hipStream_t)stream));
 }

@@ -927,7 +927,7 @@ __global__ __launch_bounds__(256) void mc_persistent_kernel(LigArgs L, GridArgs 
         const int pos = qs + idx;
         const int ch = Q.order ? __builtin_amdgcn_readfirstlane(Q.order[pos]) : pos;
         const int l = run_chain<KM, 1, false, EX>(ch, lane, w, L, G, S, R, chain_lig, M);
-        if (Q.done && lane == 0) {
+        if (Q.done && lane == 0 && l < Q.n_done) {
             const int old = atomicAdd(Q.done + l, 1);
             if (old + 1 == Q.lig_chains[l] && Q.times) Q.times[1 + l] = __builtin_amdgcn_s_memrealtime();
         }

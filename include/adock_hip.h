@@ -160,6 +160,33 @@ int adock_mc_batch(int AM, int KM,
 int adock_batch_reset(int* counter, int n_counters, int* done, int n_complexes, unsigned long long* times,
                       adock_stream_t stream);
 
+/* adock_mc_batch for callers that cannot allocate (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc). No counterpart in the reference. Same arguments as adock_mc_batch minus the workspace (counter, done)
+ * and plus n_complexes (the length of lig_chains / times): the queue counters and the per-complex finished-chain counts
+ * live in a static device array that the library owns, one per GPU (264 KiB, held until the process ends), cleared by
+ * a kernel at the start of every call. The call allocates nothing and frees nothing. Limits: n_queues <= 1024,
+ * n_complexes <= 65536 (ADOCK_E_ARG beyond). Calls that overlap on one GPU (two streams, or two host threads) share
+ * the array and race: serialise them. Results are those of adock_mc_batch bit for bit. */
+#define ADOCK_NOALLOC_MAX_QUEUES 1024
+#define ADOCK_NOALLOC_MAX_COMPLEXES 65536
+int adock_mc_batch_noalloc(int AM, int KM,
+                           const float* ref, const int* typ, const unsigned char* tm, const int* ax, const float* axp,
+                           const float* axu, const int* pr, const int* na, const int* nt, const int* np,
+                           const float* rg, int P_MAX, const int* gslot, const unsigned short* inc, const int* ist,
+                           int DEG_MAX,
+                           const float* grid, const float* origin, const float* lo, const float* hi,
+                           const int* rec_idx, const long long* goffs, const long long* gdims, int T, float spacing,
+                           float pen,
+                           const float* prm, const float* rad, const int* hyd, const int* don, const int* acc,
+                           const float* rxyz, const int* rtyp, const int* rn, int RN_MAX,
+                           int exact,
+                           const int* chain_lig, const int* chain_id, int n_chains, const int* nbr, const int* nbr_n,
+                           int NB_MAX, float* cur, float* cur_e, float* best, float* best_e, int init_opt,
+                           int mc_steps, int bfgs_iters, float temp, float amp_pos, float amp_rot,
+                           unsigned long long seed, int step_offset, const int* order, const int* queue_start,
+                           int n_queues, int n_complexes, const int* lig_chains, unsigned long long* times,
+                           int blocks, unsigned long long* eval_count, adock_stream_t stream);
+
 /* ---- learned MC proposal network (128 -> 256 -> 256 -> 23, ReLU). No reference counterpart: the reference proposes
  * moves at random only (src/adock/mckernel.py:381-398). ---- */
 
