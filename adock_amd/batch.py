@@ -95,6 +95,7 @@ class BatchRunner:
         self.counter = torch.zeros(max(self.nq, 1), dtype=torch.int32, device=dev)
         self.done = torch.zeros(self.L, dtype=torch.int32, device=dev)
         self.times = torch.zeros(1 + self.L, dtype=torch.int64, device=dev)
+        self.lig_evals = torch.zeros(self.L, dtype=torch.int64, device=dev)   # evaluations per complex (count_evals)
         self.blocks = 0
 
     def launch(self, chain_id, cur, cur_e, best, best_e, init_opt, mc_steps, bfgs_iters, temp, amp_pos, amp_rot, seed,
@@ -116,7 +117,8 @@ class BatchRunner:
                                    int(seed) & (2 ** 64 - 1), int(step_offset), _lib.ptr(self.order),
                                    _lib.ptr(self.qstart), self.nq, _lib.ptr(self.counter), _lib.ptr(self.done), _lib.ptr(self.lig_chains),
                                    _lib.ptr(self.times), int(self.blocks),
-                                   _lib.ptr(kp.eval_count) if count_evals else None, st])
+                                   _lib.ptr(kp.eval_count) if count_evals else None,
+                                   _lib.ptr(self.lig_evals) if count_evals else None, st])
         _lib.check(L.adock_mc_batch(*args))
 
     def latencies(self):

@@ -171,7 +171,8 @@ int adock_mc_batch(int AM, int KM, LIG_DECL, GRID_DECL, SF_DECL, REC_DECL, int e
                    float* cur_e, float* best, float* best_e, int init_opt, int mc_steps, int bfgs_iters, float temp,
                    float amp_pos, float amp_rot, unsigned long long seed, int step_offset, const int* order,
                    const int* queue_start, int n_queues, int* counter, int* done, const int* lig_chains,
-                   unsigned long long* times, int blocks, unsigned long long* eval_count, adock_stream_t stream) {
+                   unsigned long long* times, int blocks, unsigned long long* eval_count,
+                   unsigned long long* lig_evals, adock_stream_t stream) {
     if (int st = common_checks(AM, KM, P_MAX, ref, inc, ist, DEG_MAX, grid, prm, exact, rxyz, rn, RN_MAX, chain_lig,
                                n_chains))
         return st;
@@ -188,6 +189,8 @@ int adock_mc_batch(int AM, int KM, LIG_DECL, GRID_DECL, SF_DECL, REC_DECL, int e
     RecArgs R{rxyz, rtyp, rn, RN_MAX};
     McArgs M{chain_id, nbr, nbr_n, NB_MAX, cur, cur_e, best, best_e, init_opt, mc_steps, bfgs_iters, temp, amp_pos,
              amp_rot, seed, step_offset, eval_count};
+    M.lig_evals = lig_evals;
+    M.n_lig_evals = 0x7fffffff;
     QueueArgs Q{order, queue_start, n_queues, counter, done, lig_chains, times};
     return hip_status(launch_mc_persistent(KM, L, G, S, R, exact, chain_lig, n_chains, M, Q, blocks,
                                            (hipStream_t)stream));
@@ -232,7 +235,7 @@ int adock_mc_batch_noalloc(int AM, int KM, LIG_DECL, GRID_DECL, SF_DECL, REC_DEC
                            float temp, float amp_pos, float amp_rot, unsigned long long seed, int step_offset,
                            const int* order, const int* queue_start, int n_queues, int n_complexes,
                            const int* lig_chains, unsigned long long* times, int blocks,
-                           unsigned long long* eval_count, adock_stream_t stream) {
+                           unsigned long long* eval_count, unsigned long long* lig_evals, adock_stream_t stream) {
     if (n_queues < 0 || n_queues > ADOCK_NOALLOC_MAX_QUEUES)
         return fail(ADOCK_E_ARG, "n_queues must be in 0..1024 for the allocation-free launch");
     if (n_complexes < 0 || n_complexes > ADOCK_NOALLOC_MAX_COMPLEXES)
@@ -264,6 +267,8 @@ int adock_mc_batch_noalloc(int AM, int KM, LIG_DECL, GRID_DECL, SF_DECL, REC_DEC
              amp_rot, seed, step_offset, eval_count};
     QueueArgs Q{order, queue_start, n_queues, counter, done, lig_chains, times};
     Q.n_done = n_complexes;   // a chain of a complex past n_complexes is run but not counted (its slot is not ours)
+    M.lig_evals = lig_evals;
+    M.n_lig_evals = n_complexes;
     return hip_status(launch_mc_persistent(KM, L, G, S, R, exact, chain_lig, n_chains, M, Q, blocks,
                                            (hipStream_t)stream));
 }

@@ -861,6 +861,7 @@ DEVI int run_chain(int ch, int lane, const WaveLDS& w, const LigArgs& L, const G
         if (lane == 0) {
             M.cur_e[ch] = ec; M.best_e[ch] = eb;
             if (M.eval_count) atomicAdd(M.eval_count, (unsigned long long)nev);
+            if (M.lig_evals && l < M.n_lig_evals) atomicAdd(M.lig_evals + l, (unsigned long long)nev);
         }
     }
     return l;

@@ -19,6 +19,8 @@ struct McArgs {
     float* cur; float* cur_e; float* best; float* best_e;
     int init_opt, mc_steps, bfgs_iters; float temp, amp_pos, amp_rot; unsigned long long seed; int step_offset;
     unsigned long long* eval_count;
+    unsigned long long* lig_evals = nullptr;   // per-complex evaluation counts [n_lig_evals] (NULL: none)
+    int n_lig_evals = 0;
     int NBS = 0;   // exact mode: receptor atoms staged in each wave's LDS (set by the launcher; 0 = read from memory)
     int NBM = 0;   // exact mode: per-atom masks over the staged atoms (set by the launcher)
 };

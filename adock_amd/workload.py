@@ -27,16 +27,17 @@ def grid_types(preps):
     return [np.unique(p.types[p.types >= 0]) for p in preps]
 
 
-def gpu_problem(receptors, preps, rec_idx, sf="vina", device="cuda", with_rec=True):
+def gpu_problem(receptors, preps, rec_idx, sf="vina", device="cuda", with_rec=True, keep_builder=False):
     """receptors: list of dicts (rec_xyz, rec_types, center, size); preps: one LigandPrep per complex; rec_idx [L].
-    Grids hold, per receptor, the union of the types of its ligands (+ C_H)."""
+    Grids hold, per receptor, the union of the types of its ligands (+ C_H). keep_builder: prob.gs.rebuild() refills
+    the grids in place from device-resident receptor atoms."""
     sfp = sf_params(CFG, sf)
     rec_idx = np.asarray(rec_idx, np.int64)
     lt = []
     for r in range(len(receptors)):
         ts = [p.types[p.types >= 0] for p, ri in zip(preps, rec_idx) if ri == r]
         lt.append(np.unique(np.concatenate(ts)) if ts else np.zeros(0, np.int64))
-    gs = make_gridset(receptors, sfp, SPACING, PENALTY, device=device, lig_types=lt)
+    gs = make_gridset(receptors, sfp, SPACING, PENALTY, device=device, lig_types=lt, keep_builder=keep_builder)
     lb = LigandBatch(preps, device)
     rec = pad_receptors(receptors, device) if with_rec else None
     return Problem(gs, lb, torch.as_tensor(rec_idx, device=device), PairPotential(sfp, device), rec=rec)
@@ -78,11 +79,12 @@ def screen(seed=0, n_ligands=512, chains=64, steps=500, sf="vina", device="cuda"
     return w, prob
 
 
-def multi(seed=0, n_receptors=64, ligands_per=32, chains=32, steps=500, sf="vina", device="cuda"):
+def multi(seed=0, n_receptors=64, ligands_per=32, chains=32, steps=500, sf="vina", device="cuda", keep_builder=False):
     """Multi-receptor batch: one grid set per receptor (its ligands' types + C_H), all resident."""
     w = synth.make_multi(seed, n_receptors, ligands_per, chains, steps)
     recs = [receptor_dict(r, np.zeros(3), b) for r, b in zip(w["receptors"], w["boxes"])]
-    prob = gpu_problem(recs, w["ligs"], w["rec_idx"], sf=sf, device=device, with_rec=False)
+    w["receptor_dicts"] = recs
+    prob = gpu_problem(recs, w["ligs"], w["rec_idx"], sf=sf, device=device, with_rec=False, keep_builder=keep_builder)
     return w, prob
 
 

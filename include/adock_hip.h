@@ -137,7 +137,8 @@ int adock_mc_counted(int AM, int KM,
  * bookkeeping: done [L] counts finished chains, lig_chains [L] gives chains per complex, times [1+L] receives the
  * 100 MHz real-time stamp of the first wave start (times[0]) and of each complex's last chain (times[1+l]).
  * counter, done and times must be reset before every launch (adock_batch_reset: a kernel, so the pair can be captured
- * in a HIP graph). eval_count may be NULL. No reference counterpart: the reference launches one block per NW chains
+ * in a HIP graph). eval_count may be NULL. lig_evals [L] (device, may be NULL) receives per complex the number of
+ * energy+gradient evaluations its chains performed (added to what it holds). No reference counterpart: the reference launches one block per NW chains
  * and compacts between segments on the host (src/adock/search.py:614-636). */
 int adock_mc_batch(int AM, int KM,
                    const float* ref, const int* typ, const unsigned char* tm, const int* ax, const float* axp,
@@ -153,7 +154,7 @@ int adock_mc_batch(int AM, int KM,
                    int bfgs_iters, float temp, float amp_pos, float amp_rot, unsigned long long seed,
                    int step_offset, const int* order, const int* queue_start, int n_queues, int* counter,
                    int* done, const int* lig_chains, unsigned long long* times, int blocks,
-                   unsigned long long* eval_count, adock_stream_t stream);
+                   unsigned long long* eval_count, unsigned long long* lig_evals, adock_stream_t stream);
 
 /* Resets the work queues of adock_mc_batch with a kernel launch: counter[0..n_counters) = 0, done[0..L) = 0,
  * times[0] = ~0, times[1..L] = 0 (done / times may be NULL). n_counters in 1..1024. */
@@ -165,7 +166,7 @@ int adock_batch_reset(int* counter, int n_counters, int* done, int n_complexes, 
  * and plus n_complexes (the length of lig_chains / times): the queue counters and the per-complex finished-chain counts
  * live in a static device array that the library owns, one per GPU (264 KiB, held until the process ends), cleared by
  * a kernel at the start of every call. The call allocates nothing and frees nothing. Limits: n_queues <= 1024,
- * n_complexes <= 65536 (ADOCK_E_ARG beyond). Calls that overlap on one GPU (two streams, or two host threads) share
+ * n_complexes <= 65536 (ADOCK_E_ARG beyond; lig_evals then has n_complexes entries). Calls that overlap on one GPU (two streams, or two host threads) share
  * the array and race: serialise them. Results are those of adock_mc_batch bit for bit. */
 #define ADOCK_NOALLOC_MAX_QUEUES 1024
 #define ADOCK_NOALLOC_MAX_COMPLEXES 65536
@@ -185,7 +186,8 @@ int adock_mc_batch_noalloc(int AM, int KM,
                            int mc_steps, int bfgs_iters, float temp, float amp_pos, float amp_rot,
                            unsigned long long seed, int step_offset, const int* order, const int* queue_start,
                            int n_queues, int n_complexes, const int* lig_chains, unsigned long long* times,
-                           int blocks, unsigned long long* eval_count, adock_stream_t stream);
+                           int blocks, unsigned long long* eval_count, unsigned long long* lig_evals,
+                           adock_stream_t stream);
 
 /* ---- learned MC proposal network (128 -> 256 -> 256 -> 23, ReLU). No reference counterpart: the reference proposes
  * moves at random only (src/adock/mckernel.py:381-398). ---- */
