@@ -16,6 +16,47 @@ from collections import deque
 import numpy as np
 
 
+def _components(n, bonds):
+    """Connected components of n atoms under `bonds` as labels 0.. numbered by each component's lowest atom."""
+    root = list(range(n))
+
+    def find(a):
+        while root[a] != a:
+            root[a] = root[root[a]]
+            a = root[a]
+        return a
+
+    for i, j in bonds:
+        ri, rj = find(i), find(j)
+        if ri != rj:
+            root[max(ri, rj)] = min(ri, rj)     # the lower atom stays the representative
+    rep = np.array([find(a) for a in range(n)], np.int64)
+    _, labels = np.unique(rep, return_inverse=True)   # representatives ascend with their lowest atom
+    return labels.astype(np.int64)
+
+
+def _level_sweep(nbrs, start):
+    """Breadth-first sweep of the fragment graph from `start`, one level at a time (a level's fragments in discovery
+    order, each one's neighbours in adjacency order). Returns (discovery order, depth array (-1 = unreached), parent
+    link per fragment: (parent, atom on the parent side, atom on this side), None at the start)."""
+    nf = len(nbrs)
+    depth = np.full(nf, -1, np.int64)
+    up = [None] * nf
+    depth[start] = 0
+    order, level = [start], [start]
+    while level:
+        nxt = []
+        for u in level:
+            for v, a, b in nbrs[u]:
+                if depth[v] < 0:
+                    depth[v] = depth[u] + 1
+                    up[v] = (u, a, b)
+                    nxt.append(v)
+        order += nxt
+        level = nxt
+    return order, depth, up
+
+
 class LigandPrep:
     """Heavy-atom ligand with its torsion tree.
 
@@ -47,73 +88,38 @@ class LigandPrep:
         return adj
 
     def _build_tree(self):
+        """Rigid fragments and the torsion tree. Fragments: connected components of the bond graph without the
+        rotatable bonds, numbered by their lowest atom index. Root: the fragment whose farthest fragment (in bonds of
+        the fragment tree) is nearest, ties to more heavy atoms, then the lower number. Torsion of a non-root fragment
+        f: axis (atom on the parent side, atom in f) of the bond to its parent, moving f and everything below it;
+        torsions leaf-to-root by depth, ties in breadth-first discovery order from the root (fragments adjacent in
+        rotatable-bond order)."""
         n = self.n
-        adj = self._adjacency(self.rot_bonds)
-        frag = -np.ones(n, dtype=np.int64)
-        nf = 0
-        for s in range(n):
-            if frag[s] >= 0:
-                continue
-            frag[s] = nf
-            dq = deque([s])
-            while dq:
-                u = dq.popleft()
-                for v in adj[u]:
-                    if frag[v] < 0:
-                        frag[v] = nf
-                        dq.append(v)
-            nf += 1
+        frag = _components(n, [b for b in self.bonds if b not in set(self.rot_bonds)])
+        nf = int(frag.max()) + 1 if n else 0
         self.frag = frag
-        fadj = [[] for _ in range(nf)]
+        # fragment adjacency, in rotatable-bond order: (neighbour fragment, atom here, atom there)
+        nbrs = [[] for _ in range(nf)]
         for i, j in self.rot_bonds:
-            fadj[frag[i]].append((frag[j], i, j))
-            fadj[frag[j]].append((frag[i], j, i))
-
-        def bfs(root):
-            depth, par, order = {root: 0}, {root: None}, [root]
-            dq = deque([root])
-            while dq:
-                u = dq.popleft()
-                for v, a, b in fadj[u]:
-                    if v not in depth:
-                        depth[v] = depth[u] + 1
-                        par[v] = (u, a, b)
-                        order.append(v)
-                        dq.append(v)
-            return depth, par, order
-
-        nheavy = np.bincount(frag[self.heavy], minlength=nf)
-        keys = [(max(bfs(f)[0].values()), -int(nheavy[f]), f) for f in range(nf)]
-        root = min(keys)[2]
-        depth, par, order = bfs(root)
-        if len(depth) != nf:
+            nbrs[frag[i]].append((frag[j], i, j))
+            nbrs[frag[j]].append((frag[i], j, i))
+        heavy_per_frag = np.bincount(frag[self.heavy], minlength=nf)
+        sweeps = [_level_sweep(nbrs, f) for f in range(nf)]
+        root = min(range(nf), key=lambda f: (int(sweeps[f][1][sweeps[f][0]].max()), -int(heavy_per_frag[f]), f))
+        order, depth, up = sweeps[root]
+        if len(order) != nf:
             raise ValueError("ligand graph is disconnected")
         self.root = root
-        children = {f: [] for f in range(nf)}
-        for f in order:
-            if par[f] is not None:
-                children[par[f][0]].append(f)
-
-        def subtree(f):
-            out, st = [], [f]
-            while st:
-                g = st.pop()
-                out.extend(np.where(frag == g)[0].tolist())
-                st.extend(children[g])
-            return out
-
-        tors = []
-        for f in order:
-            if par[f] is None:
-                continue
-            _, a, b = par[f]
-            mask = np.zeros(n, dtype=bool)
-            mask[subtree(f)] = True
-            tors.append((depth[f], a, b, mask))
-        tors.sort(key=lambda x: -x[0])   # stable: leaf-to-root
-        self.tors_axis = np.array([[a, b] for _, a, b, _ in tors], dtype=np.int64).reshape(-1, 2)
-        self.tors_mask = np.array([m for *_, m in tors], dtype=bool).reshape(-1, n)
-        self.n_tors = len(tors)
+        # atoms moved by each fragment's torsion: its own atoms and its descendants', gathered leaf-up
+        moved = np.zeros((nf, n), bool)
+        moved[frag, np.arange(n)] = True
+        for f in order[:0:-1]:
+            moved[up[f][0]] |= moved[f]
+        rank = {f: k for k, f in enumerate(order)}
+        kids = sorted(order[1:], key=lambda f: (-depth[f], rank[f]))
+        self.tors_axis = np.array([[up[f][1], up[f][2]] for f in kids], np.int64).reshape(-1, 2)
+        self.tors_mask = moved[kids].reshape(-1, n)
+        self.n_tors = len(kids)
 
     def topo_distance(self):
         adj = self._adjacency()

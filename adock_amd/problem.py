@@ -9,64 +9,64 @@ import math
 import numpy as np
 import torch
 
+from .grid import trilinear
 from .scoring import pair_terms
 
 K_BUCKET = 8
 
 
 class LigandBatch:
-    """Padded tensors of L ligand preps (one per complex), the torsion count bucketed to a multiple of 8
-    (src/adock/ligand.py:125-182): ref [L,A,3], types [L,A] (-1 = hydrogen / padding), heavy, amask, axis [L,K,2],
-    tmask [L,K,A], tvalid [L,K], pairs [L,P,2], pmask [L,P], rg, n_tors_vina, axis_p / axis_u [L,K,3]."""
+    """L ligands (one per complex) as padded tensors (the attribute set of the reference's LigandBatch,
+    src/adock/ligand.py:125-182): ref [L,A,3], types [L,A] (-1 = hydrogen or padding), heavy, amask, axis [L,K,2],
+    tmask [L,K,A], tvalid [L,K], pairs [L,P,2], pmask [L,P], rg, n_tors_vina, heavy_count, axis_p / axis_u [L,K,3]
+    (torsion axis point and unit vector in the ligand frame). K is the largest torsion count rounded up to a multiple
+    of 8; A and P are the largest atom and pair counts."""
+
+    # per-ligand tensors (first axis = ligand): what subset() slices
+    FIELDS = ("ref", "types", "heavy", "amask", "axis", "tmask", "tvalid", "pairs", "pmask", "rg", "n_tors_vina",
+              "heavy_count", "axis_p", "axis_u")
 
     def __init__(self, preps, device):
-        self.preps = preps
-        L = len(preps)
-        A = max(p.n for p in preps)
-        K = max(max(p.n_tors for p in preps), 1)
-        K = K_BUCKET * math.ceil(K / K_BUCKET)
-        P = max(max(len(p.pairs) for p in preps), 1)
-        ref = np.zeros((L, A, 3), np.float32)
-        types = -np.ones((L, A), np.int64)
-        amask = np.zeros((L, A), bool)
-        axis = np.zeros((L, K, 2), np.int64)
-        tmask = np.zeros((L, K, A), bool)
-        tvalid = np.zeros((L, K), bool)
-        pairs = np.zeros((L, P, 2), np.int64)
-        pmask = np.zeros((L, P), bool)
-        for l, p in enumerate(preps):
-            ref[l, :p.n] = p.ref
-            types[l, :p.n] = p.types
-            amask[l, :p.n] = True
-            axis[l, :p.n_tors] = p.tors_axis
-            tmask[l, :p.n_tors, :p.n] = p.tors_mask
-            tvalid[l, :p.n_tors] = True
-            pairs[l, :len(p.pairs)] = p.pairs
-            pmask[l, :len(p.pairs)] = True
-        T = lambda a: torch.as_tensor(a, device=device)
-        self.L, self.A, self.K, self.P = L, A, K, P
-        self.ref, self.types, self.amask = T(ref), T(types), T(amask)
+        self.preps = list(preps)
+        self.L = len(self.preps)
+        self.A = max(p.n for p in self.preps)
+        self.K = K_BUCKET * math.ceil(max(1, max(p.n_tors for p in self.preps)) / K_BUCKET)
+        self.P = max(1, max(len(p.pairs) for p in self.preps))
+        dev = torch.device(device)
+
+        def stack(get, tail, fill, dtype):
+            out = np.full((self.L,) + tail, fill, dtype)
+            for l, p in enumerate(self.preps):
+                v = np.asarray(get(p))
+                out[(l,) + tuple(slice(0, n) for n in v.shape)] = v
+            return torch.as_tensor(out, device=dev)
+
+        A, K, P = self.A, self.K, self.P
+        self.ref = stack(lambda p: p.ref, (A, 3), 0.0, np.float32)
+        self.types = stack(lambda p: p.types, (A,), -1, np.int64)
+        self.amask = stack(lambda p: np.ones(p.n, bool), (A,), False, bool)
+        self.axis = stack(lambda p: p.tors_axis.reshape(-1, 2), (K, 2), 0, np.int64)
+        self.tmask = stack(lambda p: p.tors_mask.reshape(-1, p.n), (K, A), False, bool)
+        self.tvalid = stack(lambda p: np.ones(p.n_tors, bool), (K,), False, bool)
+        self.pairs = stack(lambda p: p.pairs.reshape(-1, 2), (P, 2), 0, np.int64)
+        self.pmask = stack(lambda p: np.ones(len(p.pairs), bool), (P,), False, bool)
+        self.rg = torch.as_tensor(np.array([p.rg for p in self.preps], np.float32), device=dev)
+        self.n_tors_vina = torch.as_tensor(np.array([p.n_tors_vina for p in self.preps], np.float32), device=dev)
         self.heavy = self.types >= 0
-        self.axis, self.tmask, self.tvalid = T(axis), T(tmask), T(tvalid)
-        self.pairs, self.pmask = T(pairs), T(pmask)
-        self.rg = T(np.array([p.rg for p in preps], np.float32))
-        self.n_tors_vina = T(np.array([p.n_tors_vina for p in preps], np.float32))
         self.heavy_count = self.heavy.sum(1)
-        li = torch.arange(L, device=device)[:, None]
-        pa, pb = self.ref[li, self.axis[..., 0]], self.ref[li, self.axis[..., 1]]
-        u = pb - pa
-        self.axis_p = pa
-        self.axis_u = u / u.norm(dim=-1, keepdim=True).clamp(min=1e-6)
+        ends = self.ref.gather(1, self.axis.reshape(self.L, 2 * K, 1).expand(-1, -1, 3)).view(self.L, K, 2, 3)
+        self.axis_p = ends[:, :, 0]
+        axis = ends[:, :, 1] - ends[:, :, 0]
+        self.axis_u = axis / axis.norm(dim=-1, keepdim=True).clamp(min=1e-6)
 
     def subset(self, idx):
-        new = LigandBatch.__new__(LigandBatch)
-        new.preps = [self.preps[i] for i in idx.tolist()]
-        new.A, new.K, new.P = self.A, self.K, self.P
-        new.L = len(idx)
-        for k in ("ref", "types", "heavy", "amask", "axis", "tmask", "tvalid", "pairs", "pmask", "rg", "n_tors_vina",
-                  "heavy_count", "axis_p", "axis_u"):
-            setattr(new, k, getattr(self, k)[idx])
-        return new
+        """The ligands idx (a long tensor) as a new batch with the same padded sizes A, K, P."""
+        sub = LigandBatch.__new__(LigandBatch)
+        sub.preps = [self.preps[i] for i in idx.tolist()]
+        sub.L, sub.A, sub.K, sub.P = len(sub.preps), self.A, self.K, self.P
+        for name in self.FIELDS:
+            setattr(sub, name, getattr(self, name)[idx])
+        return sub
 
 
 class PairPotential:
@@ -99,28 +99,35 @@ class PairPotential:
 
 
 class Problem:
+    """A docking batch: grid set `gs`, ligand batch `lb`, receptor index of each complex `rec_idx`, pair potential
+    `pot`, and optionally the padded receptor atoms `rec` = (xyz [R,N,3], types [R,N], mask [R,N]) for exact
+    (grid-free) energies. Attribute names are the reference's (src/adock/search.py:65-71)."""
+
     def __init__(self, gridset, lb, rec_idx, pot, rec=None):
-        self.gs, self.lb, self.rec_idx, self.pot = gridset, lb, rec_idx, pot
-        self.rec = rec   # optional (xyz [R,N,3], types [R,N], mask [R,N]) for the exact (grid-free) energy
+        self.gs, self.lb, self.rec_idx, self.pot, self.rec = gridset, lb, rec_idx, pot, rec
 
     def subset(self, idx):
+        """The complexes idx (long tensor), sharing grids, potential and receptors."""
         return Problem(self.gs, self.lb.subset(idx), self.rec_idx[idx], self.pot, self.rec)
 
+    def _atom_grid_energy(self, X, penalty):
+        """Per heavy atom of poses X [L, M, A, 3]: trilinear grid energy (+ out-of-box penalty when `penalty`), zero on
+        hydrogens and padding."""
+        lb, gs = self.lb, self.gs
+        L, M, A = X.shape[:3]
+        rows = (L, M, A)
+        t = lb.types[:, None, :].expand(rows)
+        ri = self.rec_idx[:, None, None].expand(rows)
+        return trilinear(gs.flat, gs.origin, gs.lo, gs.hi, X, t, ri, lb.heavy[:, None, :].expand(rows), gs.dims,
+                         gs.offs, gs.slot, gs.spacing, gs.penalty if penalty else 0.0)
+
     def inter(self, X):
-        """Grid inter energy + out-of-box penalty [L, M] of heavy atoms X [L, M, A, 3] (src/adock/search.py:91-97)."""
-        lb = self.lb
-        L, M, A, _ = X.shape
-        t = lb.types[:, None, :].expand(L, M, A)
-        ri = self.rec_idx[:, None, None].expand(L, M, A)
-        return self.gs.lookup(X, t, ri, mask=lb.heavy[:, None, :].expand(L, M, A)).sum(-1)
+        """Grid inter energy + out-of-box penalty [L, M] of poses X [L, M, A, 3] (src/adock/search.py:91-97)."""
+        return self._atom_grid_energy(X, True).sum(-1)
 
     def inter_nopen(self, X):
-        """Grid inter energy without the out-of-box penalty (src/adock/search.py:99-105)."""
-        pen, self.gs.penalty = self.gs.penalty, 0.0
-        try:
-            return self.inter(X)
-        finally:
-            self.gs.penalty = pen
+        """Grid inter energy without the out-of-box penalty [L, M] (src/adock/search.py:99-105)."""
+        return self._atom_grid_energy(X, False).sum(-1)
 
     def inter_exact(self, X, chunk=20_000_000):
         """Direct pairwise ligand-receptor energy [L, M], no grid (src/adock/search.py:73-89). For cuda poses and a
@@ -171,16 +178,16 @@ class Problem:
         return ei + ea, ei, ea, X
 
     def intra(self, X):
-        """Intra-ligand energy [L, M] of coordinates X [L, M, A, 3] (src/adock/search.py:108-119)."""
+        """Intra-ligand energy [L, M] of poses X [L, M, A, 3]: the pair potential over each ligand's pair list
+        (src/adock/search.py:108-119)."""
         lb = self.lb
-        L, M, A, _ = X.shape
-        pi = lb.pairs[:, None, :, :].expand(L, M, lb.P, 2)
-        xi = torch.gather(X, 2, pi[..., 0:1].expand(L, M, lb.P, 3))
-        xj = torch.gather(X, 2, pi[..., 1:2].expand(L, M, lb.P, 3))
-        r = (xi - xj).norm(dim=-1)
-        ti = torch.gather(lb.types, 1, lb.pairs[..., 0])[:, None].expand(L, M, lb.P)
-        tj = torch.gather(lb.types, 1, lb.pairs[..., 1])[:, None].expand(L, M, lb.P)
-        e = self.pot.energy(r, ti.clamp(min=0), tj.clamp(min=0))
+        L, M = X.shape[:2]
+        P = lb.pairs.shape[1]
+        ends = lb.pairs.reshape(L, 1, 2 * P, 1).expand(L, M, 2 * P, 3)
+        Y = X.gather(2, ends).view(L, M, P, 2, 3)
+        r = (Y[..., 0, :] - Y[..., 1, :]).norm(dim=-1)
+        tt = lb.types.gather(1, lb.pairs.reshape(L, 2 * P)).view(L, 1, P, 2).clamp(min=0).expand(L, M, P, 2)
+        e = self.pot.energy(r, tt[..., 0], tt[..., 1])
         return torch.where(lb.pmask[:, None, :], e, torch.zeros_like(e)).sum(-1)
 
 

@@ -1,16 +1,21 @@
-"""The search driver around the kernels: Stage A start poses, Monte Carlo + BFGS in segments, refinement, exact
-polish and finalize (exact rescoring, clustering, affinities).
+"""The search driver around the kernels: Stage A start poses, Monte Carlo + BFGS in segments, refinement of every
+chain's best, exact-pairwise polish and finalize (exact rescoring, clustering, affinities).
 
-dock_problem_kernel reproduces the reference's engine-v2 control flow (src/adock/search.py:544-671): chain widening
-for small batches, Stage A (stage_a_kernel, src/adock/search.py:322-366), MC segments of mc_segment steps with
-init_opt on segment 0 and step offsets sg * mc_segment (so a split run equals an unsplit one), compaction of
-finished / early-stopped complexes, the size-scaled segment budget (mc_size_ref), the longer BFGS refinement of every
-chain's best state, the exact-pairwise polish of the polish_top best chains per complex with receptor neighbour lists,
-and finalize_kernel (src/adock/search.py:506-537). The kernels come from a KernelProblem (libadock_hip.so, MI355X);
-tests may pass another object with the same methods as `kernel_problem`.
+Contract kept from the reference's engine-v2 driver (src/adock/search.py:322-366, 506-671), because it fixes the
+results at a given seed: the widening rule for small batches, Stage A's draws from the torch Generator (torsion sets,
+then rotations) and its selection of start poses, MC segments of `mc_segment` steps whose step offsets are
+segment * mc_segment (so a split run equals an unsplit one), init_opt on the first segment only, the size-scaled
+budget (`mc_size_ref`), optional early stop, the refine and polish passes and their RNG seeds.
+
+How it runs here: the whole MC schedule (how many segments each complex runs, and so which chains each launch holds) is
+planned on the host from ligand sizes known before the search starts, so the segment loop launches back to back
+without reading anything from the device; only early stop (off in the presets) and the log callback need the
+per-segment best scores on the host. The kernels come from a KernelProblem (libadock_hip.so, MI355X); tests may pass
+another object with the same methods as `kernel_problem`.
 """
 import math
 import time
+from dataclasses import dataclass
 
 import numpy as np
 import torch
@@ -18,6 +23,7 @@ import torch
 from .pose import internal_coords, pose_coords, random_quat
 
 
+# ------------------------------------------------------------------------------------------------------ Stage A
 def _translation_lattices(prob, boxes_c, boxes_s, cfg):
     """(trans [L, maxT, 3] f32, nT [L]): each complex's translation lattice (step stageA_translation_step inside its
     box shrunk by 1 A), points whose nearest-grid C_H probe energy exceeds stageA_probe_max dropped (the box centre
@@ -51,45 +57,58 @@ def _translation_lattices(prob, boxes_c, boxes_s, cfg):
     return trans, torch.as_tensor(nT_h, device=dev)
 
 
+def _conformer_pool(prob, gen, trials, keep):
+    """Torsion sets for Stage A: `trials` draws per ligand, uniform in [-pi, pi) on its valid torsions, with draw 0 the
+    input conformation (all zero); the `keep` lowest in intra-ligand energy survive.
+    Returns (theta [L, C, K], intra energy [L, C], coordinates [L, C, A, 3])."""
+    lb = prob.lb
+    u = torch.rand((lb.L, trials, lb.K), generator=gen, device=lb.ref.device)
+    theta = (2.0 * u - 1.0) * math.pi
+    theta[:, 0] = 0.0
+    theta = theta * lb.tvalid[:, None, :]
+    e = prob.intra(internal_coords(lb, theta))
+    pick = torch.topk(-e, min(keep, trials), dim=1).indices
+    theta = theta.gather(1, pick[..., None].expand(-1, -1, lb.K))
+    return theta, e.gather(1, pick), internal_coords(lb, theta)
+
+
+def _heavy_slots(lb, AM):
+    """[L, AM] indices of each ligand's heavy atoms in kernel order (padding slots repeat atom 0)."""
+    idx = np.zeros((lb.L, AM), np.int64)
+    for l, p in enumerate(lb.preps):
+        hv = np.flatnonzero(p.heavy)
+        idx[l, :len(hv)] = hv
+    return torch.as_tensor(idx, device=lb.ref.device)
+
+
 @torch.no_grad()
 def stage_a_kernel(prob, kp, boxes_c, boxes_s, cfg, gen):
-    """Rigid enumeration: NT random torsion draws per ligand, the C lowest-intra-energy conformers x Q random rotations
-    x the complex's translation lattice (step stageA_translation_step inside the box shrunk by 1 A, points whose
-    nearest-grid C_H probe energy exceeds stageA_probe_max dropped), scored by kp.stage_a with the per-atom cap
-    stageA_atom_cap; the n_chains best (conformer, rotation, translation) triples per complex become the chains' start
-    poses. Returns (t [L,M,3], q [L,M,4], theta [L,M,K], e [L,M])."""
-    lb, gs = prob.lb, prob.gs
+    """Rigid enumeration for the chains' start poses: stageA_conformer_trials torsion draws per ligand, the
+    stageA_conformers lowest in intra energy, stageA_rotations random rotations, each complex's probe-filtered
+    translation lattice; kp.stage_a scores every (conformer, rotation) over the lattice (per-atom cap stageA_atom_cap)
+    and the n_chains lowest (rigid + intra) energies become the start poses (cycled when there are fewer candidates).
+    Returns (t [L, M, 3], q [L, M, 4], theta [L, M, K], e [L, M])."""
+    lb = prob.lb
     dev = lb.ref.device
-    L, K = lb.L, lb.K
-    C, NT, Q, M = cfg["stageA_conformers"], cfg["stageA_conformer_trials"], cfg["stageA_rotations"], cfg["n_chains"]
-    th = (torch.rand((L, NT, K), generator=gen, device=dev) * 2 - 1) * math.pi
-    th[:, 0] = 0.0
-    th = th * lb.tvalid[:, None, :]
-    e_intra = prob.intra(internal_coords(lb, th))
-    ci = torch.topk(-e_intra, min(C, NT), dim=1).indices
-    th_c = torch.gather(th, 1, ci[..., None].expand(L, ci.shape[1], K))
-    e_c = torch.gather(e_intra, 1, ci)
-    Xc = internal_coords(lb, th_c)
-    C = Xc.shape[1]
-    hidx_h = np.zeros((L, kp.AM), np.int64)
-    for l, p in enumerate(lb.preps):
-        hv = np.where(p.heavy)[0]
-        hidx_h[l, :len(hv)] = hv
-    hidx = torch.as_tensor(hidx_h, device=dev)
-    conf_h = torch.gather(Xc, 2, hidx[:, None, :, None].expand(L, C, kp.AM, 3))
-    quats = random_quat((Q,), gen, dev)
-    trans, nT = _translation_lattices(prob, boxes_c, boxes_s, cfg)
-    best_e, best_i = kp.stage_a(conf_h, quats, trans, nT, cfg.get("stageA_atom_cap", 1e9))
-    best_e = best_e + e_c.repeat_interleave(Q, dim=1)
-    k = min(M, C * Q)
-    top = torch.topk(-best_e, k, dim=1).indices
-    if k < M:
-        top = top[:, torch.arange(M, device=dev) % k]
-    li = torch.arange(L, device=dev)[:, None]
-    t0 = trans[li, best_i[li, top]]
-    return t0, quats[top % Q], th_c[li, top // Q], best_e[li, top]
+    M, Q = cfg["n_chains"], cfg["stageA_rotations"]
+    theta_c, e_conf, X = _conformer_pool(prob, gen, cfg["stageA_conformer_trials"], cfg["stageA_conformers"])
+    C = theta_c.shape[1]
+    hs = _heavy_slots(lb, kp.AM)
+    conf = X.gather(2, hs[:, None, :, None].expand(-1, C, -1, 3))
+    rot = random_quat((Q,), gen, dev)
+    trans, n_trans = _translation_lattices(prob, boxes_c, boxes_s, cfg)
+    rigid_e, rigid_i = kp.stage_a(conf, rot, trans, n_trans, cfg.get("stageA_atom_cap", 1e9))
+    # candidate c * Q + r = conformer c under rotation r
+    score = rigid_e + e_conf.repeat_interleave(Q, dim=1)
+    n_best = min(M, C * Q)
+    order = torch.topk(-score, n_best, dim=1).indices
+    if n_best < M:
+        order = order[:, torch.arange(M, device=dev) % n_best]
+    rows = torch.arange(lb.L, device=dev)[:, None]
+    return (trans[rows, rigid_i[rows, order]], rot[order % Q], theta_c[rows, order // Q], score[rows, order])
 
 
+# ----------------------------------------------------------------------------------------------------- finalize
 @torch.no_grad()
 def finalize_kernel(prob, kp, states, chain_lig, L, P, cfg):
     """Exact total energies (pairwise inter + penalty + intra, grid total without receptor atoms) of the L x P final
@@ -173,6 +192,135 @@ def finalize(prob, bt, bq, bth, cfg):
                         cfg["n_out"], inter=einter)
 
 
+# ------------------------------------------------------------------------------------------------- MC schedule
+@dataclass
+class MCSchedule:
+    """How the MC phase runs: chains per complex, steps per chain, steps per launch (segment) and, per complex, the
+    number of segments it runs (all equal unless the budget scales with ligand size)."""
+    widen: int
+    chains: int
+    steps: int
+    segment: int
+    seg_limit: np.ndarray
+
+    @property
+    def n_seg(self):
+        return int(self.seg_limit.max()) if self.steps > 0 and len(self.seg_limit) else 0
+
+
+def widen_factor(n_complexes, cfg):
+    """The reference's widening for small batches: the largest power of two f with
+    n_complexes * n_chains * f <= min_total_chains and mc_steps // f >= min_chain_steps (f = 1 when none)."""
+    target, floor = cfg.get("min_total_chains", 0), cfg.get("min_chain_steps", 32)
+    f = 1
+    while n_complexes * cfg["n_chains"] * 2 * f <= target and cfg["mc_steps"] // (2 * f) >= floor:
+        f *= 2
+    return f
+
+
+def plan_schedule(cfg, heavy_counts):
+    """MCSchedule of a search over complexes with these heavy-atom counts, and the cfg with the widened chain and step
+    counts. Size-scaled budget (mc_size_ref > 0, after Vina's 50 + heavy atoms): complex l runs
+    ceil(n_seg * (50 + nh_l) / (50 + mc_size_ref)) segments (at least one), computed in float32."""
+    L = len(heavy_counts)
+    f = widen_factor(L, cfg)
+    if f > 1:
+        cfg = dict(cfg, n_chains=cfg["n_chains"] * f, mc_steps=cfg["mc_steps"] // f,
+                   mc_segment=max(1, cfg.get("mc_segment", cfg["mc_steps"]) // f))
+    seg = max(1, cfg.get("mc_segment", cfg["mc_steps"]))
+    base = max(1, math.ceil(cfg["mc_steps"] / seg)) if cfg["mc_steps"] > 0 else 0
+    ref = cfg.get("mc_size_ref", 0)
+    if ref > 0 and base > 0:
+        nh = np.asarray(heavy_counts, np.float32)
+        lim = np.ceil(np.float32(base) * (np.float32(50.0) + nh) / np.float32(50 + ref)).astype(np.int64)
+        lim = np.maximum(lim, 1)
+    else:
+        lim = np.full(L, base, np.int64)
+    return MCSchedule(f, cfg["n_chains"], cfg["mc_steps"], seg, lim), cfg
+
+
+class _Chains:
+    """The chains of a search: state buffers [N, SD], their complex and global id, and launches over all of them or
+    over a subset (gathered into contiguous buffers and scattered back)."""
+
+    def __init__(self, kp, t, q, theta, L, M, dev):
+        N = L * M
+        self.kp, self.L, self.M, self.N = kp, L, M, N
+        self.cur = kp.pack(t.reshape(N, 3), q.reshape(N, 4), theta.reshape(N, -1))
+        self.best = self.cur.clone()
+        self.cur_e = torch.zeros(N, device=dev)
+        self.best_e = torch.full((N,), float("inf"), device=dev)
+        self.lig = torch.arange(L, device=dev, dtype=torch.int32).repeat_interleave(M)
+        self.gid = torch.arange(N, device=dev, dtype=torch.int32)
+
+    def launch(self, sel, init_opt, steps, mc, step_offset):
+        bufs = (self.cur, self.cur_e, self.best, self.best_e)
+        if sel is None:
+            self.kp.run_mc(self.lig, self.gid, *bufs, init_opt, steps, *mc, step_offset)
+            return
+        part = [b[sel].contiguous() for b in bufs]
+        self.kp.run_mc(self.lig[sel], self.gid[sel], *part, init_opt, steps, *mc, step_offset)
+        for b, p in zip(bufs, part):
+            b[sel] = p
+
+    def complex_best(self):
+        return self.best_e.view(self.L, self.M).min(1).values
+
+
+def _run_segments(ch, plan, cfg, seed64, dev, log):
+    """The MC phase: segment sg launches the chains of every complex still within its segment budget (and not stopped
+    early) for plan.segment steps at step offset sg * segment, init_opt on segment 0. Returns the chain-steps run."""
+    mc = (cfg["bfgs_iters"], cfg["mc_temperature"], cfg["mc_amp_pos"], cfg["mc_amp_rot"], seed64)
+    if plan.n_seg == 0:
+        ch.launch(None, True, 0, mc, 0)
+        return 0
+    es_rounds, es_tol = cfg.get("early_stop_rounds", 0), cfg.get("early_stop_tol", 0.01)
+    stopped = np.zeros(ch.L, bool)
+    record = torch.full((ch.L,), float("inf"), device=dev)
+    stall = torch.zeros(ch.L, dtype=torch.long, device=dev)
+    done_steps = 0
+    for sg in range(plan.n_seg):
+        run = (plan.seg_limit > sg) & ~stopped
+        n_run = int(run.sum())
+        if n_run == 0:
+            break
+        sel = None if n_run == ch.L else torch.as_tensor(
+            (np.flatnonzero(run)[:, None] * ch.M + np.arange(ch.M)).reshape(-1), device=dev)
+        ch.launch(sel, sg == 0, plan.segment, mc, sg * plan.segment)
+        done_steps += n_run * ch.M * plan.segment
+        # per-complex progress, kept on the device unless early stop or the log needs it on the host
+        cb = ch.complex_best()
+        stall = torch.where(cb < record - es_tol, torch.zeros_like(stall), stall + 1)
+        record = torch.minimum(record, cb)
+        if es_rounds > 0:
+            stopped |= run & (stall.cpu().numpy() >= es_rounds)
+        if log is not None:
+            log(f"segment {sg}: active {int(((plan.seg_limit > sg + 1) & ~stopped).sum())}/{ch.L}, "
+                f"best E mean {record.mean().item():.3f}")
+    return done_steps
+
+
+def _polish(prob, kp, ch, cfg, seed64, dev, mark):
+    """Exact-pairwise polish of each complex's polish_top lowest-grid-energy chain bests (polish_iters BFGS
+    iterations over receptor neighbour lists). Returns (states, chains per complex, complex of each state)."""
+    P = min(cfg.get("polish_top", 0), ch.M)
+    if P <= 0 or cfg.get("polish_iters", 0) <= 0 or prob.rec is None:
+        return ch.best, ch.M, ch.lig
+    top = torch.topk(-ch.best_e.view(ch.L, ch.M), P, dim=1).indices
+    sel = (top + torch.arange(ch.L, device=dev)[:, None] * ch.M).reshape(-1)
+    start = ch.best[sel].contiguous()
+    out = start.clone()
+    e_cur = torch.zeros(ch.L * P, device=dev)
+    e_best = torch.full((ch.L * P,), float("inf"), device=dev)
+    margin = cfg.get("polish_nbr_margin", 2.0)
+    nbr, nbr_n = kp.receptor_neighbours(prob, ch.lig[sel], start, prob.lb.K, margin) if margin > 0 else (None, None)
+    mark("neighbours")
+    kp.run_mc(ch.lig[sel], ch.gid[sel], start, e_cur, out, e_best, True, 0, cfg["polish_iters"],
+              cfg["mc_temperature"], cfg["mc_amp_pos"], cfg["mc_amp_rot"], seed64, 0, exact=True, nbr=nbr,
+              nbr_n=nbr_n)
+    return out, P, ch.lig[sel]
+
+
 def dock_problem_kernel(prob, boxes_c, boxes_s, cfg, seed=0, log=None, stats=None, kernel_problem=None):
     """Full search of every complex of `prob` (boxes_c / boxes_s: per-complex box centre and edge lengths), settings
     from a search preset (config.load_config()["search"]["v2_default"] etc.). Returns one dict per complex (coords
@@ -182,19 +330,10 @@ def dock_problem_kernel(prob, boxes_c, boxes_s, cfg, seed=0, log=None, stats=Non
     dev = lb.ref.device
     if kernel_problem is None:
         from .kernel import KernelProblem as kernel_problem
+    plan, cfg = plan_schedule(cfg, [int(p.heavy.sum()) for p in lb.preps])
     gen = torch.Generator(device=dev)
     gen.manual_seed(seed)
-    L = lb.L
-    # small batches leave the GPU idle and latency is set by chain depth: widen (more, shorter chains) at constant
-    # chain-steps until L * n_chains >= min_total_chains, keeping at least min_chain_steps steps per chain
-    f = 1
-    while (L * cfg["n_chains"] * f * 2 <= cfg.get("min_total_chains", 0)
-           and cfg["mc_steps"] // (f * 2) >= cfg.get("min_chain_steps", 32)):
-        f *= 2
-    if f > 1:
-        cfg = dict(cfg, n_chains=cfg["n_chains"] * f, mc_steps=cfg["mc_steps"] // f,
-                   mc_segment=max(1, cfg.get("mc_segment", cfg["mc_steps"]) // f))
-    M = cfg["n_chains"]
+    seed64 = int(seed) * 1_000_003 + 12345
     tick = {}
 
     def mark(k):
@@ -204,86 +343,30 @@ def dock_problem_kernel(prob, boxes_c, boxes_s, cfg, seed=0, log=None, stats=Non
 
     mark("t0")
     kp = kernel_problem(prob)
-    t, q0, th, _ = stage_a_kernel(prob, kp, boxes_c, boxes_s, cfg, gen)
+    t, q, theta, _ = stage_a_kernel(prob, kp, boxes_c, boxes_s, cfg, gen)
     mark("stageA")
-    N = L * M
-    cur = kp.pack(t.reshape(N, 3), q0.reshape(N, 4), th.reshape(N, -1))
-    best = cur.clone()
-    cur_e = torch.zeros(N, device=dev)
-    best_e = torch.full((N,), float("inf"), device=dev)
-    chain_lig = torch.arange(L, device=dev, dtype=torch.int32).repeat_interleave(M)
-    chain_id = torch.arange(N, device=dev, dtype=torch.int32)
-    seg = max(1, cfg.get("mc_segment", cfg["mc_steps"]))
-    n_seg = max(1, math.ceil(cfg["mc_steps"] / seg)) if cfg["mc_steps"] > 0 else 0
-    active = torch.ones(L, dtype=torch.bool, device=dev)
-    stall = torch.zeros(L, dtype=torch.long, device=dev)
-    glob = torch.full((L,), float("inf"), device=dev)
-    es_rounds, es_tol = cfg.get("early_stop_rounds", 0), cfg.get("early_stop_tol", 0.01)
-    # size-scaled budget (Vina scales its MC steps with 50 + heavy atoms): complex l runs
-    # n_seg * (50 + nh_l) / (50 + mc_size_ref) segments; the loop runs to the largest of them
-    if cfg.get("mc_size_ref", 0) > 0:
-        nh = lb.heavy.sum(1).float()
-        seg_lim = torch.ceil(n_seg * (50 + nh) / (50 + cfg["mc_size_ref"])).long().clamp(min=1)
-        n_seg = int(seg_lim.max()) if n_seg > 0 else 0
-    else:
-        seg_lim = torch.full((L,), n_seg, dtype=torch.long, device=dev)
-    chain_steps = 0
-    seed64 = int(seed) * 1_000_003 + 12345
-    mc_args = (cfg["bfgs_iters"], cfg["mc_temperature"], cfg["mc_amp_pos"], cfg["mc_amp_rot"], seed64)
-    if n_seg == 0:
-        kp.run_mc(chain_lig, chain_id, cur, cur_e, best, best_e, True, 0, *mc_args, 0)
-    for sg in range(n_seg):
-        active = active & (seg_lim > sg)
-        idx = active.repeat_interleave(M).nonzero().squeeze(1)
-        if len(idx) == 0:
-            break
-        if len(idx) == N:
-            kp.run_mc(chain_lig, chain_id, cur, cur_e, best, best_e, sg == 0, seg, *mc_args, sg * seg)
-        else:   # compaction: only the chains of still-active complexes are launched
-            c_, ce_ = cur[idx].contiguous(), cur_e[idx].contiguous()
-            b_, be_ = best[idx].contiguous(), best_e[idx].contiguous()
-            kp.run_mc(chain_lig[idx], chain_id[idx], c_, ce_, b_, be_, sg == 0, seg, *mc_args, sg * seg)
-            cur[idx], cur_e[idx], best[idx], best_e[idx] = c_, ce_, b_, be_
-        chain_steps += len(idx) * seg
-        gb = best_e.view(L, M).min(1).values
-        improved = gb < glob - es_tol
-        glob = torch.minimum(glob, gb)
-        stall = torch.where(improved, torch.zeros_like(stall), stall + 1)
-        if es_rounds > 0:
-            active = active & (stall < es_rounds)
-        if log is not None:
-            log(f"segment {sg}: active {int(active.sum())}/{L}, best E mean {glob.mean().item():.3f}")
+    ch = _Chains(kp, t, q, theta, lb.L, plan.chains, dev)
+    chain_steps = _run_segments(ch, plan, cfg, seed64, dev, log)
     mark("mc")
-    # refine every chain's best state with a longer BFGS
     if cfg.get("final_bfgs_iters", 0) > 0:
-        fe = torch.zeros(N, device=dev)
-        bb = best.clone()
-        kp.run_mc(chain_lig, chain_id, best, fe, bb, best_e, True, 0, cfg["final_bfgs_iters"], *mc_args[1:], 0)
-        best = bb
+        # each chain's best state gets a longer BFGS (its MC state is left as it was)
+        e0 = torch.zeros(ch.N, device=dev)
+        refined = ch.best.clone()
+        kp.run_mc(ch.lig, ch.gid, ch.best, e0, refined, ch.best_e, True, 0, cfg["final_bfgs_iters"],
+                  cfg["mc_temperature"], cfg["mc_amp_pos"], cfg["mc_amp_rot"], seed64, 0)
+        ch.best = refined
     mark("refine")
-    # exact polish: the polish_top lowest-grid-energy chain bests per complex get polish_iters BFGS iterations on the
-    # exact pairwise energy, each over its receptor neighbour list
-    P = min(cfg.get("polish_top", 0), M)
-    if P > 0 and cfg.get("polish_iters", 0) > 0 and prob.rec is not None:
-        top = torch.topk(-best_e.view(L, M), P, dim=1).indices
-        sel = (top + torch.arange(L, device=dev)[:, None] * M).reshape(-1)
-        pc, pb = best[sel].contiguous(), best[sel].clone()
-        pe, pbe = torch.zeros(L * P, device=dev), torch.full((L * P,), float("inf"), device=dev)
-        margin = cfg.get("polish_nbr_margin", 2.0)
-        nb = kp.receptor_neighbours(prob, chain_lig[sel], pc, lb.K, margin) if margin > 0 else (None, None)
-        mark("neighbours")
-        kp.run_mc(chain_lig[sel], chain_id[sel], pc, pe, pb, pbe, True, 0, cfg["polish_iters"], *mc_args[1:], 0,
-                  exact=True, nbr=nb[0], nbr_n=nb[1])
-        best, M, chain_lig = pb, P, chain_lig[sel]
+    states, P, lig_of = _polish(prob, kp, ch, cfg, seed64, dev, mark)
     mark("polish")
     if cfg.get("finalize_torch", False):
-        bt, bq, bth = kp.unpack(best, lb.K)
-        res = finalize(prob, bt.view(L, M, 3), bq.view(L, M, 4), bth.view(L, M, lb.K), cfg)
+        bt, bq, bth = kp.unpack(states, lb.K)
+        res = finalize(prob, bt.view(lb.L, P, 3), bq.view(lb.L, P, 4), bth.view(lb.L, P, lb.K), cfg)
     else:
-        res = finalize_kernel(prob, kp, best, chain_lig, L, M, cfg)
+        res = finalize_kernel(prob, kp, states, lig_of, lb.L, P, cfg)
     mark("final")
     if stats is not None:
         ks = list(tick)
-        stats.update(widen=f, chain_steps=chain_steps, chain_steps_full=int(seg_lim.sum()) * cfg["n_chains"] * seg,
+        stats.update(widen=plan.widen, chain_steps=chain_steps,
+                     chain_steps_full=int(plan.seg_limit.sum()) * plan.chains * plan.segment if plan.n_seg else 0,
                      **{f"t_{b}": round(tick[b] - tick[a], 3) for a, b in zip(ks[:-1], ks[1:])})
     return res
