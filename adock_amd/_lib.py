@@ -66,7 +66,7 @@ def lib():
         L.adock_get_line_search_width.restype = I
         L.adock_set_exact_staging.argtypes = [I]
         L.adock_get_exact_staging.restype = I
-        L.adock_mc_resident_waves.argtypes = [I, I, I]
+        L.adock_mc_resident_waves.argtypes = [I, I, I, I]
         L.adock_mc_resident_waves.restype = I
         for name in ("adock_build_grid", "adock_build_grid_fit", "adock_build_grid_fit_sets", "adock_inter_exact_fit", "adock_eg", "adock_mc", "adock_mc_counted", "adock_mc_batch",
                      "adock_mc_batch_noalloc", "adock_batch_reset", "adock_set_chains_per_block", "adock_set_line_search_width", "adock_set_exact_staging", "adock_mc_step",

@@ -22,10 +22,13 @@ static int hip_status(int st) {
     return st;
 }
 
+// AM: 64 (up to 64 heavy atoms, one per lane) or the 32-multiple buckets 96 and 128 (two atoms per lane); KM: the
+// 8-multiple buckets up to 56 (state lanes 7 + KM of one wave)
 static int check_buckets(int AM, int KM, int P_MAX) {
-    if (AM != 64) return fail(ADOCK_E_BUCKET, "AM must be 64 (at most 64 heavy atoms per ligand)");
-    if (KM != 8 && KM != 16 && KM != 24 && KM != 32) return fail(ADOCK_E_BUCKET, "KM must be one of 8, 16, 24, 32");
-    if (P_MAX < 1 || !lds_budget_ok(KM, P_MAX))
+    if (AM != 64 && AM != 96 && AM != 128)
+        return fail(ADOCK_E_BUCKET, "AM must be 64, 96 or 128 (at most 128 heavy atoms per ligand)");
+    if (KM < 8 || KM > 56 || (KM & 7)) return fail(ADOCK_E_BUCKET, "KM must be a multiple of 8 in 8..56");
+    if (P_MAX < 1 || P_MAX > 32767 || !lds_budget_ok(KM, P_MAX, AM))
         return fail(ADOCK_E_LDS, "P_MAX out of range for the per-wave LDS budget");
     return 0;
 }
@@ -111,8 +114,8 @@ static int common_checks(int AM, int KM, int P_MAX, const void* ref, const void*
                          const void* grid, const void* prm, int exact, const void* rxyz, const void* rn, int RN_MAX,
                          const void* chain_lig, int n_chains) {
     if (int st = check_buckets(AM, KM, P_MAX)) return st;
-    if (DEG_MAX < 0 || DEG_MAX > 64 || (DEG_MAX & 7))
-        return fail(ADOCK_E_ARG, "DEG_MAX must be a multiple of 8 in 0..64");
+    if (DEG_MAX < 0 || DEG_MAX > AM || (DEG_MAX & 7))
+        return fail(ADOCK_E_ARG, "DEG_MAX must be a multiple of 8 in 0..AM");
     if (n_chains < 0) return fail(ADOCK_E_ARG, "negative n_chains");
     if (!ref || !inc || !ist || !grid || !prm || (n_chains > 0 && !chain_lig)) return fail(ADOCK_E_ARG, "null pointer");
     if (exact && (!rxyz || !rn || RN_MAX <= 0)) return fail(ADOCK_E_ARG, "exact mode needs receptor atoms");
@@ -126,7 +129,7 @@ int adock_eg(int AM, int KM, LIG_DECL, GRID_DECL, SF_DECL, REC_DECL, int exact, 
         return st;
     if (n_chains == 0) return 0;
     if (!state || !e_out || !g_out) return fail(ADOCK_E_ARG, "null pointer");
-    LigArgs L{ref, typ, tm, ax, axp, axu, pr, na, nt, np, rg, P_MAX, gslot, inc, ist, DEG_MAX};
+    LigArgs L{ref, typ, tm, ax, axp, axu, pr, na, nt, np, rg, P_MAX, gslot, inc, ist, DEG_MAX, AM};
     GridArgs G{grid, origin, lo, hi, rec_idx, goffs, gdims, T, spacing, pen};
     SFArgs S{prm, rad, hyd, don, acc};
     RecArgs R{rxyz, rtyp, rn, RN_MAX};
@@ -145,7 +148,7 @@ int adock_mc_counted(int AM, int KM, LIG_DECL, GRID_DECL, SF_DECL, REC_DECL, int
     if (!chain_id || !cur || !cur_e || !best || !best_e) return fail(ADOCK_E_ARG, "null pointer");
     if (mc_steps < 0) return fail(ADOCK_E_ARG, "negative mc_steps");
     if (nbr && (!nbr_n || NB_MAX <= 0)) return fail(ADOCK_E_ARG, "nbr needs nbr_n and NB_MAX > 0");
-    LigArgs L{ref, typ, tm, ax, axp, axu, pr, na, nt, np, rg, P_MAX, gslot, inc, ist, DEG_MAX};
+    LigArgs L{ref, typ, tm, ax, axp, axu, pr, na, nt, np, rg, P_MAX, gslot, inc, ist, DEG_MAX, AM};
     GridArgs G{grid, origin, lo, hi, rec_idx, goffs, gdims, T, spacing, pen};
     SFArgs S{prm, rad, hyd, don, acc};
     RecArgs R{rxyz, rtyp, rn, RN_MAX};
@@ -183,7 +186,7 @@ int adock_mc_batch(int AM, int KM, LIG_DECL, GRID_DECL, SF_DECL, REC_DECL, int e
         return fail(ADOCK_E_ARG, "n_queues must be in 0..1024, with queue_start when above 1");
     if (mc_steps < 0) return fail(ADOCK_E_ARG, "negative mc_steps");
     if (nbr && (!nbr_n || NB_MAX <= 0)) return fail(ADOCK_E_ARG, "nbr needs nbr_n and NB_MAX > 0");
-    LigArgs L{ref, typ, tm, ax, axp, axu, pr, na, nt, np, rg, P_MAX, gslot, inc, ist, DEG_MAX};
+    LigArgs L{ref, typ, tm, ax, axp, axu, pr, na, nt, np, rg, P_MAX, gslot, inc, ist, DEG_MAX, AM};
     GridArgs G{grid, origin, lo, hi, rec_idx, goffs, gdims, T, spacing, pen};
     SFArgs S{prm, rad, hyd, don, acc};
     RecArgs R{rxyz, rtyp, rn, RN_MAX};
@@ -259,7 +262,7 @@ int adock_mc_batch_noalloc(int AM, int KM, LIG_DECL, GRID_DECL, SF_DECL, REC_DEC
     if (int st = hip_status(launch_batch_reset(counter, nc, done, done ? n_complexes : 0, times,
                                                times ? 1 + n_complexes : 0, (hipStream_t)stream)))
         return st;
-    LigArgs L{ref, typ, tm, ax, axp, axu, pr, na, nt, np, rg, P_MAX, gslot, inc, ist, DEG_MAX};
+    LigArgs L{ref, typ, tm, ax, axp, axu, pr, na, nt, np, rg, P_MAX, gslot, inc, ist, DEG_MAX, AM};
     GridArgs G{grid, origin, lo, hi, rec_idx, goffs, gdims, T, spacing, pen};
     SFArgs S{prm, rad, hyd, don, acc};
     RecArgs R{rxyz, rtyp, rn, RN_MAX};
@@ -290,11 +293,12 @@ int adock_mc_step(int AM, int KM, LIG_DECL, GRID_DECL, SF_DECL, REC_DECL, const 
     if (int st = common_checks(AM, KM, P_MAX, ref, inc, ist, DEG_MAX, grid, prm, 0, rxyz, rn, RN_MAX, chain_lig,
                                n_chains))
         return st;
+    if (AM != 64) return fail(ADOCK_E_BUCKET, "the learned step kernel takes ligands of at most 64 heavy atoms");
     if (n_chains == 0) return 0;
     if (!chain_id || !cur || !cur_e || !best || !best_e) return fail(ADOCK_E_ARG, "null pointer");
     if (log_buf && (!log_count || !feat || log_cap < 0)) return fail(ADOCK_E_ARG, "logging needs feat and a counter");
     if (use_net && (!Y || !feat)) return fail(ADOCK_E_ARG, "network moves need Y and feat");
-    LigArgs L{ref, typ, tm, ax, axp, axu, pr, na, nt, np, rg, P_MAX, gslot, inc, ist, DEG_MAX};
+    LigArgs L{ref, typ, tm, ax, axp, axu, pr, na, nt, np, rg, P_MAX, gslot, inc, ist, DEG_MAX, AM};
     GridArgs G{grid, origin, lo, hi, rec_idx, goffs, gdims, T, spacing, pen};
     SFArgs S{prm, rad, hyd, don, acc};
     RecArgs R{rxyz, rtyp, rn, RN_MAX};
@@ -364,7 +368,7 @@ int adock_stage_a(int AM, const float* conf, const int* gslot, const int* na, in
                   const float* lo, const float* hi, const int* rec_idx, const long long* goffs,
                   const long long* gdims, float spacing, float pen, float cap, float* best_e, int* best_i,
                   adock_stream_t stream) {
-    if (AM != 64) return fail(ADOCK_E_BUCKET, "AM must be 64 (at most 64 heavy atoms per ligand)");
+    if (AM < 64 || AM > 256 || (AM & 31)) return fail(ADOCK_E_BUCKET, "AM must be a multiple of 32 in 64..256");
     if (L < 0 || C < 0 || Q < 0 || NT_MAX < 0) return fail(ADOCK_E_ARG, "negative size");
     if ((long long)L * C * Q == 0) return 0;
     if ((long long)L * C * Q > 0x7fffffffll) return fail(ADOCK_E_ARG, "L * C * Q too large");
@@ -394,9 +398,9 @@ int adock_set_exact_staging(int on) {
 }
 int adock_get_exact_staging(void) { return get_exact_staging(); }
 
-int adock_mc_resident_waves(int KM, int P_MAX, int persistent) {
-    if (int st = check_buckets(64, KM, P_MAX)) return st;
-    const int w = mc_resident_waves(KM, P_MAX, persistent ? 1 : 0);
+int adock_mc_resident_waves(int AM, int KM, int P_MAX, int persistent) {
+    if (int st = check_buckets(AM, KM, P_MAX)) return st;
+    const int w = mc_resident_waves(KM, P_MAX, persistent ? 1 : 0, AM);
     return w < 0 ? fail(ADOCK_E_ARG, "occupancy query failed") : w;
 }
 

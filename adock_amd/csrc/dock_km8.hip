@@ -1,0 +1,6 @@
+// Chain kernels of the torsion bucket KM = 8 (the shared implementation is in dock_impl.hpp).
+#include "dock_impl.hpp"
+
+namespace adock {
+ADOCK_BUCKET_DEFINE(8)
+}  // namespace adock

@@ -7,6 +7,7 @@ struct LigArgs {
     const float* ref; const int* typ; const unsigned char* tm; const int* ax; const float* axp; const float* axu;
     const int* pr; const int* na; const int* nt; const int* np; const float* rg; int P_MAX; const int* gslot;
     const unsigned short* inc; const int* ist; int DEG_MAX;
+    int AM = 64;   // heavy-atom bucket of the packed arrays: 64 (one atom per lane) or 96 / 128 (two per lane)
 };
 struct GridArgs {
     const float* grid; const float* origin; const float* lo; const float* hi; const int* rec_idx;
@@ -81,6 +82,6 @@ int set_line_search_width(int ns);
 int get_line_search_width();
 int set_exact_staging(int on);
 int get_exact_staging();
-int lds_budget_ok(int KM, int P_MAX);
-int mc_resident_waves(int KM, int P_MAX, int persistent);
+int lds_budget_ok(int KM, int P_MAX, int AM);
+int mc_resident_waves(int KM, int P_MAX, int persistent, int AM);
 }  // namespace adock

@@ -29,13 +29,13 @@ __global__ __launch_bounds__(SA_THREADS) void stage_a_kernel(const float* __rest
                                                               const int* __restrict__ ntrans, int NT_MAX, GridArgs G,
                                                               float cap, float* __restrict__ best_e,
                                                               int* __restrict__ best_i) {
-    __shared__ float4 XS[64];
+    __shared__ float4 XS[SA_THREADS];   // rotated heavy atoms, AM <= SA_THREADS
     __shared__ float wbe[SA_WAVES];
     __shared__ int wbi[SA_WAVES];
     const long long b = blockIdx.x;
     const int q = (int)(b % Q), c = (int)((b / Q) % C);
     const int l = (int)(b / ((long long)Q * C));
-    const int n_at = min(na[l], 64);
+    const int n_at = min(na[l], min(AM, SA_THREADS));
     const int tid = threadIdx.x;
     if (tid < n_at) {
         float qw = quats[4 * q], qx = quats[4 * q + 1], qy = quats[4 * q + 2], qz = quats[4 * q + 3];

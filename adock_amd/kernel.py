@@ -38,8 +38,9 @@ class KernelProblem:
     def __init__(self, prob):
         _lib.require_gpu()
         self.AM, self.KM, lig = ligand_arrays(prob)
-        if self.AM != 64:
-            raise _lib.AdockError("ligands above 64 heavy atoms are not supported by the HIP kernels")
+        if self.AM > 128 or self.KM > 56:
+            raise _lib.AdockError(f"ligand bucket AM={self.AM}, KM={self.KM}: the HIP kernels take at most 128 heavy "
+                                  "atoms and 56 torsions per ligand")
         gs = prob.gs
         dev = gs.flat.device
         if dev.type != "cuda":
@@ -190,8 +191,8 @@ class KernelProblem:
         only pays where it does not lose a wave per SIMD to its extra registers."""
         if self._queue_occupancy is None:
             L = _lib.lib()
-            a = L.adock_mc_resident_waves(self.KM, self.P_MAX, 1)
-            b = L.adock_mc_resident_waves(self.KM, self.P_MAX, 0)
+            a = L.adock_mc_resident_waves(self.AM, self.KM, self.P_MAX, 1)
+            b = L.adock_mc_resident_waves(self.AM, self.KM, self.P_MAX, 0)
             self._queue_occupancy = (a, b)
         a, b = self._queue_occupancy
         return a > 0 and b > 0 and a >= b

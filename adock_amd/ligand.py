@@ -146,9 +146,10 @@ def pad128(P):
     return (int(P) + 127) // 128 * 128
 
 
-def inc_stride(P_MAX):
-    """Length of one ligand's row of slot lists: two entries per pair plus padding of each atom's list to 8."""
-    return 2 * pad128(P_MAX) + 512
+def inc_stride(P_MAX, AM=64):
+    """Length of one ligand's row of slot lists: two entries per pair plus up to 7 padding entries per atom (each
+    atom's list is padded to a multiple of 8)."""
+    return 2 * pad128(P_MAX) + 8 * AM
 
 
 def slot_lists(pairs, P_MAX, AM=64):
@@ -157,7 +158,7 @@ def slot_lists(pairs, P_MAX, AM=64):
     entries with the always-zero slot pad128(P_MAX). Returns (inc [inc_stride] uint16, ist [AM] int32 =
     start | padded_count << 16)."""
     z = pad128(P_MAX)
-    inc = np.full(inc_stride(P_MAX), z, np.uint16)
+    inc = np.full(inc_stride(P_MAX, AM), z, np.uint16)
     ist = np.zeros(AM, np.int32)
     pairs = np.asarray(pairs, np.int64).reshape(-1, 2)
     q = np.arange(len(pairs))
@@ -221,7 +222,7 @@ def pack_ligands(preps, slot, rec_idx):
         "axp": np.zeros((L, KM, 3), np.float32), "axu": np.zeros((L, KM, 3), np.float32),
         "pr": np.zeros((L, P, 2), np.int32), "na": np.zeros(L, np.int32), "nt": np.zeros(L, np.int32),
         "np": np.zeros(L, np.int32), "rg": np.zeros(L, np.float32), "gslot": np.zeros((L, AM), np.int32),
-        "inc": np.zeros((L, inc_stride(P)), np.uint16), "ist": np.zeros((L, AM), np.int32),
+        "inc": np.zeros((L, inc_stride(P, AM)), np.uint16), "ist": np.zeros((L, AM), np.int32),
     }
     slot = np.asarray(slot)
     rec_idx = np.asarray(rec_idx)

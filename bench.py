@@ -146,7 +146,7 @@ def multi_headline(args):
     lat = prep_ms * 1e-3 + runner.latencies()
     best_l = be.view(L, chains).min(1).values.cpu().numpy()
     traffic, tkernel = measured_traffic("mc_persistent_kernel_multi_receptor_traffic")
-    kname = "mc_persistent_kernel<16, false>"
+    kname = f"mc_persistent_kernel<{kp.KM}, {2 if kp.AM > 64 else 1}, false>"
     out = {
         "metric": "docked ligands/s/GPU at matched best-score gap; score+grad evals/s",
         "value": L * args.steps / dt, "unit": "docked ligands/s", "n_gpus": 1, "steps": args.steps,

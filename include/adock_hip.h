@@ -8,14 +8,17 @@
  * Argument groups follow the reference's kernel parameter order (src/adock/mckernel.py:328-345, bound by
  * src/adock/cpukernel.py:97-101,263): ligand (13 + 3), grid (10), scoring function (5), receptor (4), exact, then the
  * entry's own parameters. The reference compiles one module per (A_MAX, K_MAX) bucket; here AM and KM are the first
- * two arguments and select a compiled instantiation (AM = 64; KM in {8, 16, 24, 32}).
+ * two arguments and select a compiled instantiation: AM = 64 (up to 64 heavy atoms, one per lane of a wavefront) or
+ * 96 / 128 (two atoms per lane; the reference's rule AM = 64 if the largest ligand has at most 64 heavy atoms, else its
+ * count rounded up to 32, src/adock/mckernel.py:553), KM = the largest torsion count rounded up to 8, 8 .. 56 (the
+ * state t(3) q(4) theta(KM) fits one wavefront). Larger ligands are refused with ADOCK_E_BUCKET.
  *
  * The ligand group is the reference's packed layout (src/adock/mckernel.py:547-592: ref, typ, tm, ax, axp, axu, pr,
  * na, nt, np, rg, P_MAX, gslot) plus two arrays of this library's own, built once per ligand set on the host
- * (adock_amd.ligand.slot_lists): inc [L, 2 * P128 + 512] uint16 and ist [L, AM] int32, P128 = P_MAX rounded up to a
- * multiple of 128. Atom a's list inc[l, s .. s + c) (ist = s | c << 16, s and c multiples of 8) names the intra pairs
+ * (adock_amd.ligand.slot_lists): inc [L, 2 * P128 + 8 * AM] uint16 and ist [L, AM] int32, P128 = P_MAX rounded up to
+ * a multiple of 128. Atom a's list inc[l, s .. s + c) (ist = s | c << 16, s and c multiples of 8) names the intra pairs
  * of atom a: q for a pair q = (a, j), q | 0x8000 for a pair q = (i, a), ascending, padded with P128; DEG_MAX is the
- * largest c over all atoms and ligands (a multiple of 8, at most 64). The kernels use the lists to sum the pair
+ * largest c over all atoms and ligands (a multiple of 8, at most AM). The kernels use the lists to sum the pair
  * gradient of every atom without atomics; DEG_MAX sizes the per-atom gradient rows of the single-complex kernel.
  */
 #ifndef ADOCK_HIP_H
@@ -273,11 +276,11 @@ int adock_get_line_search_width(void);
 int adock_set_exact_staging(int on);
 int adock_get_exact_staging(void);
 
-/* Resident wavefronts per CU of the grid-mode chain kernels for bucket KM and pair capacity P_MAX, from the runtime's
- * occupancy calculator: persistent = 0 the per-chain kernel of adock_mc (as launched for many chains), 1 the persistent
- * kernel of adock_mc_batch. Lets a caller pick the launch that keeps more chains resident. No counterpart in the
- * reference. Returns the count (> 0), or a negative ADOCK_E_* code. */
-int adock_mc_resident_waves(int KM, int P_MAX, int persistent);
+/* Resident wavefronts per CU of the grid-mode chain kernels for buckets (AM, KM) and pair capacity P_MAX, from the
+ * runtime's occupancy calculator: persistent = 0 the per-chain kernel of adock_mc (as launched for many chains), 1 the
+ * persistent kernel of adock_mc_batch. Lets a caller pick the launch that keeps more chains resident. No counterpart
+ * in the reference. Returns the count (> 0), or a negative ADOCK_E_* code. */
+int adock_mc_resident_waves(int AM, int KM, int P_MAX, int persistent);
 
 #ifdef __cplusplus
 }

@@ -13,7 +13,7 @@ the files where they lie under the reference checkout, without importing any of 
     -march=x86-64-v3 instead of -march=native so the library also runs on another host.
 Output: oracle/_ref/libadock_ref_am<AM>_km<KM>.so (git-ignored). Only tests/ and bench.py's CPU-baseline leg load it.
 
-python oracle/build_ref.py [--ref /root/reference] [AM:KM ...]   (default buckets 64:8 64:16 64:24 64:32)
+python oracle/build_ref.py [--ref /root/reference] [AM:KM ...]   (default: BUCKETS below)
 """
 import argparse
 import os
@@ -25,7 +25,8 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 OUT = HERE / "_ref"
 DEFAULT_REF = Path(os.environ.get("ADOCK_REFERENCE", "/root/reference"))
-BUCKETS = [(64, 8), (64, 16), (64, 24), (64, 32)]
+# the buckets the checks use: one atom per lane at every torsion bucket to 32, two atoms per lane, and past 32 torsions
+BUCKETS = [(64, 8), (64, 16), (64, 24), (64, 32), (64, 40), (96, 16), (128, 24), (128, 48), (128, 56)]
 CXX = os.environ.get("CXX", "g++")
 FLAGS = ["-O3", "-march=x86-64-v3", "-ffast-math", "-fopenmp", "-fPIC", "-shared", "-std=c++17", "-w"]
 LANE_REWRITES = 17   # the count the reference asserts (cpukernel.py:87-88)

@@ -53,7 +53,12 @@ def test_bucket_validation_without_gpu():
     assert L.adock_set_exact_staging(1) == 0 and L.adock_get_exact_staging() == 1
     assert L.adock_set_exact_staging(2) == 0
     assert L.adock_version().startswith(b"adock_hip")
-    assert L.adock_mc_resident_waves(12, 64, 1) == -2 and L.adock_mc_resident_waves(16, -1, 0) < 0
+    assert L.adock_mc_resident_waves(64, 12, 64, 1) == -2 and L.adock_mc_resident_waves(64, 16, -1, 0) < 0
+    # buckets: AM 96 / 128 and KM up to 56 are accepted past the bucket check (then stopped at the null pointers);
+    # AM 160 and KM 64 are refused
+    for am, km, want in ((96, 16, -1), (128, 56, -1), (160, 8, -2), (64, 64, -2), (128, 60, -2)):
+        args[0], args[1] = am, km
+        assert L.adock_eg(*args) == want, (am, km)
     # fitted grid builder: null pointers and more receptor ext types than the LDS table holds are rejected,
     # empty grids are a no-op
     fit = [None, None, 1, None, None, 1, None, 16, 0.0, 0.0, 0.0, 0.375, 4, 4, 4, None, None, None]

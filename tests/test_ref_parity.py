@@ -184,3 +184,31 @@ def test_stage_a_matches_reference(small):
     assert same.mean() >= 0.99
     assert np.abs(be_r - be_o)[same].max() <= 1e-4 * (1 + np.abs(be_o[same]).max())
     assert (bi_r < nT[0]).all() and (bi_o < nT[0]).all()
+
+
+@pytest.mark.parametrize("atoms,tors,seed", [(80, 12, 780), (120, 20, 820), (64, 40, 940), (110, 45, 810),
+                                             (128, 50, 828)])
+def test_large_ligand_buckets_match_reference(atoms, tors, seed):
+    """Ligands past 64 heavy atoms (AM 96 / 128) and past 32 torsions (KM 40 / 48 / 56), packed by the reference's
+    bucket rule: energies and gradients within 1e-4, and one MC step of one BFGS iteration, against the reference's
+    kernels built for those buckets."""
+    w = synth.make_single("small", 0)
+    rec = {"rec_xyz": w["receptor"]["rec_xyz"], "rec_types": w["receptor"]["rec_types"]}
+    lig = synth.make_ligand(seed, atoms, tors)
+    op, _ = cpu_problem(rec, [lig], w["center"], 30.0)
+    _ref_or_skip(op.AM, op.KM)
+    rk = ref.RefKernels(op.AM, op.KM, op.lig, op.grid, op.sf, op.rec)
+    N = 96
+    st = states(3, N, w["center"], 24.0, lig, op.KM, sigma=2.5)
+    cl = np.zeros(N, np.int32)
+    eo, go, _, m = op.energy_grad(cl, st)
+    er, gr = rk.energy_grad(cl, st)
+    keep = m >= 1e-4
+    assert rel_err(er, eo, EG_FLOOR, EG_REL) <= EG_REL
+    assert vec_rel(gr[keep], go[keep], 1.0) <= EG_REL
+    c1, b1, ce1, be1 = st.copy(), st.copy(), er.copy(), er.copy()
+    c2, b2, ce2, be2 = st.copy(), st.copy(), er.copy(), er.copy()
+    rk.run_mc(cl, np.arange(N), c1, ce1, b1, be1, False, 1, 1, 1.2, 2.0, 2.0, 99, 0)
+    op.run_mc(cl, np.arange(N), c2, ce2, b2, be2, False, 1, 1, 1.2, 2.0, 2.0, 99, 0, f64=True)
+    same = (np.abs(ce1 - ce2) < 1e-3 * np.maximum(1, np.abs(ce2))) & (np.abs(c1 - c2).max(1) < 1e-3)
+    assert same.mean() >= 0.97, same.mean()
