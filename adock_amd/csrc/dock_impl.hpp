@@ -1356,8 +1356,6 @@ DEVI void write_features(float* f, const ChainCtx<KM, 1>& c, int lane, float x, 
     if (lane >= 39 && lane < 61) lo_v = tanhf(0.1f * gv);
     const float esum = wave_sum(c.atom[0] ? ea : 0.f);
     const float inv_na = 1.f / (float)max(c.na, 1);
-    const float X0 = 0.f;   // placeholder to keep lane-uniform code shape
-    (void)X0;
     const int nout = wave_count(c.atom[0] && ea > 1.f);
     if (lane == 61) lo_v = tanhf(0.1f * E);
     if (lane == 62) lo_v = tanhf(0.1f * esum);

@@ -227,13 +227,14 @@ def make_single(name, seed=0):
 
 
 def make_screen(seed=0, n_ligands=512, chains=64, steps=500):
-    """Screening batch: one small-workload receptor and box, n_ligands ligands with heavy atoms ~ U[10, 60] and torsions
-    ~ U[0, 15] (capped by the ligand's non-terminal bonds)."""
+    """Screening batch: the GPU-size single-complex receptor and box (4,000 atoms around an 8 A cavity, 26 A box), and
+    n_ligands ligands with heavy atoms ~ U[10, 60] and torsions ~ U[0, 15] (capped by the ligand's non-terminal
+    bonds)."""
     rng = np.random.default_rng(seed + 991)
-    rec = make_receptor(seed, **WORKLOADS["small"]["rec"])
+    rec = make_receptor(seed, **WORKLOADS["single"]["rec"])
     ligs = [make_ligand(seed * 100003 + 17 + i, int(rng.integers(10, 61)), int(rng.integers(0, 16)))
             for i in range(n_ligands)]
-    return {"receptor": rec, "ligs": ligs, "center": np.zeros(3), "size": np.full(3, WORKLOADS["small"]["box"]),
+    return {"receptor": rec, "ligs": ligs, "center": np.zeros(3), "size": np.full(3, WORKLOADS["single"]["box"]),
             "chains": chains, "steps": steps, "bfgs_iters": 0}
 
 
