@@ -66,8 +66,9 @@ class Autotuner:
 
     @staticmethod
     def apply(runner, W, blocks, cus=None):
-        """blocks: 0 = resident capacity (CUs x occupancy); -2 = twice that; -1 = one workgroup per W chains."""
-        _lib.check(_lib.lib().adock_set_chains_per_block(int(W)))
+        """W: chains per workgroup of this runner's launches (the library's process-wide default is left alone);
+        blocks: 0 = resident capacity (CUs x occupancy); -2 = twice that; -1 = one workgroup per W chains."""
+        runner.waves = int(W)
         if blocks == -1:
             runner.blocks = (runner.N + W - 1) // W
         elif blocks == -2:

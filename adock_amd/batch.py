@@ -97,6 +97,33 @@ class BatchRunner:
         self.times = torch.zeros(1 + self.L, dtype=torch.int64, device=dev)
         self.lig_evals = torch.zeros(self.L, dtype=torch.int64, device=dev)   # evaluations per complex (count_evals)
         self.blocks = 0
+        self.waves = 0   # chains per workgroup of this runner's launches (0: the library's default)
+
+    @classmethod
+    def on_device(cls, kp, chain_lig, n_complexes):
+        """A runner built without reading chain_lig on the host (no synchronisation, so it can be made inside a
+        stream's work or a HIP-graph capture): one queue over all chains, longest first by their complexes' cost (a
+        stable sort on the device: the order queue_partition gives one queue), chains per complex by a scatter-add."""
+        self = cls.__new__(cls)
+        self.kp = kp
+        dev = kp.device
+        cl = chain_lig.to(device=dev, dtype=torch.int32).contiguous()
+        self.N = int(cl.numel())
+        self.L = int(n_complexes)
+        self.chain_lig = cl
+        idx = cl.long()
+        self.order = torch.sort(-kp.cost_dev[idx], stable=True).indices.to(torch.int32)
+        self.qstart = None
+        self.nq = 1
+        self.lig_chains = torch.zeros(self.L, dtype=torch.int32, device=dev).scatter_add_(
+            0, idx, torch.ones_like(cl))
+        self.counter = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.done = torch.zeros(self.L, dtype=torch.int32, device=dev)
+        self.times = torch.zeros(1 + self.L, dtype=torch.int64, device=dev)
+        self.lig_evals = torch.zeros(self.L, dtype=torch.int64, device=dev)
+        self.blocks = 0
+        self.waves = 0
+        return self
 
     def launch(self, chain_id, cur, cur_e, best, best_e, init_opt, mc_steps, bfgs_iters, temp, amp_pos, amp_rot, seed,
                step_offset, count_evals=False):
@@ -116,7 +143,7 @@ class BatchRunner:
                                    int(mc_steps), int(bfgs_iters), float(temp), float(amp_pos), float(amp_rot),
                                    int(seed) & (2 ** 64 - 1), int(step_offset), _lib.ptr(self.order),
                                    _lib.ptr(self.qstart), self.nq, _lib.ptr(self.counter), _lib.ptr(self.done), _lib.ptr(self.lig_chains),
-                                   _lib.ptr(self.times), int(self.blocks),
+                                   _lib.ptr(self.times), int(self.blocks), int(self.waves),
                                    _lib.ptr(kp.eval_count) if count_evals else None,
                                    _lib.ptr(self.lig_evals) if count_evals else None, st])
         _lib.check(L.adock_mc_batch(*args))

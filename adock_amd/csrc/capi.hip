@@ -174,7 +174,7 @@ int adock_mc_batch(int AM, int KM, LIG_DECL, GRID_DECL, SF_DECL, REC_DECL, int e
                    float* cur_e, float* best, float* best_e, int init_opt, int mc_steps, int bfgs_iters, float temp,
                    float amp_pos, float amp_rot, unsigned long long seed, int step_offset, const int* order,
                    const int* queue_start, int n_queues, int* counter, int* done, const int* lig_chains,
-                   unsigned long long* times, int blocks, unsigned long long* eval_count,
+                   unsigned long long* times, int blocks, int waves_per_block, unsigned long long* eval_count,
                    unsigned long long* lig_evals, adock_stream_t stream) {
     if (int st = common_checks(AM, KM, P_MAX, ref, inc, ist, DEG_MAX, grid, prm, exact, rxyz, rn, RN_MAX, chain_lig,
                                n_chains))
@@ -186,6 +186,7 @@ int adock_mc_batch(int AM, int KM, LIG_DECL, GRID_DECL, SF_DECL, REC_DECL, int e
         return fail(ADOCK_E_ARG, "n_queues must be in 0..1024, with queue_start when above 1");
     if (mc_steps < 0) return fail(ADOCK_E_ARG, "negative mc_steps");
     if (nbr && (!nbr_n || NB_MAX <= 0)) return fail(ADOCK_E_ARG, "nbr needs nbr_n and NB_MAX > 0");
+    if (waves_per_block < 0 || waves_per_block > 4) return fail(ADOCK_E_ARG, "waves_per_block must be in 0..4");
     LigArgs L{ref, typ, tm, ax, axp, axu, pr, na, nt, np, rg, P_MAX, gslot, inc, ist, DEG_MAX, AM};
     GridArgs G{grid, origin, lo, hi, rec_idx, goffs, gdims, T, spacing, pen};
     SFArgs S{prm, rad, hyd, don, acc};
@@ -195,6 +196,7 @@ int adock_mc_batch(int AM, int KM, LIG_DECL, GRID_DECL, SF_DECL, REC_DECL, int e
     M.lig_evals = lig_evals;
     M.n_lig_evals = 0x7fffffff;
     QueueArgs Q{order, queue_start, n_queues, counter, done, lig_chains, times};
+    Q.waves = waves_per_block;
     return hip_status(launch_mc_persistent(KM, L, G, S, R, exact, chain_lig, n_chains, M, Q, blocks,
                                            (hipStream_t)stream));
 }
@@ -237,7 +239,7 @@ int adock_mc_batch_noalloc(int AM, int KM, LIG_DECL, GRID_DECL, SF_DECL, REC_DEC
                            float* cur_e, float* best, float* best_e, int init_opt, int mc_steps, int bfgs_iters,
                            float temp, float amp_pos, float amp_rot, unsigned long long seed, int step_offset,
                            const int* order, const int* queue_start, int n_queues, int n_complexes,
-                           const int* lig_chains, unsigned long long* times, int blocks,
+                           const int* lig_chains, unsigned long long* times, int blocks, int waves_per_block,
                            unsigned long long* eval_count, unsigned long long* lig_evals, adock_stream_t stream) {
     if (n_queues < 0 || n_queues > ADOCK_NOALLOC_MAX_QUEUES)
         return fail(ADOCK_E_ARG, "n_queues must be in 0..1024 for the allocation-free launch");
@@ -253,6 +255,7 @@ int adock_mc_batch_noalloc(int AM, int KM, LIG_DECL, GRID_DECL, SF_DECL, REC_DEC
     if (n_queues > 1 && !queue_start) return fail(ADOCK_E_ARG, "n_queues above 1 needs queue_start");
     if (mc_steps < 0) return fail(ADOCK_E_ARG, "negative mc_steps");
     if (nbr && (!nbr_n || NB_MAX <= 0)) return fail(ADOCK_E_ARG, "nbr needs nbr_n and NB_MAX > 0");
+    if (waves_per_block < 0 || waves_per_block > 4) return fail(ADOCK_E_ARG, "waves_per_block must be in 0..4");
     int* base = nullptr;
     if (int st = scratch_of_current_device(&base)) return st;
     int* counter = base;
@@ -270,6 +273,7 @@ int adock_mc_batch_noalloc(int AM, int KM, LIG_DECL, GRID_DECL, SF_DECL, REC_DEC
              amp_rot, seed, step_offset, eval_count};
     QueueArgs Q{order, queue_start, n_queues, counter, done, lig_chains, times};
     Q.n_done = n_complexes;   // a chain of a complex past n_complexes is run but not counted (its slot is not ours)
+    Q.waves = waves_per_block;
     M.lig_evals = lig_evals;
     M.n_lig_evals = n_complexes;
     return hip_status(launch_mc_persistent(KM, L, G, S, R, exact, chain_lig, n_chains, M, Q, blocks,

@@ -5,9 +5,11 @@
 
 namespace adock {
 
-int g_waves_per_block = 0;     // 0: adaptive (auto_waves)
-int g_line_search_width = 0;   // 0: adaptive (line_search_width)
-int g_exact_staging = 2;       // 0: read from memory, 1: staged, 2: staged with per-atom masks
+// Process-wide launch defaults, atomic so that a setter racing a launch on another thread is well defined (the launch
+// sees the old or the new value). The batched launch also takes its chains per workgroup per call.
+std::atomic<int> g_waves_per_block{0};     // 0: adaptive (auto_waves)
+std::atomic<int> g_line_search_width{0};   // 0: adaptive (line_search_width)
+std::atomic<int> g_exact_staging{2};       // 0: read from memory, 1: staged, 2: staged with per-atom masks
 
 int set_waves_per_block(int w) {
     if (w < 0 || w > 4) return -1;

@@ -23,6 +23,7 @@
 //   torsion k sits on the same lane in the state and in the gradient; the zero lane is inert in BFGS). So one wave
 //   holds at most 57 torsions; the buckets go to KM = 56.
 #pragma once
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -34,9 +35,9 @@
 namespace adock {
 
 // launch settings shared by every bucket's translation unit (defined in dock.hip)
-extern int g_waves_per_block;     // 0: adaptive
-extern int g_line_search_width;   // 0: adaptive
-extern int g_exact_staging;       // 0: read from memory, 1: staged, 2: staged with per-atom masks
+extern std::atomic<int> g_waves_per_block;     // 0: adaptive
+extern std::atomic<int> g_line_search_width;   // 0: adaptive
+extern std::atomic<int> g_exact_staging;       // 0: read from memory, 1: staged, 2: staged with per-atom masks
 
 namespace {
 
@@ -1133,7 +1134,10 @@ inline int auto_waves(int n_chains) {
     const int w = n_chains / (4 * cus);
     return w >= 4 ? 4 : w >= 2 ? 2 : 1;
 }
-inline int waves_for(int n_chains) { return g_waves_per_block > 0 ? g_waves_per_block : auto_waves(n_chains); }
+inline int waves_for(int n_chains) {
+    const int w = g_waves_per_block.load(std::memory_order_relaxed);
+    return w > 0 ? w : auto_waves(n_chains);
+}
 
 // chains per workgroup actually launched: the requested count, lowered until the workgroup's LDS fits
 inline int fit_waves(int W, int KM, int P_MAX, int extra = 0, int APL = 1) {
@@ -1229,7 +1233,7 @@ int launch_mc_tx(const LigArgs& L, const GridArgs& G, const SFArgs& S, const Rec
     }
     const int wave_bytes = wave_lds_bytes(KM, L.P_MAX, APL) + extra;
     int W = fit_waves(waves_for(n_chains), KM, L.P_MAX, extra, APL);
-    if (g_waves_per_block <= 0)
+    if (g_waves_per_block.load(std::memory_order_relaxed) <= 0)
         W = packed_waves(reinterpret_cast<const void*>(mc_kernel<KM, APL, EX>), W, wave_bytes, EX);
     const size_t shm = (size_t)W * wave_bytes;
     const unsigned nblk = (unsigned)((n_chains + W - 1) / W);
@@ -1244,8 +1248,9 @@ int launch_mc_persistent_tx(const LigArgs& L, const GridArgs& G, const SFArgs& S
                             const int* chain_lig, int n_chains, const McArgs& M, const QueueArgs& Q, int blocks,
                             hipStream_t stream) {
     const int wave_bytes = wave_lds_bytes(KM, L.P_MAX, APL);
-    int W = fit_waves(g_waves_per_block > 0 ? g_waves_per_block : 4, KM, L.P_MAX, 0, APL);
-    if (g_waves_per_block <= 0)
+    const int req = Q.waves > 0 ? Q.waves : g_waves_per_block.load(std::memory_order_relaxed);
+    int W = fit_waves(req > 0 ? req : 4, KM, L.P_MAX, 0, APL);
+    if (req <= 0)
         W = packed_waves(reinterpret_cast<const void*>(mc_persistent_kernel<KM, APL, EX>), W, wave_bytes);
     const size_t shm = (size_t)W * wave_bytes;
     if (blocks <= 0) {
@@ -1304,11 +1309,11 @@ int launch_eg_tx(const LigArgs& L, const GridArgs& G, const SFArgs& S, const Rec
 template <int KM, int APL>
 int mc_resident_waves_t(int P_MAX, int persistent) {
     const int wave_bytes = wave_lds_bytes(KM, P_MAX, APL);
-    int W = fit_waves(persistent ? (g_waves_per_block > 0 ? g_waves_per_block : 4) : waves_for(1 << 20), KM, P_MAX, 0,
-                      APL);
+    const int req = g_waves_per_block.load(std::memory_order_relaxed);
+    int W = fit_waves(persistent ? (req > 0 ? req : 4) : waves_for(1 << 20), KM, P_MAX, 0, APL);
     const void* fn = persistent ? reinterpret_cast<const void*>(mc_persistent_kernel<KM, APL, false>)
                                 : reinterpret_cast<const void*>(mc_kernel<KM, APL, false>);
-    if (g_waves_per_block <= 0) W = packed_waves(fn, W, wave_bytes);
+    if (req <= 0) W = packed_waves(fn, W, wave_bytes);
     const size_t shm = (size_t)W * wave_bytes;
     allow_lds(fn, shm);
     int per_cu = 0;

@@ -33,6 +33,7 @@ struct QueueArgs {
     const int* order; const int* qstart; int nq; int* counter; int* done; const int* lig_chains;
     unsigned long long* times;
     int n_done = 0x7fffffff;   // entries of done / lig_chains (times: 1 + n_done); complexes at or past it are not counted
+    int waves = 0;             // chains (wavefronts) per workgroup of this launch; 0: the process default
 };
 // learned-proposal MC step: Y [N,23] network output (NULL: plain moves), feat [N,128] features (in: at the current
 // pose; out: at the new current pose; NULL: not computed), log_buf [cap, 128+23+23+1] accepted-move records.

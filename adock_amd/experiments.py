@@ -20,7 +20,7 @@ import torch
 
 from . import synth, workload
 from .autotune import Autotuner
-from .batch import XCDS, BatchRunner, GraphedBatch, admit
+from .batch import XCDS, BatchRunner, GraphedBatch
 from .kernel import KernelProblem
 from .learned import StepSearch, jump_dataset, train_proposal
 
@@ -61,37 +61,33 @@ def screen(n_ligands=512, chains=64, steps=500, seed=0, reps=2, tune=True, log=p
             "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99))}
 
 
-def multi(n_receptors=64, ligands_per=32, chains=32, steps=500, seed=0, hbm_budget=64e9, log=print, queues=XCDS):
-    w, prob = workload.multi(seed=seed, n_receptors=n_receptors, ligands_per=ligands_per, chains=chains, steps=steps)
-    grid_bytes = [int(prob.gs.dims[r].prod()) * int((prob.gs.slot[r] >= 0).sum()) * 4 for r in range(n_receptors)]
-    groups = admit(grid_bytes, hbm_budget)
-    kp = KernelProblem(prob)
+def multi(n_receptors=64, ligands_per=32, chains=32, steps=500, seed=0, hbm_budget=None, log=print, queues=XCDS):
+    """The multi-receptor batch under HBM admission: receptors grouped so that one group's grid sets fit hbm_budget
+    bytes (default: a quarter of all grid sets, so several groups), each group's grids built, its complexes docked in
+    one persistent launch, and its grids released before the next group. Timed from the first build to the last
+    chain; best scores are those of the all-resident batch bit for bit."""
+    from .admission import AdmittedBatch, grid_bytes
+    w = synth.make_multi(seed, n_receptors, ligands_per, chains, steps)
+    recs = [workload.receptor_dict(r, np.zeros(3), b) for r, b in zip(w["receptors"], w["boxes"])]
+    total = int(sum(grid_bytes(recs, w["ligs"], w["rec_idx"])))
+    budget = float(hbm_budget) if hbm_budget else total / 4.0
+    ab = AdmittedBatch(recs, w["ligs"], w["rec_idx"], budget, chains, queues=queues)
     L = len(w["ligs"])
     boxes = [w["boxes"][r] for r in w["rec_idx"]]
-    st, cl = workload.chain_starts(seed, w["ligs"], [np.zeros(3)] * L, boxes, chains, kp.KM)
-    st = torch.as_tensor(st, device=kp.device)
-    cl_all = torch.as_tensor(cl, device=kp.device)
-    bufs = _bufs(st)
-    ci = torch.arange(len(cl), dtype=torch.int32, device=kp.device)
-    lat = np.zeros(L)
+    st, _ = workload.chain_starts(seed, w["ligs"], [np.zeros(3)] * L, boxes, chains, ab.KM)
+    bufs = _bufs(torch.as_tensor(st, device=ab.device))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for grp in groups:
-        ligs = np.nonzero(np.isin(w["rec_idx"], grp))[0]
-        sel = torch.as_tensor(np.concatenate([np.arange(l * chains, (l + 1) * chains) for l in ligs]),
-                              device=kp.device)
-        c_, ce_, b_, be_ = [x[sel].contiguous() for x in bufs]
-        runner = BatchRunner(kp, cl_all[sel], n_complexes=L, queues=queues)
-        t1 = time.perf_counter()
-        runner.launch(ci[sel].contiguous(), c_, ce_, b_, be_, True, steps, 0, 1.2, 2.0, 2.0, seed * 1_000_003 + 12345, 0)
-        torch.cuda.synchronize()
-        lat[ligs] = runner.latencies()[ligs] + (t1 - t0)
-        bufs[3][sel] = be_
+    res = ab.run(*bufs, True, steps, 0, 1.2, 2.0, 2.0, seed * 1_000_003 + 12345, 0, log=log)
+    torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     best = bufs[3].view(L, chains).min(1).values.cpu().numpy()
-    return {"workload": f"multi: {n_receptors} receptors (2000-6000 atoms, boxes 16-30 A) x {ligands_per} ligands x "
-                        f"{chains} chains x {steps} MC steps, all grid sets resident",
-            "grid_set_bytes_total": int(sum(grid_bytes)), "admission_groups": len(groups),
+    lat = res["latency_s"]
+    return {"workload": f"multi, HBM-admitted: {n_receptors} receptors (2000-6000 atoms, boxes 16-30 A) x "
+                        f"{ligands_per} ligands x {chains} chains x {steps} MC steps; receptor groups whose grid sets "
+                        f"fit the budget, built and docked one group at a time",
+            "grid_set_bytes_total": total, "hbm_budget_bytes": int(budget), "admission_groups": len(ab.groups),
+            "peak_grid_bytes": int(ab.peak_grid_bytes), "groups": res["groups"],
             "docked_ligands_per_s": L / dt, "seconds": dt, "latency_p50_s": float(np.percentile(lat, 50)),
             "latency_p99_s": float(np.percentile(lat, 99)), "best_score_mean": float(best.mean())}
 

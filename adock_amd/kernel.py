@@ -15,10 +15,10 @@ from .config import search_prm
 from .ligand import pack_ligands
 
 
-def ligand_arrays(prob):
-    """Packed ligand arrays (numpy) of a Problem: (AM, KM, dict)."""
+def ligand_arrays(prob, bucket=None):
+    """Packed ligand arrays (numpy) of a Problem: (AM, KM, dict); bucket (AM, KM): smallest buckets to pack to."""
     slot = prob.gs.slot.cpu().numpy()
-    return pack_ligands(prob.lb.preps, slot, prob.rec_idx.cpu().numpy())
+    return pack_ligands(prob.lb.preps, slot, prob.rec_idx.cpu().numpy(), floor=bucket or (0, 0))
 
 
 # from this many chains in grid mode, over complexes whose chain costs differ by more than UNEVEN_COST, and where the
@@ -35,9 +35,11 @@ class KernelProblem:
     use_queues = True
     _queue_occupancy = None
 
-    def __init__(self, prob):
+    def __init__(self, prob, bucket=None):
+        """bucket (AM, KM): smallest ligand buckets to pack to (default: those the ligands need), so that problems
+        run one after another share one state layout [N, 7 + KM] and one kernel bucket."""
         _lib.require_gpu()
-        self.AM, self.KM, lig = ligand_arrays(prob)
+        self.AM, self.KM, lig = ligand_arrays(prob, bucket)
         if self.AM > 128 or self.KM > 56:
             raise _lib.AdockError(f"ligand bucket AM={self.AM}, KM={self.KM}: the HIP kernels take at most 128 heavy "
                                   "atoms and 56 torsions per ligand")
@@ -75,6 +77,10 @@ class KernelProblem:
         from .batch import chain_cost
         self.preps = prob.lb.preps
         self.cost = chain_cost(self.preps)
+        self.cost_dev = torch.as_tensor(self.cost, dtype=torch.float64, device=dev)
+        # routing of run_mc to the queued launch: decided from all complexes of the problem (host data), so run_mc
+        # never reads chain_lig back from the device
+        self._uneven = len(self.cost) > 1 and float(np.max(self.cost)) > UNEVEN_COST * float(np.min(self.cost))
         self.eval_count = torch.zeros(1, dtype=torch.int64, device=dev)
 
     # ---- state packing ----
@@ -222,18 +228,18 @@ class KernelProblem:
             nb_args = [_lib.ptr(nbr), _lib.ptr(nbr_n), int(nbr.shape[1])]
         else:
             nb_args = [None, None, 0]
-        if self.use_queues and not exact and nbr is None and N >= QUEUE_MIN_CHAINS and self.L > 1:
-            # many uneven chains in grid mode: the persistent launch with longest-first queues, one per XCD (same
-            # results bit for bit; the tail of long chains no longer idles CUs). Chains of equal cost stay on adock_mc.
-            cl_h = cl.cpu().numpy()
-            c = np.asarray(self.cost, np.float64)[np.unique(cl_h)]
-            if c.max() > UNEVEN_COST * c.min() and self.queues_keep_occupancy():
-                from .batch import BatchRunner
-                runner = BatchRunner(self, cl_h, n_complexes=self.L)
-                runner.launch(ci, cur, cur_e, best, best_e, init_opt, mc_steps, bfgs_iters, temp, amp_pos, amp_rot,
-                              seed, step_offset, count_evals=count_evals)
-                self._keep = (cl, ci, runner)
-                return
+        if (self.use_queues and not exact and nbr is None and N >= QUEUE_MIN_CHAINS and self._uneven
+                and self.queues_keep_occupancy()):
+            # many chains over complexes of uneven cost in grid mode: the persistent launch with one longest-first
+            # queue, built on the device (same results bit for bit; the tail of long chains no longer idles CUs; no
+            # host synchronisation, so the call stays asynchronous and capturable). Problems whose complexes all
+            # cost the same stay on adock_mc.
+            from .batch import BatchRunner
+            runner = BatchRunner.on_device(self, cl, self.L)
+            runner.launch(ci, cur, cur_e, best, best_e, init_opt, mc_steps, bfgs_iters, temp, amp_pos, amp_rot,
+                          seed, step_offset, count_evals=count_evals)
+            self._keep = (cl, ci, runner)
+            return
         self._keep = (cl, ci, nbr, nbr_n)
         args = (self._args(exact) + [_lib.ptr(cl), _lib.ptr(ci), N] + nb_args +
                 [_lib.ptr(cur), _lib.ptr(cur_e), _lib.ptr(best), _lib.ptr(best_e), int(bool(init_opt)), int(mc_steps),

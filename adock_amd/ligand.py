@@ -194,16 +194,18 @@ def pair_classes(pairs, types):
     return np.where(hb, 2, np.where(hyd[ti] & hyd[tj], 1, 0))
 
 
-def pack_ligands(preps, slot, rec_idx):
+def pack_ligands(preps, slot, rec_idx, floor=(0, 0)):
     """Packed ligand arrays in the search-kernel layout (src/adock/mckernel.py:547-592).
 
-    slot [R, n_types] int (XS type -> grid slot of receptor r, -1 = not built), rec_idx [L]. Returns (AM, KM, dict of
-    numpy arrays: ref, typ, tm, ax, axp, axu, pr, na, nt, np, rg, P, gslot, inc, ist, DEG); inc / ist: slot_lists of
-    each ligand, DEG: the longest (padded) list."""
+    slot [R, n_types] int (XS type -> grid slot of receptor r, -1 = not built), rec_idx [L]. floor (AM, KM): smallest
+    buckets to use (several batches packed to one state layout and one kernel bucket). Returns (AM, KM, dict of numpy
+    arrays: ref, typ, tm, ax, axp, axu, pr, na, nt, np, rg, P, gslot, inc, ist, DEG); inc / ist: slot_lists of each
+    ligand, DEG: the longest (padded) list."""
     L = len(preps)
     nh = [int(p.heavy.sum()) for p in preps]
-    AM = 64 if max(nh) <= 64 else bucket(max(nh), 32)
-    KM = bucket(max(max(p.n_tors for p in preps), 1), 8)
+    top = max(max(nh), int(floor[0]))
+    AM = 64 if top <= 64 else bucket(top, 32)
+    KM = bucket(max(max(p.n_tors for p in preps), 1, int(floor[1])), 8)
     pair_lists = []
     for p in preps:
         hv = np.where(p.heavy)[0]

@@ -417,7 +417,8 @@ def other_workload(args):
         return out
     if wl == "multi_groups":
         return ex.multi(n_receptors=8 if args.small else 64, ligands_per=8 if args.small else 32, chains=32,
-                        steps=50 if args.small else 500, queues=args.queues)
+                        steps=50 if args.small else 500, queues=args.queues,
+                        hbm_budget=args.hbm_budget_mb * 1e6 if args.hbm_budget_mb else None)
     return (ex.learned(n_train=8, train_steps=40, n_eval=8, budgets=(10, 20, 40), full_steps=100, epochs=2,
                        target=args.target)
             if args.small else ex.learned(target=args.target))
@@ -440,6 +441,8 @@ def main():
     ap.add_argument("--small", action="store_true", help="scaled-down sizes for the opt-in workloads")
     ap.add_argument("--queues", type=int, default=8,
                     help="screen / multi_groups: work queues of the persistent launch (8: one per XCD)")
+    ap.add_argument("--hbm-budget-mb", type=float, default=0.0,
+                    help="multi_groups: device memory one admission group's grid sets may take (0: a quarter of all)")
     ap.add_argument("--target", choices=["move", "jump"], default="move",
                     help="training target of the learned proposal (--workload learned)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,

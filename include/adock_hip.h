@@ -141,8 +141,10 @@ int adock_mc_counted(int AM, int KM,
  * 100 MHz real-time stamp of the first wave start (times[0]) and of each complex's last chain (times[1+l]).
  * counter, done and times must be reset before every launch (adock_batch_reset: a kernel, so the pair can be captured
  * in a HIP graph). eval_count may be NULL. lig_evals [L] (device, may be NULL) receives per complex the number of
- * energy+gradient evaluations its chains performed (added to what it holds). No reference counterpart: the reference launches one block per NW chains
- * and compacts between segments on the host (src/adock/search.py:614-636). */
+ * energy+gradient evaluations its chains performed (added to what it holds). waves_per_block (0..4) sets the chains
+ * (wavefronts) per workgroup of this launch only; 0 takes the process default of adock_set_chains_per_block. No
+ * reference counterpart: the reference launches one block per NW chains and compacts between segments on the host
+ * (src/adock/search.py:614-636). */
 int adock_mc_batch(int AM, int KM,
                    const float* ref, const int* typ, const unsigned char* tm, const int* ax, const float* axp,
                    const float* axu, const int* pr, const int* na, const int* nt, const int* np, const float* rg,
@@ -156,7 +158,7 @@ int adock_mc_batch(int AM, int KM,
                    int NB_MAX, float* cur, float* cur_e, float* best, float* best_e, int init_opt, int mc_steps,
                    int bfgs_iters, float temp, float amp_pos, float amp_rot, unsigned long long seed,
                    int step_offset, const int* order, const int* queue_start, int n_queues, int* counter,
-                   int* done, const int* lig_chains, unsigned long long* times, int blocks,
+                   int* done, const int* lig_chains, unsigned long long* times, int blocks, int waves_per_block,
                    unsigned long long* eval_count, unsigned long long* lig_evals, adock_stream_t stream);
 
 /* Resets the work queues of adock_mc_batch with a kernel launch: counter[0..n_counters) = 0, done[0..L) = 0,
@@ -189,8 +191,8 @@ int adock_mc_batch_noalloc(int AM, int KM,
                            int mc_steps, int bfgs_iters, float temp, float amp_pos, float amp_rot,
                            unsigned long long seed, int step_offset, const int* order, const int* queue_start,
                            int n_queues, int n_complexes, const int* lig_chains, unsigned long long* times,
-                           int blocks, unsigned long long* eval_count, unsigned long long* lig_evals,
-                           adock_stream_t stream);
+                           int blocks, int waves_per_block, unsigned long long* eval_count,
+                           unsigned long long* lig_evals, adock_stream_t stream);
 
 /* ---- learned MC proposal network (128 -> 256 -> 256 -> 23, ReLU). No reference counterpart: the reference proposes
  * moves at random only (src/adock/mckernel.py:381-398). ---- */
@@ -255,8 +257,10 @@ int adock_stage_a(int AM, const float* conf, const int* gslot, const int* na, in
                   float* best_e, int* best_i, adock_stream_t stream);
 
 /* Launch shape of the chain kernels: chains (waves) per workgroup, 1..4, or 0 = adaptive (default: one chain per
- * workgroup while chains <= 8 x CUs, 2 up to 16 x CUs, else 4; the persistent batch kernel uses 4). The reference's
- * counterpart is the NW constant (src/adock/mckernel.py:525-540). Returns 0, or ADOCK_E_ARG. */
+ * workgroup while chains <= 8 x CUs, 2 up to 16 x CUs, else 4; the persistent batch kernel uses 4). A process-wide
+ * default (atomic: a launch on another thread sees the old or the new value); the batched launches take their own per
+ * call (waves_per_block). The reference's counterpart is the NW constant (src/adock/mckernel.py:525-540). Returns 0,
+ * or ADOCK_E_ARG. */
 int adock_set_chains_per_block(int chains_per_block);
 int adock_get_chains_per_block(void);
 

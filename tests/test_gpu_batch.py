@@ -77,8 +77,10 @@ def test_autotuner_picks_a_shape(small_screen):
     print("[gpu] autotune", flush=True)
     w, kp, st, cl = small_screen
     r = BatchRunner(kp, cl, n_complexes=len(w["ligs"]))
+    from adock_amd import _lib
     W, blocks = Autotuner().tune(r, st, probe_steps=1, reps=1)
-    assert W in (1, 2, 4)
+    assert W in (1, 2, 4) and r.waves == W
+    assert _lib.lib().adock_get_chains_per_block() == 0     # the shape is the runner's, not the process default
     ref = _run_plain(kp, st, cl, 2)
     N = len(cl)
     cur, best = st.clone(), st.clone()
@@ -86,8 +88,6 @@ def test_autotuner_picks_a_shape(small_screen):
     r.launch(torch.arange(N, dtype=torch.int32).cuda(), cur, ce, best, be, True, 2, 0, 1.2, 2.0, 2.0, 55, 0)
     torch.cuda.synchronize()
     np.testing.assert_array_equal(ref[3], be.cpu().numpy())
-    from adock_amd import _lib
-    _lib.check(_lib.lib().adock_set_chains_per_block(0))
 
 
 def test_run_mc_routes_many_chains_through_queues():
@@ -124,6 +124,55 @@ def test_run_mc_routes_many_chains_through_queues():
     assert a[4] == b[4] > 0
     for x, y in zip(a[:4], b[:4]):
         np.testing.assert_array_equal(x, y)
+
+
+def test_run_mc_queued_route_is_asynchronous_and_capturable():
+    """run_mc's queued route builds its one longest-first queue on the device (chain_lig is never read back): the
+    order equals the host partition's single queue, the route runs inside a HIP-graph capture, and three replays
+    (with eager resets and synchronisations between them) equal the eager run bit for bit."""
+    print("[gpu] run_mc queued route: device-built queue, graph capture", flush=True)
+    from adock_amd.batch import queue_partition
+    w, prob = workload.screen(seed=4, n_ligands=64, chains=64, steps=3)
+    kp = KernelProblem(prob)
+    L = len(w["ligs"])
+    st_h, cl_h = workload.chain_starts(4, w["ligs"], [w["center"]] * L, [w["size"]] * L, w["chains"], kp.KM)
+    st, cl = torch.as_tensor(st_h).cuda(), torch.as_tensor(cl_h).cuda()
+    N = len(cl_h)
+    r = BatchRunner.on_device(kp, cl, L)
+    want, _ = queue_partition(np.asarray(kp.cost)[cl_h], np.zeros(N, np.int64), 1)
+    np.testing.assert_array_equal(r.order.cpu().numpy(), want)
+    np.testing.assert_array_equal(r.lig_chains.cpu().numpy(), np.bincount(cl_h, minlength=L))
+    ci = torch.arange(N, dtype=torch.int32).cuda()
+
+    def bufs():
+        return st.clone(), torch.zeros(N).cuda(), st.clone(), torch.full((N,), float("inf")).cuda()
+
+    def call(b):
+        kp.run_mc(cl, ci, *b, True, w["steps"], 0, 1.2, 2.0, 2.0, 9, 0)
+
+    ref = bufs()
+    call(ref)
+    torch.cuda.synchronize()
+    assert isinstance(kp._keep[-1], BatchRunner)
+    b = bufs()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        call(b)
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        call(b)
+    keep = kp._keep
+    for rep in range(3):
+        b[0].copy_(st); b[1].zero_(); b[2].copy_(st); b[3].fill_(float("inf"))
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for x, y in zip(ref, b):
+            assert torch.equal(x, y), rep
+    del keep
 
 
 def test_skewed_queues_are_stolen_and_complete(small_screen):

@@ -177,6 +177,35 @@ def test_multi_receptor_full_size_properties():
     _full_size_properties(kp, torch.as_tensor(st_h).cuda(), cl_h, w["steps"], subset=[0, 777, 2047])
 
 
+def test_hbm_admission_groups_equal_resident_batch():
+    """HBM admission with a budget of a third of all grid bytes: several receptor groups, each group's grid sets within
+    the budget (the predicted bytes are those the build allocates), and every chain ends as in the all-resident batch
+    bit for bit (one state layout and kernel bucket for all groups; chain ids are global)."""
+    print("[gpu] HBM admission: groups under a budget vs all grid sets resident", flush=True)
+    from adock_amd.admission import AdmittedBatch
+    M, steps = 8, 3
+    w, prob = workload.multi(seed=5, n_receptors=6, ligands_per=3, chains=M, steps=steps)
+    kp = KernelProblem(prob)
+    probe = AdmittedBatch(w["receptor_dicts"], w["ligs"], w["rec_idx"], 1e18, M)
+    total = sum(probe.bytes)
+    assert total == int(prob.gs.flat.numel()) * 4
+    budget = total / 3
+    ab = AdmittedBatch(w["receptor_dicts"], w["ligs"], w["rec_idx"], budget, M)
+    assert len(ab.groups) >= 3 and (ab.AM, ab.KM) == (kp.AM, kp.KM)
+    st_h, cl_h = _multi_starts(w, kp, M, 9)
+    st = torch.as_tensor(st_h).cuda()
+    runner = BatchRunner(kp, torch.as_tensor(cl_h).cuda(), n_complexes=len(w["ligs"]))
+    ref = _batched(kp, runner, st, True, steps, 0, SEED64)
+    b = _bufs(st)
+    res = ab.run(*b, True, steps, 0, 1.2, 2.0, 2.0, SEED64, 0, log=print)
+    torch.cuda.synchronize()
+    for x, y in zip(ref, b):
+        np.testing.assert_array_equal(x, y.cpu().numpy())
+    if max(ab.bytes) <= budget:
+        assert ab.peak_grid_bytes <= budget
+    assert len(res["groups"]) == len(ab.groups) and np.all(res["latency_s"] > 0)
+
+
 # ------------------------------------------------------------------------------------------------ screening batch
 @pytest.fixture(scope="module")
 def screen_full():

@@ -166,3 +166,36 @@ def test_energy_kernel_many_states_per_wave(shuffle):
     np.testing.assert_array_equal(e.cpu().numpy(), e1.cpu().numpy())
     np.testing.assert_array_equal(g.cpu().numpy(), g1.cpu().numpy())
     assert np.isfinite(e.cpu().numpy()).all()
+
+
+def test_exact_polish_neighbour_list_matches_full():
+    """The reference's edge case (tests/test_mckernel.py:150-175): 16 poses near the box centre, pre-relaxed on the
+    grid (100 BFGS iterations), then polished in exact mode (50 iterations) once over the whole receptor and once over
+    per-chain neighbour lists (cutoff + 2 A); the polish moves poses little, so both reach the same exact energies
+    within 0.05 kcal/mol, and the lists are shorter than the receptor."""
+    print("[gpu] exact polish: neighbour lists vs the whole receptor", flush=True)
+    from adock_amd.pose import random_quat
+    w, prob = workload.single("small", seed=0)
+    kp = KernelProblem(prob)
+    lb = prob.lb
+    g = torch.Generator(device="cuda").manual_seed(4)
+    M = 16
+    c = torch.as_tensor(w["center"], dtype=torch.float32, device="cuda")
+    st = kp.pack(c + torch.randn((M, 3), generator=g, device="cuda") * 0.5, random_quat((M,), g, "cuda"),
+                 torch.zeros(M, lb.K, device="cuda"))
+    cl, ci = torch.zeros(M, dtype=torch.int32, device="cuda"), torch.arange(M, dtype=torch.int32, device="cuda")
+    b0, e0 = st.clone(), torch.full((M,), float("inf"), device="cuda")
+    kp.run_mc(cl, ci, st, torch.zeros(M, device="cuda"), b0, e0, True, 0, 100, 1.2, 2.0, 2.0, 0, 0)
+    out = []
+    for use_nb in (False, True):
+        cur, best = b0.clone(), b0.clone()
+        be = torch.full((M,), float("inf"), device="cuda")
+        nb = kp.receptor_neighbours(prob, cl, cur, lb.K, 2.0) if use_nb else (None, None)
+        kp.run_mc(cl, ci, cur, torch.zeros(M, device="cuda"), best, be, True, 0, 50, 1.2, 2.0, 2.0, 0, 0,
+                  exact=True, nbr=nb[0], nbr_n=nb[1])
+        out.append(kp.energy_grad(cl, best, exact=True)[0])
+        if use_nb:
+            assert int(nb[1].max()) < int(prob.rec[2].sum())
+    torch.cuda.synchronize()
+    print(f"  max |diff| {float((out[1] - out[0]).abs().max()):.2e}", flush=True)
+    torch.testing.assert_close(out[1], out[0], rtol=0, atol=0.05)

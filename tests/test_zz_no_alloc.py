@@ -30,8 +30,8 @@ def _noalloc(kp, cl, queues, st, steps, seed, times=True):
     ci = torch.arange(N, dtype=torch.int32, device=dev)
     args = (kp._args(False) + [_lib.ptr(cl), _lib.ptr(ci), N, None, None, 0, _lib.ptr(cur), _lib.ptr(ce),
                                _lib.ptr(best), _lib.ptr(be), 1, steps, 0, 1.2, 2.0, 2.0, seed, 0, _lib.ptr(order),
-                               _lib.ptr(qstart), len(qstart) - 1, L, _lib.ptr(lig_chains), _lib.ptr(tm), 0, None,
-                               None, _lib.stream_of(dev)])
+                               _lib.ptr(qstart), len(qstart) - 1, L, _lib.ptr(lig_chains), _lib.ptr(tm), 0, 0,
+                               None, None, _lib.stream_of(dev)])
     _lib.check(_lib.lib().adock_mc_batch_noalloc(*args))
     torch.cuda.synchronize()
     return [x.cpu().numpy() for x in (cur, ce, best, be)], (None if tm is None else tm.cpu().numpy())
@@ -86,9 +86,9 @@ def test_noalloc_refuses_sizes_beyond_its_array():
     base = kp._args(False) + [_lib.ptr(cl), _lib.ptr(cl), N, None, None, 0, _lib.ptr(st), _lib.ptr(f), _lib.ptr(st),
                               _lib.ptr(f), 1, 1, 0, 1.2, 2.0, 2.0, 1, 0, None, None]
     L = _lib.lib()
-    assert L.adock_mc_batch_noalloc(*base, 1, 65537, _lib.ptr(lc), _lib.ptr(tm), 0, None, None,
+    assert L.adock_mc_batch_noalloc(*base, 1, 65537, _lib.ptr(lc), _lib.ptr(tm), 0, 0, None, None,
                                     _lib.stream_of(kp.device)) == -1
     assert b"65536" in L.adock_last_error()
-    assert L.adock_mc_batch_noalloc(*base, 1025, 2, _lib.ptr(lc), _lib.ptr(tm), 0, None, None,
+    assert L.adock_mc_batch_noalloc(*base, 1025, 2, _lib.ptr(lc), _lib.ptr(tm), 0, 0, None, None,
                                     _lib.stream_of(kp.device)) == -1
     torch.cuda.synchronize()
