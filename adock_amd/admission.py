@@ -8,8 +8,8 @@ built, so at most one group's grids are resident. Every group is packed to the s
 chains of all groups share one state layout and run the same kernel: a complex's result does not depend on its group
 (the same bits as with every grid set resident at once).
 
-The reference has no counterpart: it builds one receptor's grids at a time and keeps no device budget
-(src/adock/dock.py, src/adock/search.py:544-671).
+The reference has no counterpart: dock_batch builds the grid sets of every complex it is given at once and keeps no
+device budget, so its callers bound memory through the batch they pass (src/adock/dock.py:69-110).
 """
 import time
 
