@@ -98,6 +98,7 @@ template <int KM> using TMask = std::conditional_t<(KM > 32), unsigned long long
 //                                  R_i1, R_i2, b_i; p -> R p + b), rebuilt every evaluation (identity past nt)
 //   TU  [KM + 1]    float4  current (world) axis unit vector u_k
 //   TM  [KM + 1]    float4  current axis moment m_k = p_k x u_k (torque about the axis: u.(X x F) + m.F)
+//   RD  [64]        float   the evaluation's 7 + KM wave sums (wave_sum_to_lanes)
 //   FS  [P128 + 4]  float   slot of pair q: (dE/dr) / r, so dE/dX of either atom is FS[q] (X_self - X_partner);
 //                           slot P128 stays zero
 //   flg [AMX]       int     atom type flags
@@ -119,13 +120,13 @@ __host__ __device__ inline int inc_row(int P_MAX, int AM) { return 2 * pad128(P_
 __host__ __device__ inline int wave_lds_bytes(int KM, int P_MAX, int APL = 1) {
     const int P = pad128(P_MAX);
     const int pw = APL == 1 ? 2 : 4;
-    return 16 * (64 * APL + 1) + 5 * 16 * (KM + 1) + 4 * (P + 4) + 4 * 64 * APL + pw * (P + 8) +
+    return 16 * (64 * APL + 1) + 5 * 16 * (KM + 1) + 4 * 64 + 4 * (P + 4) + 4 * 64 * APL + pw * (P + 8) +
            2 * inc_stride(P_MAX, APL);
 }
 __host__ __device__ inline int pad512(int P_MAX) { return (P_MAX + 511) & ~511; }
 __host__ __device__ inline int wave_lds_bytes_dense(int KM, int P_MAX, int DEG_MAX) {
     const int P = pad512(P_MAX);   // pair words and destinations padded to whole 8-round chunks
-    return 16 * (64 + 1) + 5 * 16 * (KM + 1) + 16 * (64 * DEG_MAX + 1) + 4 * 64 + 2 * (P + 8) + 4 * P;
+    return 16 * (64 + 1) + 5 * 16 * (KM + 1) + 4 * 64 + 16 * (64 * DEG_MAX + 1) + 4 * 64 + 2 * (P + 8) + 4 * P;
 }
 constexpr int LDS_PER_WORKGROUP = 160 * 1024;   // gfx950: 160 KB of LDS per CU, all of it addressable by one workgroup
 
@@ -133,6 +134,7 @@ template <int APL>
 struct WaveLDS {
     float4* X4;
     float4* TR0; float4* TR1; float4* TR2; float4* TU; float4* TM;
+    float* RD;            // the evaluation's wave reduction (64 floats)
     float* FS;
     int* flg;
     typename PairWord<APL>::T* pl;
@@ -186,6 +188,7 @@ DEVI WaveLDS<APL> carve_lds(char* base, int P_MAX, int DEG_MAX = 0) {
     w.TR2 = f; f += KM + 1;
     w.TU = f; f += KM + 1;
     w.TM = f; f += KM + 1;
+    w.RD = reinterpret_cast<float*>(f); f += 16;
     w.FS = nullptr;
     w.FD = nullptr;
     w.RN4 = nullptr;
@@ -700,14 +703,9 @@ DEVI float eval_eg(const ChainCtx<KM, APL>& c, const SFConst& sf, const WaveLDS<
             }
         }
     }
-    wave_sum_multi<7 + KM>(v);
-    float g = 0.f;
-#pragma unroll
-    for (int i = 0; i < 6; ++i) g = lane == i ? v[i] : g;
-#pragma unroll
-    for (int k = 0; k < KM; ++k) g = (lane == 7 + k && k < c.nt) ? v[7 + k] : g;
-    gout = g;
-    return v[6];
+    const float r = wave_sum_to_lanes<7 + KM>(v, w.RD, lane);
+    gout = (lane < 6 || (lane >= 7 && lane < 7 + c.nt)) ? r : 0.f;
+    return bcast(r, 6);
 }
 
 // x (+) alpha p on the lane-distributed state
