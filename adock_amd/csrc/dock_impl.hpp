@@ -98,7 +98,8 @@ template <int KM> using TMask = std::conditional_t<(KM > 32), unsigned long long
 //                                  R_i1, R_i2, b_i; p -> R p + b), rebuilt every evaluation (identity past nt)
 //   TU  [KM + 1]    float4  current (world) axis unit vector u_k
 //   TM  [KM + 1]    float4  current axis moment m_k = p_k x u_k (torque about the axis: u.(X x F) + m.F)
-//   RD  [64]        float   the evaluation's 7 + KM wave sums (wave_sum_to_lanes)
+//   RD  [128]       float   the evaluation's 7 + KM wave sums (wave_sum_to_lanes); between evaluations, BFGS's
+//                           lane-distributed vectors for broadcast reads
 //   FS  [P128 + 4]  float   slot of pair q: (dE/dr) / r, so dE/dX of either atom is FS[q] (X_self - X_partner);
 //                           slot P128 stays zero
 //   flg [AMX]       int     atom type flags
@@ -120,13 +121,13 @@ __host__ __device__ inline int inc_row(int P_MAX, int AM) { return 2 * pad128(P_
 __host__ __device__ inline int wave_lds_bytes(int KM, int P_MAX, int APL = 1) {
     const int P = pad128(P_MAX);
     const int pw = APL == 1 ? 2 : 4;
-    return 16 * (64 * APL + 1) + 5 * 16 * (KM + 1) + 4 * 64 + 4 * (P + 4) + 4 * 64 * APL + pw * (P + 8) +
+    return 16 * (64 * APL + 1) + 5 * 16 * (KM + 1) + 4 * 128 + 4 * (P + 4) + 4 * 64 * APL + pw * (P + 8) +
            2 * inc_stride(P_MAX, APL);
 }
 __host__ __device__ inline int pad512(int P_MAX) { return (P_MAX + 511) & ~511; }
 __host__ __device__ inline int wave_lds_bytes_dense(int KM, int P_MAX, int DEG_MAX) {
     const int P = pad512(P_MAX);   // pair words and destinations padded to whole 8-round chunks
-    return 16 * (64 + 1) + 5 * 16 * (KM + 1) + 4 * 64 + 16 * (64 * DEG_MAX + 1) + 4 * 64 + 2 * (P + 8) + 4 * P;
+    return 16 * (64 + 1) + 5 * 16 * (KM + 1) + 4 * 128 + 16 * (64 * DEG_MAX + 1) + 4 * 64 + 2 * (P + 8) + 4 * P;
 }
 constexpr int LDS_PER_WORKGROUP = 160 * 1024;   // gfx950: 160 KB of LDS per CU, all of it addressable by one workgroup
 
@@ -188,7 +189,7 @@ DEVI WaveLDS<APL> carve_lds(char* base, int P_MAX, int DEG_MAX = 0) {
     w.TR2 = f; f += KM + 1;
     w.TU = f; f += KM + 1;
     w.TM = f; f += KM + 1;
-    w.RD = reinterpret_cast<float*>(f); f += 16;
+    w.RD = reinterpret_cast<float*>(f); f += 32;
     w.FS = nullptr;
     w.FD = nullptr;
     w.RN4 = nullptr;
@@ -742,6 +743,22 @@ DEVI float plain_move(float x, const Ctx& c, int lane, unsigned long long seed, 
     return x;
 }
 
+// A lane-distributed vector (component j in lane j) made readable by every lane: written to this wave's LDS at
+// construction, component j then read as a broadcast (all lanes the same address; the compiler merges neighbouring
+// components into 16-byte reads).
+template <int DV>
+struct BcastVec {
+    const float4* a;
+    DEVI BcastVec(float* buf, int lane, float v) : a(reinterpret_cast<const float4*>(buf)) {
+        buf[lane] = v;
+        wave_lds_sync();
+    }
+    DEVI float operator[](int j) const {
+        const float4 q = a[j >> 2];
+        return (j & 3) == 0 ? q.x : (j & 3) == 1 ? q.y : (j & 3) == 2 ? q.z : q.w;
+    }
+};
+
 // Vina-style BFGS on the lane-distributed state x (in/out); returns the final energy. One evaluation site: the first
 // pass evaluates x itself, every later pass evaluates one line-search trial.
 template <int KM, int APL, bool DENSE = false, bool EX = false>
@@ -782,25 +799,35 @@ DEVI float bfgs(const ChainCtx<KM, APL>& c, const SFConst& sf, const WaveLDS<APL
                 }
             }
             if (!(alpha * yp < EPSF)) {
+                // the vectors' components reach every lane as LDS broadcast reads (16 bytes per read) instead of a
+                // v_readlane each: the same operands in the same order, so the same bits, with a quarter of the
+                // instructions on the VALU
                 float mhy = 0.f;
+                {
+                    const BcastVec<DV> yb(w.RD, lane, y);
 #pragma unroll
-                for (int j = 0; j < DV; ++j) mhy += h[j] * bcast(y, j);
+                    for (int j = 0; j < DV; ++j) mhy += h[j] * yb[j];
+                }
                 mhy = -mhy;
                 const float yhy = -wave_sum(y * mhy);
                 const float r = 1.f / (alpha * yp);
                 const float c2 = alpha * alpha * (r * r * yhy + r);
                 const float ar = alpha * r;
+                const BcastVec<DV> pb(w.RD, lane, p), mb(w.RD + 64, lane, mhy);
 #pragma unroll
                 for (int j = 0; j < DV; ++j) {
-                    const float pj = bcast(p, j), mj = bcast(mhy, j);
+                    const float pj = pb[j], mj = mb[j];
                     h[j] += ar * (mhy * pj + mj * p) + c2 * p * pj;
                 }
             }
             if (++step >= iters) break;
         }
         float s = 0.f;
+        {
+            const BcastVec<DV> gb(w.RD, lane, g);
 #pragma unroll
-        for (int j = 0; j < DV; ++j) s += h[j] * bcast(g, j);
+            for (int j = 0; j < DV; ++j) s += h[j] * gb[j];
+        }
         p = -s;
         pg = wave_sum(p * g);
         alpha = 1.f;
