@@ -28,7 +28,7 @@ static int check_buckets(int AM, int KM, int P_MAX) {
     if (AM != 64 && AM != 96 && AM != 128)
         return fail(ADOCK_E_BUCKET, "AM must be 64, 96 or 128 (at most 128 heavy atoms per ligand)");
     if (KM < 8 || KM > 56 || (KM & 7)) return fail(ADOCK_E_BUCKET, "KM must be a multiple of 8 in 8..56");
-    if (P_MAX < 1 || P_MAX > 32767 || !lds_budget_ok(KM, P_MAX, AM))
+    if (P_MAX < 1 || P_MAX > 16256 || !lds_budget_ok(KM, P_MAX, AM))
         return fail(ADOCK_E_LDS, "P_MAX out of range for the per-wave LDS budget");
     return 0;
 }
@@ -103,7 +103,7 @@ int adock_inter_exact_fit(const float* xyz, const int* lig_type, int n_cx, int n
 #define LIG_DECL                                                                                                     \
     const float *ref, const int *typ, const unsigned char *tm, const int *ax, const float *axp, const float *axu,    \
         const int *pr, const int *na, const int *nt, const int *np, const float *rg, int P_MAX, const int *gslot, \
-        const unsigned short *inc, const int *ist, int DEG_MAX
+        const unsigned *inc, const int *ist, int DEG_MAX
 #define GRID_DECL                                                                                                    \
     const float *grid, const float *origin, const float *lo, const float *hi, const int *rec_idx,                    \
         const long long *goffs, const long long *gdims, int T, float spacing, float pen

@@ -105,24 +105,23 @@ template <int KM> using TMask = std::conditional_t<(KM > 32), unsigned long long
 //   flg [AMX]       int     atom type flags
 //   pl  [P128 + 8]  word    pair words (PairWord<APL>), strided over lanes (pair q on lane q % 64 in round q / 64);
 //                           padding pairs (and entry P128) are (dummy, 0)
-//   inc [2 P128 + 512 APL] u16  per-atom pair-slot lists, copied from the host-built packed ligand (each atom's list
-//                           padded to 8 entries; start and padded count live in the atom's lane and slot); bit 15 =
-//                           the atom is the pair's second one (its partner is then the first)
+// The per-atom pair-slot lists stay in memory (the host-built packed ligand, read through the caches; a complex's
+// chains share them): atom a's entries are fs | px << 16, the byte offsets of the pair's slot in FS and of its partner
+// atom in X4, so an entry costs two address additions (each list padded to 8 entries; start and padded count live in
+// the atom's lane and slot; the row ends with 8 padding entries: the zero slot and the dummy atom).
 // Dense variant (the latency-bound speculative kernel, one atom per lane): FS is replaced by FD [64 DEG_MAX + 1]
 // float4, row k column a ^ k = the gradient of atom a's k-th pair (sign applied by the writer; unused slots stay zero,
 // the last one takes the padding pairs' writes), and inc by pdst [P128] u32 (destination slot of pair q's first atom
 // | second atom << 16). The per-atom sum reads one permuted row per entry (no list indirection, no bank conflicts);
 // the XOR skew spreads the writes of one atom's pairs over the banks.
 __host__ __device__ inline int pad128(int P_MAX) { return (P_MAX + 127) & ~127; }
-// LDS capacity of the slot lists (up to 64 APL atoms) and the row length of one ligand's lists in the packed arrays
-// (AM atoms, each list padded to a multiple of 8 entries); the LDS copy keeps the row's layout, and AM <= 64 APL
-__host__ __device__ inline int inc_stride(int P_MAX, int APL) { return 2 * pad128(P_MAX) + 512 * APL; }
-__host__ __device__ inline int inc_row(int P_MAX, int AM) { return 2 * pad128(P_MAX) + 8 * AM; }
+// row length of one ligand's slot lists in the packed arrays (AM atoms, each list padded to a multiple of 8 entries,
+// then one group of 8 padding entries)
+__host__ __device__ inline int inc_row(int P_MAX, int AM) { return 2 * pad128(P_MAX) + 8 * AM + 8; }
 __host__ __device__ inline int wave_lds_bytes(int KM, int P_MAX, int APL = 1) {
     const int P = pad128(P_MAX);
     const int pw = APL == 1 ? 2 : 4;
-    return 16 * (64 * APL + 1) + 5 * 16 * (KM + 1) + 4 * 128 + 4 * (P + 4) + 4 * 64 * APL + pw * (P + 8) +
-           2 * inc_stride(P_MAX, APL);
+    return 16 * (64 * APL + 1) + 5 * 16 * (KM + 1) + 4 * 128 + 4 * (P + 4) + 4 * 64 * APL + pw * (P + 8);
 }
 __host__ __device__ inline int pad512(int P_MAX) { return (P_MAX + 511) & ~511; }
 __host__ __device__ inline int wave_lds_bytes_dense(int KM, int P_MAX, int DEG_MAX) {
@@ -139,7 +138,6 @@ struct WaveLDS {
     float* FS;
     int* flg;
     typename PairWord<APL>::T* pl;
-    unsigned short* inc;
     float4* FD;           // dense variant
     unsigned* pdst;
     int P128, trash;
@@ -157,7 +155,9 @@ struct ChainCtx {
     TMask<KM> tmask[APL];
     bool atom[APL];
     const float* gp[APL];
-    int is[APL], ic8[APL];   // this atom's slot list: inc[is .. is + ic8), padded to a multiple of 8
+    int is[APL], ic8[APL];   // this atom's slot list: incg[is .. is + ic8), padded to a multiple of 8
+    const unsigned* incg;    // the ligand's row of slot lists (memory); incg[incpad ..] are 8 padding entries
+    int incpad;
     // lane = torsion k: its axis atoms; lane 7 + k: torsion k's reference axis (point, unit vector)
     int axa, axb;
     float rpx, rpy, rpz, rux, ruy, ruz;
@@ -196,7 +196,6 @@ DEVI WaveLDS<APL> carve_lds(char* base, int P_MAX, int DEG_MAX = 0) {
     w.RNF = nullptr;
     w.MK = nullptr;
     w.nrs = 0;
-    w.inc = nullptr;
     w.pdst = nullptr;
     w.trash = 64 * DEG_MAX;
     if constexpr (DENSE) { w.FD = f; f += 64 * DEG_MAX + 1; }
@@ -204,7 +203,6 @@ DEVI WaveLDS<APL> carve_lds(char* base, int P_MAX, int DEG_MAX = 0) {
     w.flg = reinterpret_cast<int*>(f);
     w.pl = reinterpret_cast<typename PairWord<APL>::T*>(w.flg + AMX);
     if constexpr (DENSE) w.pdst = reinterpret_cast<unsigned*>(w.pl + w.P128 + 8);
-    else w.inc = reinterpret_cast<unsigned short*>(w.pl + w.P128 + 8);
     return w;
 }
 
@@ -329,8 +327,10 @@ DEVI ChainCtx<KM, APL> setup_chain(int l, int lane, const LigArgs& L, const Grid
         w.pl[q] = (typename PW::T)word;
     }
     wave_lds_sync();
-    // each atom's pair-slot list (host-built): copied to LDS 8 entries per 16-byte copy, or (dense variant) turned
-    // into the destination slots of its pairs
+    // each atom's pair-slot list (host-built): read from memory in the evaluations, or (dense variant) turned into the
+    // destination slots of its pairs (the atom is the pair's second one when its partner is the pair's first)
+    c.incg = L.inc + (long long)l * inc_row(L.P_MAX, AM);
+    c.incpad = inc_row(L.P_MAX, AM) - 8;
     int d8 = 0;
 #pragma unroll
     for (int s = 0; s < APL; ++s) {
@@ -339,18 +339,17 @@ DEVI ChainCtx<KM, APL> setup_chain(int l, int lane, const LigArgs& L, const Grid
         c.is[s] = e & 0xffff;
         c.ic8[s] = min((e >> 16) & 0xffff, DENSE ? L.DEG_MAX : 0xffff);
         d8 = max(d8, c.ic8[s]);
-        const unsigned short* src = L.inc + (long long)l * inc_row(L.P_MAX, AM) + c.is[s];
         if constexpr (DENSE) {
+            const unsigned* src = c.incg + c.is[s];
             unsigned short* p16 = reinterpret_cast<unsigned short*>(w.pdst);
             for (int k = 0; k < c.ic8[s]; ++k) {
                 const unsigned v = src[k];
-                if ((v & 0x7fffu) < (unsigned)c.np)
-                    p16[2 * (v & 0x7fffu) + (v >> 15)] = (unsigned short)(k * 64 + (lane ^ k));
+                const unsigned q = (v & 0xffffu) >> 2, p = v >> 20;
+                if (q < (unsigned)c.np) {
+                    const unsigned side = pw_first<APL>(w.pl[q]) == p ? 1u : 0u;
+                    p16[2 * q + side] = (unsigned short)(k * 64 + (lane ^ k));
+                }
             }
-        } else {
-            const uint4* src4 = reinterpret_cast<const uint4*>(src);
-            uint4* dst = reinterpret_cast<uint4*>(w.inc + c.is[s]);
-            for (int g = 0; g < (c.ic8[s] >> 3); ++g) dst[g] = src4[g];
         }
     }
     c.D8 = (int)wave_max_f((float)d8);
@@ -612,30 +611,30 @@ DEVI float eval_eg(const ChainCtx<KM, APL>& c, const SFConst& sf, const WaveLDS<
             }
         }
     } else {
-        // each entry: the pair's slot (dE/dr)/r and the partner atom, from the pair word; the product is rounded
-        // before the sum (as the dense rows' (dE/dX) slots are), so both layouts give the same bits
-        const uint4* inc4 = reinterpret_cast<const uint4*>(w.inc);
+        // each entry: the byte offsets of the pair's slot (dE/dr)/r and of the partner atom; lanes past their list
+        // read the row's trailing padding (zero slot, dummy atom). The product is rounded before the sum (as the
+        // dense rows' (dE/dX) slots are), so both layouts give the same bits
+        const char* fsb = reinterpret_cast<const char*>(w.FS);
+        const char* x4b = reinterpret_cast<const char*>(w.X4);
 #pragma unroll
         for (int s = 0; s < APL; ++s) {
             for (int k = 0; k < c.D8; k += 8) {
-                const bool ok = k < c.ic8[s];
-                const uint4 v = inc4[ok ? (c.is[s] + k) >> 3 : 0];
-                const unsigned zs = (unsigned)w.P128;
-                const unsigned ws[8] = {ok ? v.x & 0xffffu : zs, ok ? v.x >> 16 : zs, ok ? v.y & 0xffffu : zs,
-                                        ok ? v.y >> 16 : zs,     ok ? v.z & 0xffffu : zs, ok ? v.z >> 16 : zs,
-                                        ok ? v.w & 0xffffu : zs, ok ? v.w >> 16 : zs};
+                const uint4* r4 = reinterpret_cast<const uint4*>(c.incg + (k < c.ic8[s] ? c.is[s] + k : c.incpad));
+                const uint4 va = r4[0], vb = r4[1];
+                const unsigned ws[8] = {va.x, va.y, va.z, va.w, vb.x, vb.y, vb.z, vb.w};
                 float sc[8];
-                unsigned pw[8];
+                float4 xp[8];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) { sc[u] = w.FS[ws[u] & 0x7fffu]; pw[u] = w.pl[ws[u] & 0x7fffu]; }
+                for (int u = 0; u < 8; ++u) {
+                    sc[u] = *reinterpret_cast<const float*>(fsb + (ws[u] & 0xffffu));
+                    xp[u] = *reinterpret_cast<const float4*>(x4b + (ws[u] >> 16));
+                }
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
 #pragma clang fp contract(off)
-                    const unsigned pt = (ws[u] & 0x8000u) ? pw_first<APL>(pw[u]) : pw_second<APL>(pw[u]);
-                    const float4 xp = w.X4[pt];
-                    pfx[s] += sc[u] * (X0[s] - xp.x);
-                    pfy[s] += sc[u] * (X1[s] - xp.y);
-                    pfz[s] += sc[u] * (X2[s] - xp.z);
+                    pfx[s] += sc[u] * (X0[s] - xp[u].x);
+                    pfy[s] += sc[u] * (X1[s] - xp[u].y);
+                    pfz[s] += sc[u] * (X2[s] - xp[u].z);
                 }
             }
         }

@@ -6,7 +6,7 @@ namespace adock {
 struct LigArgs {
     const float* ref; const int* typ; const unsigned char* tm; const int* ax; const float* axp; const float* axu;
     const int* pr; const int* na; const int* nt; const int* np; const float* rg; int P_MAX; const int* gslot;
-    const unsigned short* inc; const int* ist; int DEG_MAX;
+    const unsigned* inc; const int* ist; int DEG_MAX;
     int AM = 64;   // heavy-atom bucket of the packed arrays: 64 (one atom per lane) or 96 / 128 (two per lane)
 };
 struct GridArgs {

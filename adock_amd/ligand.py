@@ -147,30 +147,41 @@ def pad128(P):
 
 
 def inc_stride(P_MAX, AM=64):
-    """Length of one ligand's row of slot lists: two entries per pair plus up to 7 padding entries per atom (each
-    atom's list is padded to a multiple of 8)."""
-    return 2 * pad128(P_MAX) + 8 * AM
+    """Length of one ligand's row of slot lists: two entries per pair, up to 7 padding entries per atom (each atom's
+    list is padded to a multiple of 8), and a trailing group of 8 padding entries (what lanes past their list read)."""
+    return 2 * pad128(P_MAX) + 8 * AM + 8
+
+
+def pad_entry(P_MAX, AM=64):
+    """The padding entry of the slot lists: the always-zero slot pad128(P_MAX) and the dummy atom (64 per atom slot
+    of a lane: 64 for AM <= 64, else 128)."""
+    return (4 * pad128(P_MAX)) | ((16 * (64 if AM <= 64 else 128)) << 16)
 
 
 def slot_lists(pairs, P_MAX, AM=64):
-    """Per-atom pair-slot lists of one ligand for the kernels' atomic-free pair gradient: atom a's list holds q for
-    every pair q = (a, j) and q | 0x8000 (negate) for every pair q = (i, a), ascending in q, padded to a multiple of 8
-    entries with the always-zero slot pad128(P_MAX). Returns (inc [inc_stride] uint16, ist [AM] int32 =
-    start | padded_count << 16)."""
+    """Per-atom pair-slot lists of one ligand for the kernels' atomic-free pair gradient. Atom a's list holds, for
+    every intra pair q of a (ascending in q), the word fs | px << 16: fs = 4 q, the byte offset of the pair's slot
+    (dE/dr)/r in the kernels' slot array, and px = 16 p, the byte offset of the pair's other atom p in the atom array,
+    so that the atom adds slot x (X_a - X_p) with two address additions. Lists are padded to a multiple of 8 entries
+    with pad_entry (the zero slot and the dummy atom); 8 more padding entries end the row. Returns
+    (inc [inc_stride] uint32, ist [AM] int32 = start | padded_count << 16)."""
     z = pad128(P_MAX)
-    inc = np.full(inc_stride(P_MAX, AM), z, np.uint16)
+    if 4 * z > 0xffff:
+        raise ValueError(f"P_MAX = {P_MAX}: slot byte offsets exceed 16 bits (at most 16,256 pairs per ligand)")
+    inc = np.full(inc_stride(P_MAX, AM), pad_entry(P_MAX, AM), np.uint32)
     ist = np.zeros(AM, np.int32)
     pairs = np.asarray(pairs, np.int64).reshape(-1, 2)
     q = np.arange(len(pairs))
     real = pairs[:, 0] != pairs[:, 1]
     atom = np.concatenate([pairs[real, 0], pairs[real, 1]])
-    val = np.concatenate([q[real], q[real] | 0x8000])
-    order = np.lexsort((val & 0x7fff, atom))
-    atom, val = atom[order], val[order]
+    part = np.concatenate([pairs[real, 1], pairs[real, 0]])
+    qq = np.concatenate([q[real], q[real]])
+    order = np.lexsort((qq, atom))
+    atom, val = atom[order], ((4 * qq) | ((16 * part) << 16))[order].astype(np.uint32)
     cnt = np.bincount(atom, minlength=AM)[:AM]
     pc = (cnt + 7) // 8 * 8
     start = np.concatenate([[0], np.cumsum(pc)[:-1]])
-    if start[-1] + pc[-1] > len(inc):
+    if start[-1] + pc[-1] > len(inc) - 8:
         raise ValueError("slot lists exceed their row")
     first = np.concatenate([[0], np.cumsum(cnt)[:-1]])
     for a in np.nonzero(cnt)[0]:
@@ -224,7 +235,7 @@ def pack_ligands(preps, slot, rec_idx, floor=(0, 0)):
         "axp": np.zeros((L, KM, 3), np.float32), "axu": np.zeros((L, KM, 3), np.float32),
         "pr": np.zeros((L, P, 2), np.int32), "na": np.zeros(L, np.int32), "nt": np.zeros(L, np.int32),
         "np": np.zeros(L, np.int32), "rg": np.zeros(L, np.float32), "gslot": np.zeros((L, AM), np.int32),
-        "inc": np.zeros((L, inc_stride(P, AM)), np.uint16), "ist": np.zeros((L, AM), np.int32),
+        "inc": np.zeros((L, inc_stride(P, AM)), np.uint32), "ist": np.zeros((L, AM), np.int32),
     }
     slot = np.asarray(slot)
     rec_idx = np.asarray(rec_idx)

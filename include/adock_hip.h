@@ -15,11 +15,14 @@
  *
  * The ligand group is the reference's packed layout (src/adock/mckernel.py:547-592: ref, typ, tm, ax, axp, axu, pr,
  * na, nt, np, rg, P_MAX, gslot) plus two arrays of this library's own, built once per ligand set on the host
- * (adock_amd.ligand.slot_lists): inc [L, 2 * P128 + 8 * AM] uint16 and ist [L, AM] int32, P128 = P_MAX rounded up to
- * a multiple of 128. Atom a's list inc[l, s .. s + c) (ist = s | c << 16, s and c multiples of 8) names the intra pairs
- * of atom a: q for a pair q = (a, j), q | 0x8000 for a pair q = (i, a), ascending, padded with P128; DEG_MAX is the
- * largest c over all atoms and ligands (a multiple of 8, at most AM). The kernels use the lists to sum the pair
- * gradient of every atom without atomics; DEG_MAX sizes the per-atom gradient rows of the single-complex kernel.
+ * (adock_amd.ligand.slot_lists): inc [L, 2 * P128 + 8 * AM + 8] uint32 and ist [L, AM] int32, P128 = P_MAX rounded up
+ * to a multiple of 128. Atom a's list inc[l, s .. s + c) (ist = s | c << 16, s and c multiples of 8) holds, for every
+ * intra pair q of atom a, ascending in q, the word 4 q | (16 p) << 16 (p: the pair's other atom): the byte offsets of
+ * the pair's slot and of its partner atom in the kernels' per-wave arrays; lists are padded, and the row ends with 8
+ * entries, with 4 P128 | (16 D) << 16, D the dummy atom (64 for AM 64, 128 above). DEG_MAX is the largest c over all
+ * atoms and ligands (a multiple of 8, at most AM). The kernels use the lists to sum the pair gradient of every atom
+ * without atomics; DEG_MAX sizes the per-atom gradient rows of the single-complex kernel. P_MAX is at most 16,256
+ * (16-bit slot offsets; a 128-atom ligand has at most 8,128 pairs).
  */
 #ifndef ADOCK_HIP_H
 #define ADOCK_HIP_H
@@ -87,7 +90,7 @@ int adock_inter_exact_fit(const float* xyz, const int* lig_type, int n_cx, int n
 int adock_eg(int AM, int KM,
              const float* ref, const int* typ, const unsigned char* tm, const int* ax, const float* axp,
              const float* axu, const int* pr, const int* na, const int* nt, const int* np, const float* rg, int P_MAX,
-             const int* gslot, const unsigned short* inc, const int* ist, int DEG_MAX,
+             const int* gslot, const unsigned* inc, const int* ist, int DEG_MAX,
              const float* grid, const float* origin, const float* lo, const float* hi, const int* rec_idx,
              const long long* goffs, const long long* gdims, int T, float spacing, float pen,
              const float* prm, const float* rad, const int* hyd, const int* don, const int* acc,
@@ -101,7 +104,7 @@ int adock_eg(int AM, int KM,
 int adock_mc(int AM, int KM,
              const float* ref, const int* typ, const unsigned char* tm, const int* ax, const float* axp,
              const float* axu, const int* pr, const int* na, const int* nt, const int* np, const float* rg, int P_MAX,
-             const int* gslot, const unsigned short* inc, const int* ist, int DEG_MAX,
+             const int* gslot, const unsigned* inc, const int* ist, int DEG_MAX,
              const float* grid, const float* origin, const float* lo, const float* hi, const int* rec_idx,
              const long long* goffs, const long long* gdims, int T, float spacing, float pen,
              const float* prm, const float* rad, const int* hyd, const int* don, const int* acc,
@@ -118,7 +121,7 @@ int adock_mc(int AM, int KM,
 int adock_mc_counted(int AM, int KM,
                      const float* ref, const int* typ, const unsigned char* tm, const int* ax, const float* axp,
                      const float* axu, const int* pr, const int* na, const int* nt, const int* np, const float* rg,
-                     int P_MAX, const int* gslot, const unsigned short* inc, const int* ist, int DEG_MAX,
+                     int P_MAX, const int* gslot, const unsigned* inc, const int* ist, int DEG_MAX,
                      const float* grid, const float* origin, const float* lo, const float* hi, const int* rec_idx,
                      const long long* goffs, const long long* gdims, int T, float spacing, float pen,
                      const float* prm, const float* rad, const int* hyd, const int* don, const int* acc,
@@ -148,7 +151,7 @@ int adock_mc_counted(int AM, int KM,
 int adock_mc_batch(int AM, int KM,
                    const float* ref, const int* typ, const unsigned char* tm, const int* ax, const float* axp,
                    const float* axu, const int* pr, const int* na, const int* nt, const int* np, const float* rg,
-                   int P_MAX, const int* gslot, const unsigned short* inc, const int* ist, int DEG_MAX,
+                   int P_MAX, const int* gslot, const unsigned* inc, const int* ist, int DEG_MAX,
                    const float* grid, const float* origin, const float* lo, const float* hi, const int* rec_idx,
                    const long long* goffs, const long long* gdims, int T, float spacing, float pen,
                    const float* prm, const float* rad, const int* hyd, const int* don, const int* acc,
@@ -178,7 +181,7 @@ int adock_batch_reset(int* counter, int n_counters, int* done, int n_complexes, 
 int adock_mc_batch_noalloc(int AM, int KM,
                            const float* ref, const int* typ, const unsigned char* tm, const int* ax, const float* axp,
                            const float* axu, const int* pr, const int* na, const int* nt, const int* np,
-                           const float* rg, int P_MAX, const int* gslot, const unsigned short* inc, const int* ist,
+                           const float* rg, int P_MAX, const int* gslot, const unsigned* inc, const int* ist,
                            int DEG_MAX,
                            const float* grid, const float* origin, const float* lo, const float* hi,
                            const int* rec_idx, const long long* goffs, const long long* gdims, int T, float spacing,
@@ -208,7 +211,7 @@ int adock_mc_batch_noalloc(int AM, int KM,
 int adock_mc_step(int AM, int KM,
                   const float* ref, const int* typ, const unsigned char* tm, const int* ax, const float* axp,
                   const float* axu, const int* pr, const int* na, const int* nt, const int* np, const float* rg,
-                  int P_MAX, const int* gslot, const unsigned short* inc, const int* ist, int DEG_MAX,
+                  int P_MAX, const int* gslot, const unsigned* inc, const int* ist, int DEG_MAX,
                   const float* grid, const float* origin, const float* lo, const float* hi, const int* rec_idx,
                   const long long* goffs, const long long* gdims, int T, float spacing, float pen,
                   const float* prm, const float* rad, const int* hyd, const int* don, const int* acc,
