@@ -110,9 +110,9 @@ template <int KM> using TMask = std::conditional_t<(KM > 32), unsigned long long
 // atom in X4, so an entry costs two address additions (each list padded to 8 entries; start and padded count live in
 // the atom's lane and slot; the row ends with 8 padding entries: the zero slot and the dummy atom).
 // Dense variant (the latency-bound speculative kernel, one atom per lane): FS is replaced by FD [64 DEG_MAX + 1]
-// float4, row k column a ^ k = the gradient of atom a's k-th pair (sign applied by the writer; unused slots stay zero,
-// the last one takes the padding pairs' writes), and inc by pdst [P128] u32 (destination slot of pair q's first atom
-// | second atom << 16). The per-atom sum reads one permuted row per entry (no list indirection, no bank conflicts);
+// float4, row k column a ^ k = atom a's k-th pair: its separation X_a - X_partner (sign applied by the writer) and
+// slot (dE/dr)/r (unused slots stay zero, the last one takes the padding pairs' writes), and the lists by pdst [P128]
+// u32 (destination slot of pair q's first atom | second atom << 16). The per-atom sum reads one permuted row per entry (no list indirection, no bank conflicts);
 // the XOR skew spreads the writes of one atom's pairs over the banks.
 __host__ __device__ inline int pad128(int P_MAX) { return (P_MAX + 127) & ~127; }
 // row length of one ligand's slot lists in the packed arrays (AM atoms, each list padded to a multiple of 8 entries,
@@ -380,7 +380,7 @@ DEVI float pair_math(float4 xi, float4 xj, unsigned pw, const SFConst& sf, float
     const float e = pair_e_nb(rr - xi.w - xj.w, pw_class<APL>(pw), sf, de, hyd, hb);
     const float in = rr < sf.cut ? 1.f : 0.f;
     const float sc = in * de * rinv;
-    s = make_float4(sc * rx, sc * ry, sc * rz, sc);
+    s = make_float4(rx, ry, rz, sc);   // separation X_i - X_j and the slot (dE/dr)/r
     return in * e;
 }
 
@@ -580,7 +580,7 @@ DEVI float eval_eg(const ChainCtx<KM, APL>& c, const SFConst& sf, const WaveLDS<
                 float4 s0;
                 ep_l += pair_math<1>(xi[r], xj[r], pw[r], sf, s0, hyd, hb);
                 w.FD[dd[r] & 0xffffu] = s0;
-                w.FD[dd[r] >> 16] = make_float4(-s0.x, -s0.y, -s0.z, 0.f);
+                w.FD[dd[r] >> 16] = make_float4(-s0.x, -s0.y, -s0.z, s0.w);
             }
         }
     } else {
@@ -607,13 +607,13 @@ DEVI float eval_eg(const ChainCtx<KM, APL>& c, const SFConst& sf, const WaveLDS<
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 const float4 f = w.FD[(k + u) * 64 + (xk ^ u)];
-                pfx[0] += f.x; pfy[0] += f.y; pfz[0] += f.z;
+                pfx[0] = fmaf(f.w, f.x, pfx[0]); pfy[0] = fmaf(f.w, f.y, pfy[0]); pfz[0] = fmaf(f.w, f.z, pfz[0]);
             }
         }
     } else {
         // each entry: the byte offsets of the pair's slot (dE/dr)/r and of the partner atom; lanes past their list
-        // read the row's trailing padding (zero slot, dummy atom). The product is rounded before the sum (as the
-        // dense rows' (dE/dX) slots are), so both layouts give the same bits
+        // read the row's trailing padding (zero slot, dummy atom). The atom adds fma(slot, X_self - X_partner): the
+        // dense rows hold the same separation and slot, so both layouts give the same bits
         const char* fsb = reinterpret_cast<const char*>(w.FS);
         const char* x4b = reinterpret_cast<const char*>(w.X4);
 #pragma unroll
@@ -631,10 +631,9 @@ DEVI float eval_eg(const ChainCtx<KM, APL>& c, const SFConst& sf, const WaveLDS<
                 }
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
-#pragma clang fp contract(off)
-                    pfx[s] += sc[u] * (X0[s] - xp[u].x);
-                    pfy[s] += sc[u] * (X1[s] - xp[u].y);
-                    pfz[s] += sc[u] * (X2[s] - xp[u].z);
+                    pfx[s] = fmaf(sc[u], X0[s] - xp[u].x, pfx[s]);
+                    pfy[s] = fmaf(sc[u], X1[s] - xp[u].y, pfy[s]);
+                    pfz[s] = fmaf(sc[u], X2[s] - xp[u].z, pfz[s]);
                 }
             }
         }
