@@ -741,6 +741,11 @@ DEVI float plain_move(float x, const Ctx& c, int lane, unsigned long long seed, 
     return x;
 }
 
+// BFGS stops once |g| < 1e-5 (the reference's sqrt(g.g) >= 1e-5 test): sqrtf is correctly rounded and monotonic, so
+// the test equals g.g >= the smallest float whose square root rounds to at least 1e-5f, 0x2edbe6fd (~9.9999987e-11;
+// its predecessor's square root rounds to 9.999999e-06), without the square root
+#define GG_STOP __uint_as_float(0x2edbe6fdu)
+
 // A lane-distributed vector (component j in lane j) made readable by every lane: written to this wave's LDS at
 // construction, component j then read as a broadcast (all lanes the same address; the compiler merges neighbouring
 // components into 16-byte reads).
@@ -787,11 +792,12 @@ DEVI float bfgs(const ChainCtx<KM, APL>& c, const SFConst& sf, const WaveLDS<APL
             float red[3] = {ge * ge, y * p, y * y};
             wave_sum_multi<3>(red);
             const float gg = red[0], yp = red[1];
-            if (!(sqrtf(gg) >= 1e-5f)) break;
+            if (!(gg >= GG_STOP)) break;
             if (step == 0) {
                 const float yy = red[2];
                 if (fabsf(yy) > EPSF) {
                     const float dg = alpha * yp / yy;
+                    asm volatile("");   // keeps this once-per-run reset a branch, not selects on every iteration
 #pragma unroll
                     for (int j = 0; j < DV; ++j) h[j] = (j == lane) ? dg : 0.f;
                 }
@@ -910,11 +916,12 @@ DEVI float bfgs_spec(const ChainCtx<KM, 1>& c, const SFConst& sf, const WaveLDS<
         float red[3] = {g * g, y * p, y * y};
         wave_sum_multi<3>(red);
         const float gg = red[0], yp = red[1];
-        if (!(sqrtf(gg) >= 1e-5f)) break;
+        if (!(gg >= GG_STOP)) break;
         if (step == 0) {
             const float yy = red[2];
             if (fabsf(yy) > EPSF) {
                 const float dg = alpha * yp / yy;
+                asm volatile("");   // keeps this once-per-run reset a branch, not selects on every iteration
 #pragma unroll
                 for (int j = 0; j < DV; ++j) h[j] = (j == lane) ? dg : 0.f;
             }
