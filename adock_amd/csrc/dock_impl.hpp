@@ -741,6 +741,15 @@ DEVI float plain_move(float x, const Ctx& c, int lane, unsigned long long seed, 
     return x;
 }
 
+// Register budget of the throughput chain kernels (per-chain, persistent, learned step): at most 168 VGPRs, so three
+// waves share a SIMD. They issue VALU about 87 % of the time at two waves; a third wave covers the LDS and gather
+// latency that idles the rest, worth more than the few spills (mostly chain set-up) the budget costs: the
+// multi-receptor batch runs 520 -> 614 docked ligands/s.
+// Kept to the one-atom-per-lane buckets up to 32 torsions: past them (longer reductions, two atom slots per lane) the
+// budget would spill 0.4-0.9 KB per lane, and those kernels keep the compiler's own allocation.
+template <int KM, int APL> constexpr int three_waves() { return (APL == 1 && KM <= 32) ? 3 : 1; }
+#define ADOCK_WAVES(KM, APL) __attribute__((amdgpu_waves_per_eu(three_waves<KM, APL>(), three_waves<KM, APL>() == 3 ? 3 : 8)))
+
 // BFGS stops once |g| < 1e-5 (the reference's sqrt(g.g) >= 1e-5 test): sqrtf is correctly rounded and monotonic, so
 // the test equals g.g >= the smallest float whose square root rounds to at least 1e-5f, 0x2edbe6fd (~9.9999987e-11;
 // its predecessor's square root rounds to 9.999999e-06), without the square root
@@ -1031,7 +1040,7 @@ DEVI int run_chain(int ch, int lane, const WaveLDS<APL>& w, const LigArgs& L, co
 }
 
 template <int KM, int APL, bool EX>
-__global__ __launch_bounds__(256) void mc_kernel(LigArgs L, GridArgs G, SFArgs S, RecArgs R,
+__global__ __launch_bounds__(256) ADOCK_WAVES(KM, APL) void mc_kernel(LigArgs L, GridArgs G, SFArgs S, RecArgs R,
                                                   const int* __restrict__ chain_lig, int n_chains, McArgs M) {
     extern __shared__ __attribute__((aligned(16))) char lds[];
     const int lane = lane_id();
@@ -1067,7 +1076,7 @@ __global__ __launch_bounds__(256) void mc_spec_kernel(LigArgs L, GridArgs G, SFA
 // counts finished chains and stamps the 100 MHz real-time clock when the last one ends (times[0] = earliest wave
 // start).
 template <int KM, int APL, bool EX>
-__global__ __launch_bounds__(256) void mc_persistent_kernel(LigArgs L, GridArgs G, SFArgs S, RecArgs R,
+__global__ __launch_bounds__(256) ADOCK_WAVES(KM, APL) void mc_persistent_kernel(LigArgs L, GridArgs G, SFArgs S, RecArgs R,
                                                              const int* __restrict__ chain_lig, int n_chains, McArgs M,
                                                              QueueArgs Q) {
     extern __shared__ __attribute__((aligned(16))) char lds[];
@@ -1487,7 +1496,7 @@ DEVI void move_target(float cur, float x, const Ctx& c, int lane, float& t, floa
 // new current pose. init = 1: BFGS of the start pose only (the reference's init_opt). With use_net = 0 and init_opt
 // followed by steps 0..S-1 this reproduces the fused plain kernel bit for bit.
 template <int KM>
-__global__ __launch_bounds__(256) void mc_step_kernel(LigArgs L, GridArgs G, SFArgs S, RecArgs R,
+__global__ __launch_bounds__(256) ADOCK_WAVES(KM, 1) void mc_step_kernel(LigArgs L, GridArgs G, SFArgs S, RecArgs R,
                                                        const int* __restrict__ chain_lig, int n_chains, McArgs M,
                                                        StepArgs P) {
     extern __shared__ __attribute__((aligned(16))) char lds[];
