@@ -688,12 +688,12 @@ DEVI float eval_eg(const ChainCtx<KM, APL>& c, const SFConst& sf, const WaveLDS<
 #pragma unroll
         for (int kb = 0; kb < KM; kb += 8) {
             if (kb < c.nt) {
+                const unsigned t8 = (unsigned)(c.tmask[s] >> kb);
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
                     const float4 cu = w.TU[kb + u], cm = w.TM[kb + u];
-                    const float mk = ((c.tmask[s] >> (kb + u)) & 1u) ? 1.f : 0.f;
-                    const float tq = mk * (cu.x * Tx + cu.y * Ty + cu.z * Tz + cm.x * fx[s] + cm.y * fy[s] +
-                                           cm.z * fz[s]);
+                    const float tq = bit_select(bit_mask(t8, u), cu.x * Tx + cu.y * Ty + cu.z * Tz + cm.x * fx[s] +
+                                                                     cm.y * fy[s] + cm.z * fz[s], 0.f);
                     v[7 + kb + u] = s == 0 ? tq : v[7 + kb + u] + tq;
                 }
             } else if (s == 0) {
@@ -827,11 +827,10 @@ DEVI float bfgs(const ChainCtx<KM, APL>& c, const SFConst& sf, const WaveLDS<APL
                 const float c2 = alpha * alpha * (r * r * yhy + r);
                 const float ar = alpha * r;
                 const BcastVec<DV> pb(w.RD, lane, p), mb(w.RD + 64, lane, mhy);
+                // H += ar (mhy p^T + p mhy^T) + c2 p p^T, as two FMAs per entry: row i gets pj A_i + mj B_i
+                const float A = fmaf(ar, mhy, c2 * p), B = ar * p;
 #pragma unroll
-                for (int j = 0; j < DV; ++j) {
-                    const float pj = pb[j], mj = mb[j];
-                    h[j] += ar * (mhy * pj + mj * p) + c2 * p * pj;
-                }
+                for (int j = 0; j < DV; ++j) h[j] = fmaf(pb[j], A, fmaf(mb[j], B, h[j]));
             }
             if (++step >= iters) break;
         }
@@ -944,11 +943,9 @@ DEVI float bfgs_spec(const ChainCtx<KM, 1>& c, const SFConst& sf, const WaveLDS<
             const float r = 1.f / (alpha * yp);
             const float c2 = alpha * alpha * (r * r * yhy + r);
             const float ar = alpha * r;
+            const float A = fmaf(ar, mhy, c2 * p), B = ar * p;   // as in bfgs: two FMAs per entry
 #pragma unroll
-            for (int j = 0; j < DV; ++j) {
-                const float pj = bcast(p, j), mj = bcast(mhy, j);
-                h[j] += ar * (mhy * pj + mj * p) + c2 * p * pj;
-            }
+            for (int j = 0; j < DV; ++j) h[j] = fmaf(bcast(p, j), A, fmaf(bcast(mhy, j), B, h[j]));
         }
         if (++step >= iters) break;
     }
