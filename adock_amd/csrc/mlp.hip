@@ -1,12 +1,12 @@
 // Learned Monte Carlo proposal network (128 -> 256 -> 256 -> 23, ReLU): inference on MFMA, training kernels on VALU.
 //
-// Inference (mlp_infer_kernel): BF16 weights and activations, FP32 accumulation, v_mfma_f32_16x16x16_bf16. One
-// workgroup (4 wavefronts) owns a 64-row tile of the batch: the features are converted to BF16 into LDS, each wave
-// computes a 16-row strip of every layer (A fragments from LDS, B fragments straight from the L2-resident weights,
-// stored [out, in] so each lane's 4 consecutive K values are one 8-byte load), bias + ReLU is applied on the
-// accumulators and the BF16 activations go back to LDS for the next layer. Fragment layouts of the 16x16x16 BF16 MFMA
-// (CDNA3/CDNA4 ISA): A[i][k] on lane i + 16 (k/4), element k%4; B[k][j] on lane j + 16 (k/4), element k%4;
-// D[i][j] on lane j + 16 (i/4), element i%4.
+// Inference (mlp_infer_kernel): BF16 weights and activations, FP32 accumulation, gfx950's v_mfma_f32_16x16x32_bf16
+// (twice the depth of the CDNA3 16x16x16 form: half the MFMAs per layer). One workgroup (4 wavefronts) owns a 64-row
+// tile of the batch: the features are converted to BF16 into LDS, each wave computes a 16-row strip of every layer (A
+// fragments from LDS, B fragments straight from the L2-resident weights, stored [out, in] so each lane's 8 consecutive
+// K values are one 16-byte load), bias + ReLU is applied on the accumulators and the BF16 activations go back to LDS
+// for the next layer. Fragment layouts of the 16x16x32 BF16 MFMA: A[i][k] on lane i + 16 (k/8), element k%8;
+// B[k][j] on lane j + 16 (k/8), element k%8; D[i][j] on lane j + 16 (i/4), element i%4.
 //
 // Training: FP32 tiled GEMM (64x64 tiles, 16-deep K slices through LDS, 4x4 outputs per thread) with either operand
 // transposed, plus bias / ReLU / loss / Adam elementwise kernels; the host composes forward and backward passes.
@@ -16,7 +16,7 @@
 namespace adock {
 namespace {
 
-typedef short short4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
 constexpr int IN = 128, HID = 256, OUT = 23, OUTP = 32;
@@ -39,16 +39,17 @@ DEVI void layer_strip(const unsigned short* __restrict__ in, unsigned short* out
     float4_t acc[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[t] = float4_t{0.f, 0.f, 0.f, 0.f};
-    const int li = lane & 15, lk = (lane >> 4) * 4;
+    const int li = lane & 15, lk = (lane >> 4) * 8;
+    static_assert(K % 32 == 0, "layer depth must be a multiple of the MFMA's 32");
 #pragma unroll 2
-    for (int k0 = 0; k0 < K; k0 += 16) {
-        const short4_t a = *reinterpret_cast<const short4_t*>(in + (r0 + li) * K + k0 + lk);
+    for (int k0 = 0; k0 < K; k0 += 32) {
+        const bf16x8_t a = *reinterpret_cast<const bf16x8_t*>(in + (r0 + li) * K + k0 + lk);
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
             const int j = t * 16 + li;
-            short4_t bv = short4_t{0, 0, 0, 0};
-            if (j < nvalid) bv = *reinterpret_cast<const short4_t*>(W + (long long)j * K + k0 + lk);
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bv, acc[t], 0, 0, 0);
+            bf16x8_t bv = {};
+            if (j < nvalid) bv = *reinterpret_cast<const bf16x8_t*>(W + (long long)j * K + k0 + lk);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv, acc[t], 0, 0, 0);
         }
     }
 #pragma unroll
