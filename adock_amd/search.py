@@ -321,11 +321,12 @@ def _polish(prob, kp, ch, cfg, seed64, dev, mark):
     return out, P, ch.lig[sel]
 
 
-def dock_problem_kernel(prob, boxes_c, boxes_s, cfg, seed=0, log=None, stats=None, kernel_problem=None):
+def dock_problem_kernel(prob, boxes_c, boxes_s, cfg, seed=0, log=None, stats=None, kernel_problem=None, starts=None):
     """Full search of every complex of `prob` (boxes_c / boxes_s: per-complex box centre and edge lengths), settings
     from a search preset (config.load_config()["search"]["v2_default"] etc.). Returns one dict per complex (coords
     [n_out, A, 3], affinity, inter, intra, total). `stats` (dict) receives the widening factor, chain-steps and phase
-    times."""
+    times. starts = (t [L, M, 3], q [L, M, 4], theta [L, M, K]), M = n_chains: start poses to use instead of Stage A's
+    (e.g. one Stage A run shared by two engines, so that they search from identical inputs)."""
     lb = prob.lb
     dev = lb.ref.device
     if kernel_problem is None:
@@ -343,7 +344,12 @@ def dock_problem_kernel(prob, boxes_c, boxes_s, cfg, seed=0, log=None, stats=Non
 
     mark("t0")
     kp = kernel_problem(prob)
-    t, q, theta, _ = stage_a_kernel(prob, kp, boxes_c, boxes_s, cfg, gen)
+    if starts is None:
+        t, q, theta, _ = stage_a_kernel(prob, kp, boxes_c, boxes_s, cfg, gen)
+    else:
+        t, q, theta = (torch.as_tensor(a, dtype=torch.float32, device=dev) for a in starts)
+        if t.shape[:2] != (lb.L, cfg["n_chains"]) or q.shape[:2] != t.shape[:2] or theta.shape[:2] != t.shape[:2]:
+            raise ValueError("starts must hold n_chains poses per complex")
     mark("stageA")
     ch = _Chains(kp, t, q, theta, lb.L, plan.chains, dev)
     chain_steps = _run_segments(ch, plan, cfg, seed64, dev, log)

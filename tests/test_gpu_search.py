@@ -3,7 +3,9 @@
 Stage A (adock_stage_a) against the FP64 oracle on the same conformers, rotations and lattices; the search driver
 (Stage A -> MC segments -> refine -> exact polish -> finalize) on libadock_hip.so: deterministic, a run split into
 segments equals the unsplit run bit for bit, clusters ascend in energy, and the best score matches the same driver
-run on the oracle twin (independent Stage A draws) within 1.5 kcal/mol."""
+run on the oracle twin (independent Stage A draws) within 1.5 kcal/mol; from identical inputs (one Stage A run's start
+poses handed to both engines, the same counter-RNG seed) the top-1 best scores of 6 complexes agree in distribution
+(median difference <= 0.1 kcal/mol, mean <= 0.25, each within 1.0)."""
 import math
 
 import numpy as np
@@ -13,7 +15,7 @@ import torch
 from adock_amd import synth, workload
 from adock_amd.kernel import KernelProblem
 from adock_amd.pose import internal_coords, random_quat
-from adock_amd.search import dock_problem_kernel
+from adock_amd.search import dock_problem_kernel, stage_a_kernel
 from helpers import CFG, OracleKernel, cpu_search_problem
 
 pytestmark = pytest.mark.gpu
@@ -116,3 +118,37 @@ def test_search_driver_gpu():
     for r, cr in zip(res, cres):
         assert r["total"][0] < 0 and cr["total"][0] < 0
         assert abs(r["total"][0] - cr["total"][0]) <= 1.5, (r["total"][0], cr["total"][0])
+
+
+def test_search_driver_matches_oracle_twin_from_identical_starts():
+    """Six complexes (the small receptor, ligands of 12-30 heavy atoms) searched by the GPU driver and by the same
+    driver on the oracle twin from identical start poses (one GPU Stage A run) and the same seed: the MC moves come
+    from the counter RNG, so the engines differ only by float rounding (FP32 kernels vs the FP64 oracle). Rounding
+    parts the chains' paths after a few BFGS line searches (chaotic, as between any two float builds), so the bar is
+    on the top-1 distribution at a fixed seed: median |difference| <= 0.1 kcal/mol, mean <= 0.25, every complex
+    within 1.0 (measured: 0.001-0.064 for five complexes, 0.63 for the 30-atom ligand whose shallow landscape sends
+    the two engines' best chains to different minima)."""
+    print("[gpu] search driver vs oracle twin from identical Stage A starts", flush=True)
+    w1 = synth.make_single("small", 0)
+    rec = workload.receptor_dict(w1["receptor"], w1["center"], w1["size"])
+    shapes = [(20, 4), (16, 3), (12, 2), (24, 5), (30, 6), (18, 0)]
+    preps = [synth.make_ligand(900 + i, a, t) for i, (a, t) in enumerate(shapes)]
+    L = len(preps)
+    c, s = np.asarray(w1["center"], float), float(np.max(w1["size"]))
+    bc, bs = [c] * L, [np.full(3, s)] * L
+    cfg = _cfg()
+    prob = workload.gpu_problem([rec], preps, [0] * L)
+    gen = torch.Generator(device="cuda").manual_seed(11)
+    t, q, th, _ = stage_a_kernel(prob, KernelProblem(prob), bc, bs, cfg, gen)
+    starts = (t.cpu(), q.cpu(), th.cpu())
+    res = dock_problem_kernel(prob, bc, bs, cfg, seed=4, starts=starts)
+    cprob = cpu_search_problem(rec, preps, c, s)
+    cres = dock_problem_kernel(cprob, bc, bs, cfg, seed=4, kernel_problem=OracleKernel, starts=starts)
+    g = np.array([r["total"][0] for r in res])
+    o = np.array([r["total"][0] for r in cres])
+    print(f"  top-1 GPU {np.round(g, 3)}", flush=True)
+    print(f"  top-1 oracle twin {np.round(o, 3)}", flush=True)
+    d = np.abs(g - o)
+    print(f"  |difference| {np.round(d, 3)}: median {np.median(d):.3f}, mean {d.mean():.3f}", flush=True)
+    assert np.all(g < 0) and np.all(o < 0)
+    assert np.median(d) <= 0.1 and d.mean() <= 0.25 and d.max() <= 1.0, d
