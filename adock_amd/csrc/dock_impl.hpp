@@ -107,8 +107,10 @@ template <int KM> using TMask = std::conditional_t<(KM > 32), unsigned long long
 //                           padding pairs (and entry P128) are (dummy, 0)
 // The per-atom pair-slot lists stay in memory (the host-built packed ligand, read through the caches; a complex's
 // chains share them): atom a's entries are fs | px << 16, the byte offsets of the pair's slot in FS and of its partner
-// atom in X4, so an entry costs two address additions (each list padded to 8 entries; start and padded count live in
-// the atom's lane and slot; the row ends with 8 padding entries: the zero slot and the dummy atom).
+// atom in X4, so an entry costs two address additions (each list padded to 8 entries; the row ends with 8 padding
+// entries: the zero slot and the dummy atom). Each lane walks one segment (start and count in its lane and slot): its
+// atom's list, or with one atom per lane, for the longest lists, its head, while a lane past the atoms (a helper) walks
+// the tail and hands its partial sum to the atom's lane (ds_bpermute). The longest walk of the wave sets the trip count.
 // Dense variant (the latency-bound speculative kernel, one atom per lane): FS is replaced by FD [64 DEG_MAX + 1]
 // float4, row k column a ^ k = atom a's k-th pair: its separation X_a - X_partner (sign applied by the writer) and
 // slot (dE/dr)/r (unused slots stay zero, the last one takes the padding pairs' writes), and the lists by pdst [P128]
@@ -155,7 +157,13 @@ struct ChainCtx {
     TMask<KM> tmask[APL];
     bool atom[APL];
     const float* gp[APL];
-    int is[APL], ic8[APL];   // this atom's slot list: incg[is .. is + ic8), padded to a multiple of 8
+    int is[APL], ic8[APL];   // this lane's segment of slot lists: incg[is .. is + ic8), a multiple of 8 entries
+    // one atom per lane: the lane whose partial sum this lane adds (split atom: its helper) or serves (helper lane:
+    // the atom), as a byte offset for ds_bpermute; seg: the atom whose list the segment belongs to; k0: the segment's
+    // first entry within that atom's list; zc: the atom whose dense row is all zeros (the padding read)
+    int pb, seg, k0, zc;
+    bool helper;             // a lane past the atoms that walks the second half of a split atom's list
+    bool split;              // uniform: the ligand has split lists
     const unsigned* incg;    // the ligand's row of slot lists (memory); incg[incpad ..] are 8 padding entries
     int incpad;
     // lane = torsion k: its axis atoms; lane 7 + k: torsion k's reference axis (point, unit vector)
@@ -331,13 +339,24 @@ DEVI ChainCtx<KM, APL> setup_chain(int l, int lane, const LigArgs& L, const Grid
     // destination slots of its pairs (the atom is the pair's second one when its partner is the pair's first)
     c.incg = L.inc + (long long)l * inc_row(L.P_MAX, AM);
     c.incpad = inc_row(L.P_MAX, AM) - 8;
+    // ist: start | count << 16 | partner << 24 per lane (atom slot); a helper lane holds no atom and a non-empty
+    // segment, which continues its partner atom's list
     int d8 = 0;
+    c.pb = 4 * lane; c.seg = lane; c.k0 = 0; c.zc = lane; c.helper = false;
 #pragma unroll
     for (int s = 0; s < APL; ++s) {
         const int a = lane + 64 * s;
-        const int e = c.atom[s] ? L.ist[(long long)l * AM + a] : 0;
+        const int e = L.ist[(long long)l * AM + a];
         c.is[s] = e & 0xffff;
-        c.ic8[s] = min((e >> 16) & 0xffff, DENSE ? L.DEG_MAX : 0xffff);
+        c.ic8[s] = (e >> 16) & 0xff;
+        if constexpr (APL == 1) {
+            const int pl = (e >> 24) & 63;
+            c.helper = !c.atom[0] && c.ic8[0] > 0;
+            c.pb = 4 * pl;
+            c.seg = c.helper ? pl : lane;
+            if (c.helper) c.k0 = max(0, c.is[0] - (L.ist[(long long)l * AM + pl] & 0xffff));
+        }
+        if constexpr (DENSE) c.ic8[s] = max(0, min(c.ic8[s], L.DEG_MAX - c.k0));
         d8 = max(d8, c.ic8[s]);
         if constexpr (DENSE) {
             const unsigned* src = c.incg + c.is[s];
@@ -347,11 +366,15 @@ DEVI ChainCtx<KM, APL> setup_chain(int l, int lane, const LigArgs& L, const Grid
                 const unsigned q = (v & 0xffffu) >> 2, p = v >> 20;
                 if (q < (unsigned)c.np) {
                     const unsigned side = pw_first<APL>(w.pl[q]) == p ? 1u : 0u;
-                    p16[2 * q + side] = (unsigned short)(k * 64 + (lane ^ k));
+                    const int kk = c.k0 + k;
+                    p16[2 * q + side] = (unsigned short)(kk * 64 + (c.seg ^ kk));
                 }
             }
         }
     }
+    c.split = APL == 1 && __ballot(c.helper) != 0ull;
+    // split lists exist only when lanes past the atoms do, so atom 63 is absent and its dense row reads zero
+    if (c.split) c.zc = 63;
     c.D8 = (int)wave_max_f((float)d8);
     c.exact = EX ? 1 : 0;
     c.nb = nullptr; c.nbn = 0; c.staged = false; c.masked = false;
@@ -603,10 +626,13 @@ DEVI float eval_eg(const ChainCtx<KM, APL>& c, const SFConst& sf, const WaveLDS<
     for (int s = 0; s < APL; ++s) { pfx[s] = pfy[s] = pfz[s] = 0.f; }
     if constexpr (DENSE) {
         for (int k = 0; k < c.D8; k += 8) {
-            const int xk = lane ^ k;
+            // the lane's segment: entries k0 + k.. of atom seg's row; past it, the zero row of atom zc
+            const bool in = k < c.ic8[0];
+            const int kk = in ? c.k0 + k : k;
+            const int xk = (in ? c.seg : c.zc) ^ kk;
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
-                const float4 f = w.FD[(k + u) * 64 + (xk ^ u)];
+                const float4 f = w.FD[(kk + u) * 64 + (xk ^ u)];
                 pfx[0] = fmaf(f.w, f.x, pfx[0]); pfy[0] = fmaf(f.w, f.y, pfy[0]); pfz[0] = fmaf(f.w, f.z, pfz[0]);
             }
         }
@@ -618,6 +644,12 @@ DEVI float eval_eg(const ChainCtx<KM, APL>& c, const SFConst& sf, const WaveLDS<
         const char* x4b = reinterpret_cast<const char*>(w.X4);
 #pragma unroll
         for (int s = 0; s < APL; ++s) {
+            // the segment's atom position (a helper lane walks another atom's list)
+            float sx = X0[s], sy = X1[s], sz = X2[s];
+            if constexpr (APL == 1) {
+                const float4 t = w.X4[c.seg];
+                sx = t.x; sy = t.y; sz = t.z;
+            }
             for (int k = 0; k < c.D8; k += 8) {
                 const uint4* r4 = reinterpret_cast<const uint4*>(c.incg + (k < c.ic8[s] ? c.is[s] + k : c.incpad));
                 const uint4 va = r4[0], vb = r4[1];
@@ -631,11 +663,24 @@ DEVI float eval_eg(const ChainCtx<KM, APL>& c, const SFConst& sf, const WaveLDS<
                 }
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
-                    pfx[s] = fmaf(sc[u], X0[s] - xp[u].x, pfx[s]);
-                    pfy[s] = fmaf(sc[u], X1[s] - xp[u].y, pfy[s]);
-                    pfz[s] = fmaf(sc[u], X2[s] - xp[u].z, pfz[s]);
+                    pfx[s] = fmaf(sc[u], sx - xp[u].x, pfx[s]);
+                    pfy[s] = fmaf(sc[u], sy - xp[u].y, pfy[s]);
+                    pfz[s] = fmaf(sc[u], sz - xp[u].z, pfz[s]);
                 }
             }
+        }
+    }
+    if constexpr (APL == 1) {
+        // split lists: each atom adds its helper's partial sum (lanes without one add -0, which changes no value);
+        // helper lanes end with zero
+        if (c.split) {
+            const float hx = c.helper ? pfx[0] : -0.f, hy = c.helper ? pfy[0] : -0.f, hz = c.helper ? pfz[0] : -0.f;
+            const float ox = __int_as_float(__builtin_amdgcn_ds_bpermute(c.pb, __float_as_int(hx)));
+            const float oy = __int_as_float(__builtin_amdgcn_ds_bpermute(c.pb, __float_as_int(hy)));
+            const float oz = __int_as_float(__builtin_amdgcn_ds_bpermute(c.pb, __float_as_int(hz)));
+            pfx[0] = (c.helper ? 0.f : pfx[0]) + ox;
+            pfy[0] = (c.helper ? 0.f : pfy[0]) + oy;
+            pfz[0] = (c.helper ? 0.f : pfz[0]) + oz;
         }
     }
     if constexpr (EX) {

@@ -158,18 +158,25 @@ def pad_entry(P_MAX, AM=64):
     return (4 * pad128(P_MAX)) | ((16 * (64 if AM <= 64 else 128)) << 16)
 
 
-def slot_lists(pairs, P_MAX, AM=64):
+def slot_lists(pairs, P_MAX, AM=64, na=None):
     """Per-atom pair-slot lists of one ligand for the kernels' atomic-free pair gradient. Atom a's list holds, for
     every intra pair q of a (ascending in q), the word fs | px << 16: fs = 4 q, the byte offset of the pair's slot
     (dE/dr)/r in the kernels' slot array, and px = 16 p, the byte offset of the pair's other atom p in the atom array,
     so that the atom adds slot x (X_a - X_p) with two address additions. Lists are padded to a multiple of 8 entries
-    with pad_entry (the zero slot and the dummy atom); 8 more padding entries end the row. Returns
-    (inc [inc_stride] uint32, ist [AM] int32 = start | padded_count << 16)."""
+    with pad_entry (the zero slot and the dummy atom); 8 more padding entries end the row.
+
+    Helper lanes (one atom per lane, AM = 64, na atoms given): the lanes na..63 hold no atom, so each takes the second
+    half of one of the longest lists (longest first, ties by atom index): atom a keeps the first ceil(g / 2) of its g
+    groups of 8 entries, its helper the other floor(g / 2), and the kernels add the helper's partial sum to the atom's.
+    A lane walks only its own segment, so the longest walk of the wave shrinks.
+
+    Returns (inc [inc_stride] uint32, ist [AM] int32 = start | count << 16 | partner << 24): start and count
+    (multiples of 8) of the lane's segment of the row; partner = the helper lane of a split atom, the atom a helper
+    lane works for, and the lane itself otherwise."""
     z = pad128(P_MAX)
     if 4 * z > 0xffff:
         raise ValueError(f"P_MAX = {P_MAX}: slot byte offsets exceed 16 bits (at most 16,256 pairs per ligand)")
     inc = np.full(inc_stride(P_MAX, AM), pad_entry(P_MAX, AM), np.uint32)
-    ist = np.zeros(AM, np.int32)
     pairs = np.asarray(pairs, np.int64).reshape(-1, 2)
     q = np.arange(len(pairs))
     real = pairs[:, 0] != pairs[:, 1]
@@ -180,14 +187,37 @@ def slot_lists(pairs, P_MAX, AM=64):
     atom, val = atom[order], ((4 * qq) | ((16 * part) << 16))[order].astype(np.uint32)
     cnt = np.bincount(atom, minlength=AM)[:AM]
     pc = (cnt + 7) // 8 * 8
+    if pc.max(initial=0) > 0xff:
+        raise ValueError("an atom's slot list exceeds 255 entries")
     start = np.concatenate([[0], np.cumsum(pc)[:-1]])
     if start[-1] + pc[-1] > len(inc) - 8:
         raise ValueError("slot lists exceed their row")
     first = np.concatenate([[0], np.cumsum(cnt)[:-1]])
     for a in np.nonzero(cnt)[0]:
         inc[start[a]:start[a] + cnt[a]] = val[first[a]:first[a] + cnt[a]]
-    ist[:] = start | (pc << 16)
+    seg_s, seg_c, partner = start.copy(), pc.copy(), np.arange(AM)
+    if na is not None and AM == 64 and int(na) < 64:
+        na = int(na)
+        g = pc[:na] // 8
+        for h, a in zip(range(na, 64), np.argsort(-g, kind="stable")):
+            if g[a] < 2:
+                break
+            keep = 8 * ((g[a] + 1) // 2)
+            seg_c[a] = keep
+            seg_s[h], seg_c[h] = start[a] + keep, pc[a] - keep
+            partner[a], partner[h] = h, a
+    ist = (seg_s | (seg_c << 16) | (partner << 24)).astype(np.int32)
     return inc, ist
+
+
+def list_lengths(ist):
+    """Full (padded) slot-list length of each lane's atom from an ist row: the lane's segment plus its helper's."""
+    ist = np.asarray(ist, np.int64)
+    s, c, p = ist & 0xffff, (ist >> 16) & 0xff, (ist >> 24) & 0x7f
+    lane = np.arange(len(ist))
+    # a split atom's helper segment starts right after the atom's own
+    helped = (p != lane) & (s[p] == s + c) & (c > 0)
+    return np.where(helped, c + c[p], np.where((p != lane) & (s == s[p] + c[p]), 0, c))
 
 
 def bucket(x, step):
@@ -257,12 +287,12 @@ def pack_ligands(preps, slot, rec_idx, floor=(0, 0)):
             out["axu"][l, :p.n_tors] = u / np.maximum(np.linalg.norm(u, axis=1, keepdims=True), 1e-6)
         if len(pair_lists[l]):
             out["pr"][l, :len(pair_lists[l])] = pair_lists[l]
-        out["inc"][l], out["ist"][l] = slot_lists(pair_lists[l], P, AM)
+        out["inc"][l], out["ist"][l] = slot_lists(pair_lists[l], P, AM, na=len(hv))
         out["gslot"][l, :len(hv)] = slot[rec_idx[l], out["typ"][l, :len(hv)]]
         if (out["gslot"][l, :len(hv)] < 0).any():
             raise ValueError("a ligand atom type has no grid")
     out["P"] = np.int32(P)
-    out["DEG"] = np.int32(int((out["ist"] >> 16).max()) if L else 0)
+    out["DEG"] = np.int32(max(int(list_lengths(r).max()) for r in out["ist"]) if L else 0)
     return AM, KM, out
 
 

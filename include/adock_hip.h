@@ -16,13 +16,16 @@
  * The ligand group is the reference's packed layout (src/adock/mckernel.py:547-592: ref, typ, tm, ax, axp, axu, pr,
  * na, nt, np, rg, P_MAX, gslot) plus two arrays of this library's own, built once per ligand set on the host
  * (adock_amd.ligand.slot_lists): inc [L, 2 * P128 + 8 * AM + 8] uint32 and ist [L, AM] int32, P128 = P_MAX rounded up
- * to a multiple of 128. Atom a's list inc[l, s .. s + c) (ist = s | c << 16, s and c multiples of 8) holds, for every
- * intra pair q of atom a, ascending in q, the word 4 q | (16 p) << 16 (p: the pair's other atom): the byte offsets of
- * the pair's slot and of its partner atom in the kernels' per-wave arrays; lists are padded, and the row ends with 8
- * entries, with 4 P128 | (16 D) << 16, D the dummy atom (64 for AM 64, 128 above). DEG_MAX is the largest c over all
- * atoms and ligands (a multiple of 8, at most AM). The kernels use the lists to sum the pair gradient of every atom
- * without atomics; DEG_MAX sizes the per-atom gradient rows of the single-complex kernel. P_MAX is at most 16,256
- * (16-bit slot offsets; a 128-atom ligand has at most 8,128 pairs).
+ * to a multiple of 128. Atom a's list, padded to a multiple of 8 entries, holds for every intra pair q of atom a,
+ * ascending in q, the word 4 q | (16 p) << 16 (p: the pair's other atom): the byte offsets of the pair's slot and of
+ * its partner atom in the kernels' per-wave arrays; the row ends with 8 padding entries, 4 P128 | (16 D) << 16, D the
+ * dummy atom (64 for AM 64, 128 above). ist[l, lane] = s | c << 16 | h << 24 names the lane's segment inc[l, s .. s + c)
+ * (s and c multiples of 8, c < 256): normally its atom's whole list, with h the lane itself. For AM 64 the lanes past
+ * the ligand's atoms may each take the tail of one list: atom a keeps the head (h = the helper lane) and helper lane
+ * b >= na walks the tail that starts where a's segment ends (h = a); the kernels add the helper's partial sum to the
+ * atom's. DEG_MAX is the largest whole (padded) list over all atoms and ligands (at most AM). The kernels use the lists
+ * to sum the pair gradient of every atom without atomics; DEG_MAX sizes the per-atom gradient rows of the
+ * single-complex kernel. P_MAX is at most 16,256 (16-bit slot offsets; a 128-atom ligand has at most 8,128 pairs).
  */
 #ifndef ADOCK_HIP_H
 #define ADOCK_HIP_H
