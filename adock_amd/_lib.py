@@ -50,6 +50,9 @@ def lib():
         LL = ctypes.c_longlong
         L.adock_mc_step.argtypes = (LIG + GRID + SF + REC +
                                     [P, P, I, P, P, P, P, I, I, I, F, F, F, U64, P, P, P, P, I, I, F, F, F, F, P, P])
+        L.adock_mc_learned.argtypes = (LIG + GRID + SF + REC +
+                                       [P, P, I, P, P, P, P, I, I, I, I, F, F, F, U64] + [P] * 6 +
+                                       [P, P, P, I, I, F, F, F, F, P, P])
         L.adock_mlp_infer.argtypes = [I, P, P, P, P, P, P, P, P, P]
         L.adock_gemm_f32.argtypes = [I, I, I, I, I, F, P, I, P, I, F, P, I, P]
         L.adock_gemm_f32_splitk.argtypes = [I, I, I, I, I, F, P, I, P, I, F, P, I, I, P, P]
@@ -70,6 +73,7 @@ def lib():
         L.adock_mc_resident_waves.restype = I
         for name in ("adock_build_grid", "adock_build_grid_fit", "adock_build_grid_fit_sets", "adock_inter_exact_fit", "adock_eg", "adock_mc", "adock_mc_counted", "adock_mc_batch",
                      "adock_mc_batch_noalloc", "adock_batch_reset", "adock_set_chains_per_block", "adock_set_line_search_width", "adock_set_exact_staging", "adock_mc_step",
+                     "adock_mc_learned",
                      "adock_mlp_infer", "adock_gemm_f32", "adock_gemm_f32_splitk", "adock_stage_a",
                      "adock_bias_act", "adock_relu_bwd", "adock_colsum", "adock_mse", "adock_adam", "adock_to_bf16"):
             getattr(L, name).restype = I

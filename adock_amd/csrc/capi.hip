@@ -312,6 +312,34 @@ int adock_mc_step(int AM, int KM, LIG_DECL, GRID_DECL, SF_DECL, REC_DECL, const 
     return hip_status(launch_mc_step(KM, L, G, S, R, chain_lig, n_chains, M, P, (hipStream_t)stream));
 }
 
+int adock_mc_learned(int AM, int KM, LIG_DECL, GRID_DECL, SF_DECL, REC_DECL, const int* chain_lig, const int* chain_id,
+                     int n_chains, float* cur, float* cur_e, float* best, float* best_e, int init_opt, int mc_steps,
+                     int step_offset, int bfgs_iters, float temp, float amp_pos, float amp_rot,
+                     unsigned long long seed, const unsigned short* W1, const float* b1, const unsigned short* W2,
+                     const float* b2, const unsigned short* W3, const float* b3, float* feat, float* log_buf,
+                     int* log_count, int log_cap, int use_net, float p_plain, float sig_t, float sig_r, float sig_th,
+                     unsigned long long* eval_count, adock_stream_t stream) {
+    if (int st = common_checks(AM, KM, P_MAX, ref, inc, ist, DEG_MAX, grid, prm, 0, rxyz, rn, RN_MAX, chain_lig,
+                               n_chains))
+        return st;
+    if (AM != 64) return fail(ADOCK_E_BUCKET, "the learned search kernel takes ligands of at most 64 heavy atoms");
+    if (mc_steps < 0) return fail(ADOCK_E_ARG, "negative mc_steps");
+    if (n_chains == 0) return 0;
+    if (!chain_id || !cur || !cur_e || !best || !best_e) return fail(ADOCK_E_ARG, "null pointer");
+    if (log_buf && (!log_count || log_cap < 0)) return fail(ADOCK_E_ARG, "logging needs a counter");
+    if (use_net && (!W1 || !b1 || !W2 || !b2 || !W3 || !b3)) return fail(ADOCK_E_ARG, "network moves need the weights");
+    if (!(p_plain >= 0.f && p_plain <= 1.f)) return fail(ADOCK_E_ARG, "p_plain outside [0, 1]");
+    LigArgs L{ref, typ, tm, ax, axp, axu, pr, na, nt, np, rg, P_MAX, gslot, inc, ist, DEG_MAX, AM};
+    GridArgs G{grid, origin, lo, hi, rec_idx, goffs, gdims, T, spacing, pen};
+    SFArgs S{prm, rad, hyd, don, acc};
+    RecArgs R{rxyz, rtyp, rn, RN_MAX};
+    McArgs M{chain_id, nullptr, nullptr, 0, cur, cur_e, best, best_e, init_opt, mc_steps, bfgs_iters, temp, amp_pos,
+             amp_rot, seed, step_offset, eval_count};
+    StepArgs P{nullptr, feat, log_buf, log_count, log_cap, use_net, p_plain, sig_t, sig_r, sig_th, 0};
+    NetArgs N{W1, b1, W2, b2, W3, b3};
+    return hip_status(launch_mc_learned(KM, L, G, S, R, chain_lig, n_chains, M, P, N, (hipStream_t)stream));
+}
+
 // ---- learned-proposal network ----
 int adock_mlp_infer(int B, const float* X, const unsigned short* W1, const float* b1, const unsigned short* W2,
                     const float* b2, const unsigned short* W3, const float* b3, float* Y, adock_stream_t stream) {

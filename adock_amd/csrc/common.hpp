@@ -122,6 +122,20 @@ DEVI void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Proposal network shape (features -> hidden -> hidden -> move components; the last layer's 23 outputs padded to two
+// 16-wide MFMA tiles) and the BF16 operand types of gfx950's v_mfma_f32_16x16x32_bf16.
+constexpr int NET_IN = 128, NET_HID = 256, NET_OUT = 23, NET_OUTP = 32;
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+// float32 -> bfloat16, round to nearest even (NaN kept quiet)
+DEVI unsigned short f2bf(float f) {
+    unsigned u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x7fffffu)) return (unsigned short)((u >> 16) | 0x40);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+
 DEVI float fast_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
 DEVI float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 DEVI float fast_rsq(float x) { return __builtin_amdgcn_rsqf(x); }

@@ -75,6 +75,17 @@ int launch_mc_step(int KM, const LigArgs& L, const GridArgs& G, const SFArgs& S,
     }
 }
 
+#define ADOCK_CASE_LEARNED(KM) \
+    case KM: return launch_mc_learned_km##KM(L, G, S, R, chain_lig, n_chains, M, P, N, stream);
+int launch_mc_learned(int KM, const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecArgs& R,
+                      const int* chain_lig, int n_chains, const McArgs& M, const StepArgs& P, const NetArgs& N,
+                      hipStream_t stream) {
+    switch (KM) {
+        ADOCK_KM_LIST(ADOCK_CASE_LEARNED)
+        default: return -1;
+    }
+}
+
 #define ADOCK_CASE_EG(KM) \
     case KM: return launch_eg_km##KM(L, G, S, R, exact, chain_lig, n_chains, state, e_out, g_out, stream);
 int launch_eg(int KM, const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecArgs& R, int exact,

@@ -816,23 +816,30 @@ struct BcastVec {
     }
 };
 
+// The evaluation at the pose bfgs returns (KEEP): its lane-distributed gradient and this lane's atom energy. Every pass
+// of the loop ends on an evaluation of the pose it keeps, so the last evaluation is that of the returned pose, except
+// when the start pose is restored after a worse end (valid = false: its evaluation, the first, is not kept).
+struct LastEval { float g = 0.f, ea = 0.f; bool valid = false; };
+
 // Vina-style BFGS on the lane-distributed state x (in/out); returns the final energy. One evaluation site: the first
 // pass evaluates x itself, every later pass evaluates one line-search trial.
-template <int KM, int APL, bool DENSE = false, bool EX = false>
+template <int KM, int APL, bool DENSE = false, bool EX = false, bool KEEP = false>
 DEVI float bfgs(const ChainCtx<KM, APL>& c, const SFConst& sf, const WaveLDS<APL>& w, int lane, float& x, int iters,
-                int& nev) {
+                int& nev, LastEval* last = nullptr) {
     constexpr int DV = 7 + KM;
     float h[DV];
 #pragma unroll
     for (int j = 0; j < DV; ++j) h[j] = (j == lane) ? 1.f : 0.f;
     const float xo = x;
     float g = 0.f, p = 0.f, f0 = 0.f, f_orig = 0.f, pg = 0.f, alpha = 1.f;
+    float kg = 0.f, kea = 0.f;
     int step = 0, trial = -1;
     while (true) {
         const float xe = trial < 0 ? x : conf_inc(x, p, alpha, lane);
         float ge, ea_, x0_, x1_, x2_;
         const float fe = eval_eg<KM, APL, DENSE, EX>(c, sf, w, lane, xe, ge, ea_, x0_, x1_, x2_);
         ++nev;
+        if constexpr (KEEP) { kg = ge; kea = ea_; }
         if (trial < 0) {
             f0 = fe; f_orig = fe; g = ge;
             if (iters <= 0) break;
@@ -890,7 +897,9 @@ DEVI float bfgs(const ChainCtx<KM, APL>& c, const SFConst& sf, const WaveLDS<APL
         alpha = 1.f;
         trial = 0;
     }
-    if (!(f0 <= f_orig)) { f0 = f_orig; x = xo; }
+    const bool restored = !(f0 <= f_orig);
+    if (restored) { f0 = f_orig; x = xo; }
+    if constexpr (KEEP) *last = LastEval{kg, kea, !restored};
     return f0;
 }
 
@@ -1402,7 +1411,7 @@ int mc_resident_waves_t(int P_MAX, int persistent) {
     return per_cu * W;
 }
 
-constexpr int NFEAT = 128, NOUT = 23, LOG_W = NFEAT + 2 * NOUT + 1;
+constexpr int NFEAT = NET_IN, NOUT = NET_OUT, LOG_W = NFEAT + 2 * NOUT + 1;
 // Counter-RNG slots of the learned step. The plain move uses slots 0 (entity), 1 (torsion angle), 2 (acceptance) and
 // 10 .. 105 (unit-ball tries 10 + 3k, k < 32); the step counter sits 8 bits up, so every slot must stay below 256 or
 // it aliases a slot of another step. The network move draws its coin at SLOT_NET_PLAIN and one normal per lane from
@@ -1624,6 +1633,191 @@ int launch_mc_step_t(const LigArgs& L, const GridArgs& G, const SFArgs& S, const
 }
 
 
+// ---- the proposal network inside the chain loop ----
+// One chain's forward pass on its wave's own MFMAs, the operation of mlp_infer_kernel for one row: every row of the
+// A operand is this chain's activation vector (BF16, the wave's LDS), B a 16-output tile of W^T read from the
+// L2-resident weights ([out, in]: each lane's 8 consecutive K values are one 16-byte load). Each output is the same
+// K-ordered chain of v_mfma_f32_16x16x32_bf16, the same bias add, ReLU and BF16 rounding as in the batched kernel, so
+// the move equals the batched inference's bit for bit. D[i][j] sits on lane j + 16 (i / 4), element i % 4; with all
+// rows equal, lane j of every 16-lane quarter holds output j of the tile. 208 MFMAs per forward pass.
+struct NetLDS { float* feat; unsigned short* a0; unsigned short* a1; };   // feat [128] FP32, a0 / a1 [256] BF16
+constexpr int NET_LDS_BYTES = 4 * NET_IN + 2 * 2 * NET_HID;
+
+DEVI NetLDS carve_net(char* base) {
+    float* f = reinterpret_cast<float*>(base);
+    unsigned short* a = reinterpret_cast<unsigned short*>(f + NET_IN);
+    return NetLDS{f, a, a + NET_HID};
+}
+
+// out[0 .. N) = act(W in + b) for the wave's vector in[0 .. K) (LDS, BF16), out written as BF16 by lanes 0-15
+template <int K, int N>
+DEVI void net_hidden(const unsigned short* in, unsigned short* out, const unsigned short* __restrict__ W,
+                     const float* __restrict__ b, int lane) {
+    const int li = lane & 15, lk = (lane >> 4) * 8;
+#pragma unroll 2
+    for (int t = 0; t < N / 16; ++t) {
+        const int j = t * 16 + li;
+        const unsigned short* wr = W + (long long)j * K + lk;
+        float4_t acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int k0 = 0; k0 < K; k0 += 32) {
+            const bf16x8_t a = *reinterpret_cast<const bf16x8_t*>(in + k0 + lk);
+            const bf16x8_t bv = *reinterpret_cast<const bf16x8_t*>(wr + k0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv, acc, 0, 0, 0);
+        }
+        const float v = fmaxf(acc[0] + b[j], 0.f);
+        if (lane < 16) out[j] = f2bf(v);
+    }
+    wave_lds_sync();
+}
+
+// the network's move for the features in n.feat: component j on lane j (j < 23), 0 on the other lanes
+DEVI float net_forward(const NetLDS& n, const NetArgs& N, int lane) {
+    n.a0[lane] = f2bf(n.feat[lane]);
+    n.a0[64 + lane] = f2bf(n.feat[64 + lane]);
+    wave_lds_sync();
+    net_hidden<NET_IN, NET_HID>(n.a0, n.a1, N.W1, N.b1, lane);
+    net_hidden<NET_HID, NET_HID>(n.a1, n.a0, N.W2, N.b2, lane);
+    // last layer: outputs 0-15 and 16-31 (23 valid; rows past 22 of W3 do not exist and enter as zeros)
+    const int li = lane & 15, lk = (lane >> 4) * 8;
+    const bool hi_ok = 16 + li < NET_OUT;
+    const unsigned short* w0 = N.W3 + (long long)li * NET_HID + lk;
+    const unsigned short* w1 = N.W3 + (long long)(hi_ok ? 16 + li : 0) * NET_HID + lk;
+    float4_t acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k0 = 0; k0 < NET_HID; k0 += 32) {
+        const bf16x8_t a = *reinterpret_cast<const bf16x8_t*>(n.a0 + k0 + lk);
+        const bf16x8_t b0 = *reinterpret_cast<const bf16x8_t*>(w0 + k0);
+        bf16x8_t b1 = {};
+        if (hi_ok) b1 = *reinterpret_cast<const bf16x8_t*>(w1 + k0);
+        acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b0, acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b1, acc1, 0, 0, 0);
+    }
+    // lane j < 16: output j (first tile); lane 16 + i: output 16 + i (second tile, quarter 1)
+    float y = 0.f;
+    if (lane < 16) y = acc0[0] + N.b3[lane];
+    else if (lane < NET_OUT) y = acc1[0] + N.b3[lane];
+    return y;
+}
+
+// The learned search in one launch: each wave runs its chain's init BFGS and mc_steps MC steps with the chain set up
+// once; before a step whose coin (slot SLOT_NET_PLAIN) calls for the network, the wave runs the network on the
+// features of its current pose (net_forward), and after an accepted move it takes the new pose's features from the
+// evaluation BFGS ended on (no extra evaluation; a fresh one only when BFGS restored its start). Moves, BFGS,
+// Metropolis test and move log are those of mc_step_kernel, so a launch equals the step-by-step search (init step,
+// then adock_mlp_infer + one step launch per MC step) bit for bit in poses, energies and logged records (in another
+// order); it skips the step kernel's extra evaluation per step (eval_count differs). With use_net = 0 and no log it
+// is mc_kernel's search. P.feat (may be NULL) receives the features of every chain's final current pose.
+template <int KM>
+__global__ __launch_bounds__(256) ADOCK_WAVES(KM, 1) void mc_learned_kernel(LigArgs L, GridArgs G, SFArgs S, RecArgs R,
+                                                          const int* __restrict__ chain_lig, int n_chains, McArgs M,
+                                                          StepArgs P, NetArgs N) {
+    extern __shared__ __attribute__((aligned(16))) char lds[];
+    constexpr int SD = 7 + KM;
+    const int lane = lane_id();
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+    const int ch = blockIdx.x * (blockDim.x / WAVE) + wid;
+    if (ch >= n_chains) return;
+    const int wl = wave_lds_bytes(KM, L.P_MAX);
+    const WaveLDS<1> w = carve_lds<KM, 1>(lds + wid * (wl + NET_LDS_BYTES), L.P_MAX);
+    const NetLDS nl = carve_net(lds + wid * (wl + NET_LDS_BYTES) + wl);
+    const SFConst sf = make_sf(S.prm);
+    const int l = __builtin_amdgcn_readfirstlane(chain_lig[ch]);
+    const ChainCtx<KM, 1> c = setup_chain<KM, 1>(l, lane, L, G, S, R, w);
+    const int cid = __builtin_amdgcn_readfirstlane(M.chain_id[ch]);
+    const int S_ = 7 + c.nt;
+    const int iters = M.bfgs_iters > 0 ? M.bfgs_iters : (25 + c.na) / 3;
+    const bool use_net = P.use_net != 0;
+    const bool feat_on = use_net || P.log_buf != nullptr || P.feat != nullptr;
+    float* gcur = M.cur + (long long)ch * SD;
+    float* gbest = M.best + (long long)ch * SD;
+    float cur = lane < S_ ? gcur[lane] : 0.f;
+    float best = lane < S_ ? gbest[lane] : 0.f;
+    float eb = M.best_e[ch], ec = M.cur_e[ch];
+    int nev = 0;
+    const unsigned long long seed = M.seed;
+    // features of the current pose from an evaluation of it (le valid) or a fresh one
+    auto features = [&](float E, const LastEval& le) {
+        float g = le.g, ea = le.ea;
+        if (!le.valid) {
+            float a0, a1, a2;
+            E = eval_eg<KM, 1>(c, sf, w, lane, cur, g, ea, a0, a1, a2);
+            ++nev;
+        }
+        write_features(nl.feat, c, lane, cur, E, g, ea);
+        wave_lds_sync();
+    };
+    if (M.init_opt) {
+        float x = cur;
+        LastEval le;
+        ec = bfgs<KM, 1, false, false, true>(c, sf, w, lane, x, iters, nev, &le);
+        cur = x;
+        if (ec < eb) { eb = ec; best = cur; }
+        if (feat_on) features(ec, le);
+    } else {
+        if (ec < eb) { eb = ec; best = cur; }
+        if (feat_on) features(0.f, LastEval{});
+    }
+    for (int st = 0; st < M.mc_steps; ++st) {
+        const int sg = M.step_offset + st;
+        bool plain = !use_net || urand(seed, cid, sg, SLOT_NET_PLAIN) < P.p_plain;
+        float y = 0.f;
+        if (!plain) {
+            y = net_forward(nl, N, lane);
+            plain = wave_count(!isfinite(y)) > 0;
+        }
+        float x = plain ? plain_move(cur, c, lane, seed, cid, sg, M.amp_pos, M.amp_rot)
+                        : net_move(cur, y, c, lane, seed, cid, sg, P.sig_t, P.sig_r, P.sig_th);
+        LastEval le;
+        const float en = bfgs<KM, 1, false, false, true>(c, sf, w, lane, x, iters, nev, &le);
+        const bool accept = (en < ec) || (urand(seed, cid, sg, 2) < expf((ec - en) / M.temp));
+        if (accept) {
+            if (P.log_buf) {
+                int slot = 0;
+                if (lane == 0) slot = atomicAdd(P.log_count, 1);
+                slot = bcast_i(slot, 0);
+                if (slot < P.log_cap) {
+                    float* rec = P.log_buf + (long long)slot * LOG_W;
+                    rec[lane] = nl.feat[lane];
+                    rec[64 + lane] = nl.feat[64 + lane];
+                    float t, m;
+                    move_target(cur, x, c, lane, t, m);
+                    if (lane < NOUT) { rec[NFEAT + lane] = t; rec[NFEAT + NOUT + lane] = m; }
+                    if (lane == 0) rec[NFEAT + 2 * NOUT] = ec - en;
+                }
+            }
+            ec = en; cur = x;
+            if (en < eb) { eb = en; best = x; }
+            if (feat_on) features(en, le);
+        }
+    }
+    if (P.feat) {
+        P.feat[(long long)ch * NFEAT + lane] = nl.feat[lane];
+        P.feat[(long long)ch * NFEAT + 64 + lane] = nl.feat[64 + lane];
+    }
+    if (lane < S_) { gcur[lane] = cur; gbest[lane] = best; }
+    if (lane == 0) {
+        M.cur_e[ch] = ec; M.best_e[ch] = eb;
+        if (M.eval_count) atomicAdd(M.eval_count, (unsigned long long)nev);
+        if (M.lig_evals && l < M.n_lig_evals) atomicAdd(M.lig_evals + l, (unsigned long long)nev);
+    }
+}
+
+template <int KM>
+int launch_mc_learned_t(const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecArgs& R, const int* chain_lig,
+                        int n_chains, const McArgs& M, const StepArgs& P, const NetArgs& N, hipStream_t stream) {
+    const int wave_bytes = wave_lds_bytes(KM, L.P_MAX) + NET_LDS_BYTES;
+    const void* fn = reinterpret_cast<const void*>(mc_learned_kernel<KM>);
+    int W = fit_waves(waves_for(n_chains), KM, L.P_MAX, NET_LDS_BYTES);
+    if (g_waves_per_block.load(std::memory_order_relaxed) <= 0) W = packed_waves(fn, W, wave_bytes);
+    const size_t shm = (size_t)W * wave_bytes;
+    const unsigned nblk = (unsigned)((n_chains + W - 1) / W);
+    allow_lds(fn, shm);
+    hipLaunchKernelGGL(mc_learned_kernel<KM>, dim3(nblk), dim3(W * WAVE), shm, stream, L, G, S, R, chain_lig, n_chains,
+                       M, P, N);
+    return (int)hipGetLastError();
+}
+
 }  // namespace
 
 // The external entry points of one torsion bucket (declared in dock_buckets.hpp): one translation unit per bucket
@@ -1658,6 +1852,11 @@ int launch_mc_step_t(const LigArgs& L, const GridArgs& G, const SFArgs& S, const
     int launch_mc_step_km##KM(const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecArgs& R, const int* cl,   \
                               int n, const McArgs& M, const StepArgs& P, hipStream_t st) {                             \
         return L.AM > 64 ? -1 : launch_mc_step_t<KM>(L, G, S, R, cl, n, M, P, st);                                     \
+    }                                                                                                                  \
+    int launch_mc_learned_km##KM(const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecArgs& R,               \
+                                 const int* cl, int n, const McArgs& M, const StepArgs& P, const NetArgs& N,           \
+                                 hipStream_t st) {                                                                     \
+        return L.AM > 64 ? -1 : launch_mc_learned_t<KM>(L, G, S, R, cl, n, M, P, N, st);                               \
     }                                                                                                                  \
     int mc_resident_waves_km##KM(int P_MAX, int persistent, int APL) {                                                 \
         return APL > 1 ? mc_resident_waves_t<KM, 2>(P_MAX, persistent) : mc_resident_waves_t<KM, 1>(P_MAX, persistent); \

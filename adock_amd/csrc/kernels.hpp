@@ -41,6 +41,12 @@ struct StepArgs {
     const float* Y; float* feat; float* log_buf; int* log_count; int log_cap;
     int use_net; float p_plain, sig_t, sig_r, sig_th; int init;
 };
+// proposal network weights for inference inside the chain loop: BF16 [out, in] (W1 [256,128], W2 [256,256],
+// W3 [23,256]) and FP32 biases
+struct NetArgs {
+    const unsigned short* W1; const float* b1; const unsigned short* W2; const float* b2; const unsigned short* W3;
+    const float* b3;
+};
 int launch_build_grid(const float*, const int*, int, const float*, const int*, const int*, const int*, const int*,
                       int, float, float, float, float, int, int, int, const float*, int, float*, hipStream_t);
 int launch_build_grid_fit(const float*, const int*, int, const float*, const float*, int, const float*, int, float,
@@ -74,6 +80,8 @@ int launch_adam(long long n, float* w, const float* g, float* m, float* v, float
 int launch_to_bf16(long long n, const float* x, unsigned short* y, hipStream_t s);
 int launch_mc_step(int KM, const LigArgs&, const GridArgs&, const SFArgs&, const RecArgs&, const int*, int,
                    const McArgs&, const StepArgs&, hipStream_t);
+int launch_mc_learned(int KM, const LigArgs&, const GridArgs&, const SFArgs&, const RecArgs&, const int*, int,
+                      const McArgs&, const StepArgs&, const NetArgs&, hipStream_t);
 int launch_stage_a(const float* conf, const int* gslot, const int* na, int AM, int L, int C, const float* quats,
                    int Q, const float* trans, const int* ntrans, int NT_MAX, const GridArgs& G, float cap,
                    float* best_e, int* best_i, hipStream_t stream);

@@ -16,19 +16,8 @@
 namespace adock {
 namespace {
 
-typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
-typedef float float4_t __attribute__((ext_vector_type(4)));
-
-constexpr int IN = 128, HID = 256, OUT = 23, OUTP = 32;
+constexpr int IN = NET_IN, HID = NET_HID, OUT = NET_OUT, OUTP = NET_OUTP;
 constexpr int ROWS = 64;
-
-// float32 -> bfloat16, round to nearest even (NaN kept quiet)
-DEVI unsigned short f2bf(float f) {
-    unsigned u = __float_as_uint(f);
-    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x7fffffu)) return (unsigned short)((u >> 16) | 0x40);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (unsigned short)(u >> 16);
-}
 
 // One layer for this wave's 16-row strip: out[16 x N] = act(in[16 x K] * W^T + b), W [N, K] bf16 (row = output).
 // in/out are LDS bf16 row-major [ROWS][K] / [ROWS][N]; rows r0..r0+15. N multiple of 16.

@@ -1,7 +1,9 @@
-"""Learned Monte Carlo proposal: step-by-step search with the proposal network between steps.
+"""Learned Monte Carlo proposal: the search with the proposal network inside the chain loop, and step by step.
 
-Each MC step is one adock_mc_step launch over all chains; with the network on, adock_mlp_infer (BF16 MFMA) runs on the
-features of every chain's current pose right before it. The whole step sequence can be captured once in a HIP graph
+StepSearch.fused runs the whole search in one launch (adock_mc_learned): every chain is set up once and its wavefront
+runs the network on MFMA before each network move. StepSearch.run is the step-by-step form: one adock_mc_step launch
+per MC step over all chains, with adock_mlp_infer (BF16 MFMA) on the features of every chain's current pose right
+before it; the two give the same results bit for bit. The step sequence can be captured once in a HIP graph
 (torch.cuda.CUDAGraph) and replayed. The exact Vina score and the reference's Metropolis test decide every move; the
 reference's random mutation is the fallback (probability p_plain, or whenever the network output is not finite).
 The network proposal is not symmetric, so the chain no longer satisfies detailed balance: it is an optimiser, not a
@@ -69,6 +71,32 @@ class StepSearch:
         for st in range(steps):
             self._step(cur, cur_e, best, best_e, 0, step_offset + st, bfgs_iters, temp, amp_pos, amp_rot, seed,
                        use_net, p_plain, sig, log, feat_on, count_evals)
+
+    def fused(self, cur, cur_e, best, best_e, steps, bfgs_iters=0, temp=1.2, amp_pos=2.0, amp_rot=2.0, seed=0,
+              step_offset=0, init_opt=True, use_net=False, p_plain=0.25, sig=(0.5, 0.15, 0.3), log=False,
+              count_evals=False, features=False):
+        """run(...) in one launch (adock_mc_learned): each chain is set up once and its wavefront runs the network on
+        MFMA inside its step loop, so there is no launch, chain set-up or feature evaluation per MC step. Same poses,
+        energies and logged moves (in another order) as run(...) bit for bit; self.feat holds the features of the
+        final current poses when the network, the log or `features` is on (a later run(..., init_opt=False) can
+        continue from them)."""
+        if use_net and self.net is None:
+            raise _lib.AdockError("use_net needs a ProposalNet")
+        kp = self.kp
+        kp.check_chain_buffers(self.N, cur, cur_e, best, best_e)
+        net = self.net if use_net else None
+        w = ([_lib.ptr(net.bf["W1"]), _lib.ptr(net.p["b1"]), _lib.ptr(net.bf["W2"]), _lib.ptr(net.p["b2"]),
+              _lib.ptr(net.bf["W3"]), _lib.ptr(net.p["b3"])] if net is not None else [None] * 6)
+        feat_on = bool(use_net or log or features)
+        args = (kp._args(False)[:-1] +
+                [_lib.ptr(self.chain_lig), _lib.ptr(self.chain_id), self.N, _lib.ptr(cur), _lib.ptr(cur_e),
+                 _lib.ptr(best), _lib.ptr(best_e), int(bool(init_opt)), int(steps), int(step_offset), int(bfgs_iters),
+                 float(temp), float(amp_pos), float(amp_rot), int(seed) & (2 ** 64 - 1)] + w +
+                [_lib.ptr(self.feat) if feat_on else None,
+                 _lib.ptr(self.log_buf) if log and self.log_buf is not None else None, _lib.ptr(self.log_count),
+                 self.log_cap, int(bool(use_net)), float(p_plain), float(sig[0]), float(sig[1]), float(sig[2]),
+                 _lib.ptr(kp.eval_count) if count_evals else None, _lib.stream_of(kp.device)])
+        _lib.check(_lib.lib().adock_mc_learned(*args))
 
     def capture(self, cur, cur_e, best, best_e, steps, **kw):
         """Captures run(...) on these buffers in a HIP graph; returns the graph (replay() re-runs the search; reset the

@@ -225,6 +225,29 @@ int adock_mc_step(int AM, int KM,
                   int use_net, float p_plain, float sig_t, float sig_r, float sig_th, unsigned long long* eval_count,
                   adock_stream_t stream);
 
+/* The learned search in one launch: every chain's start BFGS (init_opt = 1) and mc_steps MC steps (steps step_offset
+ * .. step_offset + mc_steps - 1), the chain set up once, with the proposal network run by each chain's wavefront on
+ * MFMA inside its step loop (weights as for adock_mlp_infer, read in place; may be NULL when use_net = 0). Moves,
+ * BFGS, Metropolis test, move log and feature definition are adock_mc_step's, so one call equals the step-by-step
+ * search (init step, then adock_mlp_infer + adock_mc_step per MC step) bit for bit in cur / cur_e / best / best_e and
+ * in the logged records (in another order); eval_count counts evaluations of this kernel, which takes a new pose's
+ * features from the evaluation its BFGS ended on instead of evaluating it again. feat [n,128] (may be NULL) receives the features of each chain's final
+ * current pose. use_net = 0 without a log is adock_mc's plain search (init_opt, mc_steps). */
+int adock_mc_learned(int AM, int KM,
+                     const float* ref, const int* typ, const unsigned char* tm, const int* ax, const float* axp,
+                     const float* axu, const int* pr, const int* na, const int* nt, const int* np, const float* rg,
+                     int P_MAX, const int* gslot, const unsigned* inc, const int* ist, int DEG_MAX,
+                     const float* grid, const float* origin, const float* lo, const float* hi, const int* rec_idx,
+                     const long long* goffs, const long long* gdims, int T, float spacing, float pen,
+                     const float* prm, const float* rad, const int* hyd, const int* don, const int* acc,
+                     const float* rxyz, const int* rtyp, const int* rn, int RN_MAX,
+                     const int* chain_lig, const int* chain_id, int n_chains, float* cur, float* cur_e, float* best,
+                     float* best_e, int init_opt, int mc_steps, int step_offset, int bfgs_iters, float temp,
+                     float amp_pos, float amp_rot, unsigned long long seed, const unsigned short* W1, const float* b1,
+                     const unsigned short* W2, const float* b2, const unsigned short* W3, const float* b3,
+                     float* feat, float* log_buf, int* log_count, int log_cap, int use_net, float p_plain,
+                     float sig_t, float sig_r, float sig_th, unsigned long long* eval_count, adock_stream_t stream);
+
 /* Inference on MFMA: Y[B,23] = W3 relu(W2 relu(W1 X + b1) + b2) + b3 with BF16 weights [out, in] (W1 [256,128],
  * W2 [256,256], W3 [23,256]), BF16 activations and FP32 accumulation; X [B,128] and biases FP32. */
 int adock_mlp_infer(int B, const float* X, const unsigned short* W1, const float* b1, const unsigned short* W2,
