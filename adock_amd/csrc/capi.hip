@@ -318,7 +318,7 @@ int adock_mc_learned(int AM, int KM, LIG_DECL, GRID_DECL, SF_DECL, REC_DECL, con
                      unsigned long long seed, const unsigned short* W1, const float* b1, const unsigned short* W2,
                      const float* b2, const unsigned short* W3, const float* b3, float* feat, float* log_buf,
                      int* log_count, int log_cap, int use_net, float p_plain, float sig_t, float sig_r, float sig_th,
-                     unsigned long long* eval_count, adock_stream_t stream) {
+                     const int* order, int* counter, unsigned long long* eval_count, adock_stream_t stream) {
     if (int st = common_checks(AM, KM, P_MAX, ref, inc, ist, DEG_MAX, grid, prm, 0, rxyz, rn, RN_MAX, chain_lig,
                                n_chains))
         return st;
@@ -337,7 +337,8 @@ int adock_mc_learned(int AM, int KM, LIG_DECL, GRID_DECL, SF_DECL, REC_DECL, con
              amp_rot, seed, step_offset, eval_count};
     StepArgs P{nullptr, feat, log_buf, log_count, log_cap, use_net, p_plain, sig_t, sig_r, sig_th, 0};
     NetArgs N{W1, b1, W2, b2, W3, b3};
-    return hip_status(launch_mc_learned(KM, L, G, S, R, chain_lig, n_chains, M, P, N, (hipStream_t)stream));
+    QueueArgs Q{order, nullptr, 1, counter, nullptr, nullptr, nullptr};
+    return hip_status(launch_mc_learned(KM, L, G, S, R, chain_lig, n_chains, M, P, N, Q, (hipStream_t)stream));
 }
 
 // ---- learned-proposal network ----

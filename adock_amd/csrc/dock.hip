@@ -76,10 +76,10 @@ int launch_mc_step(int KM, const LigArgs& L, const GridArgs& G, const SFArgs& S,
 }
 
 #define ADOCK_CASE_LEARNED(KM) \
-    case KM: return launch_mc_learned_km##KM(L, G, S, R, chain_lig, n_chains, M, P, N, stream);
+    case KM: return launch_mc_learned_km##KM(L, G, S, R, chain_lig, n_chains, M, P, N, Q, stream);
 int launch_mc_learned(int KM, const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecArgs& R,
                       const int* chain_lig, int n_chains, const McArgs& M, const StepArgs& P, const NetArgs& N,
-                      hipStream_t stream) {
+                      const QueueArgs& Q, hipStream_t stream) {
     switch (KM) {
         ADOCK_KM_LIST(ADOCK_CASE_LEARNED)
         default: return -1;

@@ -17,7 +17,7 @@ namespace adock {
     int launch_mc_step_km##KM(const LigArgs&, const GridArgs&, const SFArgs&, const RecArgs&, const int*, int,        \
                               const McArgs&, const StepArgs&, hipStream_t);                                            \
     int launch_mc_learned_km##KM(const LigArgs&, const GridArgs&, const SFArgs&, const RecArgs&, const int*, int,     \
-                                 const McArgs&, const StepArgs&, const NetArgs&, hipStream_t);                         \
+                                 const McArgs&, const StepArgs&, const NetArgs&, const QueueArgs&, hipStream_t);       \
     int mc_resident_waves_km##KM(int P_MAX, int persistent, int APL);
 ADOCK_KM_LIST(ADOCK_BUCKET_DECL)
 #undef ADOCK_BUCKET_DECL

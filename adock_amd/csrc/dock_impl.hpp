@@ -1709,18 +1709,10 @@ DEVI float net_forward(const NetLDS& n, const NetArgs& N, int lane) {
 // order); it skips the step kernel's extra evaluation per step (eval_count differs). With use_net = 0 and no log it
 // is mc_kernel's search. P.feat (may be NULL) receives the features of every chain's final current pose.
 template <int KM>
-__global__ __launch_bounds__(256) ADOCK_WAVES(KM, 1) void mc_learned_kernel(LigArgs L, GridArgs G, SFArgs S, RecArgs R,
-                                                          const int* __restrict__ chain_lig, int n_chains, McArgs M,
-                                                          StepArgs P, NetArgs N) {
-    extern __shared__ __attribute__((aligned(16))) char lds[];
+DEVI void learned_chain(int ch, int lane, const WaveLDS<1>& w, const NetLDS& nl, const LigArgs& L, const GridArgs& G,
+                        const SFArgs& S, const RecArgs& R, const int* __restrict__ chain_lig, const McArgs& M,
+                        const StepArgs& P, const NetArgs& N) {
     constexpr int SD = 7 + KM;
-    const int lane = lane_id();
-    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
-    const int ch = blockIdx.x * (blockDim.x / WAVE) + wid;
-    if (ch >= n_chains) return;
-    const int wl = wave_lds_bytes(KM, L.P_MAX);
-    const WaveLDS<1> w = carve_lds<KM, 1>(lds + wid * (wl + NET_LDS_BYTES), L.P_MAX);
-    const NetLDS nl = carve_net(lds + wid * (wl + NET_LDS_BYTES) + wl);
     const SFConst sf = make_sf(S.prm);
     const int l = __builtin_amdgcn_readfirstlane(chain_lig[ch]);
     const ChainCtx<KM, 1> c = setup_chain<KM, 1>(l, lane, L, G, S, R, w);
@@ -1736,59 +1728,66 @@ __global__ __launch_bounds__(256) ADOCK_WAVES(KM, 1) void mc_learned_kernel(LigA
     float eb = M.best_e[ch], ec = M.cur_e[ch];
     int nev = 0;
     const unsigned long long seed = M.seed;
-    // features of the current pose from an evaluation of it (le valid) or a fresh one
-    auto features = [&](float E, const LastEval& le) {
-        float g = le.g, ea = le.ea;
-        if (!le.valid) {
-            float a0, a1, a2;
-            E = eval_eg<KM, 1>(c, sf, w, lane, cur, g, ea, a0, a1, a2);
-            ++nev;
-        }
-        write_features(nl.feat, c, lane, cur, E, g, ea);
-        wave_lds_sync();
-    };
-    if (M.init_opt) {
+    // One BFGS call site for the start optimisation (st = -1), the MC steps and the evaluation that gives the features
+    // of the current pose when no kept evaluation has them (fresh: BFGS with no iterations evaluates cur once), so the
+    // chain body holds one copy of the evaluation code.
+    bool fresh = feat_on && !M.init_opt;
+    if (!M.init_opt && ec < eb) { eb = ec; best = cur; }
+    for (int st = M.init_opt ? -1 : 0; st < M.mc_steps || fresh;) {
+        const int sg = M.step_offset + (st < 0 ? 0 : st);
         float x = cur;
-        LastEval le;
-        ec = bfgs<KM, 1, false, false, true>(c, sf, w, lane, x, iters, nev, &le);
-        cur = x;
-        if (ec < eb) { eb = ec; best = cur; }
-        if (feat_on) features(ec, le);
-    } else {
-        if (ec < eb) { eb = ec; best = cur; }
-        if (feat_on) features(0.f, LastEval{});
-    }
-    for (int st = 0; st < M.mc_steps; ++st) {
-        const int sg = M.step_offset + st;
-        bool plain = !use_net || urand(seed, cid, sg, SLOT_NET_PLAIN) < P.p_plain;
-        float y = 0.f;
-        if (!plain) {
-            y = net_forward(nl, N, lane);
-            plain = wave_count(!isfinite(y)) > 0;
-        }
-        float x = plain ? plain_move(cur, c, lane, seed, cid, sg, M.amp_pos, M.amp_rot)
-                        : net_move(cur, y, c, lane, seed, cid, sg, P.sig_t, P.sig_r, P.sig_th);
-        LastEval le;
-        const float en = bfgs<KM, 1, false, false, true>(c, sf, w, lane, x, iters, nev, &le);
-        const bool accept = (en < ec) || (urand(seed, cid, sg, 2) < expf((ec - en) / M.temp));
-        if (accept) {
-            if (P.log_buf) {
-                int slot = 0;
-                if (lane == 0) slot = atomicAdd(P.log_count, 1);
-                slot = bcast_i(slot, 0);
-                if (slot < P.log_cap) {
-                    float* rec = P.log_buf + (long long)slot * LOG_W;
-                    rec[lane] = nl.feat[lane];
-                    rec[64 + lane] = nl.feat[64 + lane];
-                    float t, m;
-                    move_target(cur, x, c, lane, t, m);
-                    if (lane < NOUT) { rec[NFEAT + lane] = t; rec[NFEAT + NOUT + lane] = m; }
-                    if (lane == 0) rec[NFEAT + 2 * NOUT] = ec - en;
-                }
+        bool plain = true;
+        if (!fresh && st >= 0) {
+            plain = !use_net || urand(seed, cid, sg, SLOT_NET_PLAIN) < P.p_plain;
+            float y = 0.f;
+            if (!plain) {
+                y = net_forward(nl, N, lane);
+                plain = wave_count(!isfinite(y)) > 0;
             }
+            x = plain ? plain_move(cur, c, lane, seed, cid, sg, M.amp_pos, M.amp_rot)
+                      : net_move(cur, y, c, lane, seed, cid, sg, P.sig_t, P.sig_r, P.sig_th);
+        }
+        LastEval le;
+        const float en = bfgs<KM, 1, false, false, true>(c, sf, w, lane, x, fresh ? 0 : iters, nev, &le);
+        bool moved = false;   // cur is a new pose (its features are due)
+        if (fresh) {
+            fresh = false;
+            moved = true;
+        } else if (st < 0) {
             ec = en; cur = x;
-            if (en < eb) { eb = en; best = x; }
-            if (feat_on) features(en, le);
+            if (ec < eb) { eb = ec; best = cur; }
+            moved = true;
+            ++st;
+        } else {
+            const bool accept = (en < ec) || (urand(seed, cid, sg, 2) < expf((ec - en) / M.temp));
+            if (accept) {
+                if (P.log_buf) {
+                    int slot = 0;
+                    if (lane == 0) slot = atomicAdd(P.log_count, 1);
+                    slot = bcast_i(slot, 0);
+                    if (slot < P.log_cap) {
+                        float* rec = P.log_buf + (long long)slot * LOG_W;
+                        rec[lane] = nl.feat[lane];
+                        rec[64 + lane] = nl.feat[64 + lane];
+                        float t, m;
+                        move_target(cur, x, c, lane, t, m);
+                        if (lane < NOUT) { rec[NFEAT + lane] = t; rec[NFEAT + NOUT + lane] = m; }
+                        if (lane == 0) rec[NFEAT + 2 * NOUT] = ec - en;
+                    }
+                }
+                ec = en; cur = x;
+                if (en < eb) { eb = en; best = x; }
+                moved = true;
+            }
+            ++st;
+        }
+        if (moved && feat_on) {
+            if (le.valid) {
+                write_features(nl.feat, c, lane, cur, en, le.g, le.ea);
+                wave_lds_sync();
+            } else {
+                fresh = true;   // BFGS restored its start: evaluate cur before the next move
+            }
         }
     }
     if (P.feat) {
@@ -1803,18 +1802,65 @@ __global__ __launch_bounds__(256) ADOCK_WAVES(KM, 1) void mc_learned_kernel(LigA
     }
 }
 
+// One chain per wave (Q.counter NULL: grid = the chains), or persistent (grid = the resident capacity): every wave
+// pulls positions of Q.order (NULL: identity), e.g. longest chains first, from the counter Q.counter[0] (0 at launch)
+// until all n_chains are taken, so a wave that ends a short chain starts the next one instead of leaving its slot
+// idle behind its workgroup's longest chain.
+template <int KM>
+__global__ __launch_bounds__(256) ADOCK_WAVES(KM, 1) void mc_learned_kernel(LigArgs L, GridArgs G, SFArgs S, RecArgs R,
+                                                          const int* __restrict__ chain_lig, int n_chains, McArgs M,
+                                                          StepArgs P, NetArgs N, QueueArgs Q) {
+    extern __shared__ __attribute__((aligned(16))) char lds[];
+    const int lane = lane_id();
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+    const int wl = wave_lds_bytes(KM, L.P_MAX);
+    const WaveLDS<1> w = carve_lds<KM, 1>(lds + wid * (wl + NET_LDS_BYTES), L.P_MAX);
+    const NetLDS nl = carve_net(lds + wid * (wl + NET_LDS_BYTES) + wl);
+    // one call site of the chain body for both launch forms (grid form: one pass with the wave's own index)
+    for (int pass = 0;; ++pass) {
+        int idx = 0;
+        if (Q.counter) {
+            if (lane == 0) idx = atomicAdd(Q.counter, 1);
+            idx = bcast_i(idx, 0);
+        } else {
+            if (pass > 0) break;
+            idx = blockIdx.x * (blockDim.x / WAVE) + wid;
+        }
+        if (idx >= n_chains) break;
+        const int ch = Q.order ? __builtin_amdgcn_readfirstlane(Q.order[idx]) : idx;
+        learned_chain<KM>(ch, lane, w, nl, L, G, S, R, chain_lig, M, P, N);
+    }
+}
+
 template <int KM>
 int launch_mc_learned_t(const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecArgs& R, const int* chain_lig,
-                        int n_chains, const McArgs& M, const StepArgs& P, const NetArgs& N, hipStream_t stream) {
+                        int n_chains, const McArgs& M, const StepArgs& P, const NetArgs& N, const QueueArgs& Q,
+                        hipStream_t stream) {
     const int wave_bytes = wave_lds_bytes(KM, L.P_MAX) + NET_LDS_BYTES;
     const void* fn = reinterpret_cast<const void*>(mc_learned_kernel<KM>);
     int W = fit_waves(waves_for(n_chains), KM, L.P_MAX, NET_LDS_BYTES);
     if (g_waves_per_block.load(std::memory_order_relaxed) <= 0) W = packed_waves(fn, W, wave_bytes);
     const size_t shm = (size_t)W * wave_bytes;
-    const unsigned nblk = (unsigned)((n_chains + W - 1) / W);
     allow_lds(fn, shm);
+    unsigned nblk = (unsigned)((n_chains + W - 1) / W);
+    if (Q.counter) {
+        // resident capacity, queried once per (device, W, LDS size)
+        static std::mutex mu;
+        static std::map<long long, int> cap;
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        const long long key = ((long long)dev << 40) | ((long long)W << 32) | (long long)shm;
+        std::lock_guard<std::mutex> lock(mu);
+        auto it = cap.find(key);
+        if (it == cap.end()) {
+            int per_cu = 0;
+            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, mc_learned_kernel<KM>, W * WAVE, shm);
+            it = cap.emplace(key, cu_count() * (per_cu > 0 ? per_cu : 1)).first;
+        }
+        nblk = std::min(nblk, (unsigned)it->second);
+    }
     hipLaunchKernelGGL(mc_learned_kernel<KM>, dim3(nblk), dim3(W * WAVE), shm, stream, L, G, S, R, chain_lig, n_chains,
-                       M, P, N);
+                       M, P, N, Q);
     return (int)hipGetLastError();
 }
 
@@ -1855,8 +1901,8 @@ int launch_mc_learned_t(const LigArgs& L, const GridArgs& G, const SFArgs& S, co
     }                                                                                                                  \
     int launch_mc_learned_km##KM(const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecArgs& R,               \
                                  const int* cl, int n, const McArgs& M, const StepArgs& P, const NetArgs& N,           \
-                                 hipStream_t st) {                                                                     \
-        return L.AM > 64 ? -1 : launch_mc_learned_t<KM>(L, G, S, R, cl, n, M, P, N, st);                               \
+                                 const QueueArgs& Q, hipStream_t st) {                                                 \
+        return L.AM > 64 ? -1 : launch_mc_learned_t<KM>(L, G, S, R, cl, n, M, P, N, Q, st);                            \
     }                                                                                                                  \
     int mc_resident_waves_km##KM(int P_MAX, int persistent, int APL) {                                                 \
         return APL > 1 ? mc_resident_waves_t<KM, 2>(P_MAX, persistent) : mc_resident_waves_t<KM, 1>(P_MAX, persistent); \

@@ -81,7 +81,7 @@ int launch_to_bf16(long long n, const float* x, unsigned short* y, hipStream_t s
 int launch_mc_step(int KM, const LigArgs&, const GridArgs&, const SFArgs&, const RecArgs&, const int*, int,
                    const McArgs&, const StepArgs&, hipStream_t);
 int launch_mc_learned(int KM, const LigArgs&, const GridArgs&, const SFArgs&, const RecArgs&, const int*, int,
-                      const McArgs&, const StepArgs&, const NetArgs&, hipStream_t);
+                      const McArgs&, const StepArgs&, const NetArgs&, const QueueArgs&, hipStream_t);
 int launch_stage_a(const float* conf, const int* gslot, const int* na, int AM, int L, int C, const float* quats,
                    int Q, const float* trans, const int* ntrans, int NT_MAX, const GridArgs& G, float cap,
                    float* best_e, int* best_i, hipStream_t stream);

@@ -10,6 +10,7 @@ driver():  the full search driver (Stage A, MC segments, refine, exact polish, f
 wide():    the single complex widened to 4,096 chains at constant chain-steps (the reference's min_total_chains rule).
 mlp():     the proposal network's kernels alone: MFMA inference on one step's batch, one FP32 training step.
 eg():      the energy + gradient kernel alone on many independent poses of the single complex (throughput bound).
+learned_cost(): a learned MC step's cost against a plain one (one-launch learned search and step-by-step pipeline).
 learned(): trains the proposal network on accepted moves logged by the plain method on training complexes, then
            compares plain and learned search on disjoint evaluation complexes at equal MC-step budgets.
 """
@@ -22,7 +23,8 @@ from . import synth, workload
 from .autotune import Autotuner
 from .batch import XCDS, BatchRunner, GraphedBatch
 from .kernel import KernelProblem
-from .learned import StepSearch, jump_dataset, train_proposal
+from .learned import StepSearch, jump_dataset, move_weights, train_proposal
+from .mlp import ProposalNet
 
 
 def _bufs(st):
@@ -338,15 +340,95 @@ def _budget_to_gap(rows, key, target):
     return None, None
 
 
+def learned_cost(n=512, chains=64, steps=60, seed=0, p_plain=0.25, log=print):
+    """Cost of a learned MC step against a plain one on the learned-proposal workload (n single complexes x chains,
+    all chains of one step in one batch): the plain fused search (adock_mc), the one-launch learned search
+    (adock_mc_learned) with the network off and on (an untrained network; p_plain of the moves stay random), and the
+    step-by-step pipeline (one inference + one step launch per MC step, captured in a HIP graph). Per MC step, ms."""
+    _, ligs, prob = complexes(range(20_000, 20_000 + n))
+    kp = KernelProblem(prob)
+    E = len(ligs)
+    st, cl = workload.chain_starts(seed + 2, ligs, [np.zeros(3)] * E, [np.full(3, 26.0)] * E, chains, kp.KM)
+    st, cl = torch.as_tensor(st, device=kp.device), torch.as_tensor(cl, device=kp.device)
+    ci = torch.arange(len(cl), dtype=torch.int32, device=kp.device)
+    s64 = seed * 1_000_003 + 12345
+    net = ProposalNet(kp.device, seed=seed)
+    ss = StepSearch(kp, cl, net=net)
+    b = _bufs(st)
+
+    def reset():
+        for x, y in zip(b, _bufs(st)):
+            x.copy_(y)
+
+    def plain():
+        kp.run_mc(cl, ci, *b, True, steps, 0, 1.2, 2.0, 2.0, s64, 0)
+
+    def evals(fn):
+        kp.eval_count.zero_()
+        reset()
+        fn()
+        return int(kp.eval_count.item())
+
+    runs = {"plain (adock_mc)": (plain, lambda: kp.run_mc(cl, ci, *b, True, steps, 0, 1.2, 2.0, 2.0, s64, 0,
+                                                           count_evals=True)),
+            "learned kernel, network off": (lambda: ss.fused(*b, steps, seed=s64),
+                                            lambda: ss.fused(*b, steps, seed=s64, count_evals=True)),
+            "learned kernel, network on": (lambda: ss.fused(*b, steps, seed=s64, use_net=True, p_plain=p_plain),
+                                           lambda: ss.fused(*b, steps, seed=s64, use_net=True, p_plain=p_plain,
+                                                            count_evals=True))}
+    out = {"workload": f"{E} complexes (4,000-atom receptor, 35-atom / 8-torsion ligand) x {chains} chains "
+                       f"({E * chains} chains per step), {steps} MC steps + init BFGS", "p_plain": p_plain,
+           "ms_per_step": {}, "ns_per_evaluation": {}, "evaluations": {}}
+    for name, (fn, counted) in runs.items():
+        reset()
+        fn()   # module load, first-launch costs
+        reset()
+        dt = _timed(fn)
+        n_ev = evals(counted)
+        out["ms_per_step"][name] = 1e3 * dt / steps
+        out["evaluations"][name] = n_ev
+        out["ns_per_evaluation"][name] = 1e9 * dt / n_ev
+        log(f"{name}: {1e3 * dt / steps:.3f} ms per MC step, {n_ev} evaluations, {1e9 * dt / n_ev:.3f} ns each")
+    reset()
+    g = ss.capture(*b, steps, seed=s64, use_net=True, p_plain=p_plain)
+    reset()
+    dt = _timed(g.replay)
+    out["ms_per_step"]["step pipeline (graph), network on"] = 1e3 * dt / steps
+    log(f"step pipeline: {1e3 * dt / steps:.3f} ms per MC step")
+    ms, ns = out["ms_per_step"], out["ns_per_evaluation"]
+    out["learned_over_plain"] = ms["learned kernel, network on"] / ms["plain (adock_mc)"]
+    out["learned_over_plain_per_evaluation"] = ns["learned kernel, network on"] / ns["plain (adock_mc)"]
+    out["step_pipeline_over_plain"] = ms["step pipeline (graph), network on"] / ms["plain (adock_mc)"]
+    return out
+
+
+def _time_to_gap(rows, key, target):
+    """Search time at which the method's mean gap reaches `target`: log-log interpolation between the last budget
+    above it and the first at or below it; None if no budget reaches it or the first one already does."""
+    prev = None
+    for r in rows:
+        g, t = r[key + "_gap"], r[key + "_s"]
+        if g <= target:
+            if prev is None or prev[0] <= 0 or g <= 0:
+                return None if prev is None else t
+            f = (np.log(prev[0]) - np.log(target)) / (np.log(prev[0]) - np.log(g))
+            return float(np.exp(np.log(prev[1]) + f * (np.log(t) - np.log(prev[1]))))
+        prev = (g, t)
+    return None
+
+
 def learned(n_train=512, train_chains=64, train_steps=400, n_eval=512, eval_chains=64,
-            budgets=(10, 15, 20, 30, 40, 60, 80, 120, 160, 240, 320, 480), full_steps=1000, epochs=5, p_plain=0.25,
-            gap_target=0.1, seed=0, target="move", every=2, log=print):
+            budgets=(10, 15, 20, 30, 40, 60, 80, 120, 160, 240, 320, 480), full_steps=1000, epochs=5, p_plain=0.5,
+            gap_target=0.1, seed=0, target="move", every=2, weighting="improving", log=print):
     """Learned proposal: the proposal network trained on the plain method's runs over n_train complexes, then plain (fused
-    kernel, one launch) and learned (step kernel + MFMA inference, the whole search captured in one HIP graph)
-    searches on n_eval disjoint complexes at a range of MC-step budgets. Gap = mean over complexes of
+    kernel, one launch) and learned (adock_mc_learned, one launch with MFMA inference inside the chain loop) searches
+    on n_eval disjoint complexes at a range of MC-step budgets. Gap = mean over complexes of
     best(budget) - best(plain, full_steps). Reports, per method, the smallest budget reaching gap_target and the
     docked ligands/s there. target "move": regress the accepted moves (features before the move -> the move);
-    "jump": regress, from the pose every `every` steps, the jump to the best pose its chain ends with."""
+    "jump": regress, from the pose every `every` steps, the jump to the best pose its chain ends with. weighting (move
+    target): "accepted" weighs every accepted move (downhill ones by their gain), "improving" only downhill moves, by
+    their gain. The speed-up at a matched gap compares the search times at which the two methods' mean gaps reach
+    the same value (interpolated between budgets), at gap_target and at a few other gaps."""
     t_all = time.perf_counter()
     # 1. accepted moves of the plain method on the training complexes
     _, tligs, tprob = complexes(range(10_000, 10_000 + n_train))
@@ -359,9 +441,9 @@ def learned(n_train=512, train_chains=64, train_steps=400, n_eval=512, eval_chai
         cap = int(len(cl) * train_steps * 0.85) + 1024
         ss = StepSearch(tkp, cl, log_cap=cap)
         log(f"training set: {L} complexes x {train_chains} chains x {train_steps} steps (log capacity {cap} moves)")
-        t_log = _timed(lambda: ss.run(*_bufs(st), train_steps, seed=seed + 777, log=True))
+        t_log = _timed(lambda: ss.fused(*_bufs(st), train_steps, seed=seed + 777, log=True))
         X, T, M, gain = ss.logged()
-        W = None
+        W = move_weights(M, gain, mode=weighting)
     else:
         ss = StepSearch(tkp, cl)
         log(f"training set: {L} complexes x {train_chains} chains x {train_steps} steps, jumps every {every}")
@@ -393,22 +475,35 @@ def learned(n_train=512, train_chains=64, train_steps=400, n_eval=512, eval_chai
     ref, t_full = plain(full_steps)
     log(f"plain reference: {full_steps} steps in {t_full:.2f} s, mean best {ref.mean():.3f}")
     ss = StepSearch(ekp, cl, net=net)
+    # control: the same search with a network whose output is zero (every network move is the current pose plus the
+    # Gaussian noise alone), which separates what the learned move mean adds from what the move shape adds
+    zero = ProposalNet(ekp.device, seed=seed)
+    zero.p["W3"].zero_()
+    zero.p["b3"].zero_()
+    zero.refresh_bf16()
+    sz = StepSearch(ekp, cl, net=zero)
     rows = []
     for S in budgets:
         bp, tp = plain(S)
         b = _bufs(st)
-        g = ss.capture(*b, S, seed=s64, use_net=True, p_plain=p_plain)
-        for x, y in zip(b, _bufs(st)):
-            x.copy_(y)
-        tl = _timed(g.replay)
+        tl = _timed(lambda: ss.fused(*b, S, seed=s64, use_net=True, p_plain=p_plain))
         bl = b[3].view(E, eval_chains).min(1).values.cpu().numpy()
-        del g
+        b = _bufs(st)
+        tz = _timed(lambda: sz.fused(*b, S, seed=s64, use_net=True, p_plain=p_plain))
+        bz = b[3].view(E, eval_chains).min(1).values.cpu().numpy()
         rows.append({"steps": S, "plain_gap": float((bp - ref).mean()), "learned_gap": float((bl - ref).mean()),
-                     "plain_s": tp, "learned_s": tl})
+                     "control_gap": float((bz - ref).mean()), "plain_s": tp, "learned_s": tl, "control_s": tz})
         log(f"S={S}: plain gap {rows[-1]['plain_gap']:.3f} ({tp:.2f} s), learned gap {rows[-1]['learned_gap']:.3f} "
-            f"({tl:.2f} s)")
+            f"({tl:.2f} s), zero-output control gap {rows[-1]['control_gap']:.3f}")
     sp, tp = _budget_to_gap(rows, "plain", gap_target)
     sl, tl = _budget_to_gap(rows, "learned", gap_target)
+    matched = {}
+    for g in sorted({0.3, 0.2, gap_target, 0.05}, reverse=True):
+        a, b, z = (_time_to_gap(rows, k, g) for k in ("plain", "learned", "control"))
+        matched[f"{g:g}"] = {"plain_s": a, "learned_s": b, "control_s": z, "speedup": (a / b) if (a and b) else None,
+                             "control_speedup": (a / z) if (a and z) else None}
+        log(f"gap {g:g}: plain {a} s, learned {b} s, speed-up {matched[f'{g:g}']['speedup']}, "
+            f"zero-output control {matched[f'{g:g}']['control_speedup']}")
     out = {"workload": f"learned proposal ({target} target): trained on {n_moves} samples from {L} single complexes, "
                        f"evaluated on {E} disjoint complexes x {eval_chains} chains ({E * eval_chains} chains per step)",
            "gap_target_kcal": gap_target, "full_plain_steps": full_steps, "full_plain_s": t_full, "log_s": t_log,
@@ -416,5 +511,9 @@ def learned(n_train=512, train_chains=64, train_steps=400, n_eval=512, eval_chai
            "plain_steps_to_gap": sp, "learned_steps_to_gap": sl,
            "plain_docked_ligands_per_s": E / tp if tp else None,
            "learned_docked_ligands_per_s": E / tl if tl else None}
-    out["speedup_at_matched_gap"] = (tp / tl) if (tp and tl) else None
+    out["speedup_at_matched_gap"] = matched[f"{gap_target:g}"]["speedup"]
+    out["matched_gaps"] = matched
+    out["weighting"] = weighting
+    out["learned_over_plain_time_at_equal_steps"] = (sum(r["learned_s"] for r in rows) /
+                                                     sum(r["plain_s"] for r in rows))
     return out

@@ -38,6 +38,9 @@ class StepSearch:
         self.log_buf = torch.zeros(max(self.log_cap, 1), LOG_W, device=dev) if self.log_cap else None
         self.log_count = torch.zeros(1, dtype=torch.int32, device=dev)
         self.graphs = {}
+        # the one-launch search's chain queue: longest chains first (their complexes' cost), pulled from a counter
+        self.order = torch.sort(-kp.cost_dev[self.chain_lig.long()], stable=True).indices.to(torch.int32)
+        self.counter = torch.zeros(1, dtype=torch.int32, device=dev)
 
     def _step(self, cur, cur_e, best, best_e, init, step, bfgs_iters, temp, amp_pos, amp_rot, seed, use_net, p_plain,
               sig, log, feat_on, count_evals):
@@ -74,12 +77,13 @@ class StepSearch:
 
     def fused(self, cur, cur_e, best, best_e, steps, bfgs_iters=0, temp=1.2, amp_pos=2.0, amp_rot=2.0, seed=0,
               step_offset=0, init_opt=True, use_net=False, p_plain=0.25, sig=(0.5, 0.15, 0.3), log=False,
-              count_evals=False, features=False):
+              count_evals=False, features=False, queued=True):
         """run(...) in one launch (adock_mc_learned): each chain is set up once and its wavefront runs the network on
         MFMA inside its step loop, so there is no launch, chain set-up or feature evaluation per MC step. Same poses,
         energies and logged moves (in another order) as run(...) bit for bit; self.feat holds the features of the
         final current poses when the network, the log or `features` is on (a later run(..., init_opt=False) can
-        continue from them)."""
+        continue from them). queued: persistent waves take the chains longest first from a queue (else one wave per
+        chain in chain order); the results are the same."""
         if use_net and self.net is None:
             raise _lib.AdockError("use_net needs a ProposalNet")
         kp = self.kp
@@ -95,7 +99,10 @@ class StepSearch:
                 [_lib.ptr(self.feat) if feat_on else None,
                  _lib.ptr(self.log_buf) if log and self.log_buf is not None else None, _lib.ptr(self.log_count),
                  self.log_cap, int(bool(use_net)), float(p_plain), float(sig[0]), float(sig[1]), float(sig[2]),
+                 _lib.ptr(self.order) if queued else None, _lib.ptr(self.counter) if queued else None,
                  _lib.ptr(kp.eval_count) if count_evals else None, _lib.stream_of(kp.device)])
+        if queued:
+            self.counter.zero_()
         _lib.check(_lib.lib().adock_mc_learned(*args))
 
     def capture(self, cur, cur_e, best, best_e, steps, **kw):
@@ -174,10 +181,16 @@ def jump_dataset(search, start, steps, seed=0, every=4, max_gain=5.0):
     return torch.cat(X), torch.cat(T), torch.cat(W)
 
 
-def move_weights(mask, gain):
-    """Per-component regression weights: moves that lowered the energy count 1 + min(gain, 5), uphill (Metropolis)
-    moves 0.25, missing components 0."""
-    w = torch.where(gain > 0, 1.0 + gain.clamp(max=5.0), torch.full_like(gain, 0.25))
+def move_weights(mask, gain, mode="accepted"):
+    """Per-component regression weights, 0 on missing components. mode "accepted": moves that lowered the energy count
+    1 + min(gain, 5), uphill (Metropolis) moves 0.25; "improving": only moves that lowered the energy, weighted
+    min(gain, 5)."""
+    if mode == "accepted":
+        w = torch.where(gain > 0, 1.0 + gain.clamp(max=5.0), torch.full_like(gain, 0.25))
+    elif mode == "improving":
+        w = gain.clamp(min=0.0, max=5.0)
+    else:
+        raise ValueError(f"unknown weighting {mode!r}")
     return (mask * w[:, None]).contiguous()
 
 

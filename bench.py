@@ -419,9 +419,11 @@ def other_workload(args):
         return ex.multi(n_receptors=8 if args.small else 64, ligands_per=8 if args.small else 32, chains=32,
                         steps=50 if args.small else 500, queues=args.queues,
                         hbm_budget=args.hbm_budget_mb * 1e6 if args.hbm_budget_mb else None)
+    if wl == "learned_cost":
+        return ex.learned_cost(n=16 if args.small else 512, steps=10 if args.small else 60)
     return (ex.learned(n_train=8, train_steps=40, n_eval=8, budgets=(10, 20, 40), full_steps=100, epochs=2,
-                       target=args.target)
-            if args.small else ex.learned(target=args.target))
+                       target=args.target, weighting=args.weighting, p_plain=args.p_plain)
+            if args.small else ex.learned(target=args.target, weighting=args.weighting, p_plain=args.p_plain))
 
 
 def main():
@@ -435,8 +437,8 @@ def main():
     ap.add_argument("--chains-per-block", type=int, default=0)
     ap.add_argument("--line-search-width", type=int, default=0, help="trials per speculative batch (0 = adaptive)")
     ap.add_argument("--workload", default="multi",
-                    choices=["multi", "single", "screen", "multi_groups", "learned", "grids", "eg", "wide", "driver",
-                             "driver_fitted", "mlp"],
+                    choices=["multi", "single", "screen", "multi_groups", "learned", "learned_cost", "grids", "eg",
+                             "wide", "driver", "driver_fitted", "mlp"],
                     help="multi = the headline line (default); the others print their own JSON line")
     ap.add_argument("--small", action="store_true", help="scaled-down sizes for the opt-in workloads")
     ap.add_argument("--queues", type=int, default=8,
@@ -445,6 +447,10 @@ def main():
                     help="multi_groups: device memory one admission group's grid sets may take (0: a quarter of all)")
     ap.add_argument("--target", choices=["move", "jump"], default="move",
                     help="training target of the learned proposal (--workload learned)")
+    ap.add_argument("--weighting", choices=["accepted", "improving"], default="improving",
+                    help="regression weights of logged moves (--workload learned, move target)")
+    ap.add_argument("--p-plain", type=float, default=0.5,
+                    help="share of random (plain) moves in the learned search (--workload learned)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="after the timed steps, one device-to-device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()

@@ -232,7 +232,10 @@ int adock_mc_step(int AM, int KM,
  * search (init step, then adock_mlp_infer + adock_mc_step per MC step) bit for bit in cur / cur_e / best / best_e and
  * in the logged records (in another order); eval_count counts evaluations of this kernel, which takes a new pose's
  * features from the evaluation its BFGS ended on instead of evaluating it again. feat [n,128] (may be NULL) receives the features of each chain's final
- * current pose. use_net = 0 without a log is adock_mc's plain search (init_opt, mc_steps). */
+ * current pose. use_net = 0 without a log is adock_mc's plain search (init_opt, mc_steps). counter NULL: one wavefront
+ * per chain; else (one int, 0 at the call) persistent waves take the chains in the order `order` [n_chains] (a
+ * permutation of 0 .. n_chains - 1, e.g. longest chains first; NULL: identity) from that counter, which ends at n_chains plus the waves launched:
+ * reset it before every call. Results do not depend on the order. */
 int adock_mc_learned(int AM, int KM,
                      const float* ref, const int* typ, const unsigned char* tm, const int* ax, const float* axp,
                      const float* axu, const int* pr, const int* na, const int* nt, const int* np, const float* rg,
@@ -246,7 +249,8 @@ int adock_mc_learned(int AM, int KM,
                      float amp_pos, float amp_rot, unsigned long long seed, const unsigned short* W1, const float* b1,
                      const unsigned short* W2, const float* b2, const unsigned short* W3, const float* b3,
                      float* feat, float* log_buf, int* log_count, int log_cap, int use_net, float p_plain,
-                     float sig_t, float sig_r, float sig_th, unsigned long long* eval_count, adock_stream_t stream);
+                     float sig_t, float sig_r, float sig_th, const int* order, int* counter,
+                     unsigned long long* eval_count, adock_stream_t stream);
 
 /* Inference on MFMA: Y[B,23] = W3 relu(W2 relu(W1 X + b1) + b2) + b3 with BF16 weights [out, in] (W1 [256,128],
  * W2 [256,256], W3 [23,256]), BF16 activations and FP32 accumulation; X [B,128] and biases FP32. */

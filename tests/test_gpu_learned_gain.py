@@ -1,13 +1,16 @@
 """What the trained proposal network gains over the plain random move, measured at reduced size (the full-size
-experiment is `bench.py --workload learned`): the network is trained with the library's own kernels on accepted moves
-logged by the plain method on 16 training complexes, then plain and learned search run on 16 disjoint complexes at
-equal MC-step budgets, each scored by its mean best-score gap to a long plain run.
+experiment is `bench.py --workload learned`): the network is trained with the library's own kernels on moves logged by
+the plain method on 16 training complexes, then plain search, learned search (one launch, inference inside the chain
+loop) and a control with a zero-output network (network moves = the Gaussian noise alone) run on 16 disjoint complexes
+at equal MC-step budgets, each scored by its mean best-score gap to a long plain run.
 
-The test records the gaps, the time per learned step against a plain step and the speed-up at a matched gap (printed,
-and returned in the experiment's rows). It asserts what must hold whatever the gain: the training loss goes down, every
-gap is finite, and the learned search is not worse than plain by more than 0.5 kcal/mol at the largest budget (the
-exact score still judges every move). At full size the measured gain is none: both reach a 0.1 kcal/mol gap at the
-same budget and a learned step costs more than a plain one."""
+The test records the gaps, the learned search's time against the plain one's at equal steps and the speed-up at a
+matched gap (printed, and returned in the experiment's rows). It asserts what must hold whatever the gain: the training
+loss goes down, every gap is finite, and the learned search is not worse than plain by more than 0.5 kcal/mol at the
+largest budget (the exact score still judges every move). At full size (512 unseen complexes x 64 chains) the learned
+search reaches a 0.1 kcal/mol gap 0.98-1.08x as fast as plain, and the zero-output control 1.05x: the small gain comes
+from the shape of the network move (all degrees of freedom at once, mixed with the plain move), not from what the
+network learned."""
 import math
 
 import pytest
