@@ -42,7 +42,7 @@ def lib():
                                        [I, P, P, I, P, P, I, P, P, P, P, I, I, I, F, F, F, U64, I, P, P])
         L.adock_mc_batch.argtypes = (LIG + GRID + SF + REC +
                                      [I, P, P, I, P, P, I, P, P, P, P, I, I, I, F, F, F, U64, I, P, P, I, P, P, P, P, I,
-                                      I, P, P, P])
+                                      I, I, P, P, P])
         L.adock_mc_batch_noalloc.argtypes = (LIG + GRID + SF + REC +
                                              [I, P, P, I, P, P, I, P, P, P, P, I, I, I, F, F, F, U64, I, P, P, I, I, P,
                                               P, I, I, P, P, P])

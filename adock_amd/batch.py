@@ -98,6 +98,7 @@ class BatchRunner:
         self.lig_evals = torch.zeros(self.L, dtype=torch.int64, device=dev)   # evaluations per complex (count_evals)
         self.blocks = 0
         self.waves = 0   # chains per workgroup of this runner's launches (0: the library's default)
+        self.lds_layout = 0   # pair-gradient LDS tile: 0 compact slots, 1 dense per-atom rows (same results)
 
     @classmethod
     def on_device(cls, kp, chain_lig, n_complexes):
@@ -123,6 +124,7 @@ class BatchRunner:
         self.lig_evals = torch.zeros(self.L, dtype=torch.int64, device=dev)
         self.blocks = 0
         self.waves = 0
+        self.lds_layout = 0
         return self
 
     def launch(self, chain_id, cur, cur_e, best, best_e, init_opt, mc_steps, bfgs_iters, temp, amp_pos, amp_rot, seed,
@@ -143,7 +145,7 @@ class BatchRunner:
                                    int(mc_steps), int(bfgs_iters), float(temp), float(amp_pos), float(amp_rot),
                                    int(seed) & (2 ** 64 - 1), int(step_offset), _lib.ptr(self.order),
                                    _lib.ptr(self.qstart), self.nq, _lib.ptr(self.counter), _lib.ptr(self.done), _lib.ptr(self.lig_chains),
-                                   _lib.ptr(self.times), int(self.blocks), int(self.waves),
+                                   _lib.ptr(self.times), int(self.blocks), int(self.waves), int(self.lds_layout),
                                    _lib.ptr(kp.eval_count) if count_evals else None,
                                    _lib.ptr(self.lig_evals) if count_evals else None, st])
         _lib.check(L.adock_mc_batch(*args))

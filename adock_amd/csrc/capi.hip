@@ -174,8 +174,8 @@ int adock_mc_batch(int AM, int KM, LIG_DECL, GRID_DECL, SF_DECL, REC_DECL, int e
                    float* cur_e, float* best, float* best_e, int init_opt, int mc_steps, int bfgs_iters, float temp,
                    float amp_pos, float amp_rot, unsigned long long seed, int step_offset, const int* order,
                    const int* queue_start, int n_queues, int* counter, int* done, const int* lig_chains,
-                   unsigned long long* times, int blocks, int waves_per_block, unsigned long long* eval_count,
-                   unsigned long long* lig_evals, adock_stream_t stream) {
+                   unsigned long long* times, int blocks, int waves_per_block, int lds_layout,
+                   unsigned long long* eval_count, unsigned long long* lig_evals, adock_stream_t stream) {
     if (int st = common_checks(AM, KM, P_MAX, ref, inc, ist, DEG_MAX, grid, prm, exact, rxyz, rn, RN_MAX, chain_lig,
                                n_chains))
         return st;
@@ -187,6 +187,7 @@ int adock_mc_batch(int AM, int KM, LIG_DECL, GRID_DECL, SF_DECL, REC_DECL, int e
     if (mc_steps < 0) return fail(ADOCK_E_ARG, "negative mc_steps");
     if (nbr && (!nbr_n || NB_MAX <= 0)) return fail(ADOCK_E_ARG, "nbr needs nbr_n and NB_MAX > 0");
     if (waves_per_block < 0 || waves_per_block > 4) return fail(ADOCK_E_ARG, "waves_per_block must be in 0..4");
+    if (lds_layout != 0 && lds_layout != 1) return fail(ADOCK_E_ARG, "lds_layout must be 0 (compact) or 1 (dense)");
     LigArgs L{ref, typ, tm, ax, axp, axu, pr, na, nt, np, rg, P_MAX, gslot, inc, ist, DEG_MAX, AM};
     GridArgs G{grid, origin, lo, hi, rec_idx, goffs, gdims, T, spacing, pen};
     SFArgs S{prm, rad, hyd, don, acc};
@@ -197,6 +198,7 @@ int adock_mc_batch(int AM, int KM, LIG_DECL, GRID_DECL, SF_DECL, REC_DECL, int e
     M.n_lig_evals = 0x7fffffff;
     QueueArgs Q{order, queue_start, n_queues, counter, done, lig_chains, times};
     Q.waves = waves_per_block;
+    Q.dense = lds_layout;
     return hip_status(launch_mc_persistent(KM, L, G, S, R, exact, chain_lig, n_chains, M, Q, blocks,
                                            (hipStream_t)stream));
 }

@@ -794,6 +794,10 @@ DEVI float plain_move(float x, const Ctx& c, int lane, unsigned long long seed, 
 // budget would spill 0.4-0.9 KB per lane, and those kernels keep the compiler's own allocation.
 template <int KM, int APL> constexpr int three_waves() { return (APL == 1 && KM <= 32) ? 3 : 1; }
 #define ADOCK_WAVES(KM, APL) __attribute__((amdgpu_waves_per_eu(three_waves<KM, APL>(), three_waves<KM, APL>() == 3 ? 3 : 8)))
+// the dense LDS tile (up to ~40 KB per wave) leaves room for few waves per CU: the compiler's own register allocation
+#define ADOCK_WAVES_D(KM, APL, DENSE)                                                                                  \
+    __attribute__((amdgpu_waves_per_eu(DENSE ? 1 : three_waves<KM, APL>(),                                            \
+                                       (!DENSE && three_waves<KM, APL>() == 3) ? 3 : 8)))
 
 // BFGS stops once |g| < 1e-5 (the reference's sqrt(g.g) >= 1e-5 test): sqrtf is correctly rounded and monotonic, so
 // the test equals g.g >= the smallest float whose square root rounds to at least 1e-5f, 0x2edbe6fd (~9.9999987e-11;
@@ -1126,14 +1130,19 @@ __global__ __launch_bounds__(256) void mc_spec_kernel(LigArgs L, GridArgs G, SFA
 // queue is clamped to [0, n) and a wave leaves a queue for good once it is drained, so the loop ends. Per complex it
 // counts finished chains and stamps the 100 MHz real-time clock when the last one ends (times[0] = earliest wave
 // start).
-template <int KM, int APL, bool EX>
-__global__ __launch_bounds__(256) ADOCK_WAVES(KM, APL) void mc_persistent_kernel(LigArgs L, GridArgs G, SFArgs S, RecArgs R,
+// DENSE: the pair gradients of each wave in the dense LDS tile (per-atom rows of DEG_MAX float4 entries, one atom per
+// lane) instead of one scalar slot per pair with the per-atom lists read from memory; the same bits either way, so
+// the layout is a launch-time choice (more LDS per wave, fewer instructions per evaluation).
+template <int KM, int APL, bool EX, bool DENSE = false>
+__global__ __launch_bounds__(256) ADOCK_WAVES_D(KM, APL, DENSE) void mc_persistent_kernel(LigArgs L, GridArgs G, SFArgs S, RecArgs R,
                                                              const int* __restrict__ chain_lig, int n_chains, McArgs M,
                                                              QueueArgs Q) {
+    static_assert(!DENSE || (APL == 1 && !EX), "the dense tile holds one atom per lane, grid mode");
     extern __shared__ __attribute__((aligned(16))) char lds[];
     const int lane = lane_id();
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
-    const WaveLDS<APL> w = carve_lds<KM, APL>(lds + wid * wave_lds_bytes(KM, L.P_MAX, APL), L.P_MAX);
+    const int wb = DENSE ? wave_lds_bytes_dense(KM, L.P_MAX, L.DEG_MAX) : wave_lds_bytes(KM, L.P_MAX, APL);
+    const WaveLDS<APL> w = carve_lds<KM, APL, DENSE>(lds + wid * wb, L.P_MAX, L.DEG_MAX);
     if (Q.times && lane == 0) atomicMin(Q.times, (unsigned long long)__builtin_amdgcn_s_memrealtime());
     const int nq = Q.qstart && Q.nq > 0 ? Q.nq : 1;
     int q = (int)(blockIdx.x % (unsigned)nq);
@@ -1150,7 +1159,7 @@ __global__ __launch_bounds__(256) ADOCK_WAVES(KM, APL) void mc_persistent_kernel
         }
         const int pos = qs + idx;
         const int ch = Q.order ? __builtin_amdgcn_readfirstlane(Q.order[pos]) : pos;
-        const int l = run_chain<KM, APL, 1, false, EX>(ch, lane, w, L, G, S, R, chain_lig, M);
+        const int l = run_chain<KM, APL, 1, DENSE, EX>(ch, lane, w, L, G, S, R, chain_lig, M);
         if (Q.done && lane == 0 && l < Q.n_done) {
             const int old = atomicAdd(Q.done + l, 1);
             if (old + 1 == Q.lig_chains[l] && Q.times) Q.times[1 + l] = __builtin_amdgcn_s_memrealtime();
@@ -1333,15 +1342,20 @@ int launch_mc_tx(const LigArgs& L, const GridArgs& G, const SFArgs& S, const Rec
     return (int)hipGetLastError();
 }
 
-template <int KM, int APL, bool EX>
+template <int KM, int APL, bool EX, bool DENSE = false>
 int launch_mc_persistent_tx(const LigArgs& L, const GridArgs& G, const SFArgs& S, const RecArgs& R,
                             const int* chain_lig, int n_chains, const McArgs& M, const QueueArgs& Q, int blocks,
                             hipStream_t stream) {
-    const int wave_bytes = wave_lds_bytes(KM, L.P_MAX, APL);
+    if constexpr (APL == 1 && !EX && !DENSE) {
+        if (Q.dense) return launch_mc_persistent_tx<KM, 1, false, true>(L, G, S, R, chain_lig, n_chains, M, Q, blocks,
+                                                                        stream);
+    }
+    const int wave_bytes = DENSE ? wave_lds_bytes_dense(KM, L.P_MAX, L.DEG_MAX) : wave_lds_bytes(KM, L.P_MAX, APL);
     const int req = Q.waves > 0 ? Q.waves : g_waves_per_block.load(std::memory_order_relaxed);
-    int W = fit_waves(req > 0 ? req : 4, KM, L.P_MAX, 0, APL);
+    int W = req > 0 ? req : 4;
+    while (W > 1 && (size_t)W * wave_bytes > (size_t)LDS_PER_WORKGROUP) --W;
     if (req <= 0)
-        W = packed_waves(reinterpret_cast<const void*>(mc_persistent_kernel<KM, APL, EX>), W, wave_bytes);
+        W = packed_waves(reinterpret_cast<const void*>(mc_persistent_kernel<KM, APL, EX, DENSE>), W, wave_bytes);
     const size_t shm = (size_t)W * wave_bytes;
     if (blocks <= 0) {
         // resident capacity, queried once per (device, kernel, W, LDS size) so later launches (and graph captures)
@@ -1350,22 +1364,22 @@ int launch_mc_persistent_tx(const LigArgs& L, const GridArgs& G, const SFArgs& S
         static std::map<long long, int> cap;
         int dev = 0;
         (void)hipGetDevice(&dev);
-        const long long key = ((long long)dev << 40) | ((long long)W << 32) | (long long)shm;
+        const long long key = ((long long)dev << 41) | ((long long)DENSE << 40) | ((long long)W << 32) | (long long)shm;
         std::lock_guard<std::mutex> lock(mu);
         auto it = cap.find(key);
         if (it == cap.end()) {
             int cus = 0, per_cu = 0;
             (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, mc_persistent_kernel<KM, APL, EX>, W * WAVE,
-                                                               shm);
+            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, mc_persistent_kernel<KM, APL, EX, DENSE>,
+                                                               W * WAVE, shm);
             it = cap.emplace(key, cus * (per_cu > 0 ? per_cu : 1)).first;
         }
         blocks = it->second;
     }
     const int need = (n_chains + W - 1) / W;
     if (blocks > need) blocks = need;
-    allow_lds(reinterpret_cast<const void*>(mc_persistent_kernel<KM, APL, EX>), shm);
-    hipLaunchKernelGGL((mc_persistent_kernel<KM, APL, EX>), dim3(blocks), dim3(W * WAVE), shm, stream, L, G, S, R,
+    allow_lds(reinterpret_cast<const void*>(mc_persistent_kernel<KM, APL, EX, DENSE>), shm);
+    hipLaunchKernelGGL((mc_persistent_kernel<KM, APL, EX, DENSE>), dim3(blocks), dim3(W * WAVE), shm, stream, L, G, S, R,
                        chain_lig, n_chains, M, Q);
     return (int)hipGetLastError();
 }

@@ -34,6 +34,8 @@ struct QueueArgs {
     unsigned long long* times;
     int n_done = 0x7fffffff;   // entries of done / lig_chains (times: 1 + n_done); complexes at or past it are not counted
     int waves = 0;             // chains (wavefronts) per workgroup of this launch; 0: the process default
+    int dense = 0;             // pair-gradient LDS tile: 0 compact (a slot per pair), 1 dense rows (one atom per lane,
+                               // grid mode; other launches keep the compact one)
 };
 // learned-proposal MC step: Y [N,23] network output (NULL: plain moves), feat [N,128] features (in: at the current
 // pose; out: at the new current pose; NULL: not computed), log_buf [cap, 128+23+23+1] accepted-move records.

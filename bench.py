@@ -84,6 +84,7 @@ def multi_headline(args):
     N = len(cl_h)
     ci = torch.arange(N, dtype=torch.int32, device=dev)
     runner = BatchRunner(kp, cl, n_complexes=L)
+    runner.lds_layout = args.lds_layout
     cur, best = st0.clone(), st0.clone()
     ce, be = torch.zeros(N, device=dev), torch.full((N,), float("inf"), device=dev)
     seed64 = 0 * 1_000_003 + 12345
@@ -159,7 +160,8 @@ def multi_headline(args):
                                f"builds + {L} complexes in one persistent launch (2 HIP graphs)",
                    "receptors": n_rec, "complexes": L, "chains": N, "mc_steps": steps,
                    "grid_types_per_receptor": [int(x) for x in np.unique((gs.slot >= 0).sum(1).cpu().numpy())[[0, -1]]],
-                   "grid_bytes": int(gs.flat.numel() * 4), "KM": kp.KM},
+                   "grid_bytes": int(gs.flat.numel() * 4), "KM": kp.KM,
+                   "lds_layout": "dense" if args.lds_layout else "compact"},
         "score_grad_evals_per_s": n_eval / dt,
         "evals_per_step": n_eval / args.steps,
         "grid_build_and_reset_ms": prep_ms, "search_kernel_ms": search_ms,
@@ -451,6 +453,8 @@ def main():
                     help="regression weights of logged moves (--workload learned, move target)")
     ap.add_argument("--p-plain", type=float, default=0.5,
                     help="share of random (plain) moves in the learned search (--workload learned)")
+    ap.add_argument("--lds-layout", type=int, choices=[0, 1], default=0,
+                    help="multi: pair-gradient LDS tile of the persistent launch (0 compact slots, 1 dense rows)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="after the timed steps, one device-to-device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()

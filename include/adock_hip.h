@@ -148,9 +148,11 @@ int adock_mc_counted(int AM, int KM,
  * counter, done and times must be reset before every launch (adock_batch_reset: a kernel, so the pair can be captured
  * in a HIP graph). eval_count may be NULL. lig_evals [L] (device, may be NULL) receives per complex the number of
  * energy+gradient evaluations its chains performed (added to what it holds). waves_per_block (0..4) sets the chains
- * (wavefronts) per workgroup of this launch only; 0 takes the process default of adock_set_chains_per_block. No
- * reference counterpart: the reference launches one block per NW chains and compacts between segments on the host
- * (src/adock/search.py:614-636). */
+ * (wavefronts) per workgroup of this launch only; 0 takes the process default of adock_set_chains_per_block.
+ * lds_layout picks the wave's pair-gradient LDS tile: 0 compact (one slot per intra pair, per-atom lists read from
+ * memory), 1 dense (per-atom rows of DEG_MAX entries; used in grid mode with AM 64, otherwise compact); results are
+ * the same bit for bit, only speed and occupancy differ. No reference counterpart: the reference launches one block
+ * per NW chains and compacts between segments on the host (src/adock/search.py:614-636). */
 int adock_mc_batch(int AM, int KM,
                    const float* ref, const int* typ, const unsigned char* tm, const int* ax, const float* axp,
                    const float* axu, const int* pr, const int* na, const int* nt, const int* np, const float* rg,
@@ -165,7 +167,8 @@ int adock_mc_batch(int AM, int KM,
                    int bfgs_iters, float temp, float amp_pos, float amp_rot, unsigned long long seed,
                    int step_offset, const int* order, const int* queue_start, int n_queues, int* counter,
                    int* done, const int* lig_chains, unsigned long long* times, int blocks, int waves_per_block,
-                   unsigned long long* eval_count, unsigned long long* lig_evals, adock_stream_t stream);
+                   int lds_layout, unsigned long long* eval_count, unsigned long long* lig_evals,
+                   adock_stream_t stream);
 
 /* Resets the work queues of adock_mc_batch with a kernel launch: counter[0..n_counters) = 0, done[0..L) = 0,
  * times[0] = ~0, times[1..L] = 0 (done / times may be NULL). n_counters in 1..1024. */

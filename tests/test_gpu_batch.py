@@ -78,8 +78,9 @@ def test_autotuner_picks_a_shape(small_screen):
     w, kp, st, cl = small_screen
     r = BatchRunner(kp, cl, n_complexes=len(w["ligs"]))
     from adock_amd import _lib
-    W, blocks = Autotuner().tune(r, st, probe_steps=1, reps=1)
+    W, blocks, layout = Autotuner().tune(r, st, probe_steps=1, reps=1)
     assert W in (1, 2, 4) and r.waves == W
+    assert layout in (0, 1) and r.lds_layout == layout
     assert _lib.lib().adock_get_chains_per_block() == 0     # the shape is the runner's, not the process default
     ref = _run_plain(kp, st, cl, 2)
     N = len(cl)
@@ -88,6 +89,27 @@ def test_autotuner_picks_a_shape(small_screen):
     r.launch(torch.arange(N, dtype=torch.int32).cuda(), cur, ce, best, be, True, 2, 0, 1.2, 2.0, 2.0, 55, 0)
     torch.cuda.synchronize()
     np.testing.assert_array_equal(ref[3], be.cpu().numpy())
+
+
+@pytest.mark.parametrize("waves", [4, 1])
+def test_dense_lds_tile_equals_compact(small_screen, waves):
+    """The persistent launch with the dense pair-gradient LDS tile gives the compact tile's results bit for bit."""
+    print(f"[gpu] persistent launch: dense vs compact LDS tile, {waves} chains per workgroup", flush=True)
+    w, kp, st, cl = small_screen
+    N = len(cl)
+    out = []
+    for layout in (0, 1):
+        r = BatchRunner(kp, cl, n_complexes=len(w["ligs"]), queues=3)
+        r.waves, r.lds_layout = waves, layout
+        cur, best = st.clone(), st.clone()
+        ce, be = torch.zeros(N).cuda(), torch.full((N,), float("inf")).cuda()
+        r.launch(torch.arange(N, dtype=torch.int32).cuda(), cur, ce, best, be, True, 4, 0, 1.2, 2.0, 2.0, 91, 0,
+                 count_evals=True)
+        torch.cuda.synchronize()
+        out.append([x.cpu().numpy() for x in (cur, ce, best, be, r.lig_evals)])
+        assert (r.done.cpu().numpy() == w["chains"]).all()
+    for a, b in zip(*out):
+        np.testing.assert_array_equal(a, b)
 
 
 def test_run_mc_routes_many_chains_through_queues():
