@@ -2,10 +2,10 @@
 //
 // Inference (mlp_infer_kernel): BF16 weights and activations, FP32 accumulation, gfx950's v_mfma_f32_16x16x32_bf16
 // (twice the depth of the CDNA3 16x16x16 form: half the MFMAs per layer). One workgroup (4 wavefronts) owns a 64-row
-// tile of the batch: the features are converted to BF16 into LDS, each wave computes a 16-row strip of every layer (A
-// fragments from LDS, B fragments straight from the L2-resident weights, stored [out, in] so each lane's 8 consecutive
-// K values are one 16-byte load), bias + ReLU is applied on the accumulators and the BF16 activations go back to LDS
-// for the next layer. Fragment layouts of the 16x16x32 BF16 MFMA: A[i][k] on lane i + 16 (k/8), element k%8;
+// tile of the batch: the features are converted to BF16 into LDS, each wave computes a quarter of every layer's
+// output tiles for all 64 rows (A fragments from LDS, B fragments straight from the L2-resident weights, stored
+// [out, in] so each lane's 8 consecutive K values are one 16-byte load, each fetched once per workgroup), bias + ReLU
+// is applied on the accumulators and the BF16 activations go back to LDS for the next layer. Fragment layouts of the 16x16x32 BF16 MFMA: A[i][k] on lane i + 16 (k/8), element k%8;
 // B[k][j] on lane j + 16 (k/8), element k%8; D[i][j] on lane j + 16 (i/4), element i%4.
 //
 // Training: FP32 tiled GEMM (64x64 tiles, 16-deep K slices through LDS, 4x4 outputs per thread) with either operand
@@ -19,40 +19,64 @@ namespace {
 constexpr int IN = NET_IN, HID = NET_HID, OUT = NET_OUT, OUTP = NET_OUTP;
 constexpr int ROWS = 64;
 
-// One layer for this wave's 16-row strip: out[16 x N] = act(in[16 x K] * W^T + b), W [N, K] bf16 (row = output).
-// in/out are LDS bf16 row-major [ROWS][K] / [ROWS][N]; rows r0..r0+15. N multiple of 16.
+// LDS element of (row, col) in a BF16 activation tile with K columns: each row's 16-byte chunks are XOR-swizzled by
+// the row (chunk c of row i at c ^ (i % 16)), so the 16 rows one A-fragment read touches (same K offset, row pitch a
+// multiple of 256 B) land on 16 different 4-bank groups instead of one. K / 8 >= 16 chunks.
+DEVI int swz(int row, int col, int K) { return row * K + ((((col >> 3) ^ (row & 15)) << 3) | (col & 7)); }
+
+// One layer for the workgroup's 64-row tile: out[64 x N] = act(in[64 x K] * W^T + b), W [N, K] bf16 (row = output),
+// in / out LDS bf16 [ROWS][K] / [ROWS][N] (swizzled). The N / 16 output tiles are dealt over the 4 waves (tile t to
+// wave t % 4) and each wave computes its tiles for all four 16-row strips, so every B fragment (16 outputs x 32 K,
+// one 16-byte load per lane from the L2-resident weights) is fetched once per workgroup and feeds 4 MFMAs. Each
+// output is the same K-ordered MFMA chain as in a one-strip-per-wave mapping (the in-loop inference of the chain
+// kernels relies on that: the same bits).
 template <int K, int N, bool RELU>
-DEVI void layer_strip(const unsigned short* __restrict__ in, unsigned short* out, float* outf,
-                      const unsigned short* __restrict__ W, const float* __restrict__ b, int r0, int lane, int nvalid) {
-    constexpr int NT = N / 16;
-    float4_t acc[NT];
+DEVI void layer_tiles(const unsigned short* __restrict__ in, unsigned short* out, float* outf,
+                      const unsigned short* __restrict__ W, const float* __restrict__ b, int wave, int lane,
+                      int nvalid) {
+    constexpr int NT = N / 16, TPW = (NT + 3) / 4;
+    static_assert(K % 32 == 0 && K / 8 >= 16, "layer depth: a multiple of the MFMA's 32, at least 16 chunks");
+    float4_t acc[TPW][4];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) acc[t] = float4_t{0.f, 0.f, 0.f, 0.f};
+    for (int u = 0; u < TPW; ++u)
+#pragma unroll
+        for (int st = 0; st < 4; ++st) acc[u][st] = float4_t{0.f, 0.f, 0.f, 0.f};
     const int li = lane & 15, lk = (lane >> 4) * 8;
-    static_assert(K % 32 == 0, "layer depth must be a multiple of the MFMA's 32");
 #pragma unroll 2
     for (int k0 = 0; k0 < K; k0 += 32) {
-        const bf16x8_t a = *reinterpret_cast<const bf16x8_t*>(in + (r0 + li) * K + k0 + lk);
+        bf16x8_t a[4];
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            const int j = t * 16 + li;
-            bf16x8_t bv = {};
-            if (j < nvalid) bv = *reinterpret_cast<const bf16x8_t*>(W + (long long)j * K + k0 + lk);
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv, acc[t], 0, 0, 0);
+        for (int st = 0; st < 4; ++st)
+            a[st] = *reinterpret_cast<const bf16x8_t*>(in + swz(st * 16 + li, k0 + lk, K));
+#pragma unroll
+        for (int u = 0; u < TPW; ++u) {
+            const int t = wave + 4 * u;
+            if (t < NT) {
+                const int j = t * 16 + li;
+                bf16x8_t bv = {};
+                if (j < nvalid) bv = *reinterpret_cast<const bf16x8_t*>(W + (long long)j * K + k0 + lk);
+#pragma unroll
+                for (int st = 0; st < 4; ++st)
+                    acc[u][st] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[st], bv, acc[u][st], 0, 0, 0);
+            }
         }
     }
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
+    for (int u = 0; u < TPW; ++u) {
+        const int t = wave + 4 * u;
+        if (t >= NT) continue;
         const int j = t * 16 + li;
         const float bj = j < nvalid ? b[j] : 0.f;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int i = r0 + (lane >> 4) * 4 + r;
-            float v = acc[t][r] + bj;
-            if (RELU) v = fmaxf(v, 0.f);
-            if (out) out[i * N + j] = f2bf(v);
-            if (outf && j < nvalid) outf[i * OUTP + j] = v;
-        }
+        for (int st = 0; st < 4; ++st)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = st * 16 + (lane >> 4) * 4 + r;
+                float v = acc[u][st][r] + bj;
+                if (RELU) v = fmaxf(v, 0.f);
+                if (out) out[swz(i, j, N)] = f2bf(v);
+                if (outf && j < nvalid) outf[i * OUTP + j] = v;
+            }
     }
 }
 
@@ -68,15 +92,14 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(int B, const float* __re
     const long long row0 = (long long)blockIdx.x * ROWS;
     for (int e = tid; e < ROWS * IN; e += 256) {
         const long long r = row0 + e / IN;
-        s_a[e] = f2bf(r < B ? X[r * IN + e % IN] : 0.f);
+        s_a[swz(e / IN, e % IN, IN)] = f2bf(r < B ? X[r * IN + e % IN] : 0.f);
     }
     __syncthreads();
-    const int r0 = wave * 16;
-    layer_strip<IN, HID, true>(s_a, s_b, nullptr, W1, b1, r0, lane, HID);
+    layer_tiles<IN, HID, true>(s_a, s_b, nullptr, W1, b1, wave, lane, HID);
     __syncthreads();
-    layer_strip<HID, HID, true>(s_b, s_a, nullptr, W2, b2, r0, lane, HID);
+    layer_tiles<HID, HID, true>(s_b, s_a, nullptr, W2, b2, wave, lane, HID);
     __syncthreads();
-    layer_strip<HID, OUTP, false>(s_a, nullptr, s_y, W3, b3, r0, lane, OUT);
+    layer_tiles<HID, OUTP, false>(s_a, nullptr, s_y, W3, b3, wave, lane, OUT);
     __syncthreads();
     for (int e = tid; e < ROWS * OUT; e += 256) {
         const long long r = row0 + e / OUT;
