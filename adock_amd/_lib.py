@@ -49,10 +49,11 @@ def lib():
         L.adock_batch_reset.argtypes = [P, I, P, I, P, P]
         LL = ctypes.c_longlong
         L.adock_mc_step.argtypes = (LIG + GRID + SF + REC +
-                                    [P, P, I, P, P, P, P, I, I, I, F, F, F, U64, P, P, P, P, I, I, F, F, F, F, P, P])
+                                    [P, P, I, P, P, P, P, I, I, I, F, F, F, U64, P, P, P, P, I, I, F, F, F, F, I, F, I,
+                                     P, P])
         L.adock_mc_learned.argtypes = (LIG + GRID + SF + REC +
                                        [P, P, I, P, P, P, P, I, I, I, I, F, F, F, U64] + [P] * 6 +
-                                       [P, P, P, I, I, F, F, F, F, P, P, P, P])
+                                       [P, P, P, I, I, F, F, F, F, I, F, I, P, P, P, P])
         L.adock_mlp_infer.argtypes = [I, P, P, P, P, P, P, P, P, P]
         L.adock_gemm_f32.argtypes = [I, I, I, I, I, F, P, I, P, I, F, P, I, P]
         L.adock_gemm_f32_splitk.argtypes = [I, I, I, I, I, F, P, I, P, I, F, P, I, I, P, P]

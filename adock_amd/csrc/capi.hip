@@ -295,7 +295,8 @@ int adock_mc_step(int AM, int KM, LIG_DECL, GRID_DECL, SF_DECL, REC_DECL, const 
                   int n_chains, float* cur, float* cur_e, float* best, float* best_e, int init, int step, int bfgs_iters,
                   float temp, float amp_pos, float amp_rot, unsigned long long seed, const float* Y, float* feat,
                   float* log_buf, int* log_count, int log_cap, int use_net, float p_plain, float sig_t, float sig_r,
-                  float sig_th, unsigned long long* eval_count, adock_stream_t stream) {
+                  float sig_th, int net_mode, float tau, int log_all, unsigned long long* eval_count,
+                  adock_stream_t stream) {
     if (int st = common_checks(AM, KM, P_MAX, ref, inc, ist, DEG_MAX, grid, prm, 0, rxyz, rn, RN_MAX, chain_lig,
                                n_chains))
         return st;
@@ -310,7 +311,10 @@ int adock_mc_step(int AM, int KM, LIG_DECL, GRID_DECL, SF_DECL, REC_DECL, const 
     RecArgs R{rxyz, rtyp, rn, RN_MAX};
     McArgs M{chain_id, nullptr, nullptr, 0, cur, cur_e, best, best_e, init, 1, bfgs_iters, temp, amp_pos, amp_rot, seed,
              step, eval_count};
+    if (net_mode != 0 && net_mode != 1) return fail(ADOCK_E_ARG, "net_mode must be 0 (displacement) or 1 (entity)");
+    if (net_mode == 1 && !(tau > 0.f)) return fail(ADOCK_E_ARG, "entity choice needs tau > 0");
     StepArgs P{Y, feat, log_buf, log_count, log_cap, use_net, p_plain, sig_t, sig_r, sig_th, init};
+    P.net_mode = net_mode; P.tau = tau; P.log_all = log_all;
     return hip_status(launch_mc_step(KM, L, G, S, R, chain_lig, n_chains, M, P, (hipStream_t)stream));
 }
 
@@ -320,7 +324,8 @@ int adock_mc_learned(int AM, int KM, LIG_DECL, GRID_DECL, SF_DECL, REC_DECL, con
                      unsigned long long seed, const unsigned short* W1, const float* b1, const unsigned short* W2,
                      const float* b2, const unsigned short* W3, const float* b3, float* feat, float* log_buf,
                      int* log_count, int log_cap, int use_net, float p_plain, float sig_t, float sig_r, float sig_th,
-                     const int* order, int* counter, unsigned long long* eval_count, adock_stream_t stream) {
+                     int net_mode, float tau, int log_all, const int* order, int* counter,
+                     unsigned long long* eval_count, adock_stream_t stream) {
     if (int st = common_checks(AM, KM, P_MAX, ref, inc, ist, DEG_MAX, grid, prm, 0, rxyz, rn, RN_MAX, chain_lig,
                                n_chains))
         return st;
@@ -337,7 +342,10 @@ int adock_mc_learned(int AM, int KM, LIG_DECL, GRID_DECL, SF_DECL, REC_DECL, con
     RecArgs R{rxyz, rtyp, rn, RN_MAX};
     McArgs M{chain_id, nullptr, nullptr, 0, cur, cur_e, best, best_e, init_opt, mc_steps, bfgs_iters, temp, amp_pos,
              amp_rot, seed, step_offset, eval_count};
+    if (net_mode != 0 && net_mode != 1) return fail(ADOCK_E_ARG, "net_mode must be 0 (displacement) or 1 (entity)");
+    if (net_mode == 1 && !(tau > 0.f)) return fail(ADOCK_E_ARG, "entity choice needs tau > 0");
     StepArgs P{nullptr, feat, log_buf, log_count, log_cap, use_net, p_plain, sig_t, sig_r, sig_th, 0};
+    P.net_mode = net_mode; P.tau = tau; P.log_all = log_all;
     NetArgs N{W1, b1, W2, b2, W3, b3};
     QueueArgs Q{order, nullptr, 1, counter, nullptr, nullptr, nullptr};
     return hip_status(launch_mc_learned(KM, L, G, S, R, chain_lig, n_chains, M, P, N, Q, (hipStream_t)stream));

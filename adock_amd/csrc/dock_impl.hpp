@@ -762,11 +762,15 @@ DEVI float conf_inc(float x, float p, float alpha, int lane) {
 
 // The reference's MC mutation (one entity: translation by amp_pos * ball, rotation by amp_rot / max(rg, 1) * ball, or
 // one torsion set uniformly in [-pi, pi)), drawn from the counter RNG slots 0, 1 and 10.. of (chain, step).
+// entity the plain move perturbs: 0 translation, 1 rotation, 2 + k torsion k (uniform, slot 0)
 template <class Ctx>
-DEVI float plain_move(float x, const Ctx& c, int lane, unsigned long long seed, int cid, int sg, float amp_pos,
-                      float amp_rot) {
+DEVI int plain_entity(const Ctx& c, unsigned long long seed, int cid, int sg) {
     const int nm = 2 + c.nt;
-    const int which = min((int)(urand(seed, cid, sg, 0) * (float)nm), nm - 1);
+    return min((int)(urand(seed, cid, sg, 0) * (float)nm), nm - 1);
+}
+template <class Ctx>
+DEVI float entity_move(float x, const Ctx& c, int lane, unsigned long long seed, int cid, int sg, int which,
+                       float amp_pos, float amp_rot) {
     if (which == 0) {
         float v0, v1, v2;
         rand_ball(seed, cid, sg, 10, v0, v1, v2);
@@ -784,6 +788,11 @@ DEVI float plain_move(float x, const Ctx& c, int lane, unsigned long long seed, 
         if (lane == 7 + which - 2) x = th;
     }
     return x;
+}
+template <class Ctx>
+DEVI float plain_move(float x, const Ctx& c, int lane, unsigned long long seed, int cid, int sg, float amp_pos,
+                      float amp_rot) {
+    return entity_move(x, c, lane, seed, cid, sg, plain_entity(c, seed, cid, sg), amp_pos, amp_rot);
 }
 
 // Register budget of the throughput chain kernels (per-chain, persistent, learned step): at most 168 VGPRs, so three
@@ -1425,13 +1434,15 @@ int mc_resident_waves_t(int P_MAX, int persistent) {
     return per_cu * W;
 }
 
-constexpr int NFEAT = NET_IN, NOUT = NET_OUT, LOG_W = NFEAT + 2 * NOUT + 1;
+// move record: features before the move [128], target [23], mask [23], energy gain, entity moved (-1: a network
+// displacement move)
+constexpr int NFEAT = NET_IN, NOUT = NET_OUT, LOG_W = NFEAT + 2 * NOUT + 2;
 // Counter-RNG slots of the learned step. The plain move uses slots 0 (entity), 1 (torsion angle), 2 (acceptance) and
 // 10 .. 105 (unit-ball tries 10 + 3k, k < 32); the step counter sits 8 bits up, so every slot must stay below 256 or
 // it aliases a slot of another step. The network move draws its coin at SLOT_NET_PLAIN and one normal per lane from
 // the pair SLOT_NET_NOISE + 2 lane (+ 1): all disjoint from the plain move's slots and below 256.
-constexpr int SLOT_NET_PLAIN = 108, SLOT_NET_NOISE = 110;
-static_assert(SLOT_NET_PLAIN > 10 + 3 * 31 + 2 && SLOT_NET_PLAIN < SLOT_NET_NOISE, "net coin overlaps the plain move");
+constexpr int SLOT_NET_PLAIN = 108, SLOT_NET_ENTITY = 109, SLOT_NET_NOISE = 110;
+static_assert(SLOT_NET_PLAIN > 10 + 3 * 31 + 2 && SLOT_NET_ENTITY < SLOT_NET_NOISE, "net coins overlap the plain move");
 static_assert(SLOT_NET_NOISE + 2 * (WAVE - 1) + 1 < 256, "net noise slots alias the next step's counter");
 
 
@@ -1529,6 +1540,32 @@ DEVI float net_move(float x, float y, const Ctx& c, int lane, unsigned long long
     return out;
 }
 
+// Entity-choice network move (net_mode 1): the network's outputs score the entities (translation: output 0,
+// rotation: output 3, torsion k < 16: output 7 + k, later torsions 0), one is drawn with probability proportional to
+// exp(score / tau) (slot SLOT_NET_ENTITY), and the reference's perturbation of that entity is applied with its own
+// slots. The scores are the network's estimates of each entity's expected energy gain.
+template <class Ctx>
+DEVI int net_entity(float y, const Ctx& c, int lane, unsigned long long seed, int cid, int sg, float tau) {
+    const int ne = 2 + c.nt;
+    const int src = lane == 0 ? 0 : lane == 1 ? 3 : lane - 2 + 7;
+    const float sc = __shfl(y, src < 64 ? src : 0);
+    const bool valid = lane < ne;
+    const float scv = (lane >= 2 && lane - 2 >= 16) ? 0.f : sc;
+    const float mx = wave_max_f(valid ? scv : -INFINITY);
+    const float w = valid ? expf((scv - mx) / fmaxf(tau, 1e-6f)) : 0.f;
+    const float total = wave_sum(w);
+    // inclusive prefix of the weights over the entity lanes, then the first entity whose prefix passes u * total
+    float pre = w;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const float o = __shfl_up(pre, d);
+        if (lane >= d) pre += o;
+    }
+    const float u = urand(seed, cid, sg, SLOT_NET_ENTITY) * total;
+    const unsigned long long hit = __ballot(valid && pre > u);
+    return hit ? __builtin_ctzll(hit) : ne - 1;
+}
+
 // target of an accepted move cur -> x: translation delta, rotation dq = q_x conj(q_cur) (w >= 0), wrapped torsion
 // deltas; mask 1 on the components that exist
 template <class Ctx>
@@ -1553,6 +1590,37 @@ DEVI void move_target(float cur, float x, const Ctx& c, int lane, float& t, floa
         d = d - 6.28318531f * rintf(d * 0.159154943f);
         t = d; m = 1.f;
     }
+}
+
+// One move record at *log_count (atomic; records past log_cap are dropped): the features before the move (fin, 128
+// floats, memory or LDS), the move cur -> x as target and mask, the energy gain ec - en and the entity moved.
+template <class Ctx>
+DEVI void log_move(const StepArgs& P, const float* fin, float cur, float x, const Ctx& c, int lane, float gain,
+                   int ent) {
+    int slot = 0;
+    if (lane == 0) slot = atomicAdd(P.log_count, 1);
+    slot = bcast_i(slot, 0);
+    if (slot >= P.log_cap) return;
+    float* rec = P.log_buf + (long long)slot * LOG_W;
+    rec[lane] = fin[lane];
+    rec[64 + lane] = fin[64 + lane];
+    float t, m;
+    move_target(cur, x, c, lane, t, m);
+    if (lane < NOUT) { rec[NFEAT + lane] = t; rec[NFEAT + NOUT + lane] = m; }
+    if (lane == 0) { rec[NFEAT + 2 * NOUT] = gain; rec[NFEAT + 2 * NOUT + 1] = (float)ent; }
+}
+
+// the move of one step: the reference's random mutation (plain), the network's displacement (net_mode 0) or the
+// reference's mutation of the entity the network picks (net_mode 1); ent = the entity moved, -1 for a displacement
+template <class Ctx>
+DEVI float step_move(float cur, bool plain, float y, const Ctx& c, int lane, unsigned long long seed, int cid, int sg,
+                     const McArgs& M, const StepArgs& P, int& ent) {
+    if (plain || P.net_mode == 1) {
+        ent = plain ? plain_entity(c, seed, cid, sg) : net_entity(y, c, lane, seed, cid, sg, P.tau);
+        return entity_move(cur, c, lane, seed, cid, sg, ent, M.amp_pos, M.amp_rot);
+    }
+    ent = -1;
+    return net_move(cur, y, c, lane, seed, cid, sg, P.sig_t, P.sig_r, P.sig_th);
 }
 
 // One MC step of every chain (launched once per step): optional features at the current pose (feat), a move from the
@@ -1597,26 +1665,13 @@ __global__ __launch_bounds__(256) ADOCK_WAVES(KM, 1) void mc_step_kernel(LigArgs
             y = lane < NOUT ? P.Y[(long long)ch * NOUT + lane] : 0.f;
             plain = wave_count(!isfinite(y)) > 0;
         }
-        float x = plain ? plain_move(cur, c, lane, seed, cid, sg, M.amp_pos, M.amp_rot)
-                        : net_move(cur, y, c, lane, seed, cid, sg, P.sig_t, P.sig_r, P.sig_th);
+        int ent = -1;
+        float x = step_move(cur, plain, y, c, lane, seed, cid, sg, M, P, ent);
         const float en = bfgs<KM, 1>(c, sf, w, lane, x, iters, nev);
         const bool accept = (en < ec) || (urand(seed, cid, sg, 2) < expf((ec - en) / M.temp));
+        if ((accept || P.log_all) && P.log_buf && P.feat)
+            log_move(P, P.feat + (long long)ch * NFEAT, cur, x, c, lane, ec - en, ent);
         if (accept) {
-            if (P.log_buf && P.feat) {
-                int slot = 0;
-                if (lane == 0) slot = atomicAdd(P.log_count, 1);
-                slot = bcast_i(slot, 0);
-                if (slot < P.log_cap) {
-                    float* rec = P.log_buf + (long long)slot * LOG_W;
-                    const float* fin = P.feat + (long long)ch * NFEAT;
-                    rec[lane] = fin[lane];
-                    rec[64 + lane] = fin[64 + lane];
-                    float t, m;
-                    move_target(cur, x, c, lane, t, m);
-                    if (lane < NOUT) { rec[NFEAT + lane] = t; rec[NFEAT + NOUT + lane] = m; }
-                    if (lane == 0) rec[NFEAT + 2 * NOUT] = ec - en;
-                }
-            }
             ec = en; cur = x;
             if (en < eb) { eb = en; best = x; }
         }
@@ -1746,20 +1801,25 @@ DEVI void learned_chain(int ch, int lane, const WaveLDS<1>& w, const NetLDS& nl,
     // of the current pose when no kept evaluation has them (fresh: BFGS with no iterations evaluates cur once), so the
     // chain body holds one copy of the evaluation code.
     bool fresh = feat_on && !M.init_opt;
+    // the network's output at the current pose: a rejected move leaves the pose and its features as they were, so
+    // the next network step reuses it instead of running the network again (the same values)
+    float ynet = 0.f;
+    bool ynet_ok = false;
     if (!M.init_opt && ec < eb) { eb = ec; best = cur; }
     for (int st = M.init_opt ? -1 : 0; st < M.mc_steps || fresh;) {
         const int sg = M.step_offset + (st < 0 ? 0 : st);
         float x = cur;
         bool plain = true;
+        int ent = -1;
         if (!fresh && st >= 0) {
             plain = !use_net || urand(seed, cid, sg, SLOT_NET_PLAIN) < P.p_plain;
             float y = 0.f;
             if (!plain) {
-                y = net_forward(nl, N, lane);
+                if (!ynet_ok) { ynet = net_forward(nl, N, lane); ynet_ok = true; }
+                y = ynet;
                 plain = wave_count(!isfinite(y)) > 0;
             }
-            x = plain ? plain_move(cur, c, lane, seed, cid, sg, M.amp_pos, M.amp_rot)
-                      : net_move(cur, y, c, lane, seed, cid, sg, P.sig_t, P.sig_r, P.sig_th);
+            x = step_move(cur, plain, y, c, lane, seed, cid, sg, M, P, ent);
         }
         LastEval le;
         const float en = bfgs<KM, 1, false, false, true>(c, sf, w, lane, x, fresh ? 0 : iters, nev, &le);
@@ -1774,27 +1834,15 @@ DEVI void learned_chain(int ch, int lane, const WaveLDS<1>& w, const NetLDS& nl,
             ++st;
         } else {
             const bool accept = (en < ec) || (urand(seed, cid, sg, 2) < expf((ec - en) / M.temp));
+            if ((accept || P.log_all) && P.log_buf) log_move(P, nl.feat, cur, x, c, lane, ec - en, ent);
             if (accept) {
-                if (P.log_buf) {
-                    int slot = 0;
-                    if (lane == 0) slot = atomicAdd(P.log_count, 1);
-                    slot = bcast_i(slot, 0);
-                    if (slot < P.log_cap) {
-                        float* rec = P.log_buf + (long long)slot * LOG_W;
-                        rec[lane] = nl.feat[lane];
-                        rec[64 + lane] = nl.feat[64 + lane];
-                        float t, m;
-                        move_target(cur, x, c, lane, t, m);
-                        if (lane < NOUT) { rec[NFEAT + lane] = t; rec[NFEAT + NOUT + lane] = m; }
-                        if (lane == 0) rec[NFEAT + 2 * NOUT] = ec - en;
-                    }
-                }
                 ec = en; cur = x;
                 if (en < eb) { eb = en; best = x; }
                 moved = true;
             }
             ++st;
         }
+        if (moved) ynet_ok = false;
         if (moved && feat_on) {
             if (le.valid) {
                 write_features(nl.feat, c, lane, cur, en, le.g, le.ea);

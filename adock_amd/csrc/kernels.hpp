@@ -38,10 +38,13 @@ struct QueueArgs {
                                // grid mode; other launches keep the compact one)
 };
 // learned-proposal MC step: Y [N,23] network output (NULL: plain moves), feat [N,128] features (in: at the current
-// pose; out: at the new current pose; NULL: not computed), log_buf [cap, 128+23+23+1] accepted-move records.
+// pose; out: at the new current pose; NULL: not computed), log_buf [cap, 128+23+23+2] move records (accepted moves,
+// or every move with log_all). net_mode 0: the network's output is a displacement; 1: it scores the entities and the
+// reference's mutation of the drawn one (probability ~ exp(score / tau)) is applied.
 struct StepArgs {
     const float* Y; float* feat; float* log_buf; int* log_count; int log_cap;
     int use_net; float p_plain, sig_t, sig_r, sig_th; int init;
+    int net_mode = 0; float tau = 0.1f; int log_all = 0;
 };
 // proposal network weights for inference inside the chain loop: BF16 [out, in] (W1 [256,128], W2 [256,256],
 // W3 [23,256]) and FP32 biases

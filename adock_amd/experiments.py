@@ -23,7 +23,7 @@ from . import synth, workload
 from .autotune import Autotuner
 from .batch import XCDS, BatchRunner, GraphedBatch
 from .kernel import KernelProblem
-from .learned import StepSearch, jump_dataset, move_weights, train_proposal
+from .learned import StepSearch, entity_dataset, jump_dataset, move_weights, train_proposal
 from .mlp import ProposalNet
 
 
@@ -375,7 +375,11 @@ def learned_cost(n=512, chains=64, steps=60, seed=0, p_plain=0.25, log=print):
                                             lambda: ss.fused(*b, steps, seed=s64, count_evals=True)),
             "learned kernel, network on": (lambda: ss.fused(*b, steps, seed=s64, use_net=True, p_plain=p_plain),
                                            lambda: ss.fused(*b, steps, seed=s64, use_net=True, p_plain=p_plain,
-                                                            count_evals=True))}
+                                                            count_evals=True)),
+            "learned kernel, network on, entity choice": (
+                lambda: ss.fused(*b, steps, seed=s64, use_net=True, p_plain=p_plain, net_mode=1, tau=0.1),
+                lambda: ss.fused(*b, steps, seed=s64, use_net=True, p_plain=p_plain, net_mode=1, tau=0.1,
+                                 count_evals=True))}
     out = {"workload": f"{E} complexes (4,000-atom receptor, 35-atom / 8-torsion ligand) x {chains} chains "
                        f"({E * chains} chains per step), {steps} MC steps + init BFGS", "p_plain": p_plain,
            "ms_per_step": {}, "ns_per_evaluation": {}, "evaluations": {}}
@@ -398,6 +402,9 @@ def learned_cost(n=512, chains=64, steps=60, seed=0, p_plain=0.25, log=print):
     ms, ns = out["ms_per_step"], out["ns_per_evaluation"]
     out["learned_over_plain"] = ms["learned kernel, network on"] / ms["plain (adock_mc)"]
     out["learned_over_plain_per_evaluation"] = ns["learned kernel, network on"] / ns["plain (adock_mc)"]
+    out["entity_over_plain"] = ms["learned kernel, network on, entity choice"] / ms["plain (adock_mc)"]
+    out["entity_over_plain_per_evaluation"] = (ns["learned kernel, network on, entity choice"] /
+                                               ns["plain (adock_mc)"])
     out["step_pipeline_over_plain"] = ms["step pipeline (graph), network on"] / ms["plain (adock_mc)"]
     return out
 
@@ -419,14 +426,15 @@ def _time_to_gap(rows, key, target):
 
 def learned(n_train=512, train_chains=64, train_steps=400, n_eval=512, eval_chains=64,
             budgets=(10, 15, 20, 30, 40, 60, 80, 120, 160, 240, 320, 480), full_steps=1000, epochs=5, p_plain=0.5,
-            gap_target=0.1, seed=0, target="move", every=2, weighting="improving", log=print):
+            gap_target=0.1, seed=0, target="move", every=2, weighting="improving", tau=0.1, log=print):
     """Learned proposal: the proposal network trained on the plain method's runs over n_train complexes, then plain (fused
     kernel, one launch) and learned (adock_mc_learned, one launch with MFMA inference inside the chain loop) searches
     on n_eval disjoint complexes at a range of MC-step budgets. Gap = mean over complexes of
     best(budget) - best(plain, full_steps). Reports, per method, the smallest budget reaching gap_target and the
     docked ligands/s there. target "move": regress the accepted moves (features before the move -> the move);
-    "jump": regress, from the pose every `every` steps, the jump to the best pose its chain ends with. weighting (move
-    target): "accepted" weighs every accepted move (downhill ones by their gain), "improving" only downhill moves, by
+    "jump": regress, from the pose every `every` steps, the jump to the best pose its chain ends with; "entity": learn
+    each entity's expected energy gain from every logged move (accepted or not) and let the network pick the entity
+    the reference's mutation perturbs (probability ~ exp(score / tau)). weighting (move target): "accepted" weighs every accepted move (downhill ones by their gain), "improving" only downhill moves, by
     their gain. The speed-up at a matched gap compares the search times at which the two methods' mean gaps reach
     the same value (interpolated between budgets), at gap_target and at a few other gaps."""
     t_all = time.perf_counter()
@@ -437,7 +445,16 @@ def learned(n_train=512, train_chains=64, train_steps=400, n_eval=512, eval_chai
     st, cl = workload.chain_starts(seed + 1, tligs, [np.zeros(3)] * L, [np.full(3, 26.0)] * L, train_chains, tkp.KM)
     st, cl = torch.as_tensor(st, device=tkp.device), torch.as_tensor(cl, device=tkp.device)
     holder = {}
-    if target == "move":
+    if target == "entity":
+        cap = len(cl) * train_steps + 1024
+        ss = StepSearch(tkp, cl, log_cap=cap)
+        log(f"training set: {L} complexes x {train_chains} chains x {train_steps} steps, every move logged")
+        t_log = _timed(lambda: ss.fused(*_bufs(st), train_steps, seed=seed + 777, log=True, log_all=True))
+        Xa, _, _, ga, ea = ss.logged(entity=True)
+        X, T, W = entity_dataset(Xa, ga, ea)
+        M = gain = None
+        del Xa, ga, ea
+    elif target == "move":
         cap = int(len(cl) * train_steps * 0.85) + 1024
         ss = StepSearch(tkp, cl, log_cap=cap)
         log(f"training set: {L} complexes x {train_chains} chains x {train_steps} steps (log capacity {cap} moves)")
@@ -482,14 +499,15 @@ def learned(n_train=512, train_chains=64, train_steps=400, n_eval=512, eval_chai
     zero.p["b3"].zero_()
     zero.refresh_bf16()
     sz = StepSearch(ekp, cl, net=zero)
+    mode = 1 if target == "entity" else 0
     rows = []
     for S in budgets:
         bp, tp = plain(S)
         b = _bufs(st)
-        tl = _timed(lambda: ss.fused(*b, S, seed=s64, use_net=True, p_plain=p_plain))
+        tl = _timed(lambda: ss.fused(*b, S, seed=s64, use_net=True, p_plain=p_plain, net_mode=mode, tau=tau))
         bl = b[3].view(E, eval_chains).min(1).values.cpu().numpy()
         b = _bufs(st)
-        tz = _timed(lambda: sz.fused(*b, S, seed=s64, use_net=True, p_plain=p_plain))
+        tz = _timed(lambda: sz.fused(*b, S, seed=s64, use_net=True, p_plain=p_plain, net_mode=mode, tau=tau))
         bz = b[3].view(E, eval_chains).min(1).values.cpu().numpy()
         rows.append({"steps": S, "plain_gap": float((bp - ref).mean()), "learned_gap": float((bl - ref).mean()),
                      "control_gap": float((bz - ref).mean()), "plain_s": tp, "learned_s": tl, "control_s": tz})

@@ -20,7 +20,8 @@ import torch
 from . import _lib
 from .mlp import IN, OUT, ProposalNet
 
-LOG_W = IN + 2 * OUT + 1
+LOG_W = IN + 2 * OUT + 2   # features, target, mask, energy gain, entity moved
+ENTITY_OUT = (0, 3)        # output scoring the translation / rotation (torsion k: 7 + k)
 
 
 class StepSearch:
@@ -43,7 +44,7 @@ class StepSearch:
         self.counter = torch.zeros(1, dtype=torch.int32, device=dev)
 
     def _step(self, cur, cur_e, best, best_e, init, step, bfgs_iters, temp, amp_pos, amp_rot, seed, use_net, p_plain,
-              sig, log, feat_on, count_evals):
+              sig, log, feat_on, count_evals, net_mode=0, tau=0.1, log_all=False):
         kp = self.kp
         kp.check_chain_buffers(self.N, cur, cur_e, best, best_e)
         if use_net and not init:
@@ -55,29 +56,32 @@ class StepSearch:
                  _lib.ptr(self.feat) if feat_on else None,
                  _lib.ptr(self.log_buf) if log and self.log_buf is not None else None, _lib.ptr(self.log_count),
                  self.log_cap, int(bool(use_net)), float(p_plain), float(sig[0]), float(sig[1]), float(sig[2]),
+                 int(net_mode), float(tau), int(bool(log_all)),
                  _lib.ptr(kp.eval_count) if count_evals else None, _lib.stream_of(kp.device)])
         _lib.check(_lib.lib().adock_mc_step(*args))
 
     def run(self, cur, cur_e, best, best_e, steps, bfgs_iters=0, temp=1.2, amp_pos=2.0, amp_rot=2.0, seed=0,
             step_offset=0, init_opt=True, use_net=False, p_plain=0.25, sig=(0.5, 0.15, 0.3), log=False,
-            count_evals=False, features=False):
+            count_evals=False, features=False, net_mode=0, tau=0.1, log_all=False):
         """In place on cur / cur_e / best / best_e (float32 contiguous device tensors), like KernelProblem.run_mc.
         Features (self.feat) are kept current whenever the network, the move log or `features` needs them; every step
         reads the features of its current pose, so a search continued with init_opt=False relies on self.feat being
-        the one left by the previous call on the same chains."""
+        the one left by the previous call on the same chains. net_mode 0: the network's output is a displacement;
+        1: it scores the entities (translation, rotation, torsions) and the reference's mutation of the one drawn
+        with probability ~ exp(score / tau) is applied. log_all: log rejected moves too."""
         if use_net and self.net is None:
             raise _lib.AdockError("use_net needs a ProposalNet")
         feat_on = bool(use_net or log or features)
         if init_opt:
             self._step(cur, cur_e, best, best_e, 1, step_offset, bfgs_iters, temp, amp_pos, amp_rot, seed, False,
-                       p_plain, sig, log, feat_on, count_evals)
+                       p_plain, sig, log, feat_on, count_evals, net_mode, tau, log_all)
         for st in range(steps):
             self._step(cur, cur_e, best, best_e, 0, step_offset + st, bfgs_iters, temp, amp_pos, amp_rot, seed,
-                       use_net, p_plain, sig, log, feat_on, count_evals)
+                       use_net, p_plain, sig, log, feat_on, count_evals, net_mode, tau, log_all)
 
     def fused(self, cur, cur_e, best, best_e, steps, bfgs_iters=0, temp=1.2, amp_pos=2.0, amp_rot=2.0, seed=0,
               step_offset=0, init_opt=True, use_net=False, p_plain=0.25, sig=(0.5, 0.15, 0.3), log=False,
-              count_evals=False, features=False, queued=True):
+              count_evals=False, features=False, queued=True, net_mode=0, tau=0.1, log_all=False):
         """run(...) in one launch (adock_mc_learned): each chain is set up once and its wavefront runs the network on
         MFMA inside its step loop, so there is no launch, chain set-up or feature evaluation per MC step. Same poses,
         energies and logged moves (in another order) as run(...) bit for bit; self.feat holds the features of the
@@ -99,6 +103,7 @@ class StepSearch:
                 [_lib.ptr(self.feat) if feat_on else None,
                  _lib.ptr(self.log_buf) if log and self.log_buf is not None else None, _lib.ptr(self.log_count),
                  self.log_cap, int(bool(use_net)), float(p_plain), float(sig[0]), float(sig[1]), float(sig[2]),
+                 int(net_mode), float(tau), int(bool(log_all)),
                  _lib.ptr(self.order) if queued else None, _lib.ptr(self.counter) if queued else None,
                  _lib.ptr(kp.eval_count) if count_evals else None, _lib.stream_of(kp.device)])
         if queued:
@@ -122,11 +127,31 @@ class StepSearch:
             self.run(cur, cur_e, best, best_e, steps, **kw)
         return g
 
-    def logged(self):
-        """(features [n,128], targets [n,23], masks [n,23], energy gain [n]) of the accepted moves logged so far."""
+    def logged(self, entity=False):
+        """(features [n,128], targets [n,23], masks [n,23], energy gain [n]) of the moves logged so far, and with
+        entity=True the entity each moved ([n] int64: 0 translation, 1 rotation, 2 + k torsion k, -1 a network
+        displacement)."""
         n = min(int(self.log_count.item()), self.log_cap)
         L = self.log_buf[:n]
-        return L[:, :IN], L[:, IN:IN + OUT], L[:, IN + OUT:IN + 2 * OUT], L[:, IN + 2 * OUT]
+        out = (L[:, :IN], L[:, IN:IN + OUT], L[:, IN + OUT:IN + 2 * OUT], L[:, IN + 2 * OUT])
+        return out + (L[:, IN + 2 * OUT + 1].round().long(),) if entity else out
+
+
+def entity_dataset(X, gain, entity, max_gain=5.0):
+    """Training set of the entity-choice target from moves logged with log_all (accepted and rejected): each move's
+    energy gain, clipped to [0, max_gain], is the target of the output that scores the entity it moved (translation
+    0, rotation 3, torsion k < 16: 7 + k), with weight 1 there and 0 elsewhere, so the network learns each entity's
+    expected gain from the features of the pose. Network displacement moves and torsions past 16 are dropped.
+    Returns (X, T, W)."""
+    ok = (entity >= 0) & (entity < 2 + (OUT - 7))
+    X, gain, entity = X[ok], gain[ok], entity[ok]
+    col = torch.where(entity == 0, 0, torch.where(entity == 1, 3, entity - 2 + 7))
+    rows = torch.arange(len(col), device=X.device)
+    T = torch.zeros(len(col), OUT, device=X.device)
+    W = torch.zeros(len(col), OUT, device=X.device)
+    T[rows, col] = gain.clamp(min=0.0, max=max_gain)
+    W[rows, col] = 1.0
+    return X.contiguous(), T.contiguous(), W.contiguous()
 
 
 def pose_delta(best, cur, n_tors):

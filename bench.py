@@ -424,8 +424,9 @@ def other_workload(args):
     if wl == "learned_cost":
         return ex.learned_cost(n=16 if args.small else 512, steps=10 if args.small else 60)
     return (ex.learned(n_train=8, train_steps=40, n_eval=8, budgets=(10, 20, 40), full_steps=100, epochs=2,
-                       target=args.target, weighting=args.weighting, p_plain=args.p_plain)
-            if args.small else ex.learned(target=args.target, weighting=args.weighting, p_plain=args.p_plain))
+                       target=args.target, weighting=args.weighting, p_plain=args.p_plain, tau=args.tau)
+            if args.small else ex.learned(target=args.target, weighting=args.weighting, p_plain=args.p_plain,
+                                          tau=args.tau))
 
 
 def main():
@@ -447,10 +448,12 @@ def main():
                     help="screen / multi_groups: work queues of the persistent launch (8: one per XCD)")
     ap.add_argument("--hbm-budget-mb", type=float, default=0.0,
                     help="multi_groups: device memory one admission group's grid sets may take (0: a quarter of all)")
-    ap.add_argument("--target", choices=["move", "jump"], default="move",
+    ap.add_argument("--target", choices=["move", "jump", "entity"], default="move",
                     help="training target of the learned proposal (--workload learned)")
     ap.add_argument("--weighting", choices=["accepted", "improving"], default="improving",
                     help="regression weights of logged moves (--workload learned, move target)")
+    ap.add_argument("--tau", type=float, default=0.1,
+                    help="entity target: temperature of the network's entity choice, kcal/mol (--workload learned)")
     ap.add_argument("--p-plain", type=float, default=0.5,
                     help="share of random (plain) moves in the learned search (--workload learned)")
     ap.add_argument("--lds-layout", type=int, choices=[0, 1], default=0,

@@ -210,8 +210,12 @@ int adock_mc_batch_noalloc(int AM, int KM,
  * The move comes from the network output Y [n,23] (use_net = 1; with probability p_plain, or when Y is not finite, the
  * reference's random mutation is used instead) or is the reference's mutation (use_net = 0); then BFGS and Metropolis
  * with the reference's RNG slots for step `step`. feat [n,128] (may be NULL) holds the features of each chain's current
- * pose: read for logging, rewritten after the step. log_buf [log_cap, 175] (may be NULL) receives accepted moves
- * (features before the move, 23 targets, 23 masks, energy gain) at *log_count (atomic). init = 1 runs the start
+ * pose: read for logging, rewritten after the step. log_buf [log_cap, 176] (may be NULL) receives accepted moves,
+ * or every move with log_all = 1 (features before the move, 23 targets, 23 masks, energy gain, entity moved: 0
+ * translation, 1 rotation, 2 + k torsion k, -1 a network displacement) at *log_count (atomic). net_mode 0: the
+ * network output is a displacement (translation + y_t + sig_t N, rotation normalise(y_q) q then sig_r N, torsions
+ * + y_theta + sig_th N); net_mode 1: outputs 0 (translation), 3 (rotation) and 7 + k (torsion k < 16) score the
+ * entities, one is drawn with probability ~ exp(score / tau) and the reference's mutation of it is applied. init = 1 runs the start
  * BFGS only (the reference's init_opt). With use_net = 0, init followed by steps 0..S-1 equals adock_mc(init_opt = 1,
  * mc_steps = S) bit for bit. */
 int adock_mc_step(int AM, int KM,
@@ -225,8 +229,8 @@ int adock_mc_step(int AM, int KM,
                   const int* chain_lig, const int* chain_id, int n_chains, float* cur, float* cur_e, float* best,
                   float* best_e, int init, int step, int bfgs_iters, float temp, float amp_pos, float amp_rot,
                   unsigned long long seed, const float* Y, float* feat, float* log_buf, int* log_count, int log_cap,
-                  int use_net, float p_plain, float sig_t, float sig_r, float sig_th, unsigned long long* eval_count,
-                  adock_stream_t stream);
+                  int use_net, float p_plain, float sig_t, float sig_r, float sig_th, int net_mode, float tau,
+                  int log_all, unsigned long long* eval_count, adock_stream_t stream);
 
 /* The learned search in one launch: every chain's start BFGS (init_opt = 1) and mc_steps MC steps (steps step_offset
  * .. step_offset + mc_steps - 1), the chain set up once, with the proposal network run by each chain's wavefront on
@@ -237,8 +241,9 @@ int adock_mc_step(int AM, int KM,
  * features from the evaluation its BFGS ended on instead of evaluating it again. feat [n,128] (may be NULL) receives the features of each chain's final
  * current pose. use_net = 0 without a log is adock_mc's plain search (init_opt, mc_steps). counter NULL: one wavefront
  * per chain; else (one int, 0 at the call) persistent waves take the chains in the order `order` [n_chains] (a
- * permutation of 0 .. n_chains - 1, e.g. longest chains first; NULL: identity) from that counter, which ends at n_chains plus the waves launched:
- * reset it before every call. Results do not depend on the order. */
+ * permutation of 0 .. n_chains - 1, e.g. longest chains first; NULL: identity) from that counter, which ends at
+ * n_chains plus the waves launched: reset it before every call. Results do not depend on the order. net_mode, tau
+ * and log_all as for adock_mc_step. */
 int adock_mc_learned(int AM, int KM,
                      const float* ref, const int* typ, const unsigned char* tm, const int* ax, const float* axp,
                      const float* axu, const int* pr, const int* na, const int* nt, const int* np, const float* rg,
@@ -252,8 +257,8 @@ int adock_mc_learned(int AM, int KM,
                      float amp_pos, float amp_rot, unsigned long long seed, const unsigned short* W1, const float* b1,
                      const unsigned short* W2, const float* b2, const unsigned short* W3, const float* b3,
                      float* feat, float* log_buf, int* log_count, int log_cap, int use_net, float p_plain,
-                     float sig_t, float sig_r, float sig_th, const int* order, int* counter,
-                     unsigned long long* eval_count, adock_stream_t stream);
+                     float sig_t, float sig_r, float sig_th, int net_mode, float tau, int log_all, const int* order,
+                     int* counter, unsigned long long* eval_count, adock_stream_t stream);
 
 /* Inference on MFMA: Y[B,23] = W3 relu(W2 relu(W1 X + b1) + b2) + b3 with BF16 weights [out, in] (W1 [256,128],
  * W2 [256,256], W3 [23,256]), BF16 activations and FP32 accumulation; X [B,128] and biases FP32. */

@@ -138,3 +138,57 @@ def test_fused_equals_step_pipeline_on_the_proposal_workload(proposal_cx):
     StepSearch(kp, cl, net=net).fused(*b, 6, seed=13, use_net=True, p_plain=0.25)
     torch.cuda.synchronize()
     _assert_same(a, b)
+
+
+def _bias_net(scores):
+    """A network whose output is its last-layer bias alone (W3 = 0): output j = scores.get(j, 0)."""
+    net = ProposalNet("cuda", seed=0)
+    net.p["W3"].zero_()
+    b = torch.zeros_like(net.p["b3"])
+    for j, v in scores.items():
+        b[j] = v
+    net.p["b3"].copy_(b)
+    net.refresh_bf16()
+    return net
+
+
+@pytest.mark.parametrize("out,entity", [(0, 0), (3, 1), (8, 3)])
+def test_entity_choice_follows_the_scores(small_cx, out, entity):
+    """net_mode 1 with one entity scored far above the others and a small tau: every network move perturbs that
+    entity (output 0 -> translation, 3 -> rotation, 7 + k -> torsion k), and every move is logged with log_all."""
+    print(f"[gpu] entity choice: output {out} -> entity {entity}", flush=True)
+    w, kp, st = small_cx
+    assert w["lig"].n_tors >= 2
+    N, S = st.shape[0], 6
+    cl = torch.zeros(N, dtype=torch.int32).cuda()
+    ss = StepSearch(kp, cl, net=_bias_net({out: 10.0}), log_cap=N * S + 64)
+    ss.fused(*_bufs(st), S, seed=17, use_net=True, p_plain=0.0, net_mode=1, tau=0.1, log=True, log_all=True)
+    torch.cuda.synchronize()
+    X, T, M, gain, ent = ss.logged(entity=True)
+    assert len(ent) == N * S
+    assert (ent.cpu().numpy() == entity).all()
+    # with p_plain = 1 the entities are the plain move's uniform draw over 2 + nt
+    ss.log_count.zero_()
+    ss.fused(*_bufs(st), S, seed=17, use_net=True, p_plain=1.0, net_mode=1, tau=0.1, log=True, log_all=True)
+    e = ss.logged(entity=True)[4].cpu().numpy()
+    assert len(e) == N * S and e.min() >= 0 and e.max() < 2 + w["lig"].n_tors and len(set(e.tolist())) > 2
+
+
+def test_fused_entity_search_equals_step_pipeline(small_cx):
+    print("[gpu] fused entity-choice search vs step pipeline, every move logged", flush=True)
+    w, kp, st = small_cx
+    N, S = st.shape[0], 10
+    cl = torch.zeros(N, dtype=torch.int32).cuda()
+    net = _net(5, 5.0)
+    step = StepSearch(kp, cl, net=net, log_cap=N * S + 64)
+    a = _bufs(st)
+    step.run(*a, S, seed=23, use_net=True, p_plain=0.25, net_mode=1, tau=0.2, log=True, log_all=True)
+    fused = StepSearch(kp, cl, net=net, log_cap=N * S + 64)
+    b = _bufs(st)
+    fused.fused(*b, S, seed=23, use_net=True, p_plain=0.25, net_mode=1, tau=0.2, log=True, log_all=True)
+    torch.cuda.synchronize()
+    _assert_same(a, b)
+    ra = step.log_buf[:int(step.log_count.item())].cpu().numpy()
+    rb = fused.log_buf[:int(fused.log_count.item())].cpu().numpy()
+    assert ra.shape == rb.shape == (N * S, ra.shape[1])
+    np.testing.assert_array_equal(ra[np.lexsort(ra.T[::-1])], rb[np.lexsort(rb.T[::-1])])

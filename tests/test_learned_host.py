@@ -1,0 +1,26 @@
+"""Host-side training-set builders of the learned proposal (CPU tensors)."""
+import torch
+
+from adock_amd.learned import entity_dataset, move_weights
+
+
+def test_entity_dataset_targets_the_moved_entity_only():
+    X = torch.arange(5 * 128, dtype=torch.float32).reshape(5, 128)
+    gain = torch.tensor([0.5, -1.0, 7.0, 0.2, 1.0])
+    entity = torch.tensor([0, 1, 3, -1, 20])   # translation, rotation, torsion 1, a displacement, torsion 18
+    Xo, T, W = entity_dataset(X, gain, entity)
+    assert Xo.shape == (3, 128) and torch.equal(Xo, X[:3])
+    assert T.shape == W.shape == (3, 23)
+    assert torch.equal(W.sum(1), torch.ones(3))
+    assert W[0, 0] == 1 and W[1, 3] == 1 and W[2, 8] == 1
+    assert T[0, 0] == 0.5 and T[1, 3] == 0.0 and T[2, 8] == 5.0   # clipped to [0, 5]
+    assert float((T * (1 - W)).abs().sum()) == 0.0
+
+
+def test_move_weights_modes():
+    mask = torch.ones(3, 23)
+    gain = torch.tensor([2.0, -0.5, 9.0])
+    acc = move_weights(mask, gain, "accepted")
+    imp = move_weights(mask, gain, "improving")
+    assert torch.allclose(acc[:, 0], torch.tensor([3.0, 0.25, 6.0]))
+    assert torch.allclose(imp[:, 0], torch.tensor([2.0, 0.0, 5.0]))
