@@ -392,7 +392,7 @@ def other_workload(args):
         return ex.screen(n_ligands=64 if args.small else 512, chains=64, steps=50 if args.small else 500,
                          queues=args.queues)
     if wl in ("driver", "driver_fitted"):
-        return ex.driver(n_ligands=32 if args.small else 256, fitted=wl == "driver_fitted")
+        return ex.driver(n_ligands=32 if args.small else args.n_ligands, fitted=wl == "driver_fitted")
     if wl == "mlp":
         out = ex.mlp()
         out["roofline"] = {"infer": {"bound": "mfma", "achieved": out["infer_tflops"], "peak": PEAK_BF16_TFLOPS,
@@ -444,6 +444,9 @@ def main():
                              "wide", "driver", "driver_fitted", "mlp"],
                     help="multi = the headline line (default); the others print their own JSON line")
     ap.add_argument("--small", action="store_true", help="scaled-down sizes for the opt-in workloads")
+    ap.add_argument("--n-ligands", type=int, default=256,
+                    help="driver / driver_fitted: ligands docked in the one pocket (the reference's scale report "
+                         "uses 2,000)")
     ap.add_argument("--queues", type=int, default=8,
                     help="screen / multi_groups: work queues of the persistent launch (8: one per XCD)")
     ap.add_argument("--hbm-budget-mb", type=float, default=0.0,
