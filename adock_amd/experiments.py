@@ -234,7 +234,8 @@ def driver(n_ligands=256, mode="v2_default", fitted=False, seed=0, warmup=True):
     if fitted:
         from .fit.fdock import DATA, dock_fitted, load_fitted
         spec, wts = load_fitted(DATA / "adock_fitted.json", cfg)
-        cxs = [synth.fitted_complex(rec, lg, c, s, spec, seed=i) for i, lg in enumerate(w["ligs"])]
+        # one pocket: one annotated receptor shared by every complex (one fitted cube for the screen)
+        cxs = [synth.fitted_complex(rec, lg, c, s, spec, seed=i, rec_seed=seed) for i, lg in enumerate(w["ligs"])]
 
         def run(n):
             return dock_fitted(cxs[:n], spec, wts, mode=mode, seed=seed, cfg=cfg, stats=stats)

@@ -14,6 +14,7 @@ size and lig (a LigandPrep with XS types) plus lig_ext (its ext types, terms.lig
 these from PDB / SDF files with RDKit (prepare_complex_fit, src/adock/fit/fdock.py:45-53).
 """
 import copy
+import hashlib
 import json
 import time
 from pathlib import Path
@@ -70,15 +71,37 @@ def compact_ligands(preps):
     return lig_list, ligs
 
 
+def unique_receptors(preps):
+    """(index of the first complex of each distinct receptor, receptor of every complex): complexes with the same
+    receptor atoms, ext types, box centre and box size share one grid cube and one padded receptor."""
+    first, idx, seen = [], [], {}
+    for i, p in enumerate(preps):
+        h = hashlib.sha1()
+        for a, dt in ((p["rec_xyz"], np.float32), (p["rec_ext"], np.int64), (p["center"], np.float64),
+                      (np.broadcast_to(np.asarray(p["size"], np.float64), (3,)), np.float64)):
+            h.update(np.ascontiguousarray(np.asarray(a, dt)).tobytes())
+            h.update(b"|")
+        k = h.digest()
+        if k not in seen:
+            seen[k] = len(first)
+            first.append(i)
+        idx.append(seen[k])
+    return first, np.asarray(idx, np.int64)
+
+
 def build_problem(preps, spec, w, cfg, device="cuda"):
     """Problem of fitted complexes: compact ligand types (compact_ligands), a FittedPotential, the fitted grid set and
-    the receptors with pair types TL + ext type (src/adock/fit/fdock.py:56-70)."""
+    the receptors with pair types TL + ext type (src/adock/fit/fdock.py:56-70). The reference builds a cube per
+    complex; here each distinct receptor (and box) gets one cube, which its complexes share through rec_idx (the same
+    values: a screen of many ligands against one pocket builds one cube instead of one per ligand)."""
     lig_list, ligs = compact_ligands(preps)
     pot = FittedPotential(spec, w, lig_list, device)
-    gs = make_gridset_fit(preps, pot, cfg["grid"]["spacing"], cfg["grid"]["out_of_box_penalty"], device=device)
+    first, ridx = unique_receptors(preps)
+    upreps = [preps[i] for i in first]
+    gs = make_gridset_fit(upreps, pot, cfg["grid"]["spacing"], cfg["grid"]["out_of_box_penalty"], device=device)
     lb = LigandBatch(ligs, device)
-    recs = [{"rec_xyz": p["rec_xyz"], "rec_types": np.asarray(p["rec_ext"], np.int64) + pot.TL} for p in preps]
-    return Problem(gs, lb, torch.arange(len(preps), device=device), pot, rec=pad_receptors(recs, device))
+    recs = [{"rec_xyz": p["rec_xyz"], "rec_types": np.asarray(p["rec_ext"], np.int64) + pot.TL} for p in upreps]
+    return Problem(gs, lb, torch.as_tensor(ridx, device=device), pot, rec=pad_receptors(recs, device))
 
 
 def dock_fitted(preps, spec, w, mode="v2_default", seed=0, device="cuda", cfg=None, log=None, timings=None,

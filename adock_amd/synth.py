@@ -257,31 +257,34 @@ def make_multi(seed=0, n_receptors=64, ligands_per=32, chains=32, steps=500):
             "steps": steps, "bfgs_iters": 0}
 
 
-def fitted_complex(rec, lig, center, size, spec, seed=0, p_charge=0.3, p_side_chain=0.5):
+def fitted_complex(rec, lig, center, size, spec, seed=0, p_charge=0.3, p_side_chain=0.5, rec_seed=None):
     """A complex for the fitted scoring function from a synthetic receptor and ligand: receptor buriedness from its own
     heavy atoms (count within spec.buried_radius), charge classes drawn on N (cation) and O (anion) atoms with
     probability p_charge (the annotation the reference derives from residue names and formal charges,
     src/adock/fit/terms.py:165-230), side-chain flags with probability p_side_chain. Returns the dict that
-    adock_amd.fit.fdock takes (rec_xyz, rec_ext, center, size, lig, lig_ext)."""
+    adock_amd.fit.fdock takes (rec_xyz, rec_ext, center, size, lig, lig_ext). rec_seed: draw the receptor's
+    annotation from its own seed, so that complexes of one pocket share one annotated receptor (a screen); None:
+    from the complex's seed, like its ligand's classes."""
     from .fit.terms import buried_count, ligand_ext_types, receptor_ext_types
     rng = np.random.default_rng(seed + 77)
+    rrng = rng if rec_seed is None else np.random.default_rng(rec_seed + 1077)
     names = _CFG["_xs_names"]
     elem = lambda t: names[t].split("_")[0]
 
-    def classes(types):
+    def classes(types, g):
         c = np.zeros(len(types), np.int64)
         for i, t in enumerate(types):
-            if t >= 0 and rng.random() < p_charge:
+            if t >= 0 and g.random() < p_charge:
                 c[i] = {"N": 1, "O": 2}.get(elem(int(t)), 0)
         return c
 
     xyz = np.asarray(rec["rec_xyz"], np.float32)
     ann = {"rec_types": np.asarray(rec["rec_types"], np.int64),
            "buried_count": buried_count(xyz, xyz, spec.buried_radius),
-           "rec_chg": classes(rec["rec_types"]), "rec_sc": rng.random(len(xyz)) < p_side_chain}
+           "rec_chg": classes(rec["rec_types"], rrng), "rec_sc": rrng.random(len(xyz)) < p_side_chain}
     return {"rec_xyz": xyz, "rec_ext": receptor_ext_types(ann, spec), "center": np.asarray(center, np.float64),
             "size": np.broadcast_to(np.asarray(size, np.float64), (3,)).copy(), "lig": lig,
-            "lig_ext": ligand_ext_types(lig.types, classes(lig.types), spec.nxs)}
+            "lig_ext": ligand_ext_types(lig.types, classes(lig.types, rng), spec.nxs)}
 
 
 def make_native_complex(seed, n_heavy=20, n_tors=4, shell=(3.6, 4.6), outer=(4.6, 7.5), min_sep=3.0):

@@ -83,7 +83,7 @@ def multi_headline(args):
     cl = torch.as_tensor(cl_h, device=dev)
     N = len(cl_h)
     ci = torch.arange(N, dtype=torch.int32, device=dev)
-    runner = BatchRunner(kp, cl, n_complexes=L)
+    runner = BatchRunner(kp, cl, n_complexes=L, queues=args.queues)
     runner.lds_layout = args.lds_layout
     cur, best = st0.clone(), st0.clone()
     ce, be = torch.zeros(N, device=dev), torch.full((N,), float("inf"), device=dev)
@@ -161,7 +161,7 @@ def multi_headline(args):
                    "receptors": n_rec, "complexes": L, "chains": N, "mc_steps": steps,
                    "grid_types_per_receptor": [int(x) for x in np.unique((gs.slot >= 0).sum(1).cpu().numpy())[[0, -1]]],
                    "grid_bytes": int(gs.flat.numel() * 4), "KM": kp.KM,
-                   "lds_layout": "dense" if args.lds_layout else "compact"},
+                   "lds_layout": "dense" if args.lds_layout else "compact", "work_queues": args.queues},
         "score_grad_evals_per_s": n_eval / dt,
         "evals_per_step": n_eval / args.steps,
         "grid_build_and_reset_ms": prep_ms, "search_kernel_ms": search_ms,
@@ -448,7 +448,7 @@ def main():
                     help="driver / driver_fitted: ligands docked in the one pocket (the reference's scale report "
                          "uses 2,000)")
     ap.add_argument("--queues", type=int, default=8,
-                    help="screen / multi_groups: work queues of the persistent launch (8: one per XCD)")
+                    help="multi / screen / multi_groups: work queues of the persistent launch (8: one per XCD)")
     ap.add_argument("--hbm-budget-mb", type=float, default=0.0,
                     help="multi_groups: device memory one admission group's grid sets may take (0: a quarter of all)")
     ap.add_argument("--target", choices=["move", "jump", "entity"], default="move",

@@ -233,6 +233,31 @@ def test_gridset_fit_layout():
         np.testing.assert_array_equal(gs.G[i].cpu().numpy(), g1)
 
 
+def test_build_problem_shares_identical_receptors():
+    """Complexes with the same receptor and box share one fitted cube through rec_idx; a complex with another box gets
+    its own; every complex's cube equals build_grid_fit of that complex alone."""
+    print("[gpu] fitted problem: one cube per distinct receptor", flush=True)
+    spec, w = load_fitted(DATA / "adock_fitted.json", CFG)
+    w1 = synth.make_single("small", 0)
+    rec = workload.receptor_dict(w1["receptor"], w1["center"], w1["size"])
+    s = float(np.max(w1["size"]))
+    c0 = np.asarray(w1["center"], float)
+    ligs = [w1["lig"], synth.make_ligand(31, 16, 3), synth.make_ligand(32, 12, 2)]
+    centers = [c0, c0, c0 + np.array([1.5, 0.0, -0.75])]
+    cxs = [synth.fitted_complex(rec, lg, cc, s, spec, seed=i, rec_seed=5)
+           for i, (lg, cc) in enumerate(zip(ligs, centers))]
+    prob = build_problem(cxs, spec, w, CFG, "cuda")
+    assert prob.rec_idx.cpu().tolist() == [0, 0, 1]
+    assert prob.gs.R == 2 and prob.rec[0].shape[0] == 2
+    n = prob.gs.G.shape[-1]
+    for i, c in enumerate(cxs):
+        r = int(prob.rec_idx[i])
+        o = np.asarray(c["center"], float) - np.broadcast_to(np.asarray(c["size"], float), (3,)) / 2
+        np.testing.assert_allclose(prob.gs.origin[r].cpu().numpy(), o, atol=1e-6)
+        g1 = build_grid_fit(c["rec_xyz"], c["rec_ext"], prob.pot, o, SPACING, n).cpu().numpy()
+        np.testing.assert_array_equal(prob.gs.G[r].cpu().numpy(), g1)
+
+
 def _small(**kw):
     cfg = dict(CFG["search"]["v2_default"])
     cfg.update(n_chains=32, stageA_conformers=4, stageA_conformer_trials=32, stageA_rotations=64,
