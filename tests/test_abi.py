@@ -126,3 +126,48 @@ def test_product_path_fails_loudly_without_gpu_or_library(monkeypatch, tmp_path)
     monkeypatch.setattr(_lib, "LIB_PATH", tmp_path / "missing.so")
     with pytest.raises(_lib.AdockError, match="not built"):
         _lib.lib()
+
+
+def test_learned_and_batch_argument_validation_without_gpu():
+    """adock_mc_learned / adock_mc_step / adock_mc_batch reject bad network modes, temperatures, p_plain, buckets and
+    LDS layouts before any launch (host addresses stand in for device buffers: the calls fail their checks first)."""
+    if not LIB.exists():
+        pytest.skip("libadock_hip.so not built")
+    from adock_amd import _lib
+    L = _lib.lib()
+    buf = ctypes.create_string_buffer(64)
+    h = ctypes.cast(buf, ctypes.c_void_p).value
+    lig = [64, 8] + [h] * 11 + [128, h, h, h, 8]
+    grid = [h] * 7 + [1, 0.375, 10.0]
+    sf = [h] * 5
+    rec = [h, h, h, 16]
+    chains = [h, h, 1, h, h, h, h]            # chain_lig, chain_id, n_chains, cur, cur_e, best, best_e
+
+    def learned(**kw):
+        a = dict(init_opt=1, mc_steps=2, step_offset=0, bfgs_iters=0, temp=1.2, amp_pos=2.0, amp_rot=2.0, seed=1,
+                 w=[h] * 6, feat=h, log_buf=None, log_count=h, log_cap=0, use_net=1, p_plain=0.25, sig=(0.5, 0.15, 0.3),
+                 net_mode=0, tau=0.1, log_all=0, am=64)
+        a.update(kw)
+        args = ([a["am"]] + lig[1:] + grid + sf + rec + chains +
+                [a["init_opt"], a["mc_steps"], a["step_offset"], a["bfgs_iters"], a["temp"], a["amp_pos"], a["amp_rot"],
+                 a["seed"]] + a["w"] + [a["feat"], a["log_buf"], a["log_count"], a["log_cap"], a["use_net"],
+                                        a["p_plain"], *a["sig"], a["net_mode"], a["tau"], a["log_all"], None, None, None,
+                                        None])
+        return L.adock_mc_learned(*args)
+
+    def rejected(status, code, words):
+        return status == code and words in L.adock_last_error()
+
+    assert rejected(learned(w=[None] * 6), -1, b"weights")      # network moves without weights
+    assert rejected(learned(p_plain=1.5), -1, b"p_plain")
+    assert rejected(learned(net_mode=2), -1, b"net_mode")
+    assert rejected(learned(net_mode=1, tau=0.0), -1, b"tau")
+    assert rejected(learned(am=96), -2, b"64 heavy atoms")     # two atoms per lane: not in the learned kernels
+    assert rejected(learned(mc_steps=-1), -1, b"mc_steps")
+    step = ([64] + lig[1:] + grid + sf + rec + chains +
+            [0, 0, 0, 1.2, 2.0, 2.0, 1, h, h, None, h, 0, 1, 0.25, 0.5, 0.15, 0.3, 3, 0.1, 0, None, None])
+    assert rejected(L.adock_mc_step(*step), -1, b"net_mode")
+    batch = ([64] + lig[1:] + grid + sf + rec + [0, h, h, 1, None, None, 0, h, h, h, h, 1, 2, 0, 1.2, 2.0, 2.0, 1, 0,
+                                                 None, None, 1, h, None, None, None, 0, 0, 2, None, None, None])
+    assert L.adock_mc_batch(*batch) == -1               # lds_layout 2
+    assert b"lds_layout" in L.adock_last_error()
