@@ -1,7 +1,7 @@
 """Launch-shape autotuner for the chain kernels.
 
 Chooses the wave's pair-gradient LDS tile (compact: one slot per intra pair with the per-atom lists read from memory;
-dense: per-atom rows of DEG_MAX float4 entries, more LDS per wave and fewer instructions per evaluation), chains
+dense: per-atom rows of DEG_MAX float4 entries with no list indirection, more LDS per wave), chains
 (wavefronts) per workgroup W in {1, 2, 4} and the persistent grid size (resident capacity x {1, 2}, or one workgroup
 per W chains) for a given batch by timing short probes of the real workload (the same chains, a few MC steps, on
 scratch copies of the states) and keeps the fastest. Every candidate gives the same results bit for bit. Results are

@@ -1141,7 +1141,8 @@ __global__ __launch_bounds__(256) void mc_spec_kernel(LigArgs L, GridArgs G, SFA
 // start).
 // DENSE: the pair gradients of each wave in the dense LDS tile (per-atom rows of DEG_MAX float4 entries, one atom per
 // lane) instead of one scalar slot per pair with the per-atom lists read from memory; the same bits either way, so
-// the layout is a launch-time choice (more LDS per wave, fewer instructions per evaluation).
+// the layout is a launch-time choice (the dense tile needs more LDS per wave and no list indirection, but issues
+// more VALU instructions: the batches measured so far run faster compact).
 template <int KM, int APL, bool EX, bool DENSE = false>
 __global__ __launch_bounds__(256) ADOCK_WAVES_D(KM, APL, DENSE) void mc_persistent_kernel(LigArgs L, GridArgs G, SFArgs S, RecArgs R,
                                                              const int* __restrict__ chain_lig, int n_chains, McArgs M,
