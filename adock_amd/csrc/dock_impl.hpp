@@ -1436,8 +1436,8 @@ int mc_resident_waves_t(int P_MAX, int persistent) {
 }
 
 // move record: features before the move [128], target [23], mask [23], energy gain, entity moved (-1: a network
-// displacement move)
-constexpr int NFEAT = NET_IN, NOUT = NET_OUT, LOG_W = NFEAT + 2 * NOUT + 2;
+// displacement move), the chain's best energy minus its current one before the move (<= 0)
+constexpr int NFEAT = NET_IN, NOUT = NET_OUT, LOG_W = NFEAT + 2 * NOUT + 3;
 // Counter-RNG slots of the learned step. The plain move uses slots 0 (entity), 1 (torsion angle), 2 (acceptance) and
 // 10 .. 105 (unit-ball tries 10 + 3k, k < 32); the step counter sits 8 bits up, so every slot must stay below 256 or
 // it aliases a slot of another step. The network move draws its coin at SLOT_NET_PLAIN and one normal per lane from
@@ -1594,10 +1594,11 @@ DEVI void move_target(float cur, float x, const Ctx& c, int lane, float& t, floa
 }
 
 // One move record at *log_count (atomic; records past log_cap are dropped): the features before the move (fin, 128
-// floats, memory or LDS), the move cur -> x as target and mask, the energy gain ec - en and the entity moved.
+// floats, memory or LDS), the move cur -> x as target and mask, the energy gain ec - en, the entity moved and
+// eb - ec (so gain + eb - ec > 0 marks a move that found a new chain best).
 template <class Ctx>
 DEVI void log_move(const StepArgs& P, const float* fin, float cur, float x, const Ctx& c, int lane, float gain,
-                   int ent) {
+                   int ent, float best_gap) {
     int slot = 0;
     if (lane == 0) slot = atomicAdd(P.log_count, 1);
     slot = bcast_i(slot, 0);
@@ -1608,7 +1609,11 @@ DEVI void log_move(const StepArgs& P, const float* fin, float cur, float x, cons
     float t, m;
     move_target(cur, x, c, lane, t, m);
     if (lane < NOUT) { rec[NFEAT + lane] = t; rec[NFEAT + NOUT + lane] = m; }
-    if (lane == 0) { rec[NFEAT + 2 * NOUT] = gain; rec[NFEAT + 2 * NOUT + 1] = (float)ent; }
+    if (lane == 0) {
+        rec[NFEAT + 2 * NOUT] = gain;
+        rec[NFEAT + 2 * NOUT + 1] = (float)ent;
+        rec[NFEAT + 2 * NOUT + 2] = best_gap;
+    }
 }
 
 // the move of one step: the reference's random mutation (plain), the network's displacement (net_mode 0) or the
@@ -1671,7 +1676,7 @@ __global__ __launch_bounds__(256) ADOCK_WAVES(KM, 1) void mc_step_kernel(LigArgs
         const float en = bfgs<KM, 1>(c, sf, w, lane, x, iters, nev);
         const bool accept = (en < ec) || (urand(seed, cid, sg, 2) < expf((ec - en) / M.temp));
         if ((accept || P.log_all) && P.log_buf && P.feat)
-            log_move(P, P.feat + (long long)ch * NFEAT, cur, x, c, lane, ec - en, ent);
+            log_move(P, P.feat + (long long)ch * NFEAT, cur, x, c, lane, ec - en, ent, eb - ec);
         if (accept) {
             ec = en; cur = x;
             if (en < eb) { eb = en; best = x; }
@@ -1835,7 +1840,7 @@ DEVI void learned_chain(int ch, int lane, const WaveLDS<1>& w, const NetLDS& nl,
             ++st;
         } else {
             const bool accept = (en < ec) || (urand(seed, cid, sg, 2) < expf((ec - en) / M.temp));
-            if ((accept || P.log_all) && P.log_buf) log_move(P, nl.feat, cur, x, c, lane, ec - en, ent);
+            if ((accept || P.log_all) && P.log_buf) log_move(P, nl.feat, cur, x, c, lane, ec - en, ent, eb - ec);
             if (accept) {
                 ec = en; cur = x;
                 if (en < eb) { eb = en; best = x; }

@@ -462,7 +462,8 @@ def learned(n_train=512, train_chains=64, train_steps=400, n_eval=512, eval_chai
     docked ligands/s there. target "move": regress the accepted moves (features before the move -> the move);
     "jump": regress, from the pose every `every` steps, the jump to the best pose its chain ends with; "entity": learn
     each entity's expected energy gain from every logged move (accepted or not) and let the network pick the entity
-    the reference's mutation perturbs (probability ~ exp(score / tau)). weighting (move target): "accepted" weighs every accepted move (downhill ones by their gain), "improving" only downhill moves, by
+    the reference's mutation perturbs (probability ~ exp(score / tau)); "entity_best": the same with the improvement
+    of the chain's best energy as the gain. weighting (move target): "accepted" weighs every accepted move (downhill ones by their gain), "improving" only downhill moves, by
     their gain. The speed-up at a matched gap compares the search times at which the two methods' mean gaps reach
     the same value (interpolated between budgets), at gap_target and at a few other gaps."""
     t_all = time.perf_counter()
@@ -473,15 +474,15 @@ def learned(n_train=512, train_chains=64, train_steps=400, n_eval=512, eval_chai
     st, cl = workload.chain_starts(seed + 1, tligs, [np.zeros(3)] * L, [np.full(3, 26.0)] * L, train_chains, tkp.KM)
     st, cl = torch.as_tensor(st, device=tkp.device), torch.as_tensor(cl, device=tkp.device)
     holder = {}
-    if target == "entity":
+    if target in ("entity", "entity_best"):
         cap = len(cl) * train_steps + 1024
         ss = StepSearch(tkp, cl, log_cap=cap)
         log(f"training set: {L} complexes x {train_chains} chains x {train_steps} steps, every move logged")
         t_log = _timed(lambda: ss.fused(*_bufs(st), train_steps, seed=seed + 777, log=True, log_all=True))
-        Xa, _, _, ga, ea = ss.logged(entity=True)
-        X, T, W = entity_dataset(Xa, ga, ea)
+        Xa, _, _, ga, ea, da = ss.logged(entity=True)
+        X, T, W = entity_dataset(Xa, ga, ea, best_gap=da if target == "entity_best" else None)
         M = gain = None
-        del Xa, ga, ea
+        del Xa, ga, ea, da
     elif target == "move":
         cap = int(len(cl) * train_steps * 0.85) + 1024
         ss = StepSearch(tkp, cl, log_cap=cap)
@@ -527,7 +528,7 @@ def learned(n_train=512, train_chains=64, train_steps=400, n_eval=512, eval_chai
     zero.p["b3"].zero_()
     zero.refresh_bf16()
     sz = StepSearch(ekp, cl, net=zero)
-    mode = 1 if target == "entity" else 0
+    mode = 1 if target.startswith("entity") else 0
     rows = []
     for S in budgets:
         bp, tp = plain(S)
@@ -544,7 +545,7 @@ def learned(n_train=512, train_chains=64, train_steps=400, n_eval=512, eval_chai
     sp, tp = _budget_to_gap(rows, "plain", gap_target)
     sl, tl = _budget_to_gap(rows, "learned", gap_target)
     matched = {}
-    for g in sorted({0.3, 0.2, gap_target, 0.05}, reverse=True):
+    for g in sorted({0.3, 0.2, gap_target, 0.05, 0.03, 0.02}, reverse=True):
         a, b, z = (_time_to_gap(rows, k, g) for k in ("plain", "learned", "control"))
         matched[f"{g:g}"] = {"plain_s": a, "learned_s": b, "control_s": z, "speedup": (a / b) if (a and b) else None,
                              "control_speedup": (a / z) if (a and z) else None}

@@ -20,7 +20,7 @@ import torch
 from . import _lib
 from .mlp import IN, OUT, ProposalNet
 
-LOG_W = IN + 2 * OUT + 2   # features, target, mask, energy gain, entity moved
+LOG_W = IN + 2 * OUT + 3   # features, target, mask, energy gain, entity moved, best - current energy before
 ENTITY_OUT = (0, 3)        # output scoring the translation / rotation (torsion k: 7 + k)
 
 
@@ -129,21 +129,24 @@ class StepSearch:
 
     def logged(self, entity=False):
         """(features [n,128], targets [n,23], masks [n,23], energy gain [n]) of the moves logged so far, and with
-        entity=True the entity each moved ([n] int64: 0 translation, 1 rotation, 2 + k torsion k, -1 a network
-        displacement)."""
+        entity=True also the entity each moved ([n] int64: 0 translation, 1 rotation, 2 + k torsion k, -1 a network
+        displacement) and the chain's best energy minus its current one before the move ([n], <= 0)."""
         n = min(int(self.log_count.item()), self.log_cap)
         L = self.log_buf[:n]
         out = (L[:, :IN], L[:, IN:IN + OUT], L[:, IN + OUT:IN + 2 * OUT], L[:, IN + 2 * OUT])
-        return out + (L[:, IN + 2 * OUT + 1].round().long(),) if entity else out
+        return out + (L[:, IN + 2 * OUT + 1].round().long(), L[:, IN + 2 * OUT + 2]) if entity else out
 
 
-def entity_dataset(X, gain, entity, max_gain=5.0):
+def entity_dataset(X, gain, entity, max_gain=5.0, best_gap=None):
     """Training set of the entity-choice target from moves logged with log_all (accepted and rejected): each move's
     energy gain, clipped to [0, max_gain], is the target of the output that scores the entity it moved (translation
     0, rotation 3, torsion k < 16: 7 + k), with weight 1 there and 0 elsewhere, so the network learns each entity's
-    expected gain from the features of the pose. Network displacement moves and torsions past 16 are dropped.
-    Returns (X, T, W)."""
+    expected gain from the features of the pose. With best_gap (the chain's best minus its current energy before each
+    move) the gain counted is the improvement of the chain's best, gain + best_gap, instead. Network displacement
+    moves and torsions past 16 are dropped. Returns (X, T, W)."""
     ok = (entity >= 0) & (entity < 2 + (OUT - 7))
+    if best_gap is not None:
+        gain = gain + best_gap
     X, gain, entity = X[ok], gain[ok], entity[ok]
     col = torch.where(entity == 0, 0, torch.where(entity == 1, 3, entity - 2 + 7))
     rows = torch.arange(len(col), device=X.device)

@@ -451,7 +451,7 @@ def main():
                     help="multi / screen / multi_groups: work queues of the persistent launch (8: one per XCD)")
     ap.add_argument("--hbm-budget-mb", type=float, default=0.0,
                     help="multi_groups: device memory one admission group's grid sets may take (0: a quarter of all)")
-    ap.add_argument("--target", choices=["move", "jump", "entity"], default="move",
+    ap.add_argument("--target", choices=["move", "jump", "entity", "entity_best"], default="move",
                     help="training target of the learned proposal (--workload learned)")
     ap.add_argument("--weighting", choices=["accepted", "improving"], default="improving",
                     help="regression weights of logged moves (--workload learned, move target)")

@@ -210,9 +210,10 @@ int adock_mc_batch_noalloc(int AM, int KM,
  * The move comes from the network output Y [n,23] (use_net = 1; with probability p_plain, or when Y is not finite, the
  * reference's random mutation is used instead) or is the reference's mutation (use_net = 0); then BFGS and Metropolis
  * with the reference's RNG slots for step `step`. feat [n,128] (may be NULL) holds the features of each chain's current
- * pose: read for logging, rewritten after the step. log_buf [log_cap, 176] (may be NULL) receives accepted moves,
+ * pose: read for logging, rewritten after the step. log_buf [log_cap, 177] (may be NULL) receives accepted moves,
  * or every move with log_all = 1 (features before the move, 23 targets, 23 masks, energy gain, entity moved: 0
- * translation, 1 rotation, 2 + k torsion k, -1 a network displacement) at *log_count (atomic). net_mode 0: the
+ * translation, 1 rotation, 2 + k torsion k, -1 a network displacement, and the chain's best energy minus its current
+ * one before the move) at *log_count (atomic). net_mode 0: the
  * network output is a displacement (translation + y_t + sig_t N, rotation normalise(y_q) q then sig_r N, torsions
  * + y_theta + sig_th N); net_mode 1: outputs 0 (translation), 3 (rotation) and 7 + k (torsion k < 16) score the
  * entities, one is drawn with probability ~ exp(score / tau) and the reference's mutation of it is applied. init = 1 runs the start

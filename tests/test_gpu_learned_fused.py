@@ -164,7 +164,8 @@ def test_entity_choice_follows_the_scores(small_cx, out, entity):
     ss = StepSearch(kp, cl, net=_bias_net({out: 10.0}), log_cap=N * S + 64)
     ss.fused(*_bufs(st), S, seed=17, use_net=True, p_plain=0.0, net_mode=1, tau=0.1, log=True, log_all=True)
     torch.cuda.synchronize()
-    X, T, M, gain, ent = ss.logged(entity=True)
+    X, T, M, gain, ent, bgap = ss.logged(entity=True)
+    assert (bgap.cpu().numpy() <= 0).all()
     assert len(ent) == N * S
     assert (ent.cpu().numpy() == entity).all()
     # with p_plain = 1 the entities are the plain move's uniform draw over 2 + nt

@@ -24,3 +24,12 @@ def test_move_weights_modes():
     imp = move_weights(mask, gain, "improving")
     assert torch.allclose(acc[:, 0], torch.tensor([3.0, 0.25, 6.0]))
     assert torch.allclose(imp[:, 0], torch.tensor([2.0, 0.0, 5.0]))
+
+
+def test_entity_dataset_best_target():
+    """With best_gap the target is the improvement of the chain's best: gain + (best - current), clipped at 0."""
+    X = torch.zeros(3, 128)
+    gain = torch.tensor([2.0, 0.5, 1.0])
+    best_gap = torch.tensor([-1.0, -1.0, 0.0])      # current above best by 1, 1, 0
+    _, T, W = entity_dataset(X, gain, torch.tensor([0, 1, 2]), best_gap=best_gap)
+    assert T[0, 0] == 1.0 and T[1, 3] == 0.0 and T[2, 7] == 1.0
