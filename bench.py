@@ -426,7 +426,7 @@ def other_workload(args):
     return (ex.learned(n_train=8, train_steps=40, n_eval=8, budgets=(10, 20, 40), full_steps=100, epochs=2,
                        target=args.target, weighting=args.weighting, p_plain=args.p_plain, tau=args.tau)
             if args.small else ex.learned(target=args.target, weighting=args.weighting, p_plain=args.p_plain,
-                                          tau=args.tau))
+                                          tau=args.tau, seed=args.seed))
 
 
 def main():
@@ -457,6 +457,8 @@ def main():
                     help="regression weights of logged moves (--workload learned, move target)")
     ap.add_argument("--tau", type=float, default=0.1,
                     help="entity target: temperature of the network's entity choice, kcal/mol (--workload learned)")
+    ap.add_argument("--seed", type=int, default=0,
+                    help="learned: seed of the training and evaluation chains, the MC draws and the network's init")
     ap.add_argument("--p-plain", type=float, default=0.5,
                     help="share of random (plain) moves in the learned search (--workload learned)")
     ap.add_argument("--lds-layout", type=int, choices=[0, 1], default=0,
