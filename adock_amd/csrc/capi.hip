@@ -30,6 +30,8 @@ static int check_buckets(int AM, int KM, int P_MAX) {
     if (KM < 8 || KM > 56 || (KM & 7)) return fail(ADOCK_E_BUCKET, "KM must be a multiple of 8 in 8..56");
     if (P_MAX < 1 || P_MAX > 16256 || !lds_budget_ok(KM, P_MAX, AM))
         return fail(ADOCK_E_LDS, "P_MAX out of range for the per-wave LDS budget");
+    // 64 heavy atoms have at most 2,016 distinct pairs; the one-atom-per-lane kernels track at most 32 rounds of 128
+    if (AM == 64 && P_MAX > 4096) return fail(ADOCK_E_LDS, "P_MAX above 4096 for AM 64 (at most 2,016 distinct pairs)");
     return 0;
 }
 

@@ -81,7 +81,13 @@ template <> struct PairWord<2> {
     static constexpr unsigned M = 255u;
 };
 template <int APL> DEVI unsigned pw_first(unsigned pw) { return pw & PairWord<APL>::M; }
-template <int APL> DEVI unsigned pw_second(unsigned pw) { return (pw >> PairWord<APL>::SH) & PairWord<APL>::M; }
+// a bit-field extract held opaque (an empty asm), so the scaled LDS address takes one shift-add after it: left to
+// itself the compiler folds extract and scaling into a shift, a mask and an add
+template <int APL> DEVI unsigned pw_second(unsigned pw) {
+    unsigned j = __builtin_amdgcn_ubfe(pw, PairWord<APL>::SH, PairWord<APL>::SH);
+    asm("" : "+v"(j));
+    return j;
+}
 template <int APL> DEVI int pw_class(unsigned pw) { return (int)((pw >> PairWord<APL>::CLS) & 3u); }
 template <int APL> constexpr unsigned pw_hyd() { return 1u << PairWord<APL>::CLS; }
 template <int APL> constexpr unsigned pw_hb() { return 2u << PairWord<APL>::CLS; }
@@ -171,6 +177,10 @@ struct ChainCtx {
     float rpx, rpy, rpz, rux, ruy, ruz;
     // uniform
     int na, nt, np, np2, D8;
+    // one atom per lane: bit r set when round r of the compact pair loop (pairs 128 r .. 128 r + 127) holds a
+    // hydrophobic (rhyd) or h-bond (rhb) pair, so the loop tests a bit instead of a ballot per round (a 64-atom
+    // ligand's at most 2,016 pairs take 16 rounds; the library admits P_MAX up to 4,096 = 32 rounds there)
+    unsigned rhyd, rhb;
     float rg;
     int ny, nz;
     float o0, o1, o2, lo0, lo1, lo2, hi0, hi1, hi2, inv_s, pen, fmx0, fmx1, fmx2;
@@ -335,6 +345,14 @@ DEVI ChainCtx<KM, APL> setup_chain(int l, int lane, const LigArgs& L, const Grid
         w.pl[q] = (typename PW::T)word;
     }
     wave_lds_sync();
+    c.rhyd = c.rhb = 0u;
+    if constexpr (APL == 1) {
+        for (int q = 0; q < c.np2; q += WAVE) {
+            const unsigned pw = w.pl[q + lane];
+            if (__ballot((pw & pw_hyd<1>()) != 0u) != 0ull) c.rhyd |= 1u << (q >> 7);
+            if (__ballot((pw & pw_hb<1>()) != 0u) != 0ull) c.rhb |= 1u << (q >> 7);
+        }
+    }
     // each atom's pair-slot list (host-built): read from memory in the evaluations, or (dense variant) turned into the
     // destination slots of its pairs (the atom is the pair's second one when its partner is the pair's first)
     c.incg = L.inc + (long long)l * inc_row(L.P_MAX, AM);
@@ -607,10 +625,17 @@ DEVI float eval_eg(const ChainCtx<KM, APL>& c, const SFConst& sf, const WaveLDS<
             }
         }
     } else {
-        for (int q0 = lane; q0 < c.np2; q0 += 2 * WAVE) {
+        for (int r = 0; r * 2 * WAVE < c.np2; ++r) {
+            const int q0 = r * 2 * WAVE + lane;
             const unsigned pw0 = w.pl[q0], pw1 = w.pl[q0 + WAVE];
-            const bool hyd = __ballot(((pw0 | pw1) & pw_hyd<APL>()) != 0u) != 0ull;
-            const bool hb = __ballot(((pw0 | pw1) & pw_hb<APL>()) != 0u) != 0ull;
+            bool hyd, hb;
+            if constexpr (APL == 1) {
+                hyd = (c.rhyd >> r) & 1u;
+                hb = (c.rhb >> r) & 1u;
+            } else {
+                hyd = __ballot(((pw0 | pw1) & pw_hyd<APL>()) != 0u) != 0ull;
+                hb = __ballot(((pw0 | pw1) & pw_hb<APL>()) != 0u) != 0ull;
+            }
             float4 s0, s1;
             ep_l += pair_slot<APL>(w, pw0, sf, s0, hyd, hb);
             ep_l += pair_slot<APL>(w, pw1, sf, s1, hyd, hb);

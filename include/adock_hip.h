@@ -25,7 +25,8 @@
  * b >= na walks the tail that starts where a's segment ends (h = a); the kernels add the helper's partial sum to the
  * atom's. DEG_MAX is the largest whole (padded) list over all atoms and ligands (at most AM). The kernels use the lists
  * to sum the pair gradient of every atom without atomics; DEG_MAX sizes the per-atom gradient rows of the
- * single-complex kernel. P_MAX is at most 16,256 (16-bit slot offsets; a 128-atom ligand has at most 8,128 pairs).
+ * single-complex kernel. P_MAX is at most 16,256 (16-bit slot offsets; a 128-atom ligand has at most 8,128 pairs),
+ * and at most 4,096 with AM = 64 (64 atoms have at most 2,016 pairs); ADOCK_E_LDS above.
  */
 #ifndef ADOCK_HIP_H
 #define ADOCK_HIP_H

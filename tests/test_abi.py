@@ -59,6 +59,12 @@ def test_bucket_validation_without_gpu():
     for am, km, want in ((96, 16, -1), (128, 56, -1), (160, 8, -2), (64, 64, -2), (128, 60, -2)):
         args[0], args[1] = am, km
         assert L.adock_eg(*args) == want, (am, km)
+    # AM 64 holds at most 2,016 distinct pairs: P_MAX up to 4,096 passes the bucket check, above it is refused
+    args[0], args[1], args[13] = 64, 16, 4096
+    assert L.adock_eg(*args) == -1
+    args[13] = 4097
+    assert L.adock_eg(*args) == -3 and b"P_MAX" in L.adock_last_error()
+    args[13] = 1
     # fitted grid builder: null pointers and more receptor ext types than the LDS table holds are rejected,
     # empty grids are a no-op
     fit = [None, None, 1, None, None, 1, None, 16, 0.0, 0.0, 0.0, 0.375, 4, 4, 4, None, None, None]
