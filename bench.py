@@ -409,6 +409,10 @@ def other_workload(args):
         tf = out["evals_per_s"] * fl / 1e12
         out["roofline"] = {"bound": "valu", "compute_unit": FP32_VALU, "achieved": tf, "peak": PEAK_FP32_TFLOPS,
                            "unit": "TFLOP/s", "frac": tf / PEAK_FP32_TFLOPS, "flops_per_eval": fl}
+        if "large" in out:
+            tl = out["large"]["evals_per_s"] * fl / 1e12
+            out["large"]["roofline"] = {"bound": "valu", "achieved": tl, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                                        "frac": tl / PEAK_FP32_TFLOPS}
         return out
     if wl == "grids":
         out = ex.grids()
