@@ -33,3 +33,17 @@ def test_entity_dataset_best_target():
     best_gap = torch.tensor([-1.0, -1.0, 0.0])      # current above best by 1, 1, 0
     _, T, W = entity_dataset(X, gain, torch.tensor([0, 1, 2]), best_gap=best_gap)
     assert T[0, 0] == 1.0 and T[1, 3] == 0.0 and T[2, 7] == 1.0
+
+
+def test_time_to_gap_interpolates_in_log_log():
+    """The matched-gap time: log-log interpolation between the last budget above the target and the first at or
+    below it; exact at a budget; None when no budget reaches the target or the first one already does."""
+    import math
+    from adock_amd.experiments import _time_to_gap
+    rows = [{"m_gap": 0.4, "m_s": 1.0}, {"m_gap": 0.1, "m_s": 4.0}, {"m_gap": 0.025, "m_s": 16.0}]
+    # gap 0.2 lies halfway (in log) between 0.4 and 0.1, so the time is halfway (in log) between 1 and 4
+    assert math.isclose(_time_to_gap(rows, "m", 0.2), 2.0, rel_tol=1e-12)
+    assert math.isclose(_time_to_gap(rows, "m", 0.05), 8.0, rel_tol=1e-12)
+    assert math.isclose(_time_to_gap(rows, "m", 0.1), 4.0, rel_tol=1e-12)
+    assert _time_to_gap(rows, "m", 0.01) is None
+    assert _time_to_gap(rows, "m", 0.5) is None
